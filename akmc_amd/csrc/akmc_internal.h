@@ -1,0 +1,101 @@
+// Internal device/host structures of libakmc_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define AKMC_MAXSPEC 8
+#define AKMC_MAXPAIR (AKMC_MAXSPEC * AKMC_MAXSPEC)
+// neighbour entry: bits 0-19 local atom index, 20-23 / 24-27 / 28-31 the
+// image shift (+7) along a, b, c.  Minimum-image lists store shift 0.
+#define AKMC_NB_JMASK 0xFFFFFu
+#define AKMC_NB_ZERO_SHIFT (0x777u << 20)
+#define AKMC_MAX_WALKER_ATOMS (1 << 20)
+
+// Sutton-Chen pair parameters, FP32 for the fast path and FP64 for the
+// verification path
+template <typename T>
+struct SCCoef {
+    T a, n, m, eps;
+    T rho_coef;  // (eps c)^2
+    T fs_coef;   // n eps
+    T fm_coef;   // 0.5 m (eps c)^2
+};
+struct SCPairDev {
+    SCCoef<float> f;
+    SCCoef<double> d;
+    int n_int;   // n when n is a non-negative integer <= 64, else -1
+    int m_int;
+    int on;      // 1 when the species pair has a potential
+};
+
+struct IonPairDev {
+    float A, inv_rho, C;
+    double Ad, inv_rhod, Cd;
+    int on;
+};
+
+struct FieldDev {
+    int kind;        // 0 many-body SC, 1 ionic
+    int nspec;
+    int min_image;
+    int do_ewald;
+    double rc;
+    double rc2;      // FP64 cutoff decision
+    double rlist2;   // (rc + skin)^2
+    double half_skin2;
+    double eta;
+    double cfact;
+    SCPairDev sc[AKMC_MAXPAIR];
+    IonPairDev ion[AKMC_MAXPAIR];
+};
+
+struct WalkerDev {
+    int off, n;          // first atom, atom count
+    int cell_off;        // first cell of this walker's grid
+    int nc[3];           // cells per axis
+    int shells[3];       // explicit image shells (no-image fields)
+    int k_off, nk;       // slice of the k-vector arrays
+    int gmax;            // largest |n| along any axis of this walker's k set
+    double L[3], invL[3];
+};
+
+// per-walker outputs of one force evaluation (FP64)
+struct WalkerOut {
+    double total, rcp, real, many, vdw, self;
+    double stress[6];
+};
+
+struct BatchDev {
+    int A, W, maxnb, ncells_total, nk_total, G1, maxnk;
+    const FieldDev *field;
+    const WalkerDev *walkers;
+    const int *wk;        // atom -> walker
+    double *x, *y, *z;    // positions (FP64 master copy)
+    double *xr, *yr, *zr; // positions at the last list build
+    const int *type;
+    const double *q;
+    double *fx, *fy, *fz;
+    uint32_t *nbr;        // [maxnb][A] column-major
+    int *nnb;
+    int *rebuild;         // per walker flag
+    int *overflow;        // sticky capacity flag
+    int *cell_count, *cell_start, *cell_atoms, *atom_cell, *atom_slot;
+    double *rho;          // SC density
+    float *inv_sqrt_rho;  // SC 1/sqrt(rho), -1 when rho < 1e-8
+    double *e_a, *e_b;    // per-atom energies (SC: many, pair; ionic: real, vdw)
+    double *e_c;          // ionic: reciprocal per atom
+    double *st_atom;      // [6][A] per-atom stress when requested
+    // Ewald
+    const int *kn;        // [3][nk_total] integer k indices
+    const double *kv;     // [3][nk_total] k vectors
+    const double *g0, *g1;
+    double2 *ptab;        // [A][3][G1] e^{i n theta}
+    double *Ck, *Sk;      // structure factors
+    WalkerOut *out;
+};
+
+// launcher (field_kernels.hip): list maintenance + one force evaluation of
+// every walker; hf is the host copy of *b.field
+void akmc_launch_forces_impl(const BatchDev &b, const FieldDev &hf, int fp64, int want_stress,
+                             hipStream_t s);
+extern "C" unsigned long long *akmc_rebuild_counter_ptr();

@@ -1,0 +1,764 @@
+// C ABI of libakmc_hip.so: potentials text, field setup and walker batches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/akmc.h"
+#include "akmc_internal.h"
+
+namespace {
+
+// e^2/(4 pi eps0) in eV A, the value with which the reference reproduces
+// its own example energy (docs/examples/gdc/basis:3)
+constexpr double kCTOEV = 14.3997584;
+constexpr double kPI = 3.14159265358979323846;
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(AKMC_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+std::vector<std::string> split_ws(const std::string &s)
+{
+    std::istringstream is(s);
+    std::vector<std::string> w;
+    std::string t;
+    while (is >> t) w.push_back(t);
+    return w;
+}
+
+std::string lower(std::string s)
+{
+    for (auto &c : s) c = (char)std::tolower((unsigned char)c);
+    return s;
+}
+
+struct SpeciesRec {
+    std::string name;
+    double mass, charge;
+};
+struct SCPot {
+    std::string A, B;
+    double eps, a, n, m, c;
+};
+struct BuckPot {
+    std::string A, B;
+    double Ab, rho, C;
+};
+
+}  // namespace
+
+struct akmc_field_s {
+    int kind = -1;  // 0 many-body, 1 ionic
+    double rc = 0.0;
+    bool min_image = true;
+    bool nbrlist = false;
+    double verlet_shell = 1.0;
+    int do_ewald = 1;  // pair default (potentials/pair/Field.h:28)
+    double madelung_acc = 1.0e-6;
+    bool explicit_ewald = false;
+    double eta = 0.0;
+    int gcell[3] = {0, 0, 0};
+    double skin = 0.6;
+    std::vector<SpeciesRec> species;
+    std::vector<SCPot> sc;
+    std::vector<BuckPot> buck;
+    FieldDev host{};
+    FieldDev *dev = nullptr;
+    int table[AKMC_MAXPAIR];
+    // cached single-walker batch behind akmc_field_forces
+    akmc_batch_t single = nullptr;
+    int single_n = -1;
+    double single_lat[9] = {0};
+    int nk_last = 0;
+    int fp64 = 0;
+};
+
+struct akmc_batch_s {
+    akmc_field_t f = nullptr;
+    int W = 0, A = 0;
+    std::vector<WalkerDev> hw;
+    std::vector<int> hk_n;       // [3][nk]
+    std::vector<double> hk_v, hg0, hg1;
+    BatchDev d{};
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    std::vector<WalkerOut> hout;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long rebuild_base = 0;
+};
+
+namespace {
+
+int parse_potentials(akmc_field_t f, const char *text)
+{
+    std::istringstream in(text ? text : "");
+    std::string line;
+    bool closed = false;
+    auto next_line = [&](std::string &l) -> bool {
+        while (std::getline(in, l)) {
+            if (!split_ws(l).empty()) return true;
+        }
+        return false;
+    };
+    while (next_line(line)) {
+        auto w = split_ws(lower(line));
+        const std::string &k = w[0];
+        try {
+            if (k == "cutoff") {
+                f->rc = std::stod(w.at(1));
+            } else if (k == "nbrlist") {
+                f->nbrlist = true;
+                f->verlet_shell = std::stod(w.at(1));
+            } else if (k == "noimage") {
+                f->min_image = false;
+            } else if (k == "ewald") {
+                const std::string &sub = w.at(1);
+                if (sub == "precis") {
+                    f->do_ewald = 1;
+                    f->madelung_acc = std::stod(w.at(2));
+                } else if (sub == "none") {
+                    f->do_ewald = 0;
+                } else {
+                    return fail(AKMC_EUNSUPPORTED, "ewald %s is not supported on the device", sub.c_str());
+                }
+            } else if (k == "species") {
+                int n = std::stoi(w.at(1));
+                for (int s = 0; s < n; ++s) {
+                    std::string sl;
+                    if (!next_line(sl)) return fail(AKMC_EINVAL, "species: missing line");
+                    auto sw = split_ws(sl);  // species lines keep their case (Species.cpp:45-60)
+                    f->species.push_back({sw.at(0), std::stod(sw.at(1)), std::stod(sw.at(2))});
+                }
+            } else if (k == "many" || k == "manybody") {
+                f->kind = 0;
+                int n = std::stoi(w.at(1));
+                for (int p = 0; p < n; ++p) {
+                    std::string kl, pl;
+                    if (!next_line(kl) || !next_line(pl)) return fail(AKMC_EINVAL, "manybody: missing lines");
+                    std::string key = lower(split_ws(kl).at(0));
+                    if (key != "suttonchen")
+                        return fail(AKMC_EUNSUPPORTED, "many-body form '%s' is not supported on the device",
+                                    key.c_str());
+                    auto pw = split_ws(pl);  // A B eps a n m c (ManyBody.cpp:148-163)
+                    f->sc.push_back({pw.at(0), pw.at(1), std::stod(pw.at(2)), std::stod(pw.at(3)),
+                                     std::stod(pw.at(4)), std::stod(pw.at(5)), std::stod(pw.at(6))});
+                }
+            } else if (k == "twobody") {
+                f->kind = 1;
+                int n = std::stoi(w.at(1));
+                for (int p = 0; p < n; ++p) {
+                    std::string kl, pl;
+                    if (!next_line(kl) || !next_line(pl)) return fail(AKMC_EINVAL, "twobody: missing lines");
+                    std::string key = lower(split_ws(kl).at(0));
+                    if (key != "buck")
+                        return fail(AKMC_EUNSUPPORTED, "two-body form '%s' is not supported on the device",
+                                    key.c_str());
+                    auto pw = split_ws(pl);  // A B A rho C (TwoBody.cpp:98-112)
+                    f->buck.push_back({pw.at(0), pw.at(1), std::stod(pw.at(2)), std::stod(pw.at(3)),
+                                       std::stod(pw.at(4))});
+                }
+            } else if (k == "close") {
+                closed = true;
+                break;
+            } else if (k == "maxnbrs") {
+                // accepted and ignored, as the reference can never match it
+            } else {
+                return fail(AKMC_EINVAL, "unrecognised key word used in potentials file %s", k.c_str());
+            }
+        } catch (const std::exception &) {
+            return fail(AKMC_EINVAL, "malformed potentials line: %s", line.c_str());
+        }
+    }
+    (void)closed;
+    if (f->kind < 0) return fail(AKMC_EINVAL, "potentials text has no manybody or twobody section");
+    if (f->species.empty()) return fail(AKMC_EINVAL, "potentials text has no species");
+    if ((int)f->species.size() > AKMC_MAXSPEC) return fail(AKMC_ECAPACITY, "more than %d species", AKMC_MAXSPEC);
+    return AKMC_SUCCESS;
+}
+
+// species-pair tables: -1 default for many-body (ManyBody.cpp:869), 0 for
+// pair potentials (TwoBody.cpp:509)
+void build_tables(akmc_field_t f)
+{
+    int ns = (int)f->species.size();
+    FieldDev &h = f->host;
+    std::memset(&h, 0, sizeof h);
+    h.kind = f->kind;
+    h.nspec = ns;
+    h.min_image = f->min_image ? 1 : 0;
+    h.do_ewald = f->kind == 1 ? f->do_ewald : 0;
+    h.rc = f->rc;
+    h.rc2 = f->rc * f->rc;
+    double rl = f->rc + f->skin;
+    h.rlist2 = rl * rl;
+    h.half_skin2 = 0.25 * f->skin * f->skin;
+    h.cfact = kCTOEV;
+    for (int i = 0; i < AKMC_MAXPAIR; ++i) f->table[i] = f->kind == 0 ? -1 : 0;
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j < ns; ++j) {
+            int np = f->kind == 0 ? (int)f->sc.size() : (int)f->buck.size();
+            for (int k = 0; k < np; ++k) {
+                const std::string &A = f->kind == 0 ? f->sc[k].A : f->buck[k].A;
+                const std::string &B = f->kind == 0 ? f->sc[k].B : f->buck[k].B;
+                if (f->species[i].name == A && f->species[j].name == B) {
+                    f->table[j * ns + i] = k;
+                    f->table[i * ns + j] = k;
+                }
+            }
+        }
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j < ns; ++j) {
+            int k = f->table[i * ns + j];
+            if (f->kind == 0) {
+                SCPairDev &p = h.sc[i * ns + j];
+                if (k < 0) continue;
+                const SCPot &s = f->sc[k];
+                p.on = 1;
+                p.d = {s.a, s.n, s.m, s.eps, (s.eps * s.c) * (s.eps * s.c), s.n * s.eps,
+                       0.5 * s.m * (s.eps * s.c) * (s.eps * s.c)};
+                p.f = {(float)p.d.a, (float)p.d.n, (float)p.d.m, (float)p.d.eps, (float)p.d.rho_coef,
+                       (float)p.d.fs_coef, (float)p.d.fm_coef};
+                auto as_int = [](double e) { return (e >= 0 && e <= 64 && std::floor(e) == e) ? (int)e : -1; };
+                p.n_int = as_int(s.n);
+                p.m_int = as_int(s.m);
+            } else {
+                IonPairDev &p = h.ion[i * ns + j];
+                if (k < 0 || k >= (int)f->buck.size()) continue;
+                const BuckPot &b = f->buck[k];
+                p.on = 1;
+                p.Ad = b.Ab;
+                p.inv_rhod = 1.0 / b.rho;
+                p.Cd = b.C;
+                p.A = (float)p.Ad;
+                p.inv_rho = (float)p.inv_rhod;
+                p.C = (float)p.Cd;
+            }
+        }
+}
+
+bool orthorhombic(const double *lat)
+{
+    return lat[1] == 0.0 && lat[2] == 0.0 && lat[3] == 0.0 && lat[5] == 0.0 && lat[6] == 0.0 && lat[7] == 0.0 &&
+           lat[0] > 0 && lat[4] > 0 && lat[8] > 0;
+}
+
+// eta and reciprocal shells from the Madelung accuracy
+// (potentials/pair/Field.cpp:129-132, Ewald.cpp:88-120) for an orthorhombic cell
+void ewald_params(const akmc_field_s *f, const double *L, double &eta, int g[3])
+{
+    if (f->explicit_ewald) {
+        eta = f->eta;
+        for (int a = 0; a < 3; ++a) g[a] = f->gcell[a];
+        return;
+    }
+    double eps = std::fmin(std::fabs(f->madelung_acc), 0.5);
+    double tol = std::sqrt(std::fabs(std::log(eps * f->rc)));
+    eta = std::sqrt(std::fabs(std::log(eps * f->rc * tol))) / f->rc;
+    double tol1 = std::sqrt(-std::log(eps * f->rc * std::pow(2.0 * tol * eta, 2)));
+    for (int a = 0; a < 3; ++a) g[a] = (int)std::rint(0.25 + L[a] * eta * tol1 / kPI);
+}
+
+// half-space k set (Ewald.cpp:188-319) for an orthorhombic cell; rcpcut is
+// min_a g_a / L_a
+void kvectors(const akmc_field_s *f, const double *L, double eta, const int g[3], std::vector<int> &kn,
+              std::vector<double> &kv, std::vector<double> &g0, std::vector<double> &g1)
+{
+    double cut = std::min({g[0] / L[0], g[1] / L[1], g[2] / L[2]});
+    double cut2 = cut * cut, vol = L[0] * L[1] * L[2];
+    double etasq = eta * eta, gsqfct = 1.0 / (4.0 * etasq);
+    double gfct0 = kCTOEV * kPI / (etasq * vol), gfct1 = gsqfct * gfct0;
+    for (int nx = 0; nx <= g[0]; ++nx)
+        for (int ny = -g[1]; ny <= g[1]; ++ny)
+            for (int nz = -g[2]; nz <= g[2]; ++nz) {
+                double x = nx / L[0], y = ny / L[1], z = nz / L[2];
+                double gsq = x * x + y * y + z * z;
+                if (gsq < cut2 && gsq > 1.0e-8) {
+                    double kx = 2.0 * kPI * x, ky = 2.0 * kPI * y, kz = 2.0 * kPI * z;
+                    double ksq = kx * kx + ky * ky + kz * kz, xg = ksq * gsqfct;
+                    double gzz = std::exp(-xg) / xg;
+                    kn.push_back(nx);
+                    kn.push_back(ny);
+                    kn.push_back(nz);
+                    kv.push_back(kx);
+                    kv.push_back(ky);
+                    kv.push_back(kz);
+                    g0.push_back(kx < 1.0e-8 ? gzz * gfct0 : gzz * gfct0 * 2.0);
+                    g1.push_back(-2.0 * gzz * (1.0 + 1.0 / xg) * gfct1);
+                }
+            }
+    (void)f;
+}
+
+int check_cell(const akmc_field_s *f, const double *lat)
+{
+    if (!orthorhombic(lat)) return fail(AKMC_EUNSUPPORTED, "only orthorhombic cells are supported on the device");
+    if (f->rc < 1.0) return fail(AKMC_EINVAL, "short range cutoff is too small : %g", f->rc);
+    double mind = std::min({lat[0], lat[4], lat[8]});
+    // image-convention checks: metal Field.cpp:53-60, pair Field.cpp:99-106
+    double radius = f->kind == 0 ? (f->nbrlist ? f->rc + f->verlet_shell : f->rc) : f->rc + f->verlet_shell;
+    if (f->min_image && radius > mind / 2)
+        return fail(AKMC_EINVAL, "short range cutoff too large for simulation box: cutoff %g minimum dimension %g",
+                    f->rc, mind);
+    return AKMC_SUCCESS;
+}
+
+template <typename T>
+int dalloc(akmc_batch_t b, T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc((void **)p, n * sizeof(T)));
+    b->allocs.push_back((void *)*p);
+    return AKMC_SUCCESS;
+}
+
+#define TRY(expr)                      \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != AKMC_SUCCESS) return rc_; \
+    } while (0)
+
+int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akmc_batch_t b)
+{
+    b->f = f;
+    b->W = W;
+    b->hw.resize(W);
+    int A = 0, ncells = 0, G1 = 1, maxnk = 0;
+    double rl = f->rc + f->skin;
+    double max_density = 0.0;
+    for (int w = 0; w < W; ++w) {
+        const double *lw = lat + 9 * w;
+        TRY(check_cell(f, lw));
+        if (natoms[w] <= 0 || natoms[w] > AKMC_MAX_WALKER_ATOMS)
+            return fail(AKMC_EINVAL, "walker %d: atom count %d out of range", w, natoms[w]);
+        WalkerDev &h = b->hw[w];
+        std::memset(&h, 0, sizeof h);
+        h.off = A;
+        h.n = natoms[w];
+        A += h.n;
+        h.L[0] = lw[0];
+        h.L[1] = lw[4];
+        h.L[2] = lw[8];
+        for (int a = 0; a < 3; ++a) {
+            h.invL[a] = 1.0 / h.L[a];
+            h.nc[a] = std::max(1, (int)std::floor(h.L[a] / rl));
+            h.shells[a] = (int)(rl / h.L[a]) + 1;
+            if (h.shells[a] > 7) return fail(AKMC_ECAPACITY, "walker %d: more than 7 image shells", w);
+        }
+        h.cell_off = ncells;
+        ncells += h.nc[0] * h.nc[1] * h.nc[2];
+        max_density = std::max(max_density, h.n / (h.L[0] * h.L[1] * h.L[2]));
+        if (f->kind == 1 && f->do_ewald) {
+            double eta;
+            int g[3];
+            ewald_params(f, h.L, eta, g);
+            f->host.eta = eta;
+            std::vector<int> kn;
+            std::vector<double> kv, g0, g1;
+            kvectors(f, h.L, eta, g, kn, kv, g0, g1);
+            h.k_off = (int)b->hg0.size();
+            h.nk = (int)g0.size();
+            h.gmax = std::max({g[0], g[1], g[2]});
+            G1 = std::max(G1, h.gmax + 1);
+            maxnk = std::max(maxnk, h.nk);
+            b->hk_n.insert(b->hk_n.end(), kn.begin(), kn.end());
+            b->hk_v.insert(b->hk_v.end(), kv.begin(), kv.end());
+            b->hg0.insert(b->hg0.end(), g0.begin(), g0.end());
+            b->hg1.insert(b->hg1.end(), g1.begin(), g1.end());
+            f->nk_last = h.nk;
+        }
+    }
+    b->A = A;
+    int nk = (int)b->hg0.size();
+    // neighbour capacity: 1.6x the expected count in the list sphere + 64
+    double expect = 4.0 / 3.0 * kPI * rl * rl * rl * max_density;
+    int maxnb = ((int)(1.6 * expect) + 64 + 31) / 32 * 32;
+    BatchDev &d = b->d;
+    std::memset(&d, 0, sizeof d);
+    d.A = A;
+    d.W = W;
+    d.maxnb = maxnb;
+    d.ncells_total = ncells;
+    d.nk_total = nk;
+    d.G1 = G1;
+    d.maxnk = maxnk;
+    HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&b->ev0));
+    HIPCHK(hipEventCreate(&b->ev1));
+    WalkerDev *wd;
+    int *wk, *tp;
+    double *q;
+    TRY(dalloc(b, &wd, W));
+    TRY(dalloc(b, &wk, A));
+    TRY(dalloc(b, &tp, A));
+    TRY(dalloc(b, &q, A));
+    d.walkers = wd;
+    d.wk = wk;
+    d.type = tp;
+    d.q = q;
+    TRY(dalloc(b, &d.x, A));
+    TRY(dalloc(b, &d.y, A));
+    TRY(dalloc(b, &d.z, A));
+    TRY(dalloc(b, &d.xr, A));
+    TRY(dalloc(b, &d.yr, A));
+    TRY(dalloc(b, &d.zr, A));
+    TRY(dalloc(b, &d.fx, A));
+    TRY(dalloc(b, &d.fy, A));
+    TRY(dalloc(b, &d.fz, A));
+    TRY(dalloc(b, &d.nbr, (size_t)maxnb * A));
+    TRY(dalloc(b, &d.nnb, A));
+    TRY(dalloc(b, &d.rebuild, W));
+    TRY(dalloc(b, &d.overflow, 1));
+    TRY(dalloc(b, &d.cell_count, ncells));
+    TRY(dalloc(b, &d.cell_start, ncells));
+    TRY(dalloc(b, &d.cell_atoms, A));
+    TRY(dalloc(b, &d.atom_cell, A));
+    TRY(dalloc(b, &d.atom_slot, A));
+    TRY(dalloc(b, &d.rho, A));
+    TRY(dalloc(b, &d.inv_sqrt_rho, A));
+    TRY(dalloc(b, &d.e_a, A));
+    TRY(dalloc(b, &d.e_b, A));
+    TRY(dalloc(b, &d.e_c, A));
+    TRY(dalloc(b, &d.st_atom, (size_t)6 * A));
+    TRY(dalloc(b, &d.out, W));
+    if (nk > 0) {
+        int *kn;
+        double *kv, *g0, *g1;
+        TRY(dalloc(b, &kn, (size_t)3 * nk));
+        TRY(dalloc(b, &kv, (size_t)3 * nk));
+        TRY(dalloc(b, &g0, nk));
+        TRY(dalloc(b, &g1, nk));
+        // transpose to [3][nk]
+        std::vector<int> tn(3 * (size_t)nk);
+        std::vector<double> tv(3 * (size_t)nk);
+        for (int k = 0; k < nk; ++k)
+            for (int a = 0; a < 3; ++a) {
+                tn[(size_t)a * nk + k] = b->hk_n[3 * (size_t)k + a];
+                tv[(size_t)a * nk + k] = b->hk_v[3 * (size_t)k + a];
+            }
+        HIPCHK(hipMemcpy(kn, tn.data(), tn.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(kv, tv.data(), tv.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(g0, b->hg0.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(g1, b->hg1.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+        d.kn = kn;
+        d.kv = kv;
+        d.g0 = g0;
+        d.g1 = g1;
+        TRY(dalloc(b, &d.ptab, (size_t)A * 3 * G1));
+        TRY(dalloc(b, &d.Ck, nk));
+        TRY(dalloc(b, &d.Sk, nk));
+    }
+    std::vector<int> hwk(A);
+    for (int w = 0; w < W; ++w)
+        for (int i = 0; i < b->hw[w].n; ++i) hwk[b->hw[w].off + i] = w;
+    HIPCHK(hipMemcpy(wd, b->hw.data(), W * sizeof(WalkerDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(wk, hwk.data(), A * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(d.rebuild, 0, W * sizeof(int)));
+    HIPCHK(hipMemset(d.overflow, 0, sizeof(int)));
+    HIPCHK(hipMemset(d.e_c, 0, A * sizeof(double)));
+    HIPCHK(hipMemset((void *)d.q, 0, A * sizeof(double)));
+    // device copy of the field (the eta of the last walker; all walkers of a
+    // batch share the field's eta rule)
+    if (!f->dev) HIPCHK(hipMalloc(&f->dev, sizeof(FieldDev)));
+    HIPCHK(hipMemcpy(f->dev, &f->host, sizeof(FieldDev), hipMemcpyHostToDevice));
+    d.field = f->dev;
+    b->hout.resize(W);
+    unsigned long long *cnt = akmc_rebuild_counter_ptr();
+    if (!cnt) return fail(AKMC_EHIP, "rebuild counter allocation failed");
+    HIPCHK(hipMemcpy(&b->rebuild_base, cnt, sizeof b->rebuild_base, hipMemcpyDeviceToHost));
+    return AKMC_SUCCESS;
+}
+
+void batch_free(akmc_batch_t b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (void *p : b->allocs) (void)hipFree(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int run_forces(akmc_batch_t b, int want_stress)
+{
+    akmc_launch_forces_impl(b->d, b->f->host, b->f->fp64, want_stress, b->stream);
+    HIPCHK(hipGetLastError());
+    return AKMC_SUCCESS;
+}
+
+int check_overflow(akmc_batch_t b)
+{
+    int ov = 0;
+    HIPCHK(hipMemcpyAsync(&ov, b->d.overflow, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (ov) return fail(AKMC_ECAPACITY, "neighbour list capacity %d exceeded", b->d.maxnb);
+    return AKMC_SUCCESS;
+}
+
+int fetch_out(akmc_batch_t b)
+{
+    HIPCHK(hipMemcpyAsync(b->hout.data(), b->d.out, b->W * sizeof(WalkerOut), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return AKMC_SUCCESS;
+}
+
+void to_energy(const WalkerOut &o, akmc_energy *e)
+{
+    e->total = o.total;
+    e->rcp = o.rcp;
+    e->real = o.real;
+    e->many = o.many;
+    e->vdw = o.vdw;
+    e->ext = 0.0;
+}
+
+// in-place wraps of the caller's arrays, as Field::calculateForces does
+// (potentials/metal/Field.cpp:1855-1920)
+void host_wrap(bool min_image, const double *lat, int n, double *x, double *y, double *z)
+{
+    double L[3] = {lat[0], lat[4], lat[8]};
+    for (int i = 0; i < n; ++i) {
+        double s[3] = {x[i] / L[0], y[i] / L[1], z[i] / L[2]};
+        for (int a = 0; a < 3; ++a) {
+            if (min_image) {
+                s[a] -= std::rint(s[a]);
+            } else {
+                if (s[a] < 0.0) s[a] += 1.0;
+                if (s[a] >= 1.0) s[a] -= 1.0;
+            }
+        }
+        x[i] = s[0] * L[0];
+        y[i] = s[1] * L[1];
+        z[i] = s[2] * L[2];
+    }
+}
+
+int ensure_single(akmc_field_t f, int n, const double *lat)
+{
+    bool same = f->single && f->single_n == n && std::memcmp(f->single_lat, lat, sizeof f->single_lat) == 0;
+    if (same) return AKMC_SUCCESS;
+    if (f->single) batch_free(f->single);
+    f->single = new akmc_batch_s();
+    int rc = batch_build(f, 1, &n, lat, f->single);
+    if (rc != AKMC_SUCCESS) {
+        batch_free(f->single);
+        f->single = nullptr;
+        return rc;
+    }
+    f->single_n = n;
+    std::memcpy(f->single_lat, lat, sizeof f->single_lat);
+    return AKMC_SUCCESS;
+}
+
+int upload(akmc_batch_t b, const double *x, const double *y, const double *z, const int *type, const double *q)
+{
+    size_t A = b->A;
+    int ns = (int)b->f->species.size();
+    for (size_t i = 0; i < A; ++i)
+        if (type[i] < 0 || type[i] >= ns) return fail(AKMC_EINVAL, "atom %zu: species %d out of range", i, type[i]);
+    HIPCHK(hipMemcpyAsync(b->d.x, x, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d.y, y, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d.z, z, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync((void *)b->d.type, type, A * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    if (q) HIPCHK(hipMemcpyAsync((void *)b->d.q, q, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    // a fresh configuration always gets fresh lists
+    std::vector<int> ones(b->W, 1);
+    HIPCHK(hipMemcpyAsync(b->d.rebuild, ones.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return AKMC_SUCCESS;
+}
+
+}  // namespace
+
+// ======================================================================
+extern "C" {
+
+const char *akmc_last_error(void) { return g_err.c_str(); }
+
+int akmc_version(void) { return 1; }
+
+int akmc_field_create(const char *text, akmc_field_t *out)
+{
+    if (!out) return fail(AKMC_EINVAL, "null output handle");
+    akmc_field_t f = new akmc_field_s();
+    int rc = parse_potentials(f, text);
+    if (rc != AKMC_SUCCESS) {
+        delete f;
+        return rc;
+    }
+    build_tables(f);
+    *out = f;
+    return AKMC_SUCCESS;
+}
+
+void akmc_field_destroy(akmc_field_t f)
+{
+    if (!f) return;
+    if (f->single) batch_free(f->single);
+    if (f->dev) (void)hipFree(f->dev);
+    delete f;
+}
+
+int akmc_field_kind(akmc_field_t f) { return f ? f->kind : AKMC_EINVAL; }
+
+int akmc_field_set_ewald(akmc_field_t f, double eta, int gx, int gy, int gz)
+{
+    if (!f || eta <= 0 || gx < 1 || gy < 1 || gz < 1) return fail(AKMC_EINVAL, "bad Ewald parameters");
+    f->explicit_ewald = true;
+    f->eta = eta;
+    f->gcell[0] = gx;
+    f->gcell[1] = gy;
+    f->gcell[2] = gz;
+    return AKMC_SUCCESS;
+}
+
+int akmc_field_set_skin(akmc_field_t f, double skin)
+{
+    if (!f || skin < 0) return fail(AKMC_EINVAL, "bad skin");
+    f->skin = skin;
+    build_tables(f);
+    return AKMC_SUCCESS;
+}
+
+int akmc_field_set_precision(akmc_field_t f, int bits)
+{
+    if (!f || (bits != 32 && bits != 64)) return fail(AKMC_EINVAL, "precision must be 32 or 64");
+    f->fp64 = bits == 64;
+    return AKMC_SUCCESS;
+}
+
+int akmc_field_setup(akmc_field_t f, const double lat[9], int max_atoms)
+{
+    if (!f || !lat || max_atoms <= 0) return fail(AKMC_EINVAL, "bad setup arguments");
+    TRY(check_cell(f, lat));
+    return ensure_single(f, max_atoms, lat);
+}
+
+int akmc_field_num_kvectors(akmc_field_t f) { return f ? f->nk_last : 0; }
+
+int akmc_field_forces(akmc_field_t f, double *x, double *y, double *z, double *fx, double *fy, double *fz,
+                      double stress[6], const double lat[9], const double *charge, const int *type,
+                      const int *frozen, int n, akmc_energy *e)
+{
+    (void)frozen;  // unused by the reference's force call as well
+    if (!f || !x || !y || !z || !fx || !fy || !fz || !type || n <= 0) return fail(AKMC_EINVAL, "bad arguments");
+    if (f->kind == 1 && f->do_ewald && !charge) return fail(AKMC_EINVAL, "ionic field needs charges");
+    TRY(ensure_single(f, n, lat));
+    akmc_batch_t b = f->single;
+    TRY(upload(b, x, y, z, type, charge));
+    TRY(run_forces(b, stress != nullptr));
+    TRY(check_overflow(b));
+    HIPCHK(hipMemcpyAsync(fx, b->d.fx, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(fy, b->d.fy, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(fz, b->d.fz, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    TRY(fetch_out(b));
+    if (e) to_energy(b->hout[0], e);
+    if (stress)
+        for (int c = 0; c < 6; ++c) stress[c] = b->hout[0].stress[c];
+    host_wrap(f->min_image, lat, n, x, y, z);
+    return AKMC_SUCCESS;
+}
+
+int akmc_field_energy(akmc_field_t f, double *x, double *y, double *z, const double lat[9], const double *charge,
+                      const int *type, int n, akmc_energy *e)
+{
+    if (!f || n <= 0) return fail(AKMC_EINVAL, "bad arguments");
+    std::vector<double> fx(n), fy(n), fz(n);
+    return akmc_field_forces(f, x, y, z, fx.data(), fy.data(), fz.data(), nullptr, lat, charge, type, nullptr, n, e);
+}
+
+int akmc_batch_create(akmc_field_t f, int nwalkers, const int *natoms, const double *lat, akmc_batch_t *out)
+{
+    if (!f || nwalkers <= 0 || !natoms || !lat || !out) return fail(AKMC_EINVAL, "bad batch arguments");
+    akmc_batch_t b = new akmc_batch_s();
+    int rc = batch_build(f, nwalkers, natoms, lat, b);
+    if (rc != AKMC_SUCCESS) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return AKMC_SUCCESS;
+}
+
+void akmc_batch_destroy(akmc_batch_t b) { batch_free(b); }
+
+int akmc_batch_upload(akmc_batch_t b, const double *x, const double *y, const double *z, const int *type,
+                      const double *charge, const int *frozen)
+{
+    (void)frozen;
+    if (!b || !x || !y || !z || !type) return fail(AKMC_EINVAL, "bad arguments");
+    return upload(b, x, y, z, type, charge);
+}
+
+int akmc_batch_download(akmc_batch_t b, double *x, double *y, double *z, double *fx, double *fy, double *fz)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    size_t A = b->A;
+    if (x) HIPCHK(hipMemcpyAsync(x, b->d.x, A * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (y) HIPCHK(hipMemcpyAsync(y, b->d.y, A * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (z) HIPCHK(hipMemcpyAsync(z, b->d.z, A * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (fx) HIPCHK(hipMemcpyAsync(fx, b->d.fx, A * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (fy) HIPCHK(hipMemcpyAsync(fy, b->d.fy, A * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (fz) HIPCHK(hipMemcpyAsync(fz, b->d.fz, A * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_forces(akmc_batch_t b, akmc_energy *energies)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    TRY(run_forces(b, 0));
+    TRY(check_overflow(b));
+    TRY(fetch_out(b));
+    if (energies)
+        for (int w = 0; w < b->W; ++w) to_energy(b->hout[w], energies + w);
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_forces_repeat(akmc_batch_t b, int nrep, double *ms)
+{
+    if (!b || nrep <= 0) return fail(AKMC_EINVAL, "bad arguments");
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    for (int r = 0; r < nrep; ++r) TRY(run_forces(b, 0));
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    if (ms) *ms = t;
+    return check_overflow(b);
+}
+
+int akmc_batch_total_atoms(akmc_batch_t b) { return b ? b->A : 0; }
+
+long long akmc_batch_rebuilds(akmc_batch_t b)
+{
+    if (!b) return 0;
+    unsigned long long v = 0;
+    if (hipMemcpy(&v, akmc_rebuild_counter_ptr(), sizeof v, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return (long long)(v - b->rebuild_base);
+}
+
+}  // extern "C"

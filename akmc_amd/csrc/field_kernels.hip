@@ -1,0 +1,722 @@
+// Force-field kernels for gfx950: neighbour lists, Sutton-Chen two-pass
+// EAM, Buckingham + Ewald real space, Ewald reciprocal space.
+//
+// Data layout (all walkers of a batch concatenated, SoA, FP64 positions):
+//   x/y/z[A], type[A], q[A]; neighbour list nbr[maxnb][A] (column-major, so
+//   lane i of a wavefront reads entry k of atoms i..i+63 in one coalesced
+//   access); per-walker grids of cells for the minimum-image list build.
+// Precision: the cutoff test r^2 <= rc^2 and the minimum-image displacement
+// are FP64; pair physics is T (float on the fast path, double for FP64
+// verification); per-atom force/energy sums are FP64.
+#include "akmc_internal.h"
+
+#include <math.h>
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ inline int walker_of(const BatchDev &b, int i) { return b.wk[i]; }
+
+template <typename T>
+__device__ inline const SCCoef<T> &sc_coef(const SCPairDev &p);
+template <>
+__device__ inline const SCCoef<float> &sc_coef<float>(const SCPairDev &p) { return p.f; }
+template <>
+__device__ inline const SCCoef<double> &sc_coef<double>(const SCPairDev &p) { return p.d; }
+
+template <typename T>
+__device__ inline T pow_half(T u2, int k, T e)
+{
+    // u2^(e/2) with u2 = (a/r)^2; exact repeated squaring when e is an integer
+    if (k >= 0) {
+        T r = T(1), base = u2;
+        int p = k >> 1;
+        while (p) {
+            if (p & 1) r *= base;
+            base *= base;
+            p >>= 1;
+        }
+        if (k & 1) r *= sqrt(u2);
+        return r;
+    }
+    return pow(u2, e * T(0.5));
+}
+
+__device__ inline void min_image(double &dx, double &dy, double &dz, const WalkerDev &w)
+{
+    dx -= w.L[0] * rint(dx * w.invL[0]);
+    dy -= w.L[1] * rint(dy * w.invL[1]);
+    dz -= w.L[2] * rint(dz * w.invL[2]);
+}
+
+__device__ inline void entry_disp(uint32_t e, double &dx, double &dy, double &dz, const WalkerDev &w,
+                                  bool minimg)
+{
+    if (minimg) {
+        min_image(dx, dy, dz, w);
+    } else {
+        int sa = int((e >> 20) & 15u) - 7, sb = int((e >> 24) & 15u) - 7, sc = int((e >> 28) & 15u) - 7;
+        dx -= sa * w.L[0];
+        dy -= sb * w.L[1];
+        dz -= sc * w.L[2];
+    }
+}
+
+// ---------------------------------------------------------------- lists
+
+__global__ void k_nbr_check(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    int w = walker_of(b, i);
+    double dx = b.x[i] - b.xr[i], dy = b.y[i] - b.yr[i], dz = b.z[i] - b.zr[i];
+    if (dx * dx + dy * dy + dz * dz > b.field->half_skin2) b.rebuild[w] = 1;
+}
+
+// wrap flagged walkers' atoms into [0, L) and record the reference copy
+__global__ void k_wrap_ref(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    int w = walker_of(b, i);
+    if (!b.rebuild[w]) return;
+    const WalkerDev W = b.walkers[w];
+    double s0 = b.x[i] * W.invL[0], s1 = b.y[i] * W.invL[1], s2 = b.z[i] * W.invL[2];
+    s0 -= floor(s0);
+    s1 -= floor(s1);
+    s2 -= floor(s2);
+    double x = s0 * W.L[0], y = s1 * W.L[1], z = s2 * W.L[2];
+    b.x[i] = x;
+    b.y[i] = y;
+    b.z[i] = z;
+    b.xr[i] = x;
+    b.yr[i] = y;
+    b.zr[i] = z;
+}
+
+__global__ void k_cell_clear(BatchDev b)
+{
+    int w = blockIdx.x;
+    if (!b.rebuild[w]) return;
+    const WalkerDev W = b.walkers[w];
+    int nc = W.nc[0] * W.nc[1] * W.nc[2];
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) b.cell_count[W.cell_off + c] = 0;
+}
+
+__device__ inline int cell_coord(double x, double invL, int nc)
+{
+    int c = int(x * invL * nc);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__global__ void k_cell_assign(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    int w = walker_of(b, i);
+    if (!b.rebuild[w]) return;
+    const WalkerDev W = b.walkers[w];
+    int ca = cell_coord(b.x[i], W.invL[0], W.nc[0]);
+    int cb = cell_coord(b.y[i], W.invL[1], W.nc[1]);
+    int cc = cell_coord(b.z[i], W.invL[2], W.nc[2]);
+    int cid = W.cell_off + (ca * W.nc[1] + cb) * W.nc[2] + cc;
+    b.atom_cell[i] = cid;
+    b.atom_slot[i] = atomicAdd(&b.cell_count[cid], 1);
+}
+
+// exclusive scan of a walker's cell counts, one workgroup per walker
+__global__ void k_cell_scan(BatchDev b)
+{
+    __shared__ int part[kBlock];
+    __shared__ int carry;
+    int w = blockIdx.x;
+    if (!b.rebuild[w]) return;
+    const WalkerDev W = b.walkers[w];
+    int nc = W.nc[0] * W.nc[1] * W.nc[2];
+    if (threadIdx.x == 0) carry = W.off;
+    __syncthreads();
+    for (int base = 0; base < nc; base += blockDim.x) {
+        int c = base + threadIdx.x;
+        int v = (c < nc) ? b.cell_count[W.cell_off + c] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+            int t = (threadIdx.x >= (unsigned)o) ? part[threadIdx.x - o] : 0;
+            __syncthreads();
+            part[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (c < nc) b.cell_start[W.cell_off + c] = carry + part[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) carry += part[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+__global__ void k_cell_scatter(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    int w = walker_of(b, i);
+    if (!b.rebuild[w]) return;
+    b.cell_atoms[b.cell_start[b.atom_cell[i]] + b.atom_slot[i]] = i;
+}
+
+// order each cell's atoms by index so the lists (and the FP64 sums that
+// follow them) do not depend on the atomic slot order
+__global__ void k_cell_sort(BatchDev b)
+{
+    int w = blockIdx.x;
+    if (!b.rebuild[w]) return;
+    const WalkerDev W = b.walkers[w];
+    int nc = W.nc[0] * W.nc[1] * W.nc[2];
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        int s = b.cell_start[W.cell_off + c], n = b.cell_count[W.cell_off + c];
+        int *a = b.cell_atoms + s;
+        for (int k = 1; k < n; ++k) {
+            int v = a[k], m = k - 1;
+            while (m >= 0 && a[m] > v) {
+                a[m + 1] = a[m];
+                --m;
+            }
+            a[m + 1] = v;
+        }
+    }
+}
+
+__device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
+{
+    if (cnt < b.maxnb) b.nbr[(size_t)cnt * b.A + i] = e;
+    else *b.overflow = 1;
+    ++cnt;
+}
+
+// minimum-image list from the cell grid: all j != i with min-image
+// distance <= rc + skin; axes with fewer than 3 cells visit every cell once
+__global__ void k_nbr_build_cells(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    int w = walker_of(b, i);
+    if (!b.rebuild[w]) return;
+    const WalkerDev W = b.walkers[w];
+    const double rl2 = b.field->rlist2;
+    double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    int ca = cell_coord(xi, W.invL[0], W.nc[0]);
+    int cb = cell_coord(yi, W.invL[1], W.nc[1]);
+    int cc = cell_coord(zi, W.invL[2], W.nc[2]);
+    int lo[3], hi[3];
+    const int cen[3] = {ca, cb, cc};
+    for (int a = 0; a < 3; ++a) {
+        if (W.nc[a] >= 3) {
+            lo[a] = cen[a] - 1;
+            hi[a] = cen[a] + 1;
+        } else {
+            lo[a] = 0;
+            hi[a] = W.nc[a] - 1;
+        }
+    }
+    int cnt = 0;
+    for (int pa = lo[0]; pa <= hi[0]; ++pa) {
+        int qa = (pa + W.nc[0]) % W.nc[0];
+        for (int pb = lo[1]; pb <= hi[1]; ++pb) {
+            int qb = (pb + W.nc[1]) % W.nc[1];
+            for (int pc = lo[2]; pc <= hi[2]; ++pc) {
+                int qc = (pc + W.nc[2]) % W.nc[2];
+                int cid = W.cell_off + (qa * W.nc[1] + qb) * W.nc[2] + qc;
+                int s = b.cell_start[cid], n = b.cell_count[cid];
+                for (int t = 0; t < n; ++t) {
+                    int j = b.cell_atoms[s + t];
+                    if (j == i) continue;
+                    double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
+                    min_image(dx, dy, dz, W);
+                    if (dx * dx + dy * dy + dz * dz <= rl2)
+                        push_nbr(b, i, cnt, uint32_t(j - W.off) | AKMC_NB_ZERO_SHIFT);
+                }
+            }
+        }
+    }
+    b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
+}
+
+// explicit-image list (fields without the image convention): every (j, S)
+// with |r_i - r_j - S L| <= rc + skin except j == i, S == 0
+__global__ void k_nbr_build_images(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    int w = walker_of(b, i);
+    if (!b.rebuild[w]) return;
+    const WalkerDev W = b.walkers[w];
+    const double rl2 = b.field->rlist2;
+    double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    int cnt = 0;
+    for (int jl = 0; jl < W.n; ++jl) {
+        int j = W.off + jl;
+        double bx = xi - b.x[j], by = yi - b.y[j], bz = zi - b.z[j];
+        for (int sa = -W.shells[0]; sa <= W.shells[0]; ++sa) {
+            double dx = bx - sa * W.L[0];
+            for (int sb = -W.shells[1]; sb <= W.shells[1]; ++sb) {
+                double dy = by - sb * W.L[1];
+                for (int sc = -W.shells[2]; sc <= W.shells[2]; ++sc) {
+                    if (j == i && sa == 0 && sb == 0 && sc == 0) continue;
+                    double dz = bz - sc * W.L[2];
+                    if (dx * dx + dy * dy + dz * dz <= rl2)
+                        push_nbr(b, i, cnt,
+                                 uint32_t(jl) | (uint32_t(sa + 7) << 20) | (uint32_t(sb + 7) << 24) |
+                                     (uint32_t(sc + 7) << 28));
+                }
+            }
+        }
+    }
+    b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
+}
+
+__global__ void k_rebuild_done(BatchDev b, unsigned long long *count)
+{
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= b.W) return;
+    if (b.rebuild[w]) {
+        b.rebuild[w] = 0;
+        atomicAdd(count, 1ull);
+    }
+}
+
+// ---------------------------------------------------------- Sutton-Chen
+
+// pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
+// then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2
+template <typename T, bool MINIMG>
+__global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    const FieldDev &F = *b.field;
+    const WalkerDev W = b.walkers[walker_of(b, i)];
+    const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    const int ti = b.type[i];
+    const double rc2 = F.rc2;
+    const int cnt = b.nnb[i];
+    double rho = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+        uint32_t e = b.nbr[(size_t)k * b.A + i];
+        int j = W.off + int(e & AKMC_NB_JMASK);
+        double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
+        entry_disp(e, dx, dy, dz, W, MINIMG);
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
+            const SCPairDev &p = F.sc[ti * F.nspec + b.type[j]];
+            if (p.on) {
+                const SCCoef<T> &c = sc_coef<T>(p);
+                T u2 = c.a * c.a / T(r2);
+                rho += double(c.rho_coef * pow_half<T>(u2, p.m_int, c.m));
+            }
+        }
+    }
+    b.rho[i] = rho;
+    b.e_a[i] = -sqrt(rho);
+    b.inv_sqrt_rho[i] = rho < 1.0e-8 ? -1.0f : float(1.0 / sqrt(rho));
+}
+
+// pass 2: F_i = sum_j r_ij (fs - fm)/r^2, fs = n eps (a/r)^n,
+// fm = 1/2 m (eps c)^2 (1/sqrt(rho_i) + 1/sqrt(rho_j)) (a/r)^m
+// (potentials/metal/Field.cpp:587-640, ManyBody.cpp:621-757); each atom
+// also carries half of each pair's V = eps (a/r)^n
+template <typename T, bool MINIMG, bool STRESS>
+__global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    const FieldDev &F = *b.field;
+    const WalkerDev W = b.walkers[walker_of(b, i)];
+    const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    const int ti = b.type[i];
+    const double rc2 = F.rc2;
+    const int cnt = b.nnb[i];
+    const float isi = b.inv_sqrt_rho[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    for (int k = 0; k < cnt; ++k) {
+        uint32_t e = b.nbr[(size_t)k * b.A + i];
+        int j = W.off + int(e & AKMC_NB_JMASK);
+        double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
+        entry_disp(e, dx, dy, dz, W, MINIMG);
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
+            const SCPairDev &p = F.sc[ti * F.nspec + b.type[j]];
+            if (!p.on) continue;
+            const SCCoef<T> &c = sc_coef<T>(p);
+            T r2t = T(r2);
+            T u2 = c.a * c.a / r2t;
+            T un = pow_half<T>(u2, p.n_int, c.n);
+            T um = pow_half<T>(u2, p.m_int, c.m);
+            T fs = c.fs_coef * un;
+            T fm = T(0);
+            float isj = b.inv_sqrt_rho[j];
+            if (isi >= 0.0f && isj >= 0.0f) {
+                if (sizeof(T) == 8) {
+                    double ri = b.rho[i], rj = b.rho[j];
+                    fm = c.fm_coef * T(1.0 / sqrt(ri) + 1.0 / sqrt(rj)) * um;
+                } else {
+                    fm = c.fm_coef * (T(isi) + T(isj)) * um;
+                }
+            }
+            T ft = (fs - fm) / r2t;  // both orderings of the pair
+            double dft = double(ft);
+            fx += dft * dx;
+            fy += dft * dy;
+            fz += dft * dz;
+            ep += double(T(0.5) * c.eps * un);
+            if (STRESS) {
+                double h = -0.5 * dft;  // one ordered pair: -(fs-fm)/(2 r^2) r r
+                s0 += h * dx * dx;
+                s1 += h * dy * dy;
+                s2 += h * dz * dz;
+                s3 += h * dy * dz;
+                s4 += h * dx * dz;
+                s5 += h * dx * dy;
+            }
+        }
+    }
+    b.fx[i] = fx;
+    b.fy[i] = fy;
+    b.fz[i] = fz;
+    b.e_b[i] = ep;
+    if (STRESS) {
+        b.st_atom[0 * (size_t)b.A + i] = s0;
+        b.st_atom[1 * (size_t)b.A + i] = s1;
+        b.st_atom[2 * (size_t)b.A + i] = s2;
+        b.st_atom[3 * (size_t)b.A + i] = s3;
+        b.st_atom[4 * (size_t)b.A + i] = s4;
+        b.st_atom[5 * (size_t)b.A + i] = s5;
+    }
+}
+
+// ------------------------------------------------ Buckingham + Ewald real
+
+// real-space Ewald and Buckingham over the full list
+// (potentials/pair/Field.cpp:660-842, 1016-1201; hfunc1 :2218-2236;
+// TwoBody.h:203-215).  Self-image pairs carry full weight as in the
+// reference's no-image loop, other pairs half (each is listed twice).
+template <typename T, bool MINIMG, bool STRESS>
+__global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    const FieldDev &F = *b.field;
+    const WalkerDev W = b.walkers[walker_of(b, i)];
+    const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    const int ti = b.type[i];
+    const double rc2 = F.rc2;
+    const int cnt = b.nnb[i];
+    const T eta = T(F.eta);
+    const T qi = T(F.cfact * b.q[i]);
+    const T tsp = T(1.1283791670955126);  // 2/sqrt(pi)
+    double fx = 0.0, fy = 0.0, fz = 0.0, er = 0.0, ev = 0.0;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    for (int k = 0; k < cnt; ++k) {
+        uint32_t e = b.nbr[(size_t)k * b.A + i];
+        int jl = int(e & AKMC_NB_JMASK);
+        int j = W.off + jl;
+        double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
+        entry_disp(e, dx, dy, dz, W, MINIMG);
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 <= rc2) || r2 < 1.0e-16) continue;
+        T r2t = T(r2);
+        T r = sqrt(r2t);
+        T ir = T(1) / r;
+        T forc = T(0), ec = T(0), eb = T(0);
+        if (F.do_ewald) {
+            T cq = qi * T(b.q[j]);
+            T xr = eta * r;
+            T erfcv = erfc(xr);
+            T ex = exp(-xr * xr);
+            ec = cq * erfcv * ir;
+            forc = cq * (-erfcv * ir * ir * ir - tsp * eta * ex * ir * ir);
+        }
+        const IonPairDev &p = F.ion[ti * F.nspec + b.type[j]];
+        if (p.on) {
+            T A = sizeof(T) == 8 ? T(p.Ad) : T(p.A);
+            T irho = sizeof(T) == 8 ? T(p.inv_rhod) : T(p.inv_rho);
+            T Cc = sizeof(T) == 8 ? T(p.Cd) : T(p.C);
+            T e1 = A * exp(-r * irho);
+            T ir6 = ir * ir * ir;
+            ir6 *= ir6;
+            T e2 = -Cc * ir6;
+            eb = e1 + e2;
+            forc -= e1 * irho * ir + T(6) * e2 * ir * ir;
+        }
+        double wgt = (j == i) ? 1.0 : 0.5;
+        double df = double(forc);
+        fx -= df * dx;
+        fy -= df * dy;
+        fz -= df * dz;
+        er += wgt * double(ec);
+        ev += wgt * double(eb);
+        if (STRESS) {
+            double h = wgt * df;
+            s0 += h * dx * dx;
+            s1 += h * dy * dy;
+            s2 += h * dz * dz;
+            s3 += h * dy * dz;
+            s4 += h * dx * dz;
+            s5 += h * dx * dy;
+        }
+    }
+    b.fx[i] = fx;
+    b.fy[i] = fy;
+    b.fz[i] = fz;
+    b.e_a[i] = er;
+    b.e_b[i] = ev;
+    if (STRESS) {
+        b.st_atom[0 * (size_t)b.A + i] = s0;
+        b.st_atom[1 * (size_t)b.A + i] = s1;
+        b.st_atom[2 * (size_t)b.A + i] = s2;
+        b.st_atom[3 * (size_t)b.A + i] = s3;
+        b.st_atom[4 * (size_t)b.A + i] = s4;
+        b.st_atom[5 * (size_t)b.A + i] = s5;
+    }
+}
+
+// ------------------------------------------------- Ewald reciprocal (FP64)
+
+// e^{i n theta_a}, theta_a = 2 pi s_a, for n = 0..gmax by complex recurrence
+__global__ void k_ewald_tables(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    const WalkerDev W = b.walkers[walker_of(b, i)];
+    const double pos[3] = {b.x[i], b.y[i], b.z[i]};
+    for (int a = 0; a < 3; ++a) {
+        double th = 6.283185307179586 * pos[a] * W.invL[a];
+        double s, c;
+        sincos(th, &s, &c);
+        double2 *t = b.ptab + ((size_t)i * 3 + a) * b.G1;
+        double re = 1.0, im = 0.0;
+        t[0] = make_double2(1.0, 0.0);
+        for (int n = 1; n <= W.gmax; ++n) {
+            double nr = re * c - im * s, ni = re * s + im * c;
+            re = nr;
+            im = ni;
+            t[n] = make_double2(re, im);
+        }
+    }
+}
+
+__device__ inline double2 phase_of(const double2 *t, int G1, int nx, int ny, int nz)
+{
+    double2 a = t[nx];
+    double2 bb = t[G1 + (ny < 0 ? -ny : ny)];
+    if (ny < 0) bb.y = -bb.y;
+    double2 c = t[2 * G1 + (nz < 0 ? -nz : nz)];
+    if (nz < 0) c.y = -c.y;
+    double2 ab = make_double2(a.x * bb.x - a.y * bb.y, a.x * bb.y + a.y * bb.x);
+    return make_double2(ab.x * c.x - ab.y * c.y, ab.x * c.y + ab.y * c.x);
+}
+
+// C_k = sum_i q_i cos(k.r_i), S_k = sum_i q_i sin(k.r_i) (Ewald.cpp:596-611);
+// grid (k blocks, walkers), FP64 sums in atom order
+__global__ void __launch_bounds__(kBlock) k_ewald_sk(BatchDev b)
+{
+    int w = blockIdx.y;
+    const WalkerDev W = b.walkers[w];
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= W.nk) return;
+    int kk = W.k_off + k;
+    int nx = b.kn[kk], ny = b.kn[b.nk_total + kk], nz = b.kn[2 * b.nk_total + kk];
+    double C = 0.0, S = 0.0;
+    for (int i = W.off; i < W.off + W.n; ++i) {
+        double2 p = phase_of(b.ptab + (size_t)i * 3 * b.G1, b.G1, nx, ny, nz);
+        double q = b.q[i];
+        C += q * p.x;
+        S += q * p.y;
+    }
+    b.Ck[kk] = C;
+    b.Sk[kk] = S;
+}
+
+// F_i += -g0 (-q sin C + q cos S) k;  E_i = 1/2 sum_k g0 q (cos C + sin S)
+// (Ewald.cpp:636-646)
+__global__ void __launch_bounds__(kBlock) k_ewald_force(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    const WalkerDev W = b.walkers[walker_of(b, i)];
+    const double2 *t = b.ptab + (size_t)i * 3 * b.G1;
+    const double q = b.q[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+    for (int k = 0; k < W.nk; ++k) {
+        int kk = W.k_off + k;
+        double2 p = phase_of(t, b.G1, b.kn[kk], b.kn[b.nk_total + kk], b.kn[2 * b.nk_total + kk]);
+        double cs = q * p.x, sn = q * p.y, C = b.Ck[kk], S = b.Sk[kk], g0 = b.g0[kk];
+        double pf = -g0 * (-sn * C + cs * S);
+        fx += pf * b.kv[kk];
+        fy += pf * b.kv[b.nk_total + kk];
+        fz += pf * b.kv[2 * b.nk_total + kk];
+        e += g0 * (cs * C + sn * S);
+    }
+    b.fx[i] += fx;
+    b.fy[i] += fy;
+    b.fz[i] += fz;
+    b.e_c[i] = 0.5 * e;
+}
+
+// ----------------------------------------------------- walker reductions
+
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double *sh)
+{
+    for (int c = 0; c < NV; ++c) {
+        sh[threadIdx.x] = v[c];
+        __syncthreads();
+        for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
+            if (threadIdx.x < (unsigned)o) sh[threadIdx.x] += sh[threadIdx.x + o];
+            __syncthreads();
+        }
+        v[c] = sh[0];
+        __syncthreads();
+    }
+}
+
+// one workgroup per walker; fixed summation order (deterministic)
+__global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress)
+{
+    __shared__ double sh[kBlock];
+    int w = blockIdx.x;
+    const WalkerDev W = b.walkers[w];
+    const FieldDev &F = *b.field;
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = W.off + threadIdx.x; i < W.off + W.n; i += blockDim.x) {
+        v[0] += b.e_a[i];
+        v[1] += b.e_b[i];
+        if (F.kind == 1 && F.do_ewald) {
+            v[2] += b.e_c[i];
+            v[3] += b.q[i] * b.q[i];
+        }
+        if (stress)
+            for (int c = 0; c < 6; ++c) v[4 + c] += b.st_atom[(size_t)c * b.A + i];
+    }
+    double phtot = 0.0, rs[6] = {0, 0, 0, 0, 0, 0};
+    if (stress && F.kind == 1 && F.do_ewald) {
+        // reciprocal stress (Ewald.cpp:613-657) with the phase total from 0
+        for (int k = threadIdx.x; k < W.nk; k += blockDim.x) {
+            int kk = W.k_off + k;
+            double C = b.Ck[kk], S = b.Sk[kk];
+            double ph2 = 0.5 * (C * C + S * S);
+            if (ph2 > 1.0e-16) {
+                phtot += ph2 * b.g0[kk];
+                double fac = 2.0 * ph2 * b.g1[kk];
+                double vx = b.kv[kk], vy = b.kv[b.nk_total + kk], vz = b.kv[2 * b.nk_total + kk];
+                rs[0] += fac * vx * vx;
+                rs[1] += fac * vy * vy;
+                rs[2] += fac * vz * vz;
+                rs[3] += fac * vy * vz;
+                rs[4] += fac * vx * vz;
+                rs[5] += fac * vx * vy;
+            }
+        }
+    }
+    block_sum<10>(v, sh);
+    double r7[7] = {phtot, rs[0], rs[1], rs[2], rs[3], rs[4], rs[5]};
+    if (stress && F.kind == 1 && F.do_ewald) block_sum<7>(r7, sh);
+    if (threadIdx.x == 0) {
+        WalkerOut o;
+        if (F.kind == 0) {
+            o.many = v[0];
+            o.vdw = v[1];
+            o.real = o.rcp = o.self = 0.0;
+            o.total = o.many + o.vdw;
+        } else {
+            // self term -eta/sqrt(pi) CTOEV sum q^2 (pair Field.cpp:704-720)
+            o.self = F.do_ewald ? -1.1283791670955126 * 0.5 * F.cfact * F.eta * v[3] : 0.0;
+            o.real = v[0] + o.self;
+            o.vdw = v[1];
+            o.rcp = v[2];
+            o.many = 0.0;
+            o.total = o.rcp + o.real + o.vdw;
+        }
+        for (int c = 0; c < 6; ++c) o.stress[c] = stress ? v[4 + c] : 0.0;
+        if (stress && F.kind == 1 && F.do_ewald) {
+            for (int c = 0; c < 6; ++c) o.stress[c] += r7[1 + c];
+            o.stress[0] -= r7[0];
+            o.stress[1] -= r7[0];
+            o.stress[2] -= r7[0];
+        }
+        b.out[w] = o;
+    }
+}
+
+inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+
+static unsigned long long *g_rebuild_counter = nullptr;
+
+extern "C" unsigned long long *akmc_rebuild_counter_ptr()
+{
+    if (!g_rebuild_counter) {
+        if (hipMalloc(&g_rebuild_counter, sizeof(unsigned long long)) != hipSuccess) return nullptr;
+        if (hipMemset(g_rebuild_counter, 0, sizeof(unsigned long long)) != hipSuccess) return nullptr;
+    }
+    return g_rebuild_counter;
+}
+
+static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
+{
+    int gA = blocks(b.A);
+    k_nbr_check<<<gA, kBlock, 0, s>>>(b);
+    k_wrap_ref<<<gA, kBlock, 0, s>>>(b);
+    if (hf.min_image) {
+        k_cell_clear<<<b.W, kBlock, 0, s>>>(b);
+        k_cell_assign<<<gA, kBlock, 0, s>>>(b);
+        k_cell_scan<<<b.W, kBlock, 0, s>>>(b);
+        k_cell_scatter<<<gA, kBlock, 0, s>>>(b);
+        k_cell_sort<<<b.W, kBlock, 0, s>>>(b);
+        k_nbr_build_cells<<<gA, kBlock, 0, s>>>(b);
+    } else {
+        k_nbr_build_images<<<gA, kBlock, 0, s>>>(b);
+    }
+    k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, akmc_rebuild_counter_ptr());
+}
+
+template <typename T>
+static void launch_field(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
+{
+    int gA = blocks(b.A);
+    if (hf.kind == 0) {
+        if (hf.min_image) k_sc_density<T, true><<<gA, kBlock, 0, s>>>(b);
+        else k_sc_density<T, false><<<gA, kBlock, 0, s>>>(b);
+        if (hf.min_image) {
+            if (want_stress) k_sc_force<T, true, true><<<gA, kBlock, 0, s>>>(b);
+            else k_sc_force<T, true, false><<<gA, kBlock, 0, s>>>(b);
+        } else {
+            if (want_stress) k_sc_force<T, false, true><<<gA, kBlock, 0, s>>>(b);
+            else k_sc_force<T, false, false><<<gA, kBlock, 0, s>>>(b);
+        }
+    } else {
+        if (hf.min_image) {
+            if (want_stress) k_ion_real<T, true, true><<<gA, kBlock, 0, s>>>(b);
+            else k_ion_real<T, true, false><<<gA, kBlock, 0, s>>>(b);
+        } else {
+            if (want_stress) k_ion_real<T, false, true><<<gA, kBlock, 0, s>>>(b);
+            else k_ion_real<T, false, false><<<gA, kBlock, 0, s>>>(b);
+        }
+        if (hf.do_ewald) {
+            k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
+            dim3 g((b.maxnk + kBlock - 1) / kBlock, b.W);
+            k_ewald_sk<<<g, kBlock, 0, s>>>(b);
+            k_ewald_force<<<gA, kBlock, 0, s>>>(b);
+        }
+    }
+    k_walker_reduce<<<b.W, kBlock, 0, s>>>(b, want_stress);
+}
+
+void akmc_launch_forces_impl(const BatchDev &b, const FieldDev &hf, int fp64, int want_stress, hipStream_t s)
+{
+    launch_lists(b, hf, s);
+    if (fp64) launch_field<double>(b, hf, want_stress, s);
+    else launch_field<float>(b, hf, want_stress, s);
+}

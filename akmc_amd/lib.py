@@ -1,0 +1,248 @@
+"""ctypes binding of libakmc_hip.so (include/akmc.h).
+
+This is the only way Python reaches the device: there is no CPU fallback.
+If the HIP library is missing the import of any binding raises.
+The classes mirror the reference's Field plugin API
+(potentials/metal/Field.h:137-217, potentials/pair/Field.h:157-234):
+``Field.read_potential`` (readPotential), ``setup``, ``calculate_forces``
+(calculateForces, positions wrapped in place) and ``calculate_energy``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libakmc_hip.so")
+
+SUCCESS, FAILED, INTERRUPTED, TERMINATED = 0, 1, 3, 4
+EINVAL, EHIP, ECAPACITY, EUNSUPPORTED = -1, -2, -3, -4
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+class Energy(C.Structure):
+    """akmc_energy: the Energy record (potentials/metal/Energy.h:17-37)."""
+    _fields_ = [("total", C.c_double), ("rcp", C.c_double), ("real", C.c_double), ("many", C.c_double),
+                ("vdw", C.c_double), ("ext", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AkmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"akmc error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libakmc_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m akmc_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.akmc_last_error.restype = C.c_char_p
+    L.akmc_field_create.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+    L.akmc_field_destroy.argtypes = [C.c_void_p]
+    L.akmc_field_kind.argtypes = [C.c_void_p]
+    L.akmc_field_set_ewald.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]
+    L.akmc_field_set_skin.argtypes = [C.c_void_p, C.c_double]
+    L.akmc_field_set_precision.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_field_setup.argtypes = [C.c_void_p, _D, C.c_int]
+    L.akmc_field_forces.argtypes = [C.c_void_p, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I, _I, C.c_int,
+                                    C.POINTER(Energy)]
+    L.akmc_field_energy.argtypes = [C.c_void_p, _D, _D, _D, _D, _D, _I, C.c_int, C.POINTER(Energy)]
+    L.akmc_field_num_kvectors.argtypes = [C.c_void_p]
+    L.akmc_batch_create.argtypes = [C.c_void_p, C.c_int, _I, _D, C.POINTER(C.c_void_p)]
+    L.akmc_batch_destroy.argtypes = [C.c_void_p]
+    L.akmc_batch_upload.argtypes = [C.c_void_p, _D, _D, _D, _I, _D, _I]
+    L.akmc_batch_download.argtypes = [C.c_void_p, _D, _D, _D, _D, _D, _D]
+    L.akmc_batch_forces.argtypes = [C.c_void_p, C.POINTER(Energy)]
+    L.akmc_batch_forces_repeat.argtypes = [C.c_void_p, C.c_int, _D]
+    L.akmc_batch_total_atoms.argtypes = [C.c_void_p]
+    L.akmc_batch_rebuilds.argtypes = [C.c_void_p]
+    L.akmc_batch_rebuilds.restype = C.c_longlong
+    _lib = L
+    return L
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        raise AkmcError(rc, load().akmc_last_error().decode())
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Field:
+    """Force-field handle (the reference's compile-time Field plugin)."""
+
+    def __init__(self, potentials_text: str):
+        L = load()
+        h = C.c_void_p()
+        check(L.akmc_field_create(potentials_text.encode(), C.byref(h)))
+        self._h = h
+        self.kind = L.akmc_field_kind(h)
+
+    @classmethod
+    def read_potential(cls, path: str) -> "Field":
+        with open(path) as fh:
+            return cls(fh.read())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().akmc_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_ewald(self, eta, gcell):
+        g = np.broadcast_to(np.asarray(gcell, dtype=int), (3,))
+        check(load().akmc_field_set_ewald(self._h, float(eta), int(g[0]), int(g[1]), int(g[2])))
+
+    def set_skin(self, skin):
+        check(load().akmc_field_set_skin(self._h, float(skin)))
+
+    def set_precision(self, bits):
+        check(load().akmc_field_set_precision(self._h, int(bits)))
+
+    def setup(self, lat, max_atoms):
+        check(load().akmc_field_setup(self._h, _dp(_f64(lat)), int(max_atoms)))
+
+    @property
+    def num_kvectors(self):
+        return load().akmc_field_num_kvectors(self._h)
+
+    def calculate_forces(self, pos, types, lat, charges=None, stress=True):
+        """Returns dict(pos=wrapped (3,N), force (3,N), stress (6,), energy dict)."""
+        pos = np.array(pos, dtype=np.float64, copy=True).reshape(3, -1)
+        n = pos.shape[1]
+        x, y, z = [np.ascontiguousarray(pos[k]) for k in range(3)]
+        fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
+        st = np.zeros(6) if stress else None
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        q = _f64(charges) if charges is not None else None
+        e = Energy()
+        check(load().akmc_field_forces(self._h, _dp(x), _dp(y), _dp(z), _dp(fx), _dp(fy), _dp(fz), _dp(st),
+                                       _dp(_f64(lat)), _dp(q), _ip(t), None, n, C.byref(e)))
+        return dict(pos=np.stack([x, y, z]), force=np.stack([fx, fy, fz]), stress=st, energy=e.as_dict())
+
+    def calculate_energy(self, pos, types, lat, charges=None):
+        return self.calculate_forces(pos, types, lat, charges, stress=False)["energy"]
+
+
+class Batch:
+    """W walker cells resident in HBM."""
+
+    def __init__(self, field: Field, cells):
+        """cells: sequence of objects with .pos (3,N), .types, .lat, .charges."""
+        L = load()
+        self.field = field
+        self.natoms = np.array([c.pos.shape[1] for c in cells], dtype=np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(self.natoms)]).astype(np.int64)
+        lat = _f64(np.concatenate([np.asarray(c.lat, dtype=np.float64).reshape(9) for c in cells]))
+        h = C.c_void_p()
+        check(L.akmc_batch_create(field.handle, len(cells), _ip(self.natoms), _dp(lat), C.byref(h)))
+        self._h = h
+        self.W = len(cells)
+        self.A = int(self.offsets[-1])
+        self.upload(cells)
+
+    def upload(self, cells):
+        pos = np.concatenate([np.asarray(c.pos, dtype=np.float64) for c in cells], axis=1)
+        t = np.ascontiguousarray(np.concatenate([c.types for c in cells]), dtype=np.int32)
+        q = None
+        if cells[0].charges is not None:
+            q = _f64(np.concatenate([c.charges for c in cells]))
+        x, y, z = [np.ascontiguousarray(pos[k]) for k in range(3)]
+        check(load().akmc_batch_upload(self._h, _dp(x), _dp(y), _dp(z), _ip(t), _dp(q), None))
+
+    def forces(self):
+        e = (Energy * self.W)()
+        check(load().akmc_batch_forces(self._h, e))
+        return [e[w].as_dict() for w in range(self.W)]
+
+    def download(self):
+        out = [np.zeros(self.A) for _ in range(6)]
+        check(load().akmc_batch_download(self._h, *[_dp(a) for a in out]))
+        pos, frc = np.stack(out[:3]), np.stack(out[3:])
+        return ([pos[:, self.offsets[w]:self.offsets[w + 1]] for w in range(self.W)],
+                [frc[:, self.offsets[w]:self.offsets[w + 1]] for w in range(self.W)])
+
+    def forces_repeat(self, nrep):
+        ms = C.c_double()
+        check(load().akmc_batch_forces_repeat(self._h, int(nrep), C.byref(ms)))
+        return ms.value
+
+    @property
+    def rebuilds(self):
+        return load().akmc_batch_rebuilds(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().akmc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sc_potentials_text(species_pots, rc, species=None, noimage=False):
+    """Reference 'potentials' text for Sutton-Chen (field order
+    eps a n m c, potentials/metal/ManyBody.cpp:148-163)."""
+    species = species or sorted({p[0] for p in species_pots} | {p[1] for p in species_pots})
+    lines = [f"cutoff {rc!r}", f"species {len(species)}"]
+    lines += [f"{s} 63.546 0.0 29" for s in species]
+    if noimage:
+        lines.append("noimage")
+    lines.append(f"manybody {len(species_pots)} ev")
+    for (A, B, eps, a, n, m, c) in species_pots:
+        lines += ["suttonchen", f"{A} {B} {eps!r} {a!r} {n!r} {m!r} {c!r}"]
+    lines.append("close")
+    return "\n".join(lines) + "\n"
+
+
+def buck_potentials_text(species, charges, pots, rc, noimage=False, ewald_precis=None):
+    """Reference 'potentials' text for Buckingham + Ewald
+    (potentials/pair/Field.cpp:215-317, TwoBody.cpp:56-112)."""
+    lines = [f"cutoff {rc!r}", f"species {len(species)}"]
+    lines += [f"{s} 1.0 {q!r}" for s, q in zip(species, charges)]
+    if noimage:
+        lines.append("noimage")
+    if ewald_precis is not None:
+        lines.append(f"ewald precis {ewald_precis!r}")
+    lines.append(f"twobody {len(pots)} ev ev")
+    for (A, B, a, rho, c) in pots:
+        lines += ["buck", f"{A} {B} {a!r} {rho!r} {c!r}"]
+    lines.append("close")
+    return "\n".join(lines) + "\n"

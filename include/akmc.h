@@ -1,0 +1,108 @@
+/*
+ * akmc.h -- C ABI of libakmc_hip.so, the MI355X engine for AKMC's
+ * force-and-saddle-search path.
+ *
+ * Plain pointers and sizes only.  Positions and forces are SoA FP64 arrays
+ * x[n], y[n], z[n] as in the reference's Field::calculateForces; lattices
+ * are 9 doubles, lattice vectors as rows; stress is xx,yy,zz,yz,xz,xy.
+ * Every entry point returns an int status: AKMC_SUCCESS (0), AKMC_FAILED
+ * (1), AKMC_INTERRUPTED (3), AKMC_TERMINATED (4) as in Common/Status.h:10-50,
+ * or a negative library error whose text akmc_last_error() returns.  The
+ * library never calls exit().  Handles are not re-entrant: one host thread
+ * drives one handle.
+ */
+#ifndef AKMC_H
+#define AKMC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AKMC_SUCCESS 0
+#define AKMC_FAILED 1
+#define AKMC_INTERRUPTED 3
+#define AKMC_TERMINATED 4
+#define AKMC_EINVAL (-1)      /* bad argument or input text */
+#define AKMC_EHIP (-2)        /* HIP runtime error */
+#define AKMC_ECAPACITY (-3)   /* a fixed device capacity was exceeded */
+#define AKMC_EUNSUPPORTED (-4)
+
+/* Energy record: the fields of Energy (potentials/metal/Energy.h:17-37,
+ * potentials/pair/Energy.h); total = rcp + real + vdw + many + ext
+ * (Energy::getTotalEnergy, potentials/metal/Energy.h:58-59). */
+typedef struct {
+    double total;
+    double rcp;
+    double real;
+    double many;
+    double vdw;
+    double ext;
+} akmc_energy;
+
+typedef struct akmc_field_s *akmc_field_t;
+typedef struct akmc_batch_s *akmc_batch_t;
+
+const char *akmc_last_error(void);
+int akmc_version(void);
+
+/* ---- force field ("plugin") ------------------------------------------
+ * Replaces Field::readPotential (potentials/metal/Field.cpp:122-205,
+ * potentials/pair/Field.cpp:215-317): the same "potentials" text, i.e.
+ * cutoff / species / manybody|many + suttonchen / twobody + buck /
+ * ewald precis|none / noimage / close.  kind is inferred from the text. */
+int akmc_field_create(const char *potentials_text, akmc_field_t *out);
+void akmc_field_destroy(akmc_field_t f);
+/* 0 = many-body (Sutton-Chen), 1 = ionic (Buckingham + Ewald) */
+int akmc_field_kind(akmc_field_t f);
+/* Explicit Ewald parameters (eta, |n|max per axis) instead of the
+ * accuracy-derived ones of potentials/pair/Field.cpp:129-132 and
+ * Ewald.cpp:88-120; call before akmc_field_setup. */
+int akmc_field_set_ewald(akmc_field_t f, double eta, int gx, int gy, int gz);
+/* Verlet skin of the device neighbour lists (default 0.6 A). */
+int akmc_field_set_skin(akmc_field_t f, double skin);
+/* Pair-physics precision of later force calls: 32 (FP32 pair terms, the
+ * default) or 64 (FP64 throughout; the verification path).  Cutoff tests,
+ * minimum images, reciprocal sums and all accumulations are FP64 in both. */
+int akmc_field_set_precision(akmc_field_t f, int bits);
+/* Replaces Field::setup (potentials/metal/Field.cpp:24-97,
+ * potentials/pair/Field.cpp:80-169): cutoff checks, image shells,
+ * k vectors.  Orthorhombic cells only on the device. */
+int akmc_field_setup(akmc_field_t f, const double lat[9], int max_atoms);
+/* Replaces Field::calculateForces (potentials/metal/Field.cpp:211-276,
+ * potentials/pair/Field.cpp:323-381): x,y,z are wrapped in place as the
+ * reference does; f* and stress are overwritten. */
+int akmc_field_forces(akmc_field_t f, double *x, double *y, double *z, double *fx, double *fy,
+                      double *fz, double stress[6], const double lat[9], const double *charge,
+                      const int *type, const int *frozen, int n, akmc_energy *e);
+/* Replaces Field::calculateEnergy (energy only). */
+int akmc_field_energy(akmc_field_t f, double *x, double *y, double *z, const double lat[9],
+                      const double *charge, const int *type, int n, akmc_energy *e);
+/* number of k vectors of the last setup (0 for many-body fields) */
+int akmc_field_num_kvectors(akmc_field_t f);
+
+/* ---- device-resident walker batches ----------------------------------
+ * W independent cells of uneven size kept in HBM; the batched counterpart
+ * of the reference's MPI task farm (kmc/KineticMonteCarlo.cpp:352-598),
+ * one walker per search. */
+int akmc_batch_create(akmc_field_t f, int nwalkers, const int *natoms, const double *lat,
+                      akmc_batch_t *out);
+void akmc_batch_destroy(akmc_batch_t b);
+/* all walkers' atoms concatenated in walker order; charge may be NULL */
+int akmc_batch_upload(akmc_batch_t b, const double *x, const double *y, const double *z,
+                      const int *type, const double *charge, const int *frozen);
+int akmc_batch_download(akmc_batch_t b, double *x, double *y, double *z, double *fx, double *fy,
+                        double *fz);
+/* one force evaluation of every walker at its resident positions;
+ * energies[W] (may be NULL) receives each walker's energy record */
+int akmc_batch_forces(akmc_batch_t b, akmc_energy *energies);
+/* device-resident repeat: nrep force evaluations with no host copies,
+ * returns the total device time in ms measured with HIP events */
+int akmc_batch_forces_repeat(akmc_batch_t b, int nrep, double *ms);
+int akmc_batch_total_atoms(akmc_batch_t b);
+/* neighbour-list rebuild count since creation (diagnostics) */
+long long akmc_batch_rebuilds(akmc_batch_t b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+"""GPU parity of the force fields: HIP kernels through the C ABI vs the FP64
+oracle on identical seeded inputs.
+
+Tolerances (the north-star budget): forces within 1e-5 eV/A of FP64,
+energies within 1e-6 eV/atom; the FP64 verification path within 1e-9.
+The measured error against FP64 is printed for every case.
+"""
+import numpy as np
+import pytest
+
+from akmc_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CU = W.SC_CU
+F_TOL = 1e-5
+E_TOL_ATOM = 1e-6
+
+
+def _sc_pots():
+    return [("Cu", "Cu", CU["eps"], CU["a"], CU["n"], CU["m"], CU["c"])]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from akmc_amd import lib as L
+    L.load()
+    return L
+
+
+def _compare(name, ref, got, n, ftol=F_TOL, etol=E_TOL_ATOM):
+    df = np.abs(got["force"] - ref["force"]).max()
+    de = abs(got["energy"]["total"] - ref["energy"]) / n
+    print(f"{name}: max|dF| = {df:.3e} eV/A, |dE|/N = {de:.3e} eV/atom", flush=True)
+    assert df < ftol, df
+    assert de < etol, de
+    assert np.abs(got["pos"] - ref["pos"]).max() < 1e-9  # wrapped like the reference
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_sc_plain_cell(lib, oracle, fp64):
+    c = W.plain_sc_workload(seed=1)
+    f = lib.Field(lib.sc_potentials_text(_sc_pots(), 2 * CU["a"]))
+    if fp64:
+        f.set_precision(64)
+    ref = oracle.sc_forces(oracle.sc_field(["Cu"], _sc_pots(), 2 * CU["a"]), c.pos, c.types, c.lat)
+    got = f.calculate_forces(c.pos, c.types, c.lat)
+    tol = (1e-9, 1e-11) if fp64 else (F_TOL, E_TOL_ATOM)
+    _compare(f"SC C1 fp{64 if fp64 else 32}", ref, got, c.n, *tol)
+    assert abs(got["energy"]["many"] - ref["e_many"]) / c.n < tol[1]
+    # stress is a cancelling sum of ~3.5e4 O(1 eV) pair terms: FP32 pair terms
+    # leave ~1e-5 relative; the FP64 path must agree to round-off
+    stol = 1e-9 if fp64 else 1e-4
+    ds = np.abs(got["stress"] - ref["stress"]).max() / max(1.0, np.abs(ref["stress"]).max())
+    print(f"  stress rel err {ds:.2e}", flush=True)
+    assert ds < stol
+
+
+def test_sc_two_species_and_large_cell(lib, oracle):
+    c = W.sc_cell(9, 9, 9, n_vac=3, jitter=0.05, seed=7, two_species=True, a=3.7)
+    pots = [("A", "A", 0.0124, 3.61, 9.0, 6.0, 39.4), ("B", "B", 0.0158, 3.89, 10.0, 8.0, 34.4),
+            ("A", "B", 0.0140, 3.75, 9.5, 7.0, 36.8)]
+    f = lib.Field(lib.sc_potentials_text(pots, 7.3, species=["A", "B"]))
+    ref = oracle.sc_forces(oracle.sc_field(["A", "B"], pots, 7.3), c.pos, c.types, c.lat)
+    got = f.calculate_forces(c.pos, c.types, c.lat)
+    _compare("SC two-species 2913 atoms", ref, got, c.n)
+
+
+def test_sc_noimage_path(lib, oracle):
+    c = W.sc_cell(3, 3, 3, n_vac=1, jitter=0.02, seed=3)
+    f = lib.Field(lib.sc_potentials_text(_sc_pots(), 7.0, noimage=True))
+    ref = oracle.sc_forces(oracle.sc_field(["Cu"], _sc_pots(), 7.0, min_image=False), c.pos, c.types, c.lat)
+    got = f.calculate_forces(c.pos, c.types, c.lat)
+    _compare("SC no-image (rc > L/2)", ref, got, c.n)
+
+
+def _c2_field(lib, oracle, c, eta=0.35, g=8):
+    pots = [("M", "X") + W.ROCKSALT_BUCK[("M", "X")], ("X", "X") + W.ROCKSALT_BUCK[("X", "X")],
+            ("M", "M") + W.ROCKSALT_BUCK[("M", "M")]]
+    f = lib.Field(lib.buck_potentials_text(["M", "X"], [2.0, -2.0], pots, 10.0, noimage=True))
+    f.set_ewald(eta, g)
+    ew = oracle.Ewald(c.lat, eta, gcell=g)
+    of = oracle.ion_field(["M", "X"], pots, rc=10.0, ewald=ew, min_image=False, lat=c.lat)
+    return f, of
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_ionic_rocksalt_ewald(lib, oracle, fp64):
+    c = W.rocksalt_cell(seed=5)
+    f, of = _c2_field(lib, oracle, c)
+    if fp64:
+        f.set_precision(64)
+    ref = oracle.ion_forces(of, c.pos, c.types, c.charges, c.lat)
+    got = f.calculate_forces(c.pos, c.types, c.lat, c.charges)
+    assert f.num_kvectors == 1147
+    tol = (1e-8, 1e-10) if fp64 else (F_TOL, E_TOL_ATOM)
+    _compare(f"rock-salt Ewald fp{64 if fp64 else 32}", ref, got, c.n, *tol)
+    for k, rk in (("rcp", "e_rcp"), ("real", "e_real"), ("vdw", "e_vdw")):
+        assert abs(got["energy"][k] - ref[rk]) / c.n < tol[1], k
+
+
+def test_ionic_gdc_reference_energy(lib, oracle):
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "gdc_example.json")))
+    pots = [tuple(p) for p in g["buckingham"]]
+    sp = g["species"]
+    q = {s: None for s in sp}
+    for t, qq in zip(g["types"], g["charges"]):
+        q[sp[t]] = qq
+    f = lib.Field(lib.buck_potentials_text(sp, [q[s] for s in sp], pots, g["cutoff"], ewald_precis=1e-6))
+    got = f.calculate_forces(np.array(g["pos"]), np.array(g["types"]), np.array(g["lat"]), np.array(g["charges"]))
+    n = len(g["types"])
+    de = abs(got["energy"]["total"] - g["energy"]) / n
+    print(f"gdc 2549 ions: |E - E_ref|/N = {de:.3e} eV/atom", flush=True)
+    assert de < E_TOL_ATOM
+
+
+def test_batch_matches_single(lib, oracle):
+    cells = W.batched_sc_cells(n_walkers=6, seed=11)
+    f = lib.Field(lib.sc_potentials_text(_sc_pots(), 2 * CU["a"]))
+    b = lib.Batch(f, cells)
+    e = b.forces()
+    _, frc = b.download()
+    of = oracle.sc_field(["Cu"], _sc_pots(), 2 * CU["a"])
+    for w, c in enumerate(cells):
+        ref = oracle.sc_forces(of, c.pos, c.types, c.lat)
+        df = np.abs(frc[w] - ref["force"]).max()
+        de = abs(e[w]["total"] - ref["energy"]) / c.n
+        print(f"walker {w} N={c.n}: max|dF| {df:.2e} |dE|/N {de:.2e}", flush=True)
+        assert df < F_TOL and de < E_TOL_ATOM
+
+
+def test_skin_reuse_matches_fresh(lib, oracle):
+    """Forces after small moves (list reused inside the skin) equal the
+    oracle's; a move beyond half the skin triggers a rebuild."""
+    c = W.plain_sc_workload(seed=2)
+    f = lib.Field(lib.sc_potentials_text(_sc_pots(), 2 * CU["a"]))
+    b = lib.Batch(f, [c])
+    b.forces()
+    r0 = b.rebuilds
+    rng = np.random.default_rng(0)
+    of = oracle.sc_field(["Cu"], _sc_pots(), 2 * CU["a"])
+    pos = c.pos.copy()
+    for step in range(3):
+        pos = pos + rng.normal(0, 0.05, pos.shape)
+        c2 = W.Cell(pos, c.types, c.lat)
+        b.upload([c2])
+        e = b.forces()
+        _, frc = b.download()
+        ref = oracle.sc_forces(of, pos, c.types, c.lat)
+        assert np.abs(frc[0] - ref["force"]).max() < F_TOL
+        assert abs(e[0]["total"] - ref["energy"]) / c.n < E_TOL_ATOM
+    assert b.rebuilds >= r0
