@@ -92,6 +92,7 @@ struct BatchDev {
     double2 *ptab;        // [A][3][G1] e^{i n theta}
     double *Ck, *Sk;      // structure factors
     WalkerOut *out;
+    int *active;          // per walker: 0 skips the walker (null: all active)
 };
 
 // launcher (field_kernels.hip): list maintenance + one force evaluation of
@@ -99,3 +100,46 @@ struct BatchDev {
 void akmc_launch_forces_impl(const BatchDev &b, const FieldDev &hf, int fp64, int want_stress,
                              hipStream_t s);
 extern "C" unsigned long long *akmc_rebuild_counter_ptr();
+
+// ------------------------------------------------------------ searches
+
+// walker kinds and states of the device state machines
+enum : int { WK_IDLE = 0, WK_RELAX = 1, WK_DIMER = 2 };
+enum : int {
+    ST_DONE = 0,
+    ST_R_INIT = 1,   // relax: force at X (before the loop)
+    ST_R_STEP = 2,   // relax: force at the updated X
+    ST_D_F0 = 10,    // dimer: force at the centre
+    ST_D_F1 = 11,    // dimer: force at the end X + D n
+    ST_D_F1T = 12,   // dimer: force at the trial end X + D n'
+};
+
+// FIRE relax parameters (RelaxCntrl, Common/RelaxCntrl.h:15-48)
+struct RelaxParamsDev {
+    double normTol, forceTol, initTimeStep, maxTimeStep, alphaStart, alphaDecrease, stepIncrease,
+        stepDecrease, timeStepUnit;
+    int maxIter, N_min;
+};
+
+// dimer parameters (DimerCntrl names, Common/DimerCntrl.h:18-88)
+struct DimerParamsDev {
+    double deltaTheta, dimerDisplacement, maxEigenvalue, normTol, forceTol, alphaStart, alphaDecrease,
+        stepIncrease, stepDecrease, initTimeStep, maxTimeStep, maxStep, timeStepUnit;
+    int maxRotations, maxParIter, N_min;
+};
+
+struct WalkerState {
+    int kind, state, it, nrot, status, nstep, fcalls, pad;
+    double E0, C, g, t1, dt, alpha, E_last;
+};
+
+struct SearchDev {
+    double *X, *N, *V, *F0, *F1, *T;  // [3][A] each
+    const int *frozen;                // [A]
+    WalkerState *ws;                  // [W]
+    RelaxParamsDev rp;
+    DimerParamsDev dp;
+};
+
+void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st);
+void akmc_launch_probe_init(const BatchDev &b, const SearchDev &s, hipStream_t st);

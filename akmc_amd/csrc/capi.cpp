@@ -108,6 +108,13 @@ struct akmc_batch_s {
     std::vector<WalkerOut> hout;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long rebuild_base = 0;
+    // search state
+    SearchDev s{};
+    bool search_alloc = false;
+    std::vector<WalkerState> hws;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_steps = 0;
 };
 
 namespace {
@@ -443,6 +450,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     TRY(dalloc(b, &d.e_c, A));
     TRY(dalloc(b, &d.st_atom, (size_t)6 * A));
     TRY(dalloc(b, &d.out, W));
+    TRY(dalloc(b, &d.active, W));
     if (nk > 0) {
         int *kn;
         double *kv, *g0, *g1;
@@ -476,6 +484,10 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     HIPCHK(hipMemcpy(wd, b->hw.data(), W * sizeof(WalkerDev), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(wk, hwk.data(), A * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(d.rebuild, 0, W * sizeof(int)));
+    {
+        std::vector<int> one(W, 1);
+        HIPCHK(hipMemcpy(d.active, one.data(), W * sizeof(int), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemset(d.overflow, 0, sizeof(int)));
     HIPCHK(hipMemset(d.e_c, 0, A * sizeof(double)));
     HIPCHK(hipMemset((void *)d.q, 0, A * sizeof(double)));
@@ -495,6 +507,8 @@ void batch_free(akmc_batch_t b)
 {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -584,10 +598,121 @@ int upload(akmc_batch_t b, const double *x, const double *y, const double *z, co
     HIPCHK(hipMemcpyAsync(b->d.z, z, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync((void *)b->d.type, type, A * sizeof(int), hipMemcpyHostToDevice, b->stream));
     if (q) HIPCHK(hipMemcpyAsync((void *)b->d.q, q, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
-    // a fresh configuration always gets fresh lists
+    // a fresh configuration always gets fresh lists, and every walker is
+    // evaluated by plain force calls
     std::vector<int> ones(b->W, 1);
     HIPCHK(hipMemcpyAsync(b->d.rebuild, ones.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d.active, ones.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    if (b->search_alloc) {
+        HIPCHK(hipMemcpyAsync(b->s.X, x, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipMemcpyAsync(b->s.X + A, y, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipMemcpyAsync(b->s.X + 2 * A, z, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    }
     HIPCHK(hipStreamSynchronize(b->stream));
+    return AKMC_SUCCESS;
+}
+
+
+int search_alloc(akmc_batch_t b)
+{
+    if (b->search_alloc) return AKMC_SUCCESS;
+    size_t A3 = 3 * (size_t)b->A;
+    TRY(dalloc(b, &b->s.X, A3));
+    TRY(dalloc(b, &b->s.N, A3));
+    TRY(dalloc(b, &b->s.V, A3));
+    TRY(dalloc(b, &b->s.F0, A3));
+    TRY(dalloc(b, &b->s.F1, A3));
+    TRY(dalloc(b, &b->s.T, A3));
+    int *fr;
+    TRY(dalloc(b, &fr, b->A));
+    HIPCHK(hipMemset(fr, 0, b->A * sizeof(int)));
+    b->s.frozen = fr;
+    TRY(dalloc(b, &b->s.ws, b->W));
+    HIPCHK(hipMemset(b->s.ws, 0, b->W * sizeof(WalkerState)));
+    // X starts at the resident positions
+    size_t A = b->A;
+    HIPCHK(hipMemcpy(b->s.X, b->d.x, A * sizeof(double), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(b->s.X + A, b->d.y, A * sizeof(double), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(b->s.X + 2 * A, b->d.z, A * sizeof(double), hipMemcpyDeviceToDevice));
+    b->hws.assign(b->W, WalkerState{});
+    b->search_alloc = true;
+    return AKMC_SUCCESS;
+}
+
+void drop_graph(akmc_batch_t b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->graph_steps = 0;
+}
+
+void enqueue_micro_step(akmc_batch_t b)
+{
+    akmc_launch_forces_impl(b->d, b->f->host, b->f->fp64, 0, b->stream);
+    akmc_launch_search_logic(b->d, b->s, b->stream);
+}
+
+int fetch_states(akmc_batch_t b)
+{
+    HIPCHK(hipMemcpyAsync(b->hws.data(), b->s.ws, b->W * sizeof(WalkerState), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return AKMC_SUCCESS;
+}
+
+// mark walkers that are done (or idle) inactive so the force kernels skip them
+int refresh_active(akmc_batch_t b, int *n_running)
+{
+    std::vector<int> act(b->W);
+    int run = 0;
+    for (int w = 0; w < b->W; ++w) {
+        const WalkerState &ws = b->hws[w];
+        act[w] = (ws.kind != WK_IDLE && ws.state != ST_DONE) ? 1 : 0;
+        run += act[w];
+    }
+    HIPCHK(hipMemcpyAsync(b->d.active, act.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    *n_running = run;
+    return AKMC_SUCCESS;
+}
+
+int start_walkers(akmc_batch_t b, const int *walkers, int kind, double dt0, double alpha0)
+{
+    TRY(fetch_states(b));
+    for (int w = 0; w < b->W; ++w) {
+        if (walkers && !walkers[w]) continue;
+        WalkerState ws{};
+        ws.kind = kind;
+        ws.state = kind == WK_RELAX ? ST_R_INIT : ST_D_F0;
+        ws.it = kind == WK_RELAX ? 0 : 1;
+        ws.status = AKMC_FAILED;
+        ws.dt = dt0;
+        ws.alpha = alpha0;
+        ws.nstep = kind == WK_RELAX ? 1 : 0;  // Relax.cpp:49 starts nStep at 1
+        b->hws[w] = ws;
+    }
+    HIPCHK(hipMemcpyAsync(b->s.ws, b->hws.data(), b->W * sizeof(WalkerState), hipMemcpyHostToDevice, b->stream));
+    int run;
+    TRY(refresh_active(b, &run));
+    akmc_launch_probe_init(b->d, b->s, b->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));
+    drop_graph(b);
+    return AKMC_SUCCESS;
+}
+
+int capture_graph(akmc_batch_t b, int steps)
+{
+    drop_graph(b);
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < steps; ++k) enqueue_micro_step(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess) return fail(AKMC_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->graph_steps = steps;
     return AKMC_SUCCESS;
 }
 
@@ -759,6 +884,197 @@ long long akmc_batch_rebuilds(akmc_batch_t b)
     unsigned long long v = 0;
     if (hipMemcpy(&v, akmc_rebuild_counter_ptr(), sizeof v, hipMemcpyDeviceToHost) != hipSuccess) return -1;
     return (long long)(v - b->rebuild_base);
+}
+
+
+void akmc_relax_defaults(akmc_relax_params *p)
+{
+    if (!p) return;
+    p->maxIter = 3000;
+    p->normTol = 1.0e-5;
+    p->forceTol = 1.0e-4;
+    p->N_min = 5;
+    p->initTimeStep = 0.0005;
+    p->alphaDecrease = 0.99;
+    p->stepIncrease = 1.1;
+    p->stepDecrease = 0.5;
+    p->alphaStart = 0.1;
+    p->maxTimeStep = 0.00075;
+    p->timeStepUnit = 9648.530821;
+}
+
+void akmc_dimer_defaults(akmc_dimer_params *p)
+{
+    if (!p) return;
+    p->deltaTheta = 0.005;
+    p->dimerDisplacement = 0.01;
+    p->maxDeltaX = 0.5;
+    p->maxRotations = 20;
+    p->maxParIter = 1000;
+    p->maxEigenvalue = 0.0;
+    p->normTol = 0.1;
+    p->forceTol = 1.0e-2;
+    p->N_min = 5;
+    p->alphaDecrease = 0.99;
+    p->stepIncrease = 1.1;
+    p->stepDecrease = 0.5;
+    p->alphaStart = 0.1;
+    p->initTimeStep = 0.005;
+    p->maxTimeStep = 0.01;
+    p->maxStep = 0.1;
+    p->timeStepUnit = 9648.530821;
+}
+
+int akmc_batch_set_frozen(akmc_batch_t b, const int *frozen)
+{
+    if (!b || !frozen) return fail(AKMC_EINVAL, "bad arguments");
+    TRY(search_alloc(b));
+    HIPCHK(hipMemcpy((void *)b->s.frozen, frozen, b->A * sizeof(int), hipMemcpyHostToDevice));
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_start_relax(akmc_batch_t b, const int *walkers, const akmc_relax_params *p)
+{
+    if (!b || !p) return fail(AKMC_EINVAL, "bad arguments");
+    TRY(search_alloc(b));
+    RelaxParamsDev &r = b->s.rp;
+    r.normTol = p->normTol;
+    r.forceTol = p->forceTol;
+    r.initTimeStep = p->initTimeStep;
+    r.maxTimeStep = p->maxTimeStep;
+    r.alphaStart = p->alphaStart;
+    r.alphaDecrease = p->alphaDecrease;
+    r.stepIncrease = p->stepIncrease;
+    r.stepDecrease = p->stepDecrease;
+    r.timeStepUnit = p->timeStepUnit;
+    r.maxIter = p->maxIter;
+    r.N_min = p->N_min;
+    // velocities start at zero
+    if (!walkers) {
+        HIPCHK(hipMemset(b->s.V, 0, 3 * (size_t)b->A * sizeof(double)));
+    } else {
+        for (int w = 0; w < b->W; ++w)
+            if (walkers[w])
+                for (int c = 0; c < 3; ++c)
+                    HIPCHK(hipMemset(b->s.V + (size_t)c * b->A + b->hw[w].off, 0, b->hw[w].n * sizeof(double)));
+    }
+    return start_walkers(b, walkers, WK_RELAX, p->initTimeStep, p->alphaStart);
+}
+
+int akmc_batch_start_dimer(akmc_batch_t b, const int *walkers, const double *disp, const double *mode,
+                           const akmc_dimer_params *p)
+{
+    if (!b || !p || !mode) return fail(AKMC_EINVAL, "bad arguments");
+    TRY(search_alloc(b));
+    DimerParamsDev &d = b->s.dp;
+    d.deltaTheta = p->deltaTheta;
+    d.dimerDisplacement = p->dimerDisplacement;
+    d.maxEigenvalue = p->maxEigenvalue;
+    d.normTol = p->normTol;
+    d.forceTol = p->forceTol;
+    d.alphaStart = p->alphaStart;
+    d.alphaDecrease = p->alphaDecrease;
+    d.stepIncrease = p->stepIncrease;
+    d.stepDecrease = p->stepDecrease;
+    d.initTimeStep = p->initTimeStep;
+    d.maxTimeStep = p->maxTimeStep;
+    d.maxStep = p->maxStep;
+    d.timeStepUnit = p->timeStepUnit;
+    d.maxRotations = p->maxRotations;
+    d.maxParIter = p->maxParIter;
+    d.N_min = p->N_min;
+    size_t A = b->A;
+    std::vector<double> X(3 * A), N(mode, mode + 3 * A), fr(A);
+    std::vector<int> hfr(A);
+    HIPCHK(hipMemcpy(X.data(), b->s.X, 3 * A * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hfr.data(), b->s.frozen, A * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double> V(3 * A);
+    HIPCHK(hipMemcpy(V.data(), b->s.V, 3 * A * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> Nold(3 * A);
+    HIPCHK(hipMemcpy(Nold.data(), b->s.N, 3 * A * sizeof(double), hipMemcpyDeviceToHost));
+    for (int w = 0; w < b->W; ++w) {
+        if (walkers && !walkers[w]) {
+            for (int i = b->hw[w].off; i < b->hw[w].off + b->hw[w].n; ++i)
+                for (int c = 0; c < 3; ++c) N[c * A + i] = Nold[c * A + i];
+            continue;
+        }
+        // zero frozen components, normalise the walker's mode, start at rest
+        double nn = 0.0;
+        for (int i = b->hw[w].off; i < b->hw[w].off + b->hw[w].n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                if (hfr[i]) N[c * A + i] = 0.0;
+                nn += N[c * A + i] * N[c * A + i];
+                if (disp && !hfr[i]) X[c * A + i] += disp[c * A + i];
+                V[c * A + i] = 0.0;
+            }
+        if (nn <= 0.0) return fail(AKMC_EINVAL, "walker %d: zero start mode", w);
+        nn = 1.0 / std::sqrt(nn);
+        for (int i = b->hw[w].off; i < b->hw[w].off + b->hw[w].n; ++i)
+            for (int c = 0; c < 3; ++c) N[c * A + i] *= nn;
+    }
+    HIPCHK(hipMemcpy(b->s.X, X.data(), 3 * A * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->s.N, N.data(), 3 * A * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->s.V, V.data(), 3 * A * sizeof(double), hipMemcpyHostToDevice));
+    return start_walkers(b, walkers, WK_DIMER, p->initTimeStep, p->alphaStart);
+}
+
+int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_done)
+{
+    if (!b || !b->search_alloc || max_steps < 0 || sync_every < 1) return fail(AKMC_EINVAL, "bad arguments");
+    int done = 0, running = 0;
+    TRY(fetch_states(b));
+    TRY(refresh_active(b, &running));
+    while (running > 0 && done < max_steps) {
+        int k = std::min(sync_every, max_steps - done);
+        if (k > 1) {
+            if (!b->gexec || b->graph_steps != k) TRY(capture_graph(b, k));
+            HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+        } else {
+            enqueue_micro_step(b);
+            HIPCHK(hipGetLastError());
+        }
+        done += k;
+        TRY(check_overflow(b));
+        TRY(fetch_states(b));
+        TRY(refresh_active(b, &running));
+    }
+    if (steps_done) *steps_done = done;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out)
+{
+    if (!b || !out) return fail(AKMC_EINVAL, "bad arguments");
+    if (!b->search_alloc) return fail(AKMC_EINVAL, "no search has been started");
+    TRY(fetch_states(b));
+    for (int w = 0; w < b->W; ++w) {
+        const WalkerState &ws = b->hws[w];
+        out[w].kind = ws.kind;
+        out[w].status = ws.status;
+        out[w].done = ws.state == ST_DONE ? 1 : 0;
+        out[w].iterations = ws.it;
+        out[w].force_calls = ws.fcalls;
+        out[w].energy = ws.E0;
+        out[w].curvature = ws.C;
+    }
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z)
+{
+    if (!b || !b->search_alloc) return fail(AKMC_EINVAL, "no search state");
+    size_t A = b->A;
+    if (x) HIPCHK(hipMemcpy(x, b->s.X, A * sizeof(double), hipMemcpyDeviceToHost));
+    if (y) HIPCHK(hipMemcpy(y, b->s.X + A, A * sizeof(double), hipMemcpyDeviceToHost));
+    if (z) HIPCHK(hipMemcpy(z, b->s.X + 2 * A, A * sizeof(double), hipMemcpyDeviceToHost));
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_get_mode(akmc_batch_t b, double *mode)
+{
+    if (!b || !b->search_alloc || !mode) return fail(AKMC_EINVAL, "no search state");
+    HIPCHK(hipMemcpy(mode, b->s.N, 3 * (size_t)b->A * sizeof(double), hipMemcpyDeviceToHost));
+    return AKMC_SUCCESS;
 }
 
 }  // extern "C"

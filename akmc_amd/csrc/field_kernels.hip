@@ -17,6 +17,7 @@ namespace {
 constexpr int kBlock = 256;
 
 __device__ inline int walker_of(const BatchDev &b, int i) { return b.wk[i]; }
+__device__ inline bool skipped(const BatchDev &b, int w) { return b.active && !b.active[w]; }
 
 template <typename T>
 __device__ inline const SCCoef<T> &sc_coef(const SCPairDev &p);
@@ -70,29 +71,23 @@ __global__ void k_nbr_check(BatchDev b)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
     int w = walker_of(b, i);
+    if (skipped(b, w)) return;
     double dx = b.x[i] - b.xr[i], dy = b.y[i] - b.yr[i], dz = b.z[i] - b.zr[i];
     if (dx * dx + dy * dy + dz * dz > b.field->half_skin2) b.rebuild[w] = 1;
 }
 
-// wrap flagged walkers' atoms into [0, L) and record the reference copy
-__global__ void k_wrap_ref(BatchDev b)
+// record the positions the lists are built from (positions are never
+// wrapped on the device: the minimum-image and explicit-image kernels are
+// periodic in them)
+__global__ void k_ref_positions(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
     int w = walker_of(b, i);
     if (!b.rebuild[w]) return;
-    const WalkerDev W = b.walkers[w];
-    double s0 = b.x[i] * W.invL[0], s1 = b.y[i] * W.invL[1], s2 = b.z[i] * W.invL[2];
-    s0 -= floor(s0);
-    s1 -= floor(s1);
-    s2 -= floor(s2);
-    double x = s0 * W.L[0], y = s1 * W.L[1], z = s2 * W.L[2];
-    b.x[i] = x;
-    b.y[i] = y;
-    b.z[i] = z;
-    b.xr[i] = x;
-    b.yr[i] = y;
-    b.zr[i] = z;
+    b.xr[i] = b.x[i];
+    b.yr[i] = b.y[i];
+    b.zr[i] = b.z[i];
 }
 
 __global__ void k_cell_clear(BatchDev b)
@@ -106,7 +101,9 @@ __global__ void k_cell_clear(BatchDev b)
 
 __device__ inline int cell_coord(double x, double invL, int nc)
 {
-    int c = int(x * invL * nc);
+    double s = x * invL;
+    s -= floor(s);
+    int c = int(s * nc);
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
@@ -241,7 +238,8 @@ __global__ void k_nbr_build_cells(BatchDev b)
 }
 
 // explicit-image list (fields without the image convention): every (j, S)
-// with |r_i - r_j - S L| <= rc + skin except j == i, S == 0
+// with |r_i - r_j - S L| <= rc + skin except j == i, S == 0; the shell loop
+// is centred on the nearest image so unwrapped positions are handled
 __global__ void k_nbr_build_images(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,17 +253,23 @@ __global__ void k_nbr_build_images(BatchDev b)
     for (int jl = 0; jl < W.n; ++jl) {
         int j = W.off + jl;
         double bx = xi - b.x[j], by = yi - b.y[j], bz = zi - b.z[j];
-        for (int sa = -W.shells[0]; sa <= W.shells[0]; ++sa) {
+        int ma = (int)rint(bx * W.invL[0]), mb = (int)rint(by * W.invL[1]), mc = (int)rint(bz * W.invL[2]);
+        for (int sa = ma - W.shells[0]; sa <= ma + W.shells[0]; ++sa) {
             double dx = bx - sa * W.L[0];
-            for (int sb = -W.shells[1]; sb <= W.shells[1]; ++sb) {
+            for (int sb = mb - W.shells[1]; sb <= mb + W.shells[1]; ++sb) {
                 double dy = by - sb * W.L[1];
-                for (int sc = -W.shells[2]; sc <= W.shells[2]; ++sc) {
+                for (int sc = mc - W.shells[2]; sc <= mc + W.shells[2]; ++sc) {
                     if (j == i && sa == 0 && sb == 0 && sc == 0) continue;
                     double dz = bz - sc * W.L[2];
-                    if (dx * dx + dy * dy + dz * dz <= rl2)
+                    if (dx * dx + dy * dy + dz * dz <= rl2) {
+                        if (sa < -7 || sa > 7 || sb < -7 || sb > 7 || sc < -7 || sc > 7) {
+                            *b.overflow = 1;
+                            continue;
+                        }
                         push_nbr(b, i, cnt,
                                  uint32_t(jl) | (uint32_t(sa + 7) << 20) | (uint32_t(sb + 7) << 24) |
                                      (uint32_t(sc + 7) << 28));
+                    }
                 }
             }
         }
@@ -292,6 +296,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
+    if (skipped(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
@@ -328,6 +333,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
+    if (skipped(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
@@ -404,6 +410,7 @@ __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
+    if (skipped(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
@@ -486,6 +493,7 @@ __global__ void k_ewald_tables(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
+    if (skipped(b, walker_of(b, i))) return;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double pos[3] = {b.x[i], b.y[i], b.z[i]};
     for (int a = 0; a < 3; ++a) {
@@ -520,6 +528,7 @@ __device__ inline double2 phase_of(const double2 *t, int G1, int nx, int ny, int
 __global__ void __launch_bounds__(kBlock) k_ewald_sk(BatchDev b)
 {
     int w = blockIdx.y;
+    if (skipped(b, w)) return;
     const WalkerDev W = b.walkers[w];
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= W.nk) return;
@@ -542,6 +551,7 @@ __global__ void __launch_bounds__(kBlock) k_ewald_force(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
+    if (skipped(b, walker_of(b, i))) return;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double2 *t = b.ptab + (size_t)i * 3 * b.G1;
     const double q = b.q[i];
@@ -584,6 +594,7 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
 {
     __shared__ double sh[kBlock];
     int w = blockIdx.x;
+    if (skipped(b, w)) return;
     const WalkerDev W = b.walkers[w];
     const FieldDev &F = *b.field;
     double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -668,7 +679,7 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     int gA = blocks(b.A);
     k_nbr_check<<<gA, kBlock, 0, s>>>(b);
-    k_wrap_ref<<<gA, kBlock, 0, s>>>(b);
+    k_ref_positions<<<gA, kBlock, 0, s>>>(b);
     if (hf.min_image) {
         k_cell_clear<<<b.W, kBlock, 0, s>>>(b);
         k_cell_assign<<<gA, kBlock, 0, s>>>(b);

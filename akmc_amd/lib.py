@@ -33,6 +33,48 @@ class Energy(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class RelaxParams(C.Structure):
+    """akmc_relax_params: RelaxCntrl field names (Common/RelaxCntrl.h:15-48)."""
+    _fields_ = [("maxIter", C.c_int), ("normTol", C.c_double), ("forceTol", C.c_double), ("N_min", C.c_int),
+                ("initTimeStep", C.c_double), ("alphaDecrease", C.c_double), ("stepIncrease", C.c_double),
+                ("stepDecrease", C.c_double), ("alphaStart", C.c_double), ("maxTimeStep", C.c_double),
+                ("timeStepUnit", C.c_double)]
+
+
+class DimerParams(C.Structure):
+    """akmc_dimer_params: DimerCntrl field names (Common/DimerCntrl.h:18-88)."""
+    _fields_ = [("deltaTheta", C.c_double), ("dimerDisplacement", C.c_double), ("maxDeltaX", C.c_double),
+                ("maxRotations", C.c_int), ("maxParIter", C.c_int), ("maxEigenvalue", C.c_double),
+                ("normTol", C.c_double), ("forceTol", C.c_double), ("N_min", C.c_int),
+                ("alphaDecrease", C.c_double), ("stepIncrease", C.c_double), ("stepDecrease", C.c_double),
+                ("alphaStart", C.c_double), ("initTimeStep", C.c_double), ("maxTimeStep", C.c_double),
+                ("maxStep", C.c_double), ("timeStepUnit", C.c_double)]
+
+
+class WalkerStatus(C.Structure):
+    _fields_ = [("kind", C.c_int), ("status", C.c_int), ("done", C.c_int), ("iterations", C.c_int),
+                ("force_calls", C.c_int), ("energy", C.c_double), ("curvature", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def relax_defaults(**kw) -> RelaxParams:
+    p = RelaxParams()
+    load().akmc_relax_defaults(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def dimer_defaults(**kw) -> DimerParams:
+    p = DimerParams()
+    load().akmc_dimer_defaults(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
 class AkmcError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"akmc error {code}: {msg}")
@@ -71,6 +113,15 @@ def load() -> C.CDLL:
     L.akmc_batch_total_atoms.argtypes = [C.c_void_p]
     L.akmc_batch_rebuilds.argtypes = [C.c_void_p]
     L.akmc_batch_rebuilds.restype = C.c_longlong
+    L.akmc_relax_defaults.argtypes = [C.POINTER(RelaxParams)]
+    L.akmc_dimer_defaults.argtypes = [C.POINTER(DimerParams)]
+    L.akmc_batch_set_frozen.argtypes = [C.c_void_p, _I]
+    L.akmc_batch_start_relax.argtypes = [C.c_void_p, _I, C.POINTER(RelaxParams)]
+    L.akmc_batch_start_dimer.argtypes = [C.c_void_p, _I, _D, _D, C.POINTER(DimerParams)]
+    L.akmc_batch_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _I]
+    L.akmc_batch_status.argtypes = [C.c_void_p, C.POINTER(WalkerStatus)]
+    L.akmc_batch_get_positions.argtypes = [C.c_void_p, _D, _D, _D]
+    L.akmc_batch_get_mode.argtypes = [C.c_void_p, _D]
     _lib = L
     return L
 
@@ -200,6 +251,56 @@ class Batch:
         ms = C.c_double()
         check(load().akmc_batch_forces_repeat(self._h, int(nrep), C.byref(ms)))
         return ms.value
+
+    # ---- searches
+    def _flat3(self, per_walker):
+        """list of (3, N_w) arrays -> [3][A] contiguous FP64"""
+        return _f64(np.concatenate([np.asarray(a, dtype=np.float64) for a in per_walker], axis=1).reshape(-1))
+
+    def _mask(self, walkers):
+        if walkers is None:
+            return None
+        m = np.zeros(self.W, dtype=np.int32)
+        m[np.asarray(walkers, dtype=int)] = 1
+        return m
+
+    def set_frozen(self, frozen_per_walker):
+        fr = np.ascontiguousarray(np.concatenate([np.asarray(f) for f in frozen_per_walker]), dtype=np.int32)
+        check(load().akmc_batch_set_frozen(self._h, _ip(fr)))
+
+    def start_relax(self, params: RelaxParams | None = None, walkers=None):
+        p = params or relax_defaults()
+        m = self._mask(walkers)
+        check(load().akmc_batch_start_relax(self._h, _ip(m), C.byref(p)))
+
+    def start_dimer(self, mode_per_walker, disp_per_walker=None, params: DimerParams | None = None, walkers=None):
+        p = params or dimer_defaults()
+        m = self._mask(walkers)
+        mode = self._flat3(mode_per_walker)
+        disp = self._flat3(disp_per_walker) if disp_per_walker is not None else None
+        check(load().akmc_batch_start_dimer(self._h, _ip(m), _dp(disp), _dp(mode), C.byref(p)))
+
+    def run(self, max_steps, sync_every=1):
+        n = C.c_int()
+        check(load().akmc_batch_run(self._h, int(max_steps), int(sync_every), C.byref(n)))
+        return n.value
+
+    def status(self):
+        st = (WalkerStatus * self.W)()
+        check(load().akmc_batch_status(self._h, st))
+        return [st[w].as_dict() for w in range(self.W)]
+
+    def positions(self):
+        out = [np.zeros(self.A) for _ in range(3)]
+        check(load().akmc_batch_get_positions(self._h, *[_dp(a) for a in out]))
+        pos = np.stack(out)
+        return [pos[:, self.offsets[w]:self.offsets[w + 1]] for w in range(self.W)]
+
+    def mode(self):
+        m = np.zeros(3 * self.A)
+        check(load().akmc_batch_get_mode(self._h, _dp(m)))
+        m = m.reshape(3, self.A)
+        return [m[:, self.offsets[w]:self.offsets[w + 1]] for w in range(self.W)]
 
     @property
     def rebuilds(self):

@@ -102,6 +102,65 @@ int akmc_batch_total_atoms(akmc_batch_t b);
 /* neighbour-list rebuild count since creation (diagnostics) */
 long long akmc_batch_rebuilds(akmc_batch_t b);
 
+/* ---- searches on device-resident walkers ------------------------------
+ * Each walker advances by "micro-steps": one batched force evaluation at
+ * the walker's probe positions followed by one update of its state
+ * machine.  All walkers of a batch move together; finished walkers are
+ * skipped by the force kernels. */
+
+/* RelaxCntrl (Common/RelaxCntrl.h:15-48); FIRE with semi-implicit Euler
+ * as Relax::FIRERelax (optim/Relax.cpp:44-315).  time_step_unit is the
+ * eV -> 10 J/mol factor EVTODL (potentials/metal/Energy.h:40). */
+typedef struct {
+    int maxIter;
+    double normTol, forceTol;
+    int N_min;
+    double initTimeStep, alphaDecrease, stepIncrease, stepDecrease, alphaStart, maxTimeStep;
+    double timeStepUnit;
+} akmc_relax_params;
+
+/* DimerCntrl (Common/DimerCntrl.h:18-88) plus the translation time steps
+ * and per-atom step cap this build adds (the reference Dimer is a stub,
+ * optim/Dimer.cpp:15-42). */
+typedef struct {
+    double deltaTheta, dimerDisplacement, maxDeltaX;
+    int maxRotations, maxParIter;
+    double maxEigenvalue, normTol, forceTol;
+    int N_min;
+    double alphaDecrease, stepIncrease, stepDecrease, alphaStart;
+    double initTimeStep, maxTimeStep, maxStep, timeStepUnit;
+} akmc_dimer_params;
+
+typedef struct {
+    int kind;        /* 0 idle, 1 relax, 2 dimer */
+    int status;      /* AKMC_SUCCESS / AKMC_FAILED once done */
+    int done;
+    int iterations;
+    int force_calls;
+    double energy;   /* energy at the current centre */
+    double curvature;
+} akmc_walker_status;
+
+void akmc_relax_defaults(akmc_relax_params *p);
+void akmc_dimer_defaults(akmc_dimer_params *p);
+/* frozen[A] (1 = fixed), as Basis::frozen (Common/Basis.cpp:902-981) */
+int akmc_batch_set_frozen(akmc_batch_t b, const int *frozen);
+/* Relax::RelaxStructure (optim/Relax.h:116-117) for the walkers with
+ * walkers[w] != 0 (all when walkers is NULL), from their current positions */
+int akmc_batch_start_relax(akmc_batch_t b, const int *walkers, const akmc_relax_params *p);
+/* Dimer::findSaddlePoint (optim/Dimer.h:68-70): disp ([3][A], may be NULL)
+ * is added to the current positions, mode ([3][A]) is the start direction */
+int akmc_batch_start_dimer(akmc_batch_t b, const int *walkers, const double *disp, const double *mode,
+                           const akmc_dimer_params *p);
+/* run micro-steps until every walker is done or max_steps have run; the
+ * host looks at the walker states every sync_every steps (1 = the plain,
+ * host-driven loop); steps of sync_every > 1 replay a captured hipGraph */
+int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_done);
+int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out);
+/* current search positions (the dimer centre / relaxed positions) */
+int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z);
+int akmc_batch_get_mode(akmc_batch_t b, double *mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,115 @@
+"""GPU parity of the device searches (FIRE relax, dimer) against the FP64
+CPU oracle on identical seeded inputs, and bitwise equality of a walker run
+inside a batch with the same walker run alone.
+
+Bars: relaxed energies within 1e-6 eV/atom of the oracle's (the FP32 force
+path only perturbs the trajectory; both stop at |F|max < 1e-4 eV/A);
+saddle energies within 0.01 eV of the oracle's saddle and the barrier
+error of the GPU's own energy against FP64 re-evaluation <= 0.01 eV.
+"""
+import numpy as np
+import pytest
+
+from akmc_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+CU = W.SC_CU
+
+
+def _pots():
+    return [("Cu", "Cu", CU["eps"], CU["a"], CU["n"], CU["m"], CU["c"])]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from akmc_amd import lib as L
+    L.load()
+    return L
+
+
+@pytest.fixture(scope="module")
+def c1(oracle):
+    c = W.plain_sc_workload(seed=1)
+    of = oracle.sc_field(["Cu"], _pots(), 2 * CU["a"])
+    h = oracle.FieldHandle(of, 0, c.types, c.lat)
+    r = oracle.fire_relax(h, c.pos)
+    return c, of, h, r
+
+
+def test_relax_matches_oracle(lib, oracle, c1):
+    c, of, h, r = c1
+    f = lib.Field(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
+    b = lib.Batch(f, [c])
+    b.start_relax()
+    steps = b.run(4000, sync_every=1)
+    st = b.status()[0]
+    pos = b.positions()[0]
+    ref = oracle.sc_forces(of, pos, c.types, c.lat)
+    fmax = np.sqrt((ref["force"] ** 2).sum(0).max())
+    print(f"GPU relax: status {st['status']} iters {st['iterations']} steps {steps} "
+          f"E {st['energy']:.8f} (oracle {r['energy']:.8f}, {r['iters']} iters); FP64 |F|max {fmax:.2e}",
+          flush=True)
+    assert st["status"] == 0 and r["status"] == 0
+    assert abs(st["energy"] - r["energy"]) / c.n < 1e-6
+    assert abs(ref["energy"] - r["energy"]) / c.n < 1e-6
+    assert fmax < 2e-4
+
+
+def _dimer_start(c, basin, seed):
+    dv = W.min_image(basin - c.vacancies[0][:, None], c.lat)
+    frozen = (np.sqrt((dv * dv).sum(0)) >= 6.0).astype(np.int32)
+    nb = W.neighbours_of_site(W.Cell(basin, c.types, c.lat), c.vacancies[0], 12)
+    disp, mode = W.local_displacement(W.Cell(basin, c.types, c.lat), nb, 0.1, seed)
+    return frozen, disp, mode
+
+
+def test_dimer_matches_oracle(lib, oracle, c1):
+    c, of, h, r = c1
+    basin, E1 = r["pos"], r["energy"]
+    f = lib.Field(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
+    for seed in (0, 3):
+        frozen, disp, mode = _dimer_start(c, basin, seed)
+        od = oracle.dimer_search(h, basin + disp, frozen, mode[:, frozen == 0].T.reshape(-1))
+        b = lib.Batch(f, [W.Cell(basin, c.types, c.lat)])
+        b.set_frozen([frozen])
+        b.start_dimer([mode], [disp])
+        b.run(5000, sync_every=1)
+        st = b.status()[0]
+        sad = b.positions()[0]
+        e64 = oracle.sc_energy(of, sad, c.types, c.lat)[0]
+        print(f"seed {seed}: GPU dimer status {st['status']} iters {st['iterations']} calls {st['force_calls']} "
+              f"Ea {st['energy'] - E1:.5f} (FP64 at GPU saddle {e64 - E1:.5f}; oracle {od['energy'] - E1:.5f}, "
+              f"{od['iters']} iters) C {st['curvature']:.4f}", flush=True)
+        assert st["status"] == 0 and od["status"] == 0
+        assert abs((e64 - E1) - (od["energy"] - E1)) < 0.01
+        assert abs(st["energy"] - e64) < 0.01
+        assert st["curvature"] < 0
+
+
+def test_batched_walkers_equal_plain(lib, oracle, c1):
+    """The same walker in a batch of 3 (with two different cells beside it)
+    and alone: identical trajectories bit for bit."""
+    c, of, h, r = c1
+    basin = r["pos"]
+    f = lib.Field(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
+    frozen, disp, mode = _dimer_start(c, basin, 1)
+    others = W.batched_sc_cells(n_walkers=2, seed=5)
+    cells = [others[0], W.Cell(basin, c.types, c.lat), others[1]]
+    fr = [np.zeros(others[0].n, np.int32), frozen, np.zeros(others[1].n, np.int32)]
+    rng = np.random.default_rng(9)
+    modes = [rng.normal(size=(3, others[0].n)), mode, rng.normal(size=(3, others[1].n))]
+    disps = [np.zeros((3, others[0].n)), disp, np.zeros((3, others[1].n))]
+    p = lib.dimer_defaults(maxParIter=60)
+    bb = lib.Batch(f, cells)
+    bb.set_frozen(fr)
+    bb.start_dimer(modes, disps, p)
+    bb.run(400, sync_every=7)
+    b1 = lib.Batch(f, [cells[1]])
+    b1.set_frozen([frozen])
+    b1.start_dimer([mode], [disp], p)
+    b1.run(400, sync_every=1)
+    s_b, s_1 = bb.status()[1], b1.status()[0]
+    print("batched", s_b, "\nplain  ", s_1, flush=True)
+    assert s_b == s_1
+    assert np.array_equal(bb.positions()[1], b1.positions()[0])
+    assert np.array_equal(bb.mode()[1], b1.mode()[0])
