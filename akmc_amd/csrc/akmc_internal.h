@@ -43,6 +43,8 @@ struct FieldDev {
     double rc2;      // FP64 cutoff decision
     double rlist2;   // (rc + skin)^2
     double half_skin2;
+    double rnear2;   // (r_near + skin)^2: list entries inside come first and
+                     // are evaluated in FP64 on the FP32 path (0: none)
     double eta;
     double cfact;
     SCPairDev sc[AKMC_MAXPAIR];
@@ -77,6 +79,7 @@ struct BatchDev {
     double *fx, *fy, *fz;
     uint32_t *nbr;        // [maxnb][A] column-major
     int *nnb;
+    int *nnear;           // leading list entries inside the near shell
     int *rebuild;         // per walker flag
     int *overflow;        // sticky capacity flag
     int *cell_count, *cell_start, *cell_atoms, *atom_cell, *atom_slot;
@@ -92,19 +95,21 @@ struct BatchDev {
     double2 *ptab;        // [A][3][G1] e^{i n theta}
     double *Ck, *Sk;      // structure factors
     WalkerOut *out;
-    int *active;          // per walker: 0 skips the walker (null: all active)
+    int *active;          // per walker: 0 skip, 1 FP32 pair terms, 2 FP64
 };
 
 // launcher (field_kernels.hip): list maintenance + one force evaluation of
 // every walker; hf is the host copy of *b.field
 void akmc_launch_forces_impl(const BatchDev &b, const FieldDev &hf, int fp64, int want_stress,
                              hipStream_t s);
+void akmc_launch_phase(const BatchDev &b, const FieldDev &hf, int fp64, int want_stress, int phase,
+                       hipStream_t s);
 extern "C" unsigned long long *akmc_rebuild_counter_ptr();
 
 // ------------------------------------------------------------ searches
 
 // walker kinds and states of the device state machines
-enum : int { WK_IDLE = 0, WK_RELAX = 1, WK_DIMER = 2 };
+enum : int { WK_IDLE = 0, WK_RELAX = 1, WK_DIMER = 2, WK_VERIFY = 3, WK_ART = 4 };
 enum : int {
     ST_DONE = 0,
     ST_R_INIT = 1,   // relax: force at X (before the loop)
@@ -112,13 +117,25 @@ enum : int {
     ST_D_F0 = 10,    // dimer: force at the centre
     ST_D_F1 = 11,    // dimer: force at the end X + D n
     ST_D_F1T = 12,   // dimer: force at the trial end X + D n'
+    ST_V_F0 = 20,    // verify: FP64 force at the saddle
+    ST_V_LZ = 21,    // verify: FP64 force at X + delta q_k (Lanczos)
+    ST_A_L0 = 30,    // ART: force at X, start of a Lanczos evaluation
+    ST_A_LZ = 31,    // ART: force at X + deltaX q_i
 };
+#define AKMC_LZ_MAX 32
 
 // FIRE relax parameters (RelaxCntrl, Common/RelaxCntrl.h:15-48)
 struct RelaxParamsDev {
     double normTol, forceTol, initTimeStep, maxTimeStep, alphaStart, alphaDecrease, stepIncrease,
         stepDecrease, timeStepUnit;
     int maxIter, N_min;
+};
+
+// ART parameters (ARTCntrl names, Common/ArtCntrl.h:14-74)
+struct ARTParamsDev {
+    double deltaX, maxEigenvalue, eigValTol, parDamp, normTol, forceTol, initTimeStep, alphaDecrease,
+        stepIncrease, stepDecrease, alphaStart, maxTimeStep, timeStepUnit;
+    int numLanczosVectors, maxParIter, N_min;
 };
 
 // dimer parameters (DimerCntrl names, Common/DimerCntrl.h:18-88)
@@ -129,8 +146,17 @@ struct DimerParamsDev {
 };
 
 struct WalkerState {
-    int kind, state, it, nrot, status, nstep, fcalls, pad;
+    int kind, state, it, nrot, status, nstep, fcalls, lk;
     double E0, C, g, t1, dt, alpha, E_last;
+    // FP64 verification of a converged dimer: search energy/curvature,
+    // FP64 energy, max FP64 force on free atoms, two lowest Ritz values
+    double e32, c32, e64, fmax64, lam1, lam2;
+    int verified_state;  // 0 none, 1 verified data present
+    int inloop;          // ART: 0 = the Lanczos before the FIRE loop
+    // ART: norms of the modified forces before the move, F.v, half step,
+    // last and previous Lanczos eigenvalue, Lanczos beta
+    double fnpar, fnperp, fmpar, fmperp, dotfv, half, lam, lamold, lbeta;
+    int lconv, pad3;
 };
 
 struct SearchDev {
@@ -139,6 +165,14 @@ struct SearchDev {
     WalkerState *ws;                  // [W]
     RelaxParamsDev rp;
     DimerParamsDev dp;
+    ARTParamsDev ap;
+    // Lanczos basis Q [LZ_MAX][3][A]; per walker alpha, beta and the Ritz
+    // vector: lz [W][3*LZ_MAX]
+    double *Q;
+    double *lz;
+    int verify_nl;
+    double verify_delta;
+    int default_mode;                 // active mode of searching walkers (1 FP32, 2 FP64)
 };
 
 void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st);

@@ -115,6 +115,9 @@ struct akmc_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
+    SearchDev graph_args{};  // kernel arguments baked into the captured graph
+    double *basin = nullptr;  // [3][A] saved search positions
+    hipEvent_t evp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -222,6 +225,9 @@ void build_tables(akmc_field_t f)
     double rl = f->rc + f->skin;
     h.rlist2 = rl * rl;
     h.half_skin2 = 0.25 * f->skin * f->skin;
+    // ionic lists put the first shells (r <= 3.5 A) first; the FP32 path
+    // evaluates them in FP64
+    h.rnear2 = f->kind == 1 ? (3.5 + f->skin) * (3.5 + f->skin) : 0.0;
     h.cfact = kCTOEV;
     for (int i = 0; i < AKMC_MAXPAIR; ++i) f->table[i] = f->kind == 0 ? -1 : 0;
     for (int i = 0; i < ns; ++i)
@@ -436,6 +442,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     TRY(dalloc(b, &d.fz, A));
     TRY(dalloc(b, &d.nbr, (size_t)maxnb * A));
     TRY(dalloc(b, &d.nnb, A));
+    TRY(dalloc(b, &d.nnear, A));
     TRY(dalloc(b, &d.rebuild, W));
     TRY(dalloc(b, &d.overflow, 1));
     TRY(dalloc(b, &d.cell_count, ncells));
@@ -485,7 +492,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     HIPCHK(hipMemcpy(wk, hwk.data(), A * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(d.rebuild, 0, W * sizeof(int)));
     {
-        std::vector<int> one(W, 1);
+        std::vector<int> one(W, f->fp64 ? 2 : 1);
         HIPCHK(hipMemcpy(d.active, one.data(), W * sizeof(int), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemset(d.overflow, 0, sizeof(int)));
@@ -511,14 +518,18 @@ void batch_free(akmc_batch_t b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
+    for (auto &e : b->evp)
+        if (e) (void)hipEventDestroy(e);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
+int batch_mode(akmc_batch_t b) { return b->f->fp64 ? 2 : 1; }
+
 int run_forces(akmc_batch_t b, int want_stress)
 {
-    akmc_launch_forces_impl(b->d, b->f->host, b->f->fp64, want_stress, b->stream);
+    akmc_launch_forces_impl(b->d, b->f->host, batch_mode(b), want_stress, b->stream);
     HIPCHK(hipGetLastError());
     return AKMC_SUCCESS;
 }
@@ -600,9 +611,9 @@ int upload(akmc_batch_t b, const double *x, const double *y, const double *z, co
     if (q) HIPCHK(hipMemcpyAsync((void *)b->d.q, q, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
     // a fresh configuration always gets fresh lists, and every walker is
     // evaluated by plain force calls
-    std::vector<int> ones(b->W, 1);
+    std::vector<int> ones(b->W, 1), mode(b->W, batch_mode(b));
     HIPCHK(hipMemcpyAsync(b->d.rebuild, ones.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipMemcpyAsync(b->d.active, ones.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d.active, mode.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
     if (b->search_alloc) {
         HIPCHK(hipMemcpyAsync(b->s.X, x, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
         HIPCHK(hipMemcpyAsync(b->s.X + A, y, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
@@ -650,7 +661,8 @@ void drop_graph(akmc_batch_t b)
 
 void enqueue_micro_step(akmc_batch_t b)
 {
-    akmc_launch_forces_impl(b->d, b->f->host, b->f->fp64, 0, b->stream);
+    int prec = batch_mode(b) | (b->s.verify_nl > 0 ? 2 : 0);
+    akmc_launch_forces_impl(b->d, b->f->host, prec, 0, b->stream);
     akmc_launch_search_logic(b->d, b->s, b->stream);
 }
 
@@ -668,7 +680,7 @@ int refresh_active(akmc_batch_t b, int *n_running)
     int run = 0;
     for (int w = 0; w < b->W; ++w) {
         const WalkerState &ws = b->hws[w];
-        act[w] = (ws.kind != WK_IDLE && ws.state != ST_DONE) ? 1 : 0;
+        act[w] = (ws.kind != WK_IDLE && ws.state != ST_DONE) ? (ws.kind == WK_VERIFY ? 2 : batch_mode(b)) : 0;
         run += act[w];
     }
     HIPCHK(hipMemcpyAsync(b->d.active, act.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -684,12 +696,15 @@ int start_walkers(akmc_batch_t b, const int *walkers, int kind, double dt0, doub
         if (walkers && !walkers[w]) continue;
         WalkerState ws{};
         ws.kind = kind;
-        ws.state = kind == WK_RELAX ? ST_R_INIT : ST_D_F0;
-        ws.it = kind == WK_RELAX ? 0 : 1;
+        ws.state = kind == WK_RELAX ? ST_R_INIT : (kind == WK_ART ? ST_A_L0 : ST_D_F0);
+        ws.it = (kind == WK_DIMER) ? 1 : 0;
         ws.status = AKMC_FAILED;
         ws.dt = dt0;
         ws.alpha = alpha0;
-        ws.nstep = kind == WK_RELAX ? 1 : 0;  // Relax.cpp:49 starts nStep at 1
+        // Relax.cpp:49 and ART.cpp:426 start nStep at 1
+        ws.nstep = (kind == WK_RELAX || kind == WK_ART) ? 1 : 0;
+        ws.inloop = 0;
+        ws.verified_state = 0;
         b->hws[w] = ws;
     }
     HIPCHK(hipMemcpyAsync(b->s.ws, b->hws.data(), b->W * sizeof(WalkerState), hipMemcpyHostToDevice, b->stream));
@@ -698,7 +713,59 @@ int start_walkers(akmc_batch_t b, const int *walkers, int kind, double dt0, doub
     akmc_launch_probe_init(b->d, b->s, b->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(b->stream));
-    drop_graph(b);
+    return AKMC_SUCCESS;
+}
+
+int ensure_lanczos(akmc_batch_t b)
+{
+    if (b->s.Q) return AKMC_SUCCESS;
+    TRY(dalloc(b, &b->s.Q, (size_t)AKMC_LZ_MAX * 3 * b->A));
+    TRY(dalloc(b, &b->s.lz, (size_t)b->W * 3 * AKMC_LZ_MAX));
+    return AKMC_SUCCESS;
+}
+
+// per-walker start data of a saddle search: positions (optional), a
+// normalised mode with frozen components removed, zero velocity
+int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const double *mode, const int *frozen)
+{
+    size_t A = b->A;
+    std::vector<double> nw, zero;
+    std::vector<int> fr;
+    for (int w = 0; w < b->W; ++w) {
+        if (walkers && !walkers[w]) continue;
+        const int off = b->hw[w].off, n = b->hw[w].n;
+        fr.assign(n, 0);
+        if (frozen) {
+            std::copy(frozen + off, frozen + off + n, fr.begin());
+            HIPCHK(hipMemcpyAsync((void *)(b->s.frozen + off), frozen + off, n * sizeof(int), hipMemcpyHostToDevice,
+                                  b->stream));
+        } else {
+            HIPCHK(hipMemcpyAsync(fr.data(), b->s.frozen + off, n * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+            HIPCHK(hipStreamSynchronize(b->stream));
+        }
+        // zero frozen components and normalise the walker's mode
+        nw.assign(3 * (size_t)n, 0.0);
+        double nn = 0.0;
+        for (int c = 0; c < 3; ++c)
+            for (int i = 0; i < n; ++i) {
+                double v = fr[i] ? 0.0 : mode[c * A + off + i];
+                nw[(size_t)c * n + i] = v;
+                nn += v * v;
+            }
+        if (nn <= 0.0) return fail(AKMC_EINVAL, "walker %d: zero start mode", w);
+        nn = 1.0 / std::sqrt(nn);
+        for (auto &v : nw) v *= nn;
+        zero.assign(n, 0.0);
+        for (int c = 0; c < 3; ++c) {
+            size_t o = c * A + off;
+            HIPCHK(hipMemcpyAsync(b->s.N + o, nw.data() + (size_t)c * n, n * sizeof(double), hipMemcpyHostToDevice,
+                                  b->stream));
+            HIPCHK(hipMemcpyAsync(b->s.V + o, zero.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+            if (x0)
+                HIPCHK(hipMemcpyAsync(b->s.X + o, x0 + o, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+        }
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
     return AKMC_SUCCESS;
 }
 
@@ -713,6 +780,7 @@ int capture_graph(akmc_batch_t b, int steps)
     b->graph = g;
     HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     b->graph_steps = steps;
+    b->graph_args = b->s;
     return AKMC_SUCCESS;
 }
 
@@ -925,6 +993,27 @@ void akmc_dimer_defaults(akmc_dimer_params *p)
     p->timeStepUnit = 9648.530821;
 }
 
+void akmc_art_defaults(akmc_art_params *p)
+{
+    if (!p) return;
+    p->numLanczosVectors = 20;
+    p->deltaX = 0.01;
+    p->maxEigenvalue = 0.0;
+    p->eigValTol = 0.001;
+    p->parDamp = 0.5;
+    p->maxParIter = 1000;
+    p->normTol = 1.0e-4;
+    p->forceTol = 1.0e-3;
+    p->N_min = 5;
+    p->initTimeStep = 0.005;
+    p->alphaDecrease = 0.99;
+    p->stepIncrease = 1.1;
+    p->stepDecrease = 0.5;
+    p->alphaStart = 0.1;
+    p->maxTimeStep = 0.01;
+    p->timeStepUnit = 9648.530821;
+}
+
 int akmc_batch_set_frozen(akmc_batch_t b, const int *frozen)
 {
     if (!b || !frozen) return fail(AKMC_EINVAL, "bad arguments");
@@ -961,8 +1050,8 @@ int akmc_batch_start_relax(akmc_batch_t b, const int *walkers, const akmc_relax_
     return start_walkers(b, walkers, WK_RELAX, p->initTimeStep, p->alphaStart);
 }
 
-int akmc_batch_start_dimer(akmc_batch_t b, const int *walkers, const double *disp, const double *mode,
-                           const akmc_dimer_params *p)
+int akmc_batch_start_dimer(akmc_batch_t b, const int *walkers, const double *x0, const double *mode,
+                           const int *frozen, const akmc_dimer_params *p)
 {
     if (!b || !p || !mode) return fail(AKMC_EINVAL, "bad arguments");
     TRY(search_alloc(b));
@@ -983,39 +1072,37 @@ int akmc_batch_start_dimer(akmc_batch_t b, const int *walkers, const double *dis
     d.maxRotations = p->maxRotations;
     d.maxParIter = p->maxParIter;
     d.N_min = p->N_min;
-    size_t A = b->A;
-    std::vector<double> X(3 * A), N(mode, mode + 3 * A), fr(A);
-    std::vector<int> hfr(A);
-    HIPCHK(hipMemcpy(X.data(), b->s.X, 3 * A * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hfr.data(), b->s.frozen, A * sizeof(int), hipMemcpyDeviceToHost));
-    std::vector<double> V(3 * A);
-    HIPCHK(hipMemcpy(V.data(), b->s.V, 3 * A * sizeof(double), hipMemcpyDeviceToHost));
-    std::vector<double> Nold(3 * A);
-    HIPCHK(hipMemcpy(Nold.data(), b->s.N, 3 * A * sizeof(double), hipMemcpyDeviceToHost));
-    for (int w = 0; w < b->W; ++w) {
-        if (walkers && !walkers[w]) {
-            for (int i = b->hw[w].off; i < b->hw[w].off + b->hw[w].n; ++i)
-                for (int c = 0; c < 3; ++c) N[c * A + i] = Nold[c * A + i];
-            continue;
-        }
-        // zero frozen components, normalise the walker's mode, start at rest
-        double nn = 0.0;
-        for (int i = b->hw[w].off; i < b->hw[w].off + b->hw[w].n; ++i)
-            for (int c = 0; c < 3; ++c) {
-                if (hfr[i]) N[c * A + i] = 0.0;
-                nn += N[c * A + i] * N[c * A + i];
-                if (disp && !hfr[i]) X[c * A + i] += disp[c * A + i];
-                V[c * A + i] = 0.0;
-            }
-        if (nn <= 0.0) return fail(AKMC_EINVAL, "walker %d: zero start mode", w);
-        nn = 1.0 / std::sqrt(nn);
-        for (int i = b->hw[w].off; i < b->hw[w].off + b->hw[w].n; ++i)
-            for (int c = 0; c < 3; ++c) N[c * A + i] *= nn;
-    }
-    HIPCHK(hipMemcpy(b->s.X, X.data(), 3 * A * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->s.N, N.data(), 3 * A * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->s.V, V.data(), 3 * A * sizeof(double), hipMemcpyHostToDevice));
+    TRY(load_walkers(b, walkers, x0, mode, frozen));
     return start_walkers(b, walkers, WK_DIMER, p->initTimeStep, p->alphaStart);
+}
+
+int akmc_batch_start_art(akmc_batch_t b, const int *walkers, const double *x0, const double *mode,
+                         const int *frozen, const akmc_art_params *p)
+{
+    if (!b || !p || !mode) return fail(AKMC_EINVAL, "bad arguments");
+    if (p->numLanczosVectors < 2 || p->numLanczosVectors > AKMC_LZ_MAX)
+        return fail(AKMC_EINVAL, "numLanczosVectors must be in [2, %d]", AKMC_LZ_MAX);
+    TRY(search_alloc(b));
+    TRY(ensure_lanczos(b));
+    ARTParamsDev &a = b->s.ap;
+    a.deltaX = p->deltaX;
+    a.maxEigenvalue = p->maxEigenvalue;
+    a.eigValTol = p->eigValTol;
+    a.parDamp = p->parDamp;
+    a.normTol = p->normTol;
+    a.forceTol = p->forceTol;
+    a.initTimeStep = p->initTimeStep;
+    a.alphaDecrease = p->alphaDecrease;
+    a.stepIncrease = p->stepIncrease;
+    a.stepDecrease = p->stepDecrease;
+    a.alphaStart = p->alphaStart;
+    a.maxTimeStep = p->maxTimeStep;
+    a.timeStepUnit = p->timeStepUnit;
+    a.numLanczosVectors = p->numLanczosVectors;
+    a.maxParIter = p->maxParIter;
+    a.N_min = p->N_min;
+    TRY(load_walkers(b, walkers, x0, mode, frozen));
+    return start_walkers(b, walkers, WK_ART, p->initTimeStep, p->alphaStart);
 }
 
 int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_done)
@@ -1027,7 +1114,8 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
     while (running > 0 && done < max_steps) {
         int k = std::min(sync_every, max_steps - done);
         if (k > 1) {
-            if (!b->gexec || b->graph_steps != k) TRY(capture_graph(b, k));
+            if (!b->gexec || b->graph_steps != k || std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0)
+                TRY(capture_graph(b, k));
             HIPCHK(hipGraphLaunch(b->gexec, b->stream));
         } else {
             enqueue_micro_step(b);
@@ -1056,6 +1144,13 @@ int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out)
         out[w].force_calls = ws.fcalls;
         out[w].energy = ws.E0;
         out[w].curvature = ws.C;
+        out[w].verified = ws.verified_state;
+        out[w].energy32 = ws.e32;
+        out[w].curvature32 = ws.c32;
+        out[w].energy64 = ws.e64;
+        out[w].fmax64 = ws.fmax64;
+        out[w].lambda1 = ws.lam1;
+        out[w].lambda2 = ws.lam2;
     }
     return AKMC_SUCCESS;
 }
@@ -1070,10 +1165,104 @@ int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z)
     return AKMC_SUCCESS;
 }
 
+int akmc_batch_get_walker_positions(akmc_batch_t b, int walker, double *x, double *y, double *z)
+{
+    if (!b || !b->search_alloc || walker < 0 || walker >= b->W) return fail(AKMC_EINVAL, "bad arguments");
+    size_t A = b->A, off = b->hw[walker].off, n = b->hw[walker].n;
+    double *dst[3] = {x, y, z};
+    for (int c = 0; c < 3; ++c)
+        if (dst[c]) HIPCHK(hipMemcpyAsync(dst[c], b->s.X + c * A + off, n * sizeof(double), hipMemcpyDeviceToHost,
+                                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return AKMC_SUCCESS;
+}
+
 int akmc_batch_get_mode(akmc_batch_t b, double *mode)
 {
     if (!b || !b->search_alloc || !mode) return fail(AKMC_EINVAL, "no search state");
     HIPCHK(hipMemcpy(mode, b->s.N, 3 * (size_t)b->A * sizeof(double), hipMemcpyDeviceToHost));
+    return AKMC_SUCCESS;
+}
+
+
+int akmc_batch_time_phases(akmc_batch_t b, int nrep, double ms[4])
+{
+    if (!b || nrep <= 0 || !ms) return fail(AKMC_EINVAL, "bad arguments");
+    for (auto &e : b->evp)
+        if (!e) HIPCHK(hipEventCreate(&e));
+    for (int ph = 0; ph < 4; ++ph) ms[ph] = 0.0;
+    for (int r = 0; r < nrep; ++r) {
+        HIPCHK(hipEventRecord(b->evp[0], b->stream));
+        for (int ph = 0; ph < 4; ++ph) {
+            akmc_launch_phase(b->d, b->f->host, batch_mode(b), 0, ph, b->stream);
+            HIPCHK(hipEventRecord(b->evp[ph + 1], b->stream));
+        }
+        HIPCHK(hipEventSynchronize(b->evp[4]));
+        for (int ph = 0; ph < 4; ++ph) {
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, b->evp[ph], b->evp[ph + 1]));
+            ms[ph] += t;
+        }
+    }
+    return check_overflow(b);
+}
+
+int akmc_batch_save_basin(akmc_batch_t b)
+{
+    if (!b || !b->search_alloc) return fail(AKMC_EINVAL, "no search state");
+    if (!b->basin) TRY(dalloc(b, &b->basin, 3 * (size_t)b->A));
+    HIPCHK(hipMemcpy(b->basin, b->s.X, 3 * (size_t)b->A * sizeof(double), hipMemcpyDeviceToDevice));
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_restore_basin(akmc_batch_t b, const int *walkers)
+{
+    if (!b || !b->basin) return fail(AKMC_EINVAL, "no saved basin");
+    size_t A = b->A;
+    for (int w = 0; w < b->W; ++w) {
+        if (walkers && !walkers[w]) continue;
+        for (int c = 0; c < 3; ++c) {
+            size_t o = c * A + b->hw[w].off;
+            HIPCHK(hipMemcpyAsync(b->s.X + o, b->basin + o, b->hw[w].n * sizeof(double), hipMemcpyDeviceToDevice,
+                                  b->stream));
+        }
+        if (!walkers) break;
+    }
+    if (!walkers)
+        HIPCHK(hipMemcpyAsync(b->s.X, b->basin, 3 * A * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return AKMC_SUCCESS;
+}
+
+
+int akmc_batch_set_verify(akmc_batch_t b, int nl, double delta)
+{
+    if (!b || nl < 0 || nl > AKMC_LZ_MAX || (nl > 0 && delta <= 0.0))
+        return fail(AKMC_EINVAL, "verification needs 0 <= nl <= %d and delta > 0", AKMC_LZ_MAX);
+    TRY(search_alloc(b));
+    if (nl > 0) TRY(ensure_lanczos(b));
+    if (nl > 0 && (size_t)nl > (size_t)AKMC_LZ_MAX) return fail(AKMC_EINVAL, "nl too large");
+    b->s.verify_nl = nl;
+    b->s.verify_delta = delta;
+    drop_graph(b);
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_search_energies(akmc_batch_t b, int bits, double *energies)
+{
+    if (!b || !b->search_alloc || !energies || (bits != 32 && bits != 64))
+        return fail(AKMC_EINVAL, "bad arguments");
+    size_t A = b->A;
+    HIPCHK(hipMemcpyAsync(b->d.x, b->s.X, A * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d.y, b->s.X + A, A * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d.z, b->s.X + 2 * A, A * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    std::vector<int> mode(b->W, bits == 64 ? 2 : 1);
+    HIPCHK(hipMemcpyAsync(b->d.active, mode.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    akmc_launch_forces_impl(b->d, b->f->host, bits == 64 ? 2 : 1, 0, b->stream);
+    HIPCHK(hipGetLastError());
+    TRY(check_overflow(b));
+    TRY(fetch_out(b));
+    for (int w = 0; w < b->W; ++w) energies[w] = b->hout[w].total;
     return AKMC_SUCCESS;
 }
 

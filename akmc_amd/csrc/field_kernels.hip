@@ -17,7 +17,14 @@ namespace {
 constexpr int kBlock = 256;
 
 __device__ inline int walker_of(const BatchDev &b, int i) { return b.wk[i]; }
-__device__ inline bool skipped(const BatchDev &b, int w) { return b.active && !b.active[w]; }
+__device__ inline bool skipped(const BatchDev &b, int w) { return b.active[w] == 0; }
+// the pair-physics kernels of type T serve the walkers whose precision mode
+// matches (1 FP32, 2 FP64)
+template <typename T>
+__device__ inline bool other_precision(const BatchDev &b, int w)
+{
+    return b.active[w] != (sizeof(T) == 8 ? 2 : 1);
+}
 
 template <typename T>
 __device__ inline const SCCoef<T> &sc_coef(const SCPairDev &p);
@@ -214,27 +221,35 @@ __global__ void k_nbr_build_cells(BatchDev b)
             hi[a] = W.nc[a] - 1;
         }
     }
-    int cnt = 0;
-    for (int pa = lo[0]; pa <= hi[0]; ++pa) {
-        int qa = (pa + W.nc[0]) % W.nc[0];
-        for (int pb = lo[1]; pb <= hi[1]; ++pb) {
-            int qb = (pb + W.nc[1]) % W.nc[1];
-            for (int pc = lo[2]; pc <= hi[2]; ++pc) {
-                int qc = (pc + W.nc[2]) % W.nc[2];
-                int cid = W.cell_off + (qa * W.nc[1] + qb) * W.nc[2] + qc;
-                int s = b.cell_start[cid], n = b.cell_count[cid];
-                for (int t = 0; t < n; ++t) {
-                    int j = b.cell_atoms[s + t];
-                    if (j == i) continue;
-                    double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
-                    min_image(dx, dy, dz, W);
-                    if (dx * dx + dy * dy + dz * dz <= rl2)
-                        push_nbr(b, i, cnt, uint32_t(j - W.off) | AKMC_NB_ZERO_SHIFT);
+    int cnt = 0, nnear = 0;
+    const double rn2 = b.field->rnear2;
+    // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
+    for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
+        for (int pa = lo[0]; pa <= hi[0]; ++pa) {
+            int qa = (pa + W.nc[0]) % W.nc[0];
+            for (int pb = lo[1]; pb <= hi[1]; ++pb) {
+                int qb = (pb + W.nc[1]) % W.nc[1];
+                for (int pc = lo[2]; pc <= hi[2]; ++pc) {
+                    int qc = (pc + W.nc[2]) % W.nc[2];
+                    int cid = W.cell_off + (qa * W.nc[1] + qb) * W.nc[2] + qc;
+                    int s = b.cell_start[cid], n = b.cell_count[cid];
+                    for (int t = 0; t < n; ++t) {
+                        int j = b.cell_atoms[s + t];
+                        if (j == i) continue;
+                        double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
+                        min_image(dx, dy, dz, W);
+                        double r2 = dx * dx + dy * dy + dz * dz;
+                        bool near = r2 <= rn2;
+                        if (r2 <= rl2 && (pass == 0 ? near : !near))
+                            push_nbr(b, i, cnt, uint32_t(j - W.off) | AKMC_NB_ZERO_SHIFT);
+                    }
                 }
             }
         }
+        if (pass == 0) nnear = cnt;
     }
     b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
+    b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
 }
 
 // explicit-image list (fields without the image convention): every (j, S)
@@ -249,32 +264,39 @@ __global__ void k_nbr_build_images(BatchDev b)
     const WalkerDev W = b.walkers[w];
     const double rl2 = b.field->rlist2;
     double xi = b.x[i], yi = b.y[i], zi = b.z[i];
-    int cnt = 0;
-    for (int jl = 0; jl < W.n; ++jl) {
-        int j = W.off + jl;
-        double bx = xi - b.x[j], by = yi - b.y[j], bz = zi - b.z[j];
-        int ma = (int)rint(bx * W.invL[0]), mb = (int)rint(by * W.invL[1]), mc = (int)rint(bz * W.invL[2]);
-        for (int sa = ma - W.shells[0]; sa <= ma + W.shells[0]; ++sa) {
-            double dx = bx - sa * W.L[0];
-            for (int sb = mb - W.shells[1]; sb <= mb + W.shells[1]; ++sb) {
-                double dy = by - sb * W.L[1];
-                for (int sc = mc - W.shells[2]; sc <= mc + W.shells[2]; ++sc) {
-                    if (j == i && sa == 0 && sb == 0 && sc == 0) continue;
-                    double dz = bz - sc * W.L[2];
-                    if (dx * dx + dy * dy + dz * dz <= rl2) {
-                        if (sa < -7 || sa > 7 || sb < -7 || sb > 7 || sc < -7 || sc > 7) {
-                            *b.overflow = 1;
-                            continue;
+    int cnt = 0, nnear = 0;
+    const double rn2 = b.field->rnear2;
+    for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
+        for (int jl = 0; jl < W.n; ++jl) {
+            int j = W.off + jl;
+            double bx = xi - b.x[j], by = yi - b.y[j], bz = zi - b.z[j];
+            int ma = (int)rint(bx * W.invL[0]), mb = (int)rint(by * W.invL[1]), mc = (int)rint(bz * W.invL[2]);
+            for (int sa = ma - W.shells[0]; sa <= ma + W.shells[0]; ++sa) {
+                double dx = bx - sa * W.L[0];
+                for (int sb = mb - W.shells[1]; sb <= mb + W.shells[1]; ++sb) {
+                    double dy = by - sb * W.L[1];
+                    for (int sc = mc - W.shells[2]; sc <= mc + W.shells[2]; ++sc) {
+                        if (j == i && sa == 0 && sb == 0 && sc == 0) continue;
+                        double dz = bz - sc * W.L[2];
+                        double r2 = dx * dx + dy * dy + dz * dz;
+                        bool near = r2 <= rn2;
+                        if (r2 <= rl2 && (pass == 0 ? near : !near)) {
+                            if (sa < -7 || sa > 7 || sb < -7 || sb > 7 || sc < -7 || sc > 7) {
+                                *b.overflow = 1;
+                                continue;
+                            }
+                            push_nbr(b, i, cnt,
+                                     uint32_t(jl) | (uint32_t(sa + 7) << 20) | (uint32_t(sb + 7) << 24) |
+                                         (uint32_t(sc + 7) << 28));
                         }
-                        push_nbr(b, i, cnt,
-                                 uint32_t(jl) | (uint32_t(sa + 7) << 20) | (uint32_t(sb + 7) << 24) |
-                                     (uint32_t(sc + 7) << 28));
                     }
                 }
             }
         }
+        if (pass == 0) nnear = cnt;
     }
     b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
+    b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
 }
 
 __global__ void k_rebuild_done(BatchDev b, unsigned long long *count)
@@ -296,7 +318,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
-    if (skipped(b, walker_of(b, i))) return;
+    if (other_precision<T>(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
@@ -333,7 +355,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
-    if (skipped(b, walker_of(b, i))) return;
+    if (other_precision<T>(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
@@ -405,21 +427,62 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
 // (potentials/pair/Field.cpp:660-842, 1016-1201; hfunc1 :2218-2236;
 // TwoBody.h:203-215).  Self-image pairs carry full weight as in the
 // reference's no-image loop, other pairs half (each is listed twice).
+// one real-space pair: Coulomb erfc term and Buckingham, returns the radial
+// factor forc = (dE/dr)/r and the pair energies
+template <typename T>
+__device__ inline void ion_pair_terms(const FieldDev &F, const IonPairDev &p, double r2, double qiq, T &forc, T &ec,
+                                      T &eb)
+{
+    const T tsp = T(1.1283791670955126);  // 2/sqrt(pi)
+    T r2t = T(r2);
+    T r = sqrt(r2t);
+    T ir = T(1) / r;
+    forc = T(0);
+    ec = T(0);
+    eb = T(0);
+    if (F.do_ewald) {
+        T eta = T(F.eta);
+        T cq = T(qiq);
+        T xr = eta * r;
+        T erfcv = erfc(xr);
+        T ex = exp(-xr * xr);
+        ec = cq * erfcv * ir;
+        forc = cq * (-erfcv * ir * ir * ir - tsp * eta * ex * ir * ir);
+    }
+    if (p.on) {
+        T A = sizeof(T) == 8 ? T(p.Ad) : T(p.A);
+        T irho = sizeof(T) == 8 ? T(p.inv_rhod) : T(p.inv_rho);
+        T Cc = sizeof(T) == 8 ? T(p.Cd) : T(p.C);
+        T e1 = A * exp(-r * irho);
+        T ir6 = ir * ir * ir;
+        ir6 *= ir6;
+        T e2 = -Cc * ir6;
+        eb = e1 + e2;
+        forc -= e1 * irho * ir + T(6) * e2 * ir * ir;
+    }
+}
+
+// real-space Ewald and Buckingham over the full list
+// (potentials/pair/Field.cpp:660-842, 1016-1201; hfunc1 :2218-2236;
+// TwoBody.h:203-215).  Self-image pairs carry full weight as in the
+// reference's no-image loop, other pairs half (each is listed twice).
+// On the FP32 path the leading near-shell entries of each list are
+// evaluated in FP64: they carry the large, cancelling Coulomb and
+// Buckingham terms.
 template <typename T, bool MINIMG, bool STRESS>
 __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
-    if (skipped(b, walker_of(b, i))) return;
+    if (other_precision<T>(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
     const int ti = b.type[i];
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
-    const T eta = T(F.eta);
-    const T qi = T(F.cfact * b.q[i]);
-    const T tsp = T(1.1283791670955126);  // 2/sqrt(pi)
+    const int nnear = sizeof(T) == 8 ? cnt : b.nnear[i];
+    const double qi = F.cfact * b.q[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, er = 0.0, ev = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
     for (int k = 0; k < cnt; ++k) {
@@ -430,37 +493,27 @@ __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
         entry_disp(e, dx, dy, dz, W, MINIMG);
         double r2 = dx * dx + dy * dy + dz * dz;
         if (!(r2 <= rc2) || r2 < 1.0e-16) continue;
-        T r2t = T(r2);
-        T r = sqrt(r2t);
-        T ir = T(1) / r;
-        T forc = T(0), ec = T(0), eb = T(0);
-        if (F.do_ewald) {
-            T cq = qi * T(b.q[j]);
-            T xr = eta * r;
-            T erfcv = erfc(xr);
-            T ex = exp(-xr * xr);
-            ec = cq * erfcv * ir;
-            forc = cq * (-erfcv * ir * ir * ir - tsp * eta * ex * ir * ir);
-        }
         const IonPairDev &p = F.ion[ti * F.nspec + b.type[j]];
-        if (p.on) {
-            T A = sizeof(T) == 8 ? T(p.Ad) : T(p.A);
-            T irho = sizeof(T) == 8 ? T(p.inv_rhod) : T(p.inv_rho);
-            T Cc = sizeof(T) == 8 ? T(p.Cd) : T(p.C);
-            T e1 = A * exp(-r * irho);
-            T ir6 = ir * ir * ir;
-            ir6 *= ir6;
-            T e2 = -Cc * ir6;
-            eb = e1 + e2;
-            forc -= e1 * irho * ir + T(6) * e2 * ir * ir;
+        double df, dec, deb;
+        if (k < nnear) {
+            double forc, ec, eb;
+            ion_pair_terms<double>(F, p, r2, qi * b.q[j], forc, ec, eb);
+            df = forc;
+            dec = ec;
+            deb = eb;
+        } else {
+            T forc, ec, eb;
+            ion_pair_terms<T>(F, p, r2, qi * b.q[j], forc, ec, eb);
+            df = double(forc);
+            dec = double(ec);
+            deb = double(eb);
         }
         double wgt = (j == i) ? 1.0 : 0.5;
-        double df = double(forc);
         fx -= df * dx;
         fy -= df * dy;
         fz -= df * dz;
-        er += wgt * double(ec);
-        ev += wgt * double(eb);
+        er += wgt * dec;
+        ev += wgt * deb;
         if (STRESS) {
             double h = wgt * df;
             s0 += h * dx * dx;
@@ -694,19 +747,12 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 }
 
 template <typename T>
-static void launch_field(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
+static void launch_pass1(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     int gA = blocks(b.A);
     if (hf.kind == 0) {
         if (hf.min_image) k_sc_density<T, true><<<gA, kBlock, 0, s>>>(b);
         else k_sc_density<T, false><<<gA, kBlock, 0, s>>>(b);
-        if (hf.min_image) {
-            if (want_stress) k_sc_force<T, true, true><<<gA, kBlock, 0, s>>>(b);
-            else k_sc_force<T, true, false><<<gA, kBlock, 0, s>>>(b);
-        } else {
-            if (want_stress) k_sc_force<T, false, true><<<gA, kBlock, 0, s>>>(b);
-            else k_sc_force<T, false, false><<<gA, kBlock, 0, s>>>(b);
-        }
     } else {
         if (hf.min_image) {
             if (want_stress) k_ion_real<T, true, true><<<gA, kBlock, 0, s>>>(b);
@@ -715,19 +761,59 @@ static void launch_field(const BatchDev &b, const FieldDev &hf, int want_stress,
             if (want_stress) k_ion_real<T, false, true><<<gA, kBlock, 0, s>>>(b);
             else k_ion_real<T, false, false><<<gA, kBlock, 0, s>>>(b);
         }
-        if (hf.do_ewald) {
-            k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
-            dim3 g((b.maxnk + kBlock - 1) / kBlock, b.W);
-            k_ewald_sk<<<g, kBlock, 0, s>>>(b);
-            k_ewald_force<<<gA, kBlock, 0, s>>>(b);
-        }
     }
-    k_walker_reduce<<<b.W, kBlock, 0, s>>>(b, want_stress);
 }
 
-void akmc_launch_forces_impl(const BatchDev &b, const FieldDev &hf, int fp64, int want_stress, hipStream_t s)
+template <typename T>
+static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
-    launch_lists(b, hf, s);
-    if (fp64) launch_field<double>(b, hf, want_stress, s);
-    else launch_field<float>(b, hf, want_stress, s);
+    int gA = blocks(b.A);
+    if (hf.kind == 0) {
+        if (hf.min_image) {
+            if (want_stress) k_sc_force<T, true, true><<<gA, kBlock, 0, s>>>(b);
+            else k_sc_force<T, true, false><<<gA, kBlock, 0, s>>>(b);
+        } else {
+            if (want_stress) k_sc_force<T, false, true><<<gA, kBlock, 0, s>>>(b);
+            else k_sc_force<T, false, false><<<gA, kBlock, 0, s>>>(b);
+        }
+    } else if (hf.do_ewald) {
+        k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
+        dim3 g((b.maxnk + kBlock - 1) / kBlock, b.W);
+        k_ewald_sk<<<g, kBlock, 0, s>>>(b);
+        k_ewald_force<<<gA, kBlock, 0, s>>>(b);
+    }
+}
+
+// one phase of a force evaluation: 0 lists, 1 pass 1 (SC density / ionic
+// real space), 2 pass 2 (SC force / Ewald reciprocal), 3 walker reduction.
+// prec: bit 0 launches the FP32 kernels, bit 1 the FP64 ones; each serves
+// the walkers whose active mode matches.
+void akmc_launch_phase(const BatchDev &b, const FieldDev &hf, int prec, int want_stress, int phase,
+                       hipStream_t s)
+{
+    switch (phase) {
+    case 0:
+        launch_lists(b, hf, s);
+        break;
+    case 1:
+        if (prec & 1) launch_pass1<float>(b, hf, want_stress, s);
+        if (prec & 2) launch_pass1<double>(b, hf, want_stress, s);
+        break;
+    case 2:
+        if (hf.kind == 1) {
+            // the reciprocal part is FP64 for every walker
+            if (prec) launch_pass2<double>(b, hf, want_stress, s);
+        } else {
+            if (prec & 1) launch_pass2<float>(b, hf, want_stress, s);
+            if (prec & 2) launch_pass2<double>(b, hf, want_stress, s);
+        }
+        break;
+    default:
+        k_walker_reduce<<<b.W, kBlock, 0, s>>>(b, want_stress);
+    }
+}
+
+void akmc_launch_forces_impl(const BatchDev &b, const FieldDev &hf, int prec, int want_stress, hipStream_t s)
+{
+    for (int ph = 0; ph < 4; ++ph) akmc_launch_phase(b, hf, prec, want_stress, ph, s);
 }

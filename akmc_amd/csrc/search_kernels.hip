@@ -77,6 +77,30 @@ __device__ inline void write_probe(const Ctx &C_, const double *D, double a)
     }
 }
 
+// a walker that ends stops being evaluated by the force kernels
+__device__ inline void finish(const Ctx &C_, WalkerState &w)
+{
+    w.state = ST_DONE;
+    if (threadIdx.x == 0) C_.b.active[blockIdx.x] = 0;
+    write_probe(C_, nullptr, 0.0);
+}
+
+// a converged saddle is re-verified in FP64 when verification is enabled
+__device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature)
+{
+    w.e32 = w.E0;
+    w.c32 = curvature;
+    if (C_.s.verify_nl > 0) {
+        w.kind = WK_VERIFY;
+        w.state = ST_V_F0;
+        w.lk = 0;
+        if (threadIdx.x == 0) C_.b.active[blockIdx.x] = 2;
+        write_probe(C_, nullptr, 0.0);
+    } else {
+        finish(C_, w);
+    }
+}
+
 // ------------------------------------------------------------------ relax
 
 // the part of a FIRE iteration before its force call (Relax.cpp:125-193,
@@ -174,14 +198,12 @@ __device__ void relax_logic(const Ctx &C_, WalkerState &w, double *sh)
     fm2 = bmax(fm2, sh);
     if (sqrt(t[0]) < p.normTol || sqrt(fm2) < p.forceTol) {
         w.status = 0;
-        w.state = ST_DONE;
-        write_probe(C_, nullptr, 0.0);
+        finish(C_, w);
         return;
     }
     if (w.it >= p.maxIter) {
         w.status = 1;
-        w.state = ST_DONE;
-        write_probe(C_, nullptr, 0.0);
+        finish(C_, w);
         return;
     }
     w.it += 1;
@@ -266,8 +288,7 @@ __device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
     w.C = Cv;
     if (sqrt(fm2) < p.forceTol && Cv < p.maxEigenvalue) {
         w.status = 0;
-        w.state = ST_DONE;
-        write_probe(C_, nullptr, 0.0);
+        enter_verify_or_finish(C_, w, Cv);
         return;
     }
     const bool convex = Cv > 0.0;
@@ -326,9 +347,8 @@ __device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
     w.it += 1;
     if (w.it > p.maxParIter) {
         w.status = 1;
-        w.state = ST_DONE;
         __syncthreads();
-        write_probe(C_, nullptr, 0.0);
+        finish(C_, w);
         return;
     }
     w.state = ST_D_F0;
@@ -406,6 +426,482 @@ __device__ void dimer_logic(const Ctx &C_, WalkerState &w, double *sh)
     dimer_rot_eval(C_, w, sh);
 }
 
+
+// ------------------------------------------------------------ verification
+
+// eigenvalues of the k x k symmetric tridiagonal (d, e) by implicit QL
+// (values only), ascending in d; single thread
+__device__ void tridiag_values(int n, double *d, double *e)
+{
+    for (int i = 0; i < n - 1; ++i) e[i] = e[i];
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        for (;;) {
+            for (m = l; m < n - 1; ++m) {
+                double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+            }
+            if (m == l || ++iter > 60) break;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+            double sn = 1.0, c = 1.0, p = 0.0;
+            int i;
+            for (i = m - 1; i >= l; --i) {
+                double f = sn * e[i], bb = c * e[i];
+                r = hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    break;
+                }
+                sn = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * sn + 2.0 * c * bb;
+                p = sn * r;
+                d[i + 1] = g + p;
+                g = c * r - bb;
+            }
+            if (r == 0.0 && i >= l) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    for (int i = 1; i < n; ++i) {  // insertion sort
+        double v = d[i];
+        int j = i - 1;
+        while (j >= 0 && d[j] > v) {
+            d[j + 1] = d[j];
+            --j;
+        }
+        d[j + 1] = v;
+    }
+}
+
+// FP64 check of a converged saddle: max force on the free atoms, energy,
+// and the two lowest Ritz values of the finite-difference Hessian from a
+// Lanczos run started on the dimer mode (full re-orthogonalisation)
+__device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
+{
+    const double delta = C_.s.verify_delta;
+    const int nl = C_.s.verify_nl < AKMC_LZ_MAX ? C_.s.verify_nl : AKMC_LZ_MAX;
+    const size_t A3 = 3 * (size_t)C_.A;
+    double *lz = C_.s.lz + (size_t)blockIdx.x * 3 * AKMC_LZ_MAX;
+    w.fcalls += 1;
+    if (w.state == ST_V_F0) {
+        double fm2 = 0.0;
+        FOR_ATOMS(i)
+        {
+            double q = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                double F = C_.f(c, i);
+                V3(C_.s.F0, c, i) = F;
+                q += F * F;
+                V3(C_.s.Q, c, i) = V3(C_.s.N, c, i);
+            }
+            fm2 = fmax(fm2, q);
+        }
+        fm2 = bmax(fm2, sh);
+        w.fmax64 = sqrt(fm2);
+        w.e64 = C_.b.out[blockIdx.x].total;
+        w.lk = 0;
+        w.state = ST_V_LZ;
+        __syncthreads();
+        write_probe(C_, C_.s.Q, delta);
+        return;
+    }
+    const int k = w.lk;
+    const double *Qk = C_.s.Q + k * A3;
+    double a1[1] = {0.0};
+    FOR_ATOMS(i)
+    {
+        for (int c = 0; c < 3; ++c) {
+            double u = -(C_.f(c, i) - V3(C_.s.F0, c, i)) / delta;
+            V3(C_.s.T, c, i) = u;
+            a1[0] += V3(Qk, c, i) * u;
+        }
+    }
+    bsum<1>(a1, sh);
+    const double alpha = a1[0];
+    const double bprev = k > 0 ? lz[AKMC_LZ_MAX + k - 1] : 0.0;
+    const double *Qp = k > 0 ? C_.s.Q + (k - 1) * A3 : nullptr;
+    FOR_ATOMS(i)
+    {
+        for (int c = 0; c < 3; ++c) {
+            double r = V3(C_.s.T, c, i) - alpha * V3(Qk, c, i);
+            if (Qp) r -= bprev * V3(Qp, c, i);
+            V3(C_.s.T, c, i) = r;
+        }
+    }
+    for (int j = 0; j <= k; ++j) {
+        const double *Qj = C_.s.Q + j * A3;
+        double dj[1] = {0.0};
+        FOR_ATOMS(i)
+        {
+            for (int c = 0; c < 3; ++c) dj[0] += V3(C_.s.T, c, i) * V3(Qj, c, i);
+        }
+        bsum<1>(dj, sh);
+        FOR_ATOMS(i)
+        {
+            for (int c = 0; c < 3; ++c) V3(C_.s.T, c, i) -= dj[0] * V3(Qj, c, i);
+        }
+    }
+    double bb[1] = {0.0};
+    FOR_ATOMS(i)
+    {
+        for (int c = 0; c < 3; ++c) bb[0] += V3(C_.s.T, c, i) * V3(C_.s.T, c, i);
+    }
+    bsum<1>(bb, sh);
+    const double beta = sqrt(bb[0]);
+    if (threadIdx.x == 0) {
+        lz[k] = alpha;
+        lz[AKMC_LZ_MAX + k] = beta;
+    }
+    w.lk = k + 1;
+    if (w.lk >= nl || beta < 1.0e-12) {
+        __shared__ double ev[2];
+        if (threadIdx.x == 0) {
+            double d[AKMC_LZ_MAX], e[AKMC_LZ_MAX];
+            int n = w.lk;
+            for (int t = 0; t < n; ++t) {
+                d[t] = lz[t];
+                e[t] = lz[AKMC_LZ_MAX + t];
+            }
+            tridiag_values(n, d, e);
+            ev[0] = d[0];
+            ev[1] = n > 1 ? d[1] : 1.0e300;
+        }
+        __syncthreads();
+        w.lam1 = ev[0];
+        w.lam2 = ev[1];
+        w.verified_state = 1;
+        finish(C_, w);
+        return;
+    }
+    double *Qn = C_.s.Q + (size_t)(k + 1) * A3;
+    const double ib = 1.0 / beta;
+    FOR_ATOMS(i)
+    {
+        double pr[3];
+        for (int c = 0; c < 3; ++c) {
+            double q = V3(C_.s.T, c, i) * ib;
+            V3(Qn, c, i) = q;
+            pr[c] = V3(C_.s.X, c, i) + delta * q;
+        }
+        C_.probe(i, pr[0], pr[1], pr[2]);
+    }
+}
+
+// ------------------------------------------------------------------- ART
+
+// lowest eigenvector of the n x n symmetric tridiagonal (d, e) for the
+// eigenvalue lam by inverse iteration (three sweeps); single thread
+__device__ void tridiag_lowest_vector(int n, const double *d, const double *e, double lam, double *y)
+{
+    double scale = 0.0;
+    for (int i = 0; i < n; ++i) scale = fmax(scale, fabs(d[i]) + (i < n - 1 ? fabs(e[i]) : 0.0));
+    const double sigma = lam - 1.0e-10 * (scale > 0.0 ? scale : 1.0);
+    double c[AKMC_LZ_MAX], z[AKMC_LZ_MAX];
+    for (int i = 0; i < n; ++i) y[i] = 1.0;
+    for (int sweep = 0; sweep < 3; ++sweep) {
+        // Thomas algorithm on (T - sigma I) z = y
+        double piv = d[0] - sigma;
+        if (fabs(piv) < 1.0e-300) piv = 1.0e-300;
+        c[0] = n > 1 ? e[0] / piv : 0.0;
+        z[0] = y[0] / piv;
+        for (int i = 1; i < n; ++i) {
+            piv = d[i] - sigma - e[i - 1] * c[i - 1];
+            if (fabs(piv) < 1.0e-300) piv = 1.0e-300;
+            c[i] = i < n - 1 ? e[i] / piv : 0.0;
+            z[i] = (y[i] - e[i - 1] * z[i - 1]) / piv;
+        }
+        for (int i = n - 2; i >= 0; --i) z[i] -= c[i] * z[i + 1];
+        double nn = 0.0;
+        for (int i = 0; i < n; ++i) nn += z[i] * z[i];
+        nn = 1.0 / sqrt(nn);
+        for (int i = 0; i < n; ++i) y[i] = z[i] * nn;
+    }
+}
+
+__device__ void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh);
+__device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature);
+
+// start of ART::calculateLanczosForce (optim/ART.cpp:159-190): force at x,
+// q_0 = r / |r| with r the current lowest-mode estimate
+__device__ void art_lanczos_begin(const Ctx &C_, WalkerState &w, double *sh)
+{
+    const ARTParamsDev &p = C_.s.ap;
+    double nn[1] = {0.0};
+    FOR_ATOMS(i)
+    {
+        for (int c = 0; c < 3; ++c) {
+            V3(C_.s.F0, c, i) = C_.f(c, i);
+            double r = V3(C_.s.N, c, i);
+            nn[0] += r * r;
+        }
+    }
+    bsum<1>(nn, sh);
+    w.E0 = C_.b.out[blockIdx.x].total;
+    const double beta = sqrt(nn[0]);
+    w.lk = 0;
+    w.lbeta = beta;
+    w.lconv = 0;
+    w.lam = 1.0e6;  // lowestEigenValue before the first diagonalisation (ART.cpp:157)
+    FOR_ATOMS(i)
+    {
+        double pr[3];
+        for (int c = 0; c < 3; ++c) {
+            double q = V3(C_.s.N, c, i) / beta;
+            V3(C_.s.Q, c, i) = q;
+            pr[c] = V3(C_.s.X, c, i) + p.deltaX * q;
+        }
+        C_.probe(i, pr[0], pr[1], pr[2]);
+    }
+    w.state = ST_A_LZ;
+}
+
+// ART after a Lanczos evaluation: velocity half kick and convergence test
+// of the previous iteration (optim/ART.cpp:660-720), or entry into the loop
+__device__ void art_after_lanczos(const Ctx &C_, WalkerState &w, double *sh)
+{
+    const ARTParamsDev &p = C_.s.ap;
+    if (!w.inloop) {
+        w.inloop = 1;
+        w.half = 0.5 * w.dt * p.timeStepUnit;
+        art_fire_pre(C_, w, sh);
+        return;
+    }
+    const bool perp = w.lam >= p.maxEigenvalue;
+    const double dot = w.dotfv;
+    FOR_ATOMS(i)
+    {
+        for (int c = 0; c < 3; ++c) {
+            double v = V3(C_.s.N, c, i);
+            double F = perp ? -dot * v : V3(C_.s.F0, c, i) - 2.0 * dot * v;
+            V3(C_.s.V, c, i) += w.half * F;
+        }
+    }
+    if (w.it > 5 && ((w.fnpar + w.fnperp) < p.normTol || (w.fmpar + w.fmperp) < p.forceTol)) {
+        w.status = (w.lam <= p.maxEigenvalue) ? 0 : 1;
+        if (w.status == 0) enter_verify_or_finish(C_, w, w.lam);
+        else finish(C_, w);
+        return;
+    }
+    w.half = 0.5 * w.dt * p.timeStepUnit;
+    if (fabs(w.half) < 1.0e-10) {
+        w.status = 1;
+        finish(C_, w);
+        return;
+    }
+    art_fire_pre(C_, w, sh);
+}
+
+// one Lanczos step (optim/ART.cpp:191-283); at the end the Ritz vector,
+// F_par = F - 2(F.v)v and F_perp = -(F.v)v (ART.cpp:318-349)
+__device__ void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
+{
+    const ARTParamsDev &p = C_.s.ap;
+    const size_t A3 = 3 * (size_t)C_.A;
+    double *lz = C_.s.lz + (size_t)blockIdx.x * 3 * AKMC_LZ_MAX;
+    const int i = w.lk;
+    const int nl = p.numLanczosVectors;
+    const double *Qi = C_.s.Q + i * A3;
+    const double *Qp = i > 0 ? C_.s.Q + (i - 1) * A3 : nullptr;
+    const double beta = w.lbeta;
+    double a1[1] = {0.0};
+    FOR_ATOMS(k)
+    {
+        for (int c = 0; c < 3; ++c) {
+            double u = -(C_.f(c, k) - V3(C_.s.F0, c, k)) / p.deltaX;
+            double r = Qp ? u - beta * V3(Qp, c, k) : u;
+            V3(C_.s.T, c, k) = r;
+            a1[0] += V3(Qi, c, k) * r;
+        }
+    }
+    bsum<1>(a1, sh);
+    const double alpha = a1[0];
+    double bb[1] = {0.0};
+    FOR_ATOMS(k)
+    {
+        for (int c = 0; c < 3; ++c) {
+            double r = V3(C_.s.T, c, k) - alpha * V3(Qi, c, k);
+            V3(C_.s.T, c, k) = r;
+            bb[0] += r * r;
+        }
+    }
+    bsum<1>(bb, sh);
+    const double bnew = sqrt(bb[0]);
+    __shared__ double evs[1];
+    if (threadIdx.x == 0) {
+        lz[i] = alpha;
+        if (i > 0) lz[AKMC_LZ_MAX + i - 1] = beta;
+        if (i > 0) {
+            double d[AKMC_LZ_MAX], e[AKMC_LZ_MAX];
+            for (int t = 0; t <= i; ++t) {
+                d[t] = lz[t];
+                e[t] = t < i ? lz[AKMC_LZ_MAX + t] : 0.0;
+            }
+            tridiag_values(i + 1, d, e);
+            evs[0] = d[0];
+        } else {
+            evs[0] = alpha;
+        }
+    }
+    __syncthreads();
+    const double evnew = evs[0];
+    if (i == 0) {
+        w.lamold = evnew;
+    } else {
+        double delta = fabs((evnew - w.lamold) / w.lamold);
+        w.lamold = evnew;
+        w.lam = evnew;
+        if (delta < p.eigValTol) w.lconv = 1;
+    }
+    const int next = i + 1;
+    if (next < nl && !w.lconv) {
+        double *Qn = C_.s.Q + next * A3;
+        const double ib = 1.0 / bnew;
+        FOR_ATOMS(k)
+        {
+            double pr[3];
+            for (int c = 0; c < 3; ++c) {
+                double q = V3(C_.s.T, c, k) * ib;
+                V3(Qn, c, k) = q;
+                pr[c] = V3(C_.s.X, c, k) + p.deltaX * q;
+            }
+            C_.probe(k, pr[0], pr[1], pr[2]);
+        }
+        w.lk = next;
+        w.lbeta = bnew;
+        return;
+    }
+    // Ritz vector of the last diagonalised T (size next)
+    if (threadIdx.x == 0) {
+        double d[AKMC_LZ_MAX], e[AKMC_LZ_MAX];
+        for (int t = 0; t < next; ++t) {
+            d[t] = lz[t];
+            e[t] = t < next - 1 ? lz[AKMC_LZ_MAX + t] : 0.0;
+        }
+        tridiag_lowest_vector(next, d, e, evnew, lz + 2 * AKMC_LZ_MAX);
+    }
+    __syncthreads();
+    const double *y = lz + 2 * AKMC_LZ_MAX;
+    double nn[1] = {0.0};
+    FOR_ATOMS(k)
+    {
+        for (int c = 0; c < 3; ++c) {
+            double v = 0.0;
+            for (int t = 0; t < next; ++t) v += y[t] * V3(C_.s.Q + t * A3, c, k);
+            V3(C_.s.N, c, k) = v;
+            nn[0] += v * v;
+        }
+    }
+    bsum<1>(nn, sh);
+    const double inv = 1.0 / sqrt(nn[0]);
+    double dd[1] = {0.0};
+    FOR_ATOMS(k)
+    {
+        for (int c = 0; c < 3; ++c) {
+            double v = V3(C_.s.N, c, k) * inv;
+            V3(C_.s.N, c, k) = v;
+            dd[0] += V3(C_.s.F0, c, k) * v;
+        }
+    }
+    bsum<1>(dd, sh);
+    w.dotfv = dd[0];
+    w.C = w.lam;  // reported curvature: the Lanczos lowest eigenvalue
+    art_after_lanczos(C_, w, sh);
+}
+
+// the FIRE part of one ART iteration before the next Lanczos evaluation
+// (optim/ART.cpp:549-656)
+__device__ void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh)
+{
+    const ARTParamsDev &p = C_.s.ap;
+    if (w.it > p.maxParIter) {
+        w.status = 1;
+        finish(C_, w);
+        return;
+    }
+    w.it += 1;
+    const bool perp = w.lam >= p.maxEigenvalue;
+    const double dot = w.dotfv;
+    double acc[3] = {0, 0, 0};  // |F_par|^2, |F_perp|^2, |v|^2
+    double mx[3] = {0, 0, 0};
+    FOR_ATOMS(i)
+    {
+        for (int c = 0; c < 3; ++c) {
+            double nv = V3(C_.s.N, c, i);
+            double fpar = V3(C_.s.F0, c, i) - 2.0 * dot * nv, fperp = -dot * nv;
+            double v = V3(C_.s.V, c, i) + w.half * (perp ? fperp : fpar * p.parDamp);
+            V3(C_.s.V, c, i) = v;
+            acc[0] += fpar * fpar;
+            acc[1] += fperp * fperp;
+            acc[2] += v * v;
+            mx[0] = fmax(mx[0], fpar * fpar);
+            mx[1] = fmax(mx[1], fperp * fperp);
+        }
+    }
+    bsum<3>(acc, sh);
+    mx[0] = bmax(mx[0], sh);
+    mx[1] = bmax(mx[1], sh);
+    w.fnpar = sqrt(acc[0]);
+    w.fnperp = sqrt(acc[1]);
+    w.fmpar = sqrt(mx[0]);
+    w.fmperp = sqrt(mx[1]);
+    const double vn = sqrt(acc[2]);
+    const double fsel = perp ? w.fnperp : w.fnpar;
+    double P[1] = {0.0};
+    FOR_ATOMS(i)
+    {
+        for (int c = 0; c < 3; ++c) {
+            double nv = V3(C_.s.N, c, i);
+            double F = perp ? -dot * nv : V3(C_.s.F0, c, i) - 2.0 * dot * nv;
+            P[0] += F * V3(C_.s.V, c, i);
+        }
+    }
+    bsum<1>(P, sh);
+    const double al = w.alpha;
+    const bool down = P[0] > 0;
+    if (down) {
+        if (w.nstep > p.N_min) {
+            w.dt = fmin(w.dt * p.stepIncrease, p.maxTimeStep);
+            w.alpha *= p.alphaDecrease;
+        }
+        w.nstep += 1;
+    } else {
+        w.nstep = 0;
+        w.dt *= p.stepDecrease;
+        w.alpha = p.alphaStart;
+    }
+    const double dt = w.dt;
+    FOR_ATOMS(i)
+    {
+        const bool fr = C_.s.frozen[i] != 0;
+        for (int c = 0; c < 3; ++c) {
+            double nv = V3(C_.s.N, c, i);
+            double F = perp ? -dot * nv : V3(C_.s.F0, c, i) - 2.0 * dot * nv;
+            double v = down ? (1.0 - al) * V3(C_.s.V, c, i) + al * F * vn / fsel : 0.0;
+            V3(C_.s.V, c, i) = v;
+            if (!fr) V3(C_.s.X, c, i) += dt * v;
+        }
+    }
+    w.state = ST_A_L0;
+    __syncthreads();
+    write_probe(C_, nullptr, 0.0);
+}
+
+__device__ void art_logic(const Ctx &C_, WalkerState &w, double *sh)
+{
+    w.fcalls += 1;
+    if (w.state == ST_A_L0) art_lanczos_begin(C_, w, sh);
+    else art_lanczos_step(C_, w, sh);
+}
+
 __global__ void __launch_bounds__(kBlock) k_search_logic(BatchDev b, SearchDev s)
 {
     __shared__ double sh[4 * kBlock];
@@ -422,6 +918,8 @@ __global__ void __launch_bounds__(kBlock) k_search_logic(BatchDev b, SearchDev s
     WalkerState w = ws;
     if (w.kind == WK_RELAX) relax_logic(C_, w, sh);
     else if (w.kind == WK_DIMER) dimer_logic(C_, w, sh);
+    else if (w.kind == WK_VERIFY) verify_logic(C_, w, sh);
+    else if (w.kind == WK_ART) art_logic(C_, w, sh);
     if (threadIdx.x == 0) s.ws[wid] = w;
 }
 

@@ -51,9 +51,22 @@ class DimerParams(C.Structure):
                 ("maxStep", C.c_double), ("timeStepUnit", C.c_double)]
 
 
+class ARTParams(C.Structure):
+    """akmc_art_params: ARTCntrl field names (Common/ArtCntrl.h:14-74)."""
+    _fields_ = [("numLanczosVectors", C.c_int), ("deltaX", C.c_double), ("maxEigenvalue", C.c_double),
+                ("eigValTol", C.c_double), ("parDamp", C.c_double), ("maxParIter", C.c_int),
+                ("normTol", C.c_double), ("forceTol", C.c_double), ("N_min", C.c_int),
+                ("initTimeStep", C.c_double), ("alphaDecrease", C.c_double), ("stepIncrease", C.c_double),
+                ("stepDecrease", C.c_double), ("alphaStart", C.c_double), ("maxTimeStep", C.c_double),
+                ("timeStepUnit", C.c_double)]
+
+
 class WalkerStatus(C.Structure):
     _fields_ = [("kind", C.c_int), ("status", C.c_int), ("done", C.c_int), ("iterations", C.c_int),
-                ("force_calls", C.c_int), ("energy", C.c_double), ("curvature", C.c_double)]
+                ("force_calls", C.c_int), ("energy", C.c_double), ("curvature", C.c_double),
+                ("verified", C.c_int), ("energy32", C.c_double), ("curvature32", C.c_double),
+                ("energy64", C.c_double), ("fmax64", C.c_double), ("lambda1", C.c_double),
+                ("lambda2", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -62,6 +75,14 @@ class WalkerStatus(C.Structure):
 def relax_defaults(**kw) -> RelaxParams:
     p = RelaxParams()
     load().akmc_relax_defaults(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def art_defaults(**kw) -> ARTParams:
+    p = ARTParams()
+    load().akmc_art_defaults(C.byref(p))
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -117,11 +138,19 @@ def load() -> C.CDLL:
     L.akmc_dimer_defaults.argtypes = [C.POINTER(DimerParams)]
     L.akmc_batch_set_frozen.argtypes = [C.c_void_p, _I]
     L.akmc_batch_start_relax.argtypes = [C.c_void_p, _I, C.POINTER(RelaxParams)]
-    L.akmc_batch_start_dimer.argtypes = [C.c_void_p, _I, _D, _D, C.POINTER(DimerParams)]
+    L.akmc_batch_start_dimer.argtypes = [C.c_void_p, _I, _D, _D, _I, C.POINTER(DimerParams)]
     L.akmc_batch_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _I]
     L.akmc_batch_status.argtypes = [C.c_void_p, C.POINTER(WalkerStatus)]
     L.akmc_batch_get_positions.argtypes = [C.c_void_p, _D, _D, _D]
     L.akmc_batch_get_mode.argtypes = [C.c_void_p, _D]
+    L.akmc_batch_time_phases.argtypes = [C.c_void_p, C.c_int, _D]
+    L.akmc_batch_save_basin.argtypes = [C.c_void_p]
+    L.akmc_art_defaults.argtypes = [C.POINTER(ARTParams)]
+    L.akmc_batch_start_art.argtypes = [C.c_void_p, _I, _D, _D, _I, C.POINTER(ARTParams)]
+    L.akmc_batch_set_verify.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    L.akmc_batch_search_energies.argtypes = [C.c_void_p, C.c_int, _D]
+    L.akmc_batch_get_walker_positions.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
+    L.akmc_batch_restore_basin.argtypes = [C.c_void_p, _I]
     _lib = L
     return L
 
@@ -273,12 +302,67 @@ class Batch:
         m = self._mask(walkers)
         check(load().akmc_batch_start_relax(self._h, _ip(m), C.byref(p)))
 
-    def start_dimer(self, mode_per_walker, disp_per_walker=None, params: DimerParams | None = None, walkers=None):
+    def start_dimer(self, mode_per_walker, disp_per_walker=None, params: DimerParams | None = None, walkers=None,
+                    x0_per_walker=None, frozen_per_walker=None):
+        """Start dimer searches.  Either x0 (absolute start positions) or
+        disp (added to the current positions, downloaded first) may be given;
+        per-walker lists cover every walker of the batch."""
         p = params or dimer_defaults()
         m = self._mask(walkers)
         mode = self._flat3(mode_per_walker)
-        disp = self._flat3(disp_per_walker) if disp_per_walker is not None else None
-        check(load().akmc_batch_start_dimer(self._h, _ip(m), _dp(disp), _dp(mode), C.byref(p)))
+        x0 = None
+        if x0_per_walker is not None:
+            x0 = self._flat3(x0_per_walker)
+        elif disp_per_walker is not None:
+            cur = self.positions()
+            x0 = self._flat3([c + d for c, d in zip(cur, disp_per_walker)])
+        fr = None
+        if frozen_per_walker is not None:
+            fr = np.ascontiguousarray(np.concatenate([np.asarray(f) for f in frozen_per_walker]), dtype=np.int32)
+        check(load().akmc_batch_start_dimer(self._h, _ip(m), _dp(x0), _dp(mode), _ip(fr), C.byref(p)))
+
+    def _start_args(self, mode_per_walker, disp_per_walker, walkers, x0_per_walker, frozen_per_walker):
+        m = self._mask(walkers)
+        mode = self._flat3(mode_per_walker)
+        x0 = None
+        if x0_per_walker is not None:
+            x0 = self._flat3(x0_per_walker)
+        elif disp_per_walker is not None:
+            cur = self.positions()
+            x0 = self._flat3([c + d for c, d in zip(cur, disp_per_walker)])
+        fr = None
+        if frozen_per_walker is not None:
+            fr = np.ascontiguousarray(np.concatenate([np.asarray(f) for f in frozen_per_walker]), dtype=np.int32)
+        return m, x0, mode, fr
+
+    def start_art(self, mode_per_walker, disp_per_walker=None, params: ARTParams | None = None, walkers=None,
+                  x0_per_walker=None, frozen_per_walker=None):
+        """ART searches (optim/ART.cpp:17-756) from x0 (or current + disp)
+        along the unit start direction mode (the reference's rVector)."""
+        p = params or art_defaults()
+        m, x0, mode, fr = self._start_args(mode_per_walker, disp_per_walker, walkers, x0_per_walker,
+                                           frozen_per_walker)
+        check(load().akmc_batch_start_art(self._h, _ip(m), _dp(x0), _dp(mode), _ip(fr), C.byref(p)))
+
+    def walker_positions(self, w):
+        n = int(self.natoms[w])
+        out = [np.zeros(n) for _ in range(3)]
+        check(load().akmc_batch_get_walker_positions(self._h, int(w), *[_dp(a) for a in out]))
+        return np.stack(out)
+
+    def start_flat(self, kind, mask, x0, mode, frozen, params):
+        """Low-level start on flat [3][A] / [A] arrays (only the flagged
+        walkers' slices are read); kind is "dimer" or "art"."""
+        fn = load().akmc_batch_start_dimer if kind == "dimer" else load().akmc_batch_start_art
+        check(fn(self._h, _ip(mask), _dp(x0), _dp(mode), _ip(frozen), C.byref(params)))
+
+    def set_verify(self, nl=16, delta=0.01):
+        check(load().akmc_batch_set_verify(self._h, int(nl), float(delta)))
+
+    def search_energies(self, bits=64):
+        e = np.zeros(self.W)
+        check(load().akmc_batch_search_energies(self._h, int(bits), _dp(e)))
+        return e
 
     def run(self, max_steps, sync_every=1):
         n = C.c_int()
@@ -301,6 +385,17 @@ class Batch:
         check(load().akmc_batch_get_mode(self._h, _dp(m)))
         m = m.reshape(3, self.A)
         return [m[:, self.offsets[w]:self.offsets[w + 1]] for w in range(self.W)]
+
+    def time_phases(self, nrep):
+        ms = np.zeros(4)
+        check(load().akmc_batch_time_phases(self._h, int(nrep), _dp(ms)))
+        return ms / nrep
+
+    def save_basin(self):
+        check(load().akmc_batch_save_basin(self._h))
+
+    def restore_basin(self, walkers=None):
+        check(load().akmc_batch_restore_basin(self._h, _ip(self._mask(walkers))))
 
     @property
     def rebuilds(self):

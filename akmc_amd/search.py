@@ -1,0 +1,223 @@
+"""Batched saddle searches with FP64 verification on the device.
+
+``SaddleCampaign`` keeps W walker cells resident on the GPU, relaxes each
+to its basin (FIRE, optim/Relax.cpp:44-315), then runs saddle searches
+(dimer, or ART as optim/ART.cpp) from seeded local displacements around a
+vacancy with atoms beyond ``region_radius`` frozen (the local active region
+of optim/SaddleSearch.cpp:73-135, regionStyle 2).  Every walker is
+refilled with a fresh search as soon as its previous one ends -- the role
+of the reference's MPI task farm (kmc/KineticMonteCarlo.cpp:352-598).
+
+Each converged search is re-verified inside the same device walker stream
+in FP64: FP64 forces and energy at the saddle, then a Lanczos run on FP64
+finite-difference Hessian products started on the search's mode.  A saddle
+counts as FP64-verified when its largest FP64 atomic force is below
+``fmax_verify``, its FP64 barrier is within ``barrier_tol`` (0.01 eV) of
+the search's own barrier, the two lowest Lanczos curvatures bracket zero
+(exactly one negative direction) and the barrier is positive.  Verified
+saddles are de-duplicated per basin (barrier within 2 meV and the same
+moved atoms within 0.15 A).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import lib as L
+from . import workloads as W
+
+
+def active_region(cell: W.Cell, pos, site, radius):
+    """frozen mask: atoms farther than radius from site (minimum image)."""
+    d = W.min_image(pos - np.asarray(site)[:, None], cell.lat)
+    return (np.sqrt((d * d).sum(0)) >= radius).astype(np.int32)
+
+
+@dataclass
+class SaddleRecord:
+    walker: int
+    energy32: float
+    energy64: float
+    barrier32: float
+    barrier64: float
+    curvature: float
+    lanczos: tuple
+    fmax64: float
+    atoms: np.ndarray
+    disp: np.ndarray
+    verified: bool = False
+    distinct: bool = False
+
+
+@dataclass
+class CampaignStats:
+    searches: int = 0
+    converged: int = 0
+    verified: int = 0
+    distinct: int = 0
+    force_calls: int = 0
+    micro_steps: int = 0
+    search_time: float = 0.0
+    max_barrier_err: float = 0.0
+    records: list = field(default_factory=list)
+
+    def add(self, o: "CampaignStats"):
+        for k in ("searches", "converged", "verified", "distinct", "force_calls", "micro_steps"):
+            setattr(self, k, getattr(self, k) + getattr(o, k))
+        self.search_time += o.search_time
+        self.max_barrier_err = max(self.max_barrier_err, o.max_barrier_err)
+        self.records.extend(o.records)
+
+
+class SaddleCampaign:
+    def __init__(self, potentials_text, cells, method="dimer", region_radius=6.0, d0=0.1, params=None,
+                 relax=None, seed=0, fmax_verify=0.05, barrier_tol=0.01, verify_nl=16, verify_delta=0.01,
+                 proposer=None):
+        self.text = potentials_text
+        self.cells = cells
+        self.W = len(cells)
+        self.method = method
+        self.field = L.Field(potentials_text)
+        self.batch = L.Batch(self.field, cells)
+        self.region_radius = region_radius
+        self.d0 = d0
+        self.params = params or (L.dimer_defaults() if method == "dimer" else L.art_defaults())
+        self.relax = relax or L.relax_defaults()
+        self.rng = np.random.default_rng(seed)
+        self.fmax_verify = fmax_verify
+        self.barrier_tol = barrier_tol
+        self.verify_nl = verify_nl
+        self.verify_delta = verify_delta
+        self.proposer = proposer  # optional callable(campaign, w) -> (x0, mode, frozen, site)
+        self.found = [[] for _ in range(self.W)]
+        self.pending = set()
+        A = self.batch.A
+        self.off = self.batch.offsets
+        self._x0 = np.zeros(3 * A)
+        self._mode = np.zeros(3 * A)
+        self._frozen = np.zeros(A, dtype=np.int32)
+
+    # ---------------------------------------------------------- basins
+    def relax_basins(self, sync_every=50, max_steps=5000):
+        self.batch.set_frozen([np.zeros(c.n, np.int32) for c in self.cells])
+        self.batch.start_relax(self.relax)
+        self.batch.run(max_steps, sync_every=sync_every)
+        st = self.batch.status()
+        self.basin_ok = np.array([s["status"] == 0 for s in st])
+        self.basins = self.batch.positions()
+        self.batch.save_basin()
+        self.basin32 = self.batch.search_energies(32)
+        self.basin64 = self.batch.search_energies(64)
+        self.batch.set_verify(self.verify_nl, self.verify_delta)
+        return self.basin_ok
+
+    # ---------------------------------------------------------- searches
+    def proposal(self, w):
+        """Seeded random displacement (d0, centre-of-mass free) of the 12
+        neighbours of one of the walker's vacancies; the displacement
+        direction is the start mode."""
+        c = self.cells[w]
+        site = c.vacancies[int(self.rng.integers(0, len(c.vacancies)))]
+        basin = self.basins[w]
+        frozen = active_region(c, basin, site, self.region_radius)
+        cb = W.Cell(basin, c.types, c.lat)
+        nb = W.neighbours_of_site(cb, site, 12)
+        disp, mode = W.local_displacement(cb, nb, self.d0, int(self.rng.integers(0, 2**31 - 1)))
+        return basin + disp, mode, frozen, site
+
+    def start(self, walkers):
+        walkers = [int(w) for w in walkers if self.basin_ok[w]]
+        if not walkers:
+            return
+        A = self.batch.A
+        mask = np.zeros(self.W, dtype=np.int32)
+        for w in walkers:
+            x0, mode, frozen, _ = (self.proposer or SaddleCampaign.proposal)(self, w)
+            a, b = self.off[w], self.off[w + 1]
+            for c in range(3):
+                self._x0[c * A + a:c * A + b] = x0[c]
+                self._mode[c * A + a:c * A + b] = mode[c]
+            self._frozen[a:b] = frozen
+            mask[w] = 1
+            self.pending.add(w)
+        self.batch.start_flat(self.method, mask, self._x0, self._mode, self._frozen, self.params)
+
+    def harvest(self, stats: CampaignStats):
+        """Collect ended searches (their FP64 verification already ran on
+        the device); returns the walkers that ended."""
+        st = self.batch.status()
+        done = [w for w in sorted(self.pending) if st[w]["done"]]
+        for w in done:
+            s = st[w]
+            self.pending.discard(w)
+            stats.searches += 1
+            stats.force_calls += s["force_calls"]
+            if s["status"] != 0:
+                continue
+            stats.converged += 1
+            if not s["verified"]:
+                continue
+            b32 = s["energy32"] - self.basin32[w]
+            b64 = s["energy64"] - self.basin64[w]
+            err = abs(b32 - b64)
+            stats.max_barrier_err = max(stats.max_barrier_err, err)
+            pos = self.batch.walker_positions(w)
+            d = W.min_image(pos - self.basins[w], self.cells[w].lat)
+            dn = np.sqrt((d * d).sum(0))
+            atoms = np.nonzero(dn > 0.3)[0]
+            rec = SaddleRecord(w, s["energy32"], s["energy64"], float(b32), float(b64), s["curvature32"],
+                               (s["lambda1"], s["lambda2"]), s["fmax64"], atoms, d[:, atoms])
+            rec.verified = (s["fmax64"] < self.fmax_verify and err <= self.barrier_tol
+                            and s["lambda1"] < 0.0 < s["lambda2"] and b64 > 0.0)
+            if rec.verified:
+                stats.verified += 1
+                rec.distinct = all(not self._same(rec, o) for o in self.found[w])
+                if rec.distinct:
+                    self.found[w].append(rec)
+                    stats.distinct += 1
+            stats.records.append(rec)
+        return done
+
+    @staticmethod
+    def _same(a: SaddleRecord, b: SaddleRecord):
+        if abs(a.barrier64 - b.barrier64) > 2e-3 or a.atoms.size != b.atoms.size:
+            return False
+        if not np.array_equal(a.atoms, b.atoms):
+            return False
+        return float(np.abs(a.disp - b.disp).max()) < 0.15
+
+    def begin(self):
+        """Start a search on every walker."""
+        self.start(range(self.W))
+
+    def advance(self, micro_steps, sync_every=25, refill=True):
+        """Run micro_steps micro-steps (a multiple of sync_every), refilling
+        walkers whose search ended; returns the searches that ended in this
+        window.  Searches still running carry over to the next window."""
+        stats = CampaignStats()
+        t0 = time.perf_counter()
+        steps = 0
+        while steps < micro_steps and self.pending:
+            k = min(sync_every, micro_steps - steps)
+            steps += self.batch.run(k, sync_every=k)
+            done = self.harvest(stats)
+            if refill and done:
+                self.start(done)
+        stats.search_time = time.perf_counter() - t0
+        stats.micro_steps = steps
+        return stats
+
+    def run(self, sync_every=25, max_steps=10**9):
+        """One search per walker, run until all have ended."""
+        self.begin()
+        stats = CampaignStats()
+        t0 = time.perf_counter()
+        steps = 0
+        while self.pending and steps < max_steps:
+            steps += self.batch.run(sync_every, sync_every=sync_every)
+            self.harvest(stats)
+        stats.search_time = time.perf_counter() - t0
+        stats.micro_steps = steps
+        return stats

@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the force-and-saddle-search path on one MI355X.
+
+Workload (value): batched dimer searches on Sutton-Chen Cu walkers --
+512 cells of 500-4000 atoms (FCC n_x x n_y x n_z, 1-3 vacancies at seeded
+sites, N(0, 0.01 A) jitter), each relaxed to its basin (FIRE) before the
+timed region.  Every walker runs dimer searches from seeded 0.1 A
+displacements of a vacancy's 12 neighbours and is refilled with a new
+search as soon as one ends; each converged saddle is re-verified in FP64
+on the device (max force, FP64 barrier within 0.01 eV of the search's, one
+negative Lanczos curvature) and de-duplicated per basin.  One step = one
+window of 500 batched micro-steps (a micro-step is one force evaluation of
+every running walker plus its state update).
+
+value = FP64-verified distinct saddles ended in the timed windows / their
+wall time.
+Also reported: the plain unbatched GPU dimer (one walker, host-driven
+micro-steps) on the same cells, force evaluations per second, the roofline
+of the dominant kernel (Sutton-Chen pass-2 force) timed with HIP events on
+the batch's stream, and the CPU baseline (the FP64 C restatement of the
+reference, OpenMP) on a bounded sample of the same searches.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64-verified saddles/sec at ≤0.01 eV barrier error; force evals/sec vs CPU"
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def ordered_pairs(cells, positions, rc):
+    """Ordered in-cutoff pairs (i, j != i, r <= rc) per walker, minimum image."""
+    from scipy.spatial import cKDTree
+    P = []
+    for c, p in zip(cells, positions):
+        L = np.array([c.lat[0], c.lat[4], c.lat[8]])
+        q = np.mod(p.T, L)
+        q = np.where(q >= L, 0.0, q)
+        t = cKDTree(q, boxsize=L)
+        P.append(int(t.count_neighbors(t, rc)) - c.n)
+    return np.array(P)
+
+
+def plain_dimer_rate(L, W, text, cells, basins, proposal_fn, params, n_search, basin32, basin64):
+    """The plain GPU dimer: one walker at a time, host-synchronised every
+    micro-step (the reference's control flow with device forces), with the
+    same FP64 verification."""
+    f = L.Field(text)
+    t_total, ok, calls = 0.0, 0, 0
+    for k in range(n_search):
+        w = k % len(cells)
+        c = cells[w]
+        b = L.Batch(f, [W.Cell(basins[w], c.types, c.lat)])
+        x0, mode, frozen, _ = proposal_fn(w)
+        t0 = time.perf_counter()
+        b.set_verify(16, 0.01)
+        b.start_dimer([mode], params=params, x0_per_walker=[x0], frozen_per_walker=[frozen])
+        b.run(10**6, sync_every=1)
+        st = b.status()[0]
+        t_total += time.perf_counter() - t0
+        b32 = st["energy32"] - basin32[w]
+        b64 = st["energy64"] - basin64[w]
+        ok += (st["status"] == 0 and st["verified"] and st["fmax64"] < 0.05 and abs(b32 - b64) <= 0.01
+               and st["lambda1"] < 0 < st["lambda2"])
+        calls += st["force_calls"]
+        b.close()
+    return ok, t_total, calls
+
+
+def cpu_baseline(cells, basins, proposal_fn, budget_s=20.0, max_searches=3):
+    """FP64 C restatement of the reference (OpenMP Sutton-Chen force, as the
+    reference's metal build) running the same dimer searches."""
+    from oracle import oracle as O
+    from akmc_amd import workloads as W
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cu = W.SC_CU
+    f = O.sc_field(["Cu"], [("Cu", "Cu", cu["eps"], cu["a"], cu["n"], cu["m"], cu["c"])], 2 * cu["a"])
+    order = np.argsort([c.n for c in cells])
+    pick = [int(order[len(order) // 2 + k]) for k in range(max_searches)]
+    t_total, ok, calls, done = 0.0, 0, 0, 0
+    for w in pick:
+        c = cells[w]
+        h = O.FieldHandle(f, 2, c.types, c.lat, fast=True)
+        x0, mode, frozen, _ = proposal_fn(w)
+        m = mode[:, frozen == 0].T.reshape(-1)
+        t0 = time.perf_counter()
+        r = O.dimer_search(h, x0, frozen, m, O.dimer_defaults(fast=True), fast=True)
+        t_total += time.perf_counter() - t0
+        ok += r["status"] == 0
+        calls += r["fcalls"]
+        done += 1
+        if t_total > budget_s:
+            break
+    sizes = [cells[w].n for w in pick[:done]]
+    return dict(saddles_per_s=ok / t_total if t_total > 0 else 0.0, force_evals_per_s=calls / t_total,
+                cores=cores, searches=done, converged=ok, seconds=t_total, sizes=sizes)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--window", type=int, default=500, help="micro-steps per step")
+    ap.add_argument("--walkers", type=int, default=512)
+    ap.add_argument("--plain", type=int, default=4, help="plain GPU dimer searches")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the walkers of this path "
+              "would shard across GPUs with no exchange during a search -- see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from akmc_amd import lib as L
+    from akmc_amd import search as S
+    from akmc_amd import workloads as W
+    L.load()
+    cu = W.SC_CU
+    rc = 2 * cu["a"]
+    text = L.sc_potentials_text([("Cu", "Cu", cu["eps"], cu["a"], cu["n"], cu["m"], cu["c"])], rc)
+    cells = W.batched_sc_cells(n_walkers=args.walkers, seed=args.seed)
+    natoms = np.array([c.n for c in cells])
+    dimer = L.dimer_defaults(maxParIter=400)
+    camp = S.SaddleCampaign(text, cells, params=dimer, seed=args.seed)
+    t0 = time.perf_counter()
+    ok = camp.relax_basins()
+    t_relax = time.perf_counter() - t0
+    print(f"[bench] {len(cells)} walkers, {natoms.sum()} atoms; relaxed {ok.sum()} basins in {t_relax:.1f} s",
+          file=sys.stderr, flush=True)
+
+    # ---- roofline of the force kernels at the basins (lists settled)
+    camp.batch.upload([W.Cell(p, c.types, c.lat) for p, c in zip(camp.basins, cells)])
+    camp.batch.forces()
+    ph = camp.batch.time_phases(20)
+    P = ordered_pairs(cells, camp.basins, rc)
+    Ptot = int(P.sum())
+    A = int(natoms.sum())
+    force_ms = ph[2]
+    achieved_tf = 40.0 * Ptot / (force_ms * 1e-3) / 1e12
+    dens_tf = 20.0 * Ptot / (ph[1] * 1e-3) / 1e12
+    eval_ms = float(ph.sum())
+    hbm_gbs = 56.0 * A / ((ph[1] + ph[2]) * 1e-3) / 1e9
+    print(f"[bench] phases ms: lists {ph[0]:.3f} density {ph[1]:.3f} force {ph[2]:.3f} reduce {ph[3]:.3f}; "
+          f"P={Ptot}", file=sys.stderr, flush=True)
+
+    # ---- steady-state windows of batched searches with refill
+    camp.begin()
+    tot = S.CampaignStats()
+    t_timed = 0.0
+    for r in range(args.warmup + args.steps):
+        st = camp.advance(args.window, sync_every=25)
+        print(f"[bench] window {r}: {st.searches} ended, {st.converged} converged, {st.verified} verified, "
+              f"{st.distinct} distinct, {st.micro_steps} micro-steps in {st.search_time:.2f}s, "
+              f"max barrier err {st.max_barrier_err:.2e}", file=sys.stderr, flush=True)
+        if r >= args.warmup:
+            t_timed += st.search_time
+            tot.add(st)
+    value = tot.distinct / t_timed
+    verified_rate = tot.verified / t_timed
+    batched_force_evals = tot.force_calls / t_timed
+
+    # ---- plain GPU dimer on the same cells
+    n_ok, t_plain, plain_calls = plain_dimer_rate(L, W, text, cells, camp.basins, camp.proposal, dimer,
+                                                  args.plain, camp.basin32, camp.basin64)
+    plain_rate = n_ok / t_plain if t_plain > 0 else 0.0
+    print(f"[bench] plain GPU dimer: {n_ok}/{args.plain} converged in {t_plain:.2f}s "
+          f"({plain_calls} force calls)", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cells, camp.basins, camp.proposal)
+        print(f"[bench] CPU baseline: {cpu}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "saddles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_timed / args.steps, 3),
+        "micro_steps_per_step": args.window,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair terms, fp64 cutoff/accumulation",
+        "data": "synthetic (seeded FCC Sutton-Chen Cu cells with vacancies)",
+        "config": {"workload": "batched dimer searches: 512 Sutton-Chen Cu walkers of 500-4000 atoms, "
+                               "1-3 vacancies, FP64-verified", "walkers": len(cells), "atoms": A,
+                   "parallelism": "walkers batched on one GPU"},
+        "verified_saddles_per_s": round(verified_rate, 3),
+        "searches": tot.searches, "converged": tot.converged, "verified": tot.verified,
+        "distinct": tot.distinct, "max_barrier_error_eV": tot.max_barrier_err,
+        "plain_gpu_dimer_saddles_per_s": round(plain_rate, 3),
+        "batched_over_plain": round(verified_rate / plain_rate, 2) if plain_rate > 0 else None,
+        "force_evals_per_s_batched_cells": round(batched_force_evals, 1),
+        "force_eval_ms_all_walkers": round(eval_ms, 4),
+        "force_evals_per_s_cells_device": round(len(cells) / (eval_ms * 1e-3), 1),
+        "roofline": {"bound": "valu", "kernel": "k_sc_force (Sutton-Chen pass 2)",
+                     "achieved": round(achieved_tf, 3), "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved_tf / FP32_VECTOR_PEAK_TFLOPS, 4), "traffic": None,
+                     "flops_per_launch": 40 * Ptot, "ms_per_launch": round(force_ms, 4),
+                     "density_pass_tflops": round(dens_tf, 3),
+                     "hbm_algorithmic_GBs": round(hbm_gbs, 1), "hbm_peak_GBs": HBM_PEAK_GBS},
+    }
+    if cpu:
+        out["cpu_baseline"] = {"value": round(cpu["saddles_per_s"], 5), "unit": "saddles/s",
+                               "cores": cpu["cores"], "kind": "port",
+                               "sample": f"{cpu['searches']} dimer searches on cells of {cpu['sizes']} atoms "
+                                         f"({cpu['seconds']:.1f} s, O(N^2) FP64 Sutton-Chen, OpenMP)",
+                               "force_evals_per_s": round(cpu["force_evals_per_s"], 2)}
+        out["gpu_over_cpu_saddles"] = round(verified_rate / cpu["saddles_per_s"], 1) if cpu["saddles_per_s"] else None
+        out["plain_gpu_over_cpu_saddles"] = round(plain_rate / cpu["saddles_per_s"], 1) if cpu["saddles_per_s"] else None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -98,6 +98,12 @@ int akmc_batch_forces(akmc_batch_t b, akmc_energy *energies);
 /* device-resident repeat: nrep force evaluations with no host copies,
  * returns the total device time in ms measured with HIP events */
 int akmc_batch_forces_repeat(akmc_batch_t b, int nrep, double *ms);
+/* per-phase device time of nrep force evaluations at the resident
+ * positions, HIP events on the batch's stream: ms[0] list maintenance,
+ * ms[1] pass 1 (Sutton-Chen density / ionic real space), ms[2] pass 2
+ * (Sutton-Chen force / Ewald reciprocal), ms[3] per-walker reduction;
+ * totals over the nrep evaluations */
+int akmc_batch_time_phases(akmc_batch_t b, int nrep, double ms[4]);
 int akmc_batch_total_atoms(akmc_batch_t b);
 /* neighbour-list rebuild count since creation (diagnostics) */
 long long akmc_batch_rebuilds(akmc_batch_t b);
@@ -131,27 +137,54 @@ typedef struct {
     double initTimeStep, maxTimeStep, maxStep, timeStepUnit;
 } akmc_dimer_params;
 
+/* ARTCntrl (Common/ArtCntrl.h:14-74); the perpendicular pre-phase
+ * (goPeprepedicular) is not offered on the device */
 typedef struct {
-    int kind;        /* 0 idle, 1 relax, 2 dimer */
+    int numLanczosVectors;
+    double deltaX, maxEigenvalue, eigValTol, parDamp;
+    int maxParIter;
+    double normTol, forceTol;
+    int N_min;
+    double initTimeStep, alphaDecrease, stepIncrease, stepDecrease, alphaStart, maxTimeStep;
+    double timeStepUnit;
+} akmc_art_params;
+
+typedef struct {
+    int kind;        /* 0 idle, 1 relax, 2 dimer, 3 FP64 verification, 4 ART */
     int status;      /* AKMC_SUCCESS / AKMC_FAILED once done */
     int done;
     int iterations;
     int force_calls;
     double energy;   /* energy at the current centre */
     double curvature;
+    /* FP64 verification of a converged dimer (akmc_batch_set_verify) */
+    int verified;        /* 1 once the FP64 data below is present */
+    double energy32;     /* search energy at the saddle (FP32 pair terms) */
+    double curvature32;  /* dimer curvature at convergence */
+    double energy64;     /* FP64 energy at the saddle */
+    double fmax64;       /* largest FP64 atomic force on the free atoms */
+    double lambda1, lambda2; /* two lowest FP64 Lanczos curvatures */
 } akmc_walker_status;
 
 void akmc_relax_defaults(akmc_relax_params *p);
 void akmc_dimer_defaults(akmc_dimer_params *p);
+void akmc_art_defaults(akmc_art_params *p);
 /* frozen[A] (1 = fixed), as Basis::frozen (Common/Basis.cpp:902-981) */
 int akmc_batch_set_frozen(akmc_batch_t b, const int *frozen);
 /* Relax::RelaxStructure (optim/Relax.h:116-117) for the walkers with
  * walkers[w] != 0 (all when walkers is NULL), from their current positions */
 int akmc_batch_start_relax(akmc_batch_t b, const int *walkers, const akmc_relax_params *p);
-/* Dimer::findSaddlePoint (optim/Dimer.h:68-70): disp ([3][A], may be NULL)
- * is added to the current positions, mode ([3][A]) is the start direction */
-int akmc_batch_start_dimer(akmc_batch_t b, const int *walkers, const double *disp, const double *mode,
-                           const akmc_dimer_params *p);
+/* Dimer::findSaddlePoint (optim/Dimer.h:68-70) for the walkers with
+ * walkers[w] != 0 (all when NULL).  x0 ([3][A], may be NULL to keep the
+ * current positions) holds the displaced start, mode ([3][A]) the start
+ * direction, frozen ([A], may be NULL to keep the current mask) the fixed
+ * atoms; only the flagged walkers' slices of the arrays are read. */
+int akmc_batch_start_dimer(akmc_batch_t b, const int *walkers, const double *x0, const double *mode,
+                           const int *frozen, const akmc_dimer_params *p);
+/* ART::findSaddlePoint (optim/ART.cpp:17-129) without the perpendicular
+ * pre-phase: x0/mode/frozen as for the dimer, mode is the initial rVector */
+int akmc_batch_start_art(akmc_batch_t b, const int *walkers, const double *x0, const double *mode,
+                         const int *frozen, const akmc_art_params *p);
 /* run micro-steps until every walker is done or max_steps have run; the
  * host looks at the walker states every sync_every steps (1 = the plain,
  * host-driven loop); steps of sync_every > 1 replay a captured hipGraph */
@@ -160,6 +193,19 @@ int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out);
 /* current search positions (the dimer centre / relaxed positions) */
 int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z);
 int akmc_batch_get_mode(akmc_batch_t b, double *mode);
+/* search positions of one walker (x, y, z of its n atoms) */
+int akmc_batch_get_walker_positions(akmc_batch_t b, int walker, double *x, double *y, double *z);
+/* keep the current search positions of every walker as its basin, and put
+ * the walkers flagged in walkers (all when NULL) back on their basin */
+int akmc_batch_save_basin(akmc_batch_t b);
+/* FP64 re-verification of every converged dimer inside the same walker
+ * stream: nl Lanczos steps (<= 32) of FP64 finite-difference Hessian
+ * products with step delta, started on the dimer mode; nl = 0 disables */
+int akmc_batch_set_verify(akmc_batch_t b, int nl, double delta);
+/* energies of every walker at its search positions, FP32 (bits = 32) or
+ * FP64 (bits = 64) pair terms */
+int akmc_batch_search_energies(akmc_batch_t b, int bits, double *energies);
+int akmc_batch_restore_basin(akmc_batch_t b, const int *walkers);
 
 #ifdef __cplusplus
 }

@@ -751,6 +751,9 @@ double ako_field_cb(void *vctx, int n, double *x, double *y, double *z,
     if (c->kind == 0)
         return ako_sc_forces((const ako_sc_field *)c->field, n, x, y, z, c->type, c->lat, c->rcp,
                              fx, fy, fz, st, NULL, NULL, NULL);
+    if (c->kind == 2)
+        return ako_sc_forces_omp((const ako_sc_field *)c->field, n, x, y, z, c->type, c->lat, c->rcp,
+                                 fx, fy, fz, st, NULL, NULL);
     return ako_ion_forces((const ako_ion_field *)c->field, n, x, y, z, c->type, c->q, c->lat,
                           c->rcp, fx, fy, fz, st, NULL, NULL, NULL);
 }

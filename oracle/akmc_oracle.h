@@ -135,7 +135,7 @@ typedef double (*ako_force_fn)(void *ctx, int n, double *x, double *y, double *z
 
 typedef struct {
     const void *field;
-    int kind;             /* 0 = SC, 1 = ionic */
+    int kind;             /* 0 = SC, 1 = ionic, 2 = SC threaded (OpenMP) */
     const int *type;
     const double *q;
     const double *lat;

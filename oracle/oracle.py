@@ -302,15 +302,16 @@ def tridiag_eig(d, e):
 class FieldHandle:
     """Binds a field + configuration into the C force callback."""
 
-    def __init__(self, field, kind, types, lat, charges=None):
+    def __init__(self, field, kind, types, lat, charges=None, fast=False):
         self.field = field
+        self._fast = fast
         self.types = np.ascontiguousarray(types, dtype=np.int32)
         self.lat = np.ascontiguousarray(lat, dtype=np.float64)
         self.rcp = inverse_cell(self.lat)
         self.q = np.ascontiguousarray(charges if charges is not None else np.zeros(len(types)), dtype=np.float64)
         self.ctx = FieldCtx(C.cast(C.pointer(field), C.c_void_p), kind, _p(self.types, C.c_int), _p(self.q),
                             _p(self.lat), _p(self.rcp), 0)
-        self.fn = C.cast(lib().ako_field_cb, FORCE_FN)
+        self.fn = C.cast(lib(fast).ako_field_cb, FORCE_FN)
 
     @property
     def ncalls(self):
@@ -334,9 +335,9 @@ def art_defaults():
     return c
 
 
-def dimer_defaults():
+def dimer_defaults(fast=False):
     c = DimerCntrl()
-    lib().ako_dimer_defaults(C.byref(c))
+    lib(fast).ako_dimer_defaults(C.byref(c))
     return c
 
 
@@ -395,13 +396,13 @@ def art_search(h: FieldHandle, pos, frozen, rvec, cntrl=None):
     return dict(status=st, pos=np.stack([x, y, z]), energy=e.value, eigval=ev.value, iters=it.value, mode=rv)
 
 
-def dimer_search(h: FieldHandle, pos, frozen, mode, cntrl=None):
+def dimer_search(h: FieldHandle, pos, frozen, mode, cntrl=None, fast=False):
     x, y, z = _xyz(pos)
     fr = np.ascontiguousarray(frozen, dtype=np.int32)
     md = np.ascontiguousarray(mode, dtype=np.float64).copy()
     c = cntrl or dimer_defaults()
     e, cv, it, nf = C.c_double(), C.c_double(), C.c_int(), C.c_int()
-    st = lib().ako_dimer_search(h.fn, C.byref(h.ctx), x.size, _p(x), _p(y), _p(z), _p(fr, C.c_int), _p(md),
+    st = lib(fast).ako_dimer_search(h.fn, C.byref(h.ctx), x.size, _p(x), _p(y), _p(z), _p(fr, C.c_int), _p(md),
                                 C.byref(c), C.byref(e), C.byref(cv), C.byref(it), C.byref(nf))
     return dict(status=st, pos=np.stack([x, y, z]), energy=e.value, curvature=cv.value, iters=it.value,
                 fcalls=nf.value, mode=md)

@@ -113,3 +113,63 @@ def test_batched_walkers_equal_plain(lib, oracle, c1):
     assert s_b == s_1
     assert np.array_equal(bb.positions()[1], b1.positions()[0])
     assert np.array_equal(bb.mode()[1], b1.mode()[0])
+
+
+def _art_params_ref(oracle):
+    """ART settings of the reference's own example (docs/examples/gdc/control:16-23)."""
+    c = oracle.art_defaults()
+    c.initTimeStep, c.maxTimeStep, c.forceTol, c.parDamp, c.numLanczosVectors = 0.0005, 0.002, 1e-2, 0.25, 25
+    return c
+
+
+def test_art_matches_oracle(lib, oracle, c1):
+    c, of, h, r = c1
+    basin, E1 = r["pos"], r["energy"]
+    f = lib.Field(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
+    frozen, disp, mode = _dimer_start(c, basin, 5)
+    m = mode[:, frozen == 0].T.reshape(-1)
+    oa = oracle.art_search(h, basin + disp, frozen, m / np.linalg.norm(m), _art_params_ref(oracle))
+    p = lib.art_defaults(initTimeStep=0.0005, maxTimeStep=0.002, forceTol=1e-2, parDamp=0.25,
+                         numLanczosVectors=25)
+    b = lib.Batch(f, [W.Cell(basin, c.types, c.lat)])
+    b.set_frozen([frozen])
+    b.start_art([mode], [disp], p)
+    b.run(200000, sync_every=50)
+    st = b.status()[0]
+    sad = b.positions()[0]
+    e64 = oracle.sc_energy(of, sad, c.types, c.lat)[0]
+    print(f"GPU ART status {st['status']} iters {st['iterations']} calls {st['force_calls']} Ea "
+          f"{st['energy'] - E1:.5f} (FP64 at GPU saddle {e64 - E1:.5f}); oracle status {oa['status']} Ea "
+          f"{oa['energy'] - E1:.5f} iters {oa['iters']}; eig {st['curvature']:.4f} vs {oa['eigval']:.4f}",
+          flush=True)
+    assert st["status"] == 0 and oa["status"] == 0
+    assert abs((e64 - E1) - (oa["energy"] - E1)) < 0.01
+    assert st["curvature"] < 0 and abs(st["curvature"] - oa["eigval"]) < 0.05 * abs(oa["eigval"])
+
+
+def test_device_fp64_verification(lib, oracle, c1):
+    """The in-stream FP64 check of a converged dimer: its FP64 energy equals
+    the CPU oracle's at the same positions, its two lowest Lanczos
+    curvatures bracket zero and its FP64 force is small."""
+    c, of, h, r = c1
+    basin = r["pos"]
+    f = lib.Field(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
+    frozen, disp, mode = _dimer_start(c, basin, 2)
+    b = lib.Batch(f, [W.Cell(basin, c.types, c.lat)])
+    b.set_frozen([frozen])
+    b.set_verify(16, 0.01)
+    b.start_dimer([mode], [disp])
+    b.run(20000, sync_every=25)
+    st = b.status()[0]
+    sad = b.positions()[0]
+    ref = oracle.sc_forces(of, sad, c.types, c.lat)
+    fmax_ref = np.sqrt((ref["force"][:, frozen == 0] ** 2).sum(0).max())
+    print(f"verify: e64 {st['energy64']:.9f} oracle {ref['energy']:.9f} e32 {st['energy32']:.9f} "
+          f"fmax64 {st['fmax64']:.2e} (oracle {fmax_ref:.2e}) lambdas {st['lambda1']:.4f} {st['lambda2']:.4f} "
+          f"c32 {st['curvature32']:.4f}", flush=True)
+    assert st["status"] == 0 and st["verified"] == 1
+    assert abs(st["energy64"] - ref["energy"]) < 1e-9 * abs(ref["energy"])
+    assert abs(st["fmax64"] - fmax_ref) < 1e-9
+    assert st["lambda1"] < 0 < st["lambda2"]
+    assert abs(st["lambda1"] - st["curvature32"]) < 0.05 * abs(st["curvature32"])
+    assert abs(st["energy32"] - st["energy64"]) < 0.01
