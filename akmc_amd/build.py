@@ -42,6 +42,14 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    # the C++ host-API driver used by tests/test_gpu_cpp_host.py
+    drv = os.path.join(HERE, "_build", "host_driver")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "host_driver.cpp"), "-L", HERE, "-lakmc_hip",
+           "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/..", "-o", drv]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host driver build failed:\n" + r.stderr)
     return LIB
 
 

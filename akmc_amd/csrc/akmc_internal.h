@@ -73,6 +73,7 @@ struct BatchDev {
     const WalkerDev *walkers;
     const int *wk;        // atom -> walker
     double *x, *y, *z;    // positions (FP64 master copy)
+    double4 *p4;          // packed (x, y, z, type) of every atom for the gathers
     double *xr, *yr, *zr; // positions at the last list build
     const int *type;
     const double *q;

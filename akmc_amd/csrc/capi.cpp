@@ -431,6 +431,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     d.wk = wk;
     d.type = tp;
     d.q = q;
+    TRY(dalloc(b, &d.p4, A));
     TRY(dalloc(b, &d.x, A));
     TRY(dalloc(b, &d.y, A));
     TRY(dalloc(b, &d.z, A));

@@ -17,6 +17,18 @@ namespace {
 constexpr int kBlock = 256;
 
 __device__ inline int walker_of(const BatchDev &b, int i) { return b.wk[i]; }
+
+// Workgroups are dealt round-robin to the 8 XCDs; remapping the hardware
+// workgroup id so that each XCD owns one contiguous range of atoms keeps a
+// walker's positions in a single XCD's L2.  Launches use xcd_grid(n).
+constexpr int kXCD = 8;
+__host__ __device__ inline int xcd_grid(int nblocks) { return ((nblocks + kXCD - 1) / kXCD) * kXCD; }
+__device__ inline int xcd_atom(int nblocks)
+{
+    const int per = (nblocks + kXCD - 1) / kXCD;
+    const int logical = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+    return logical < nblocks ? logical * blockDim.x + threadIdx.x : -1;
+}
 __device__ inline bool skipped(const BatchDev &b, int w) { return b.active[w] == 0; }
 // the pair-physics kernels of type T serve the walkers whose precision mode
 // matches (1 FP32, 2 FP64)
@@ -81,6 +93,15 @@ __global__ void k_nbr_check(BatchDev b)
     if (skipped(b, w)) return;
     double dx = b.x[i] - b.xr[i], dy = b.y[i] - b.yr[i], dz = b.z[i] - b.zr[i];
     if (dx * dx + dy * dy + dz * dz > b.field->half_skin2) b.rebuild[w] = 1;
+}
+
+// gather copy of the positions and species: one 32-byte load per neighbour
+__global__ void k_pack(BatchDev b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.A) return;
+    if (skipped(b, walker_of(b, i))) return;
+    b.p4[i] = make_double4(b.x[i], b.y[i], b.z[i], (double)b.type[i]);
 }
 
 // record the positions the lists are built from (positions are never
@@ -316,8 +337,8 @@ __global__ void k_rebuild_done(BatchDev b, unsigned long long *count)
 template <typename T, bool MINIMG>
 __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.A) return;
+    int i = xcd_atom((b.A + kBlock - 1) / kBlock);
+    if (i < 0 || i >= b.A) return;
     if (other_precision<T>(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
@@ -329,11 +350,12 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
     for (int k = 0; k < cnt; ++k) {
         uint32_t e = b.nbr[(size_t)k * b.A + i];
         int j = W.off + int(e & AKMC_NB_JMASK);
-        double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
+        const double4 pj = b.p4[j];
+        double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
         entry_disp(e, dx, dy, dz, W, MINIMG);
         double r2 = dx * dx + dy * dy + dz * dz;
         if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
-            const SCPairDev &p = F.sc[ti * F.nspec + b.type[j]];
+            const SCPairDev &p = F.sc[ti * F.nspec + (int)pj.w];
             if (p.on) {
                 const SCCoef<T> &c = sc_coef<T>(p);
                 T u2 = c.a * c.a / T(r2);
@@ -353,8 +375,8 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 template <typename T, bool MINIMG, bool STRESS>
 __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.A) return;
+    int i = xcd_atom((b.A + kBlock - 1) / kBlock);
+    if (i < 0 || i >= b.A) return;
     if (other_precision<T>(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
@@ -368,11 +390,12 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
     for (int k = 0; k < cnt; ++k) {
         uint32_t e = b.nbr[(size_t)k * b.A + i];
         int j = W.off + int(e & AKMC_NB_JMASK);
-        double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
+        const double4 pj = b.p4[j];
+        double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
         entry_disp(e, dx, dy, dz, W, MINIMG);
         double r2 = dx * dx + dy * dy + dz * dz;
         if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
-            const SCPairDev &p = F.sc[ti * F.nspec + b.type[j]];
+            const SCPairDev &p = F.sc[ti * F.nspec + (int)pj.w];
             if (!p.on) continue;
             const SCCoef<T> &c = sc_coef<T>(p);
             T r2t = T(r2);
@@ -472,8 +495,8 @@ __device__ inline void ion_pair_terms(const FieldDev &F, const IonPairDev &p, do
 template <typename T, bool MINIMG, bool STRESS>
 __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.A) return;
+    int i = xcd_atom((b.A + kBlock - 1) / kBlock);
+    if (i < 0 || i >= b.A) return;
     if (other_precision<T>(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
@@ -489,11 +512,12 @@ __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
         uint32_t e = b.nbr[(size_t)k * b.A + i];
         int jl = int(e & AKMC_NB_JMASK);
         int j = W.off + jl;
-        double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
+        const double4 pj = b.p4[j];
+        double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
         entry_disp(e, dx, dy, dz, W, MINIMG);
         double r2 = dx * dx + dy * dy + dz * dz;
         if (!(r2 <= rc2) || r2 < 1.0e-16) continue;
-        const IonPairDev &p = F.ion[ti * F.nspec + b.type[j]];
+        const IonPairDev &p = F.ion[ti * F.nspec + (int)pj.w];
         double df, dec, deb;
         if (k < nnear) {
             double forc, ec, eb;
@@ -731,6 +755,7 @@ extern "C" unsigned long long *akmc_rebuild_counter_ptr()
 static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     int gA = blocks(b.A);
+    k_pack<<<gA, kBlock, 0, s>>>(b);
     k_nbr_check<<<gA, kBlock, 0, s>>>(b);
     k_ref_positions<<<gA, kBlock, 0, s>>>(b);
     if (hf.min_image) {
@@ -749,7 +774,7 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 template <typename T>
 static void launch_pass1(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
-    int gA = blocks(b.A);
+    int gA = xcd_grid(blocks(b.A));
     if (hf.kind == 0) {
         if (hf.min_image) k_sc_density<T, true><<<gA, kBlock, 0, s>>>(b);
         else k_sc_density<T, false><<<gA, kBlock, 0, s>>>(b);
@@ -768,13 +793,14 @@ template <typename T>
 static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     int gA = blocks(b.A);
+    int gX = xcd_grid(gA);
     if (hf.kind == 0) {
         if (hf.min_image) {
-            if (want_stress) k_sc_force<T, true, true><<<gA, kBlock, 0, s>>>(b);
-            else k_sc_force<T, true, false><<<gA, kBlock, 0, s>>>(b);
+            if (want_stress) k_sc_force<T, true, true><<<gX, kBlock, 0, s>>>(b);
+            else k_sc_force<T, true, false><<<gX, kBlock, 0, s>>>(b);
         } else {
-            if (want_stress) k_sc_force<T, false, true><<<gA, kBlock, 0, s>>>(b);
-            else k_sc_force<T, false, false><<<gA, kBlock, 0, s>>>(b);
+            if (want_stress) k_sc_force<T, false, true><<<gX, kBlock, 0, s>>>(b);
+            else k_sc_force<T, false, false><<<gX, kBlock, 0, s>>>(b);
         }
     } else if (hf.do_ewald) {
         k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
