@@ -178,3 +178,17 @@ struct SearchDev {
 
 void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st);
 void akmc_launch_probe_init(const BatchDev &b, const SearchDev &s, hipStream_t st);
+
+// learned proposal (mlp_kernels.hip)
+void akmc_mlp_launch_weights(const float *p, uint16_t *w1t, uint16_t *w2t, uint16_t *w3t, uint16_t *w2r,
+                             uint16_t *w3r, hipStream_t s);
+void akmc_mlp_launch_forward(const float *X, int M, const float *p, const uint16_t *w1t, const uint16_t *w2t,
+                             const uint16_t *w3t, float *Z1, float *H1, float *Z2, float *H2, float *Y, hipStream_t s);
+void akmc_mlp_launch_backward(const float *X, const float *T, int M, const uint16_t *w2r, const uint16_t *w3r,
+                              const float *Z1, const float *H1, const float *Z2, const float *H2, const float *Y,
+                              float *dY, float *dZ2, float *dZ1, double *loss_part, float *P, int rows_per_chunk,
+                              float *grad, hipStream_t s);
+void akmc_mlp_launch_adam(float *p, const float *g, float *m1, float *m2, int n, float lr, int t, hipStream_t s);
+void akmc_mlp_launch_gather(const float *src, int width, const int *idx, int M, float *dst, hipStream_t s);
+void akmc_mlp_launch_descriptors(const BatchDev &b, int nq, const int *qatom, const double *qvac,
+                                 const double *qmov, float *X, hipStream_t s);

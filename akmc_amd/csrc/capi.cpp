@@ -1268,3 +1268,268 @@ int akmc_batch_search_energies(akmc_batch_t b, int bits, double *energies)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ MLP
+
+struct akmc_mlp_s {
+    int rows = 0;
+    unsigned seed = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+    uint16_t *w1t = nullptr, *w2t = nullptr, *w3t = nullptr, *w2r = nullptr, *w3r = nullptr;
+    float *X = nullptr, *T = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *Y = nullptr,
+          *dY = nullptr, *dZ2 = nullptr, *dZ1 = nullptr, *P = nullptr;
+    double *lpart = nullptr;
+    int adam_t = 0;
+    static constexpr int kChunk = 256;
+};
+
+namespace {
+template <typename T>
+int malloc_m(akmc_mlp_t m, T **p, size_t n)
+{
+    HIPCHK(hipMalloc((void **)p, (n ? n : 1) * sizeof(T)));
+    m->allocs.push_back((void *)*p);
+    return AKMC_SUCCESS;
+}
+
+void mlp_free(akmc_mlp_t m)
+{
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (void *q : m->allocs) (void)hipFree(q);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+int mlp_refresh_weights(akmc_mlp_t m)
+{
+    akmc_mlp_launch_weights(m->p, m->w1t, m->w2t, m->w3t, m->w2r, m->w3r, m->stream);
+    HIPCHK(hipGetLastError());
+    return AKMC_SUCCESS;
+}
+
+// forward + loss/gradient of rows [0, M) already in m->X / m->T
+int mlp_loss_grad_dev(akmc_mlp_t m, int M, double *loss)
+{
+    akmc_mlp_launch_forward(m->X, M, m->p, m->w1t, m->w2t, m->w3t, m->Z1, m->H1, m->Z2, m->H2, m->Y, m->stream);
+    akmc_mlp_launch_backward(m->X, m->T, M, m->w2r, m->w3r, m->Z1, m->H1, m->Z2, m->H2, m->Y, m->dY, m->dZ2,
+                             m->dZ1, m->lpart, m->P, akmc_mlp_s::kChunk, m->g, m->stream);
+    HIPCHK(hipGetLastError());
+    if (loss) {
+        int nb = (M + 255) / 256;
+        std::vector<double> part(nb);
+        HIPCHK(hipMemcpyAsync(part.data(), m->lpart, nb * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        double s = 0.0;
+        for (double v : part) s += v;
+        *loss = s / M;
+    }
+    return AKMC_SUCCESS;
+}
+}  // namespace
+
+extern "C" {
+
+int akmc_mlp_create(int max_rows, unsigned seed, akmc_mlp_t *out)
+{
+    if (max_rows <= 0 || !out) return fail(AKMC_EINVAL, "bad arguments");
+    akmc_mlp_t m = new akmc_mlp_s();
+    m->rows = (max_rows + 255) / 256 * 256;
+    m->seed = seed;
+    auto bad = [&](int rc) {
+        mlp_free(m);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(fail(AKMC_EHIP, "stream creation failed"));
+    const size_t R = m->rows, NP = AKMC_MLP_NPARAM;
+    int rc = AKMC_SUCCESS;
+    if ((rc = malloc_m(m, &m->p, NP)) || (rc = malloc_m(m, &m->g, NP)) || (rc = malloc_m(m, &m->m1, NP)) ||
+        (rc = malloc_m(m, &m->m2, NP)) || (rc = malloc_m(m, &m->w1t, 128 * 48)) ||
+        (rc = malloc_m(m, &m->w2t, 128 * 128)) || (rc = malloc_m(m, &m->w3t, 16 * 128)) ||
+        (rc = malloc_m(m, &m->w2r, 128 * 128)) || (rc = malloc_m(m, &m->w3r, 128 * 16)) ||
+        (rc = malloc_m(m, &m->X, R * 48)) || (rc = malloc_m(m, &m->T, R * 3)) ||
+        (rc = malloc_m(m, &m->Z1, R * 128)) || (rc = malloc_m(m, &m->H1, R * 128)) ||
+        (rc = malloc_m(m, &m->Z2, R * 128)) || (rc = malloc_m(m, &m->H2, R * 128)) ||
+        (rc = malloc_m(m, &m->Y, R * 16)) || (rc = malloc_m(m, &m->dY, R * 16)) ||
+        (rc = malloc_m(m, &m->dZ2, R * 128)) || (rc = malloc_m(m, &m->dZ1, R * 128)) ||
+        (rc = malloc_m(m, &m->P, (R / akmc_mlp_s::kChunk + 1) * 128 * 128)) ||
+        (rc = malloc_m(m, &m->lpart, R / 256 + 1)))
+        return bad(rc);
+    HIPCHK(hipMemset(m->m1, 0, NP * sizeof(float)));
+    HIPCHK(hipMemset(m->m2, 0, NP * sizeof(float)));
+    // He-style initial weights from a seeded generator, zero biases
+    std::vector<float> h(NP, 0.0f);
+    unsigned long long st = 0x9E3779B97F4A7C15ull ^ seed;
+    auto unif = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        unsigned long long z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    auto gauss = [&]() { return std::sqrt(-2.0 * std::log(unif())) * std::cos(6.283185307179586 * unif()); };
+    size_t o = 0;
+    for (int i = 0; i < 48 * 128; ++i) h[o++] = (float)(gauss() * std::sqrt(2.0 / 48));
+    o += 128;
+    for (int i = 0; i < 128 * 128; ++i) h[o++] = (float)(gauss() * std::sqrt(2.0 / 128));
+    o += 128;
+    for (int i = 0; i < 128 * 3; ++i) h[o++] = (float)(gauss() * std::sqrt(1.0 / 128));
+    HIPCHK(hipMemcpy(m->p, h.data(), NP * sizeof(float), hipMemcpyHostToDevice));
+    if ((rc = mlp_refresh_weights(m))) return bad(rc);
+    HIPCHK(hipStreamSynchronize(m->stream));
+    *out = m;
+    return AKMC_SUCCESS;
+}
+
+void akmc_mlp_destroy(akmc_mlp_t m) { mlp_free(m); }
+
+int akmc_mlp_set_params(akmc_mlp_t m, const float *params)
+{
+    if (!m || !params) return fail(AKMC_EINVAL, "bad arguments");
+    HIPCHK(hipMemcpyAsync(m->p, params, AKMC_MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    TRY(mlp_refresh_weights(m));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return AKMC_SUCCESS;
+}
+
+int akmc_mlp_get_params(akmc_mlp_t m, float *params)
+{
+    if (!m || !params) return fail(AKMC_EINVAL, "bad arguments");
+    HIPCHK(hipMemcpyAsync(params, m->p, AKMC_MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return AKMC_SUCCESS;
+}
+
+int akmc_mlp_forward(akmc_mlp_t m, const float *X, int M, float *Y)
+{
+    if (!m || !X || !Y || M < 0) return fail(AKMC_EINVAL, "bad arguments");
+    std::vector<float> y16;
+    for (int r0 = 0; r0 < M; r0 += m->rows) {
+        int n = std::min(m->rows, M - r0);
+        HIPCHK(hipMemcpyAsync(m->X, X + (size_t)r0 * 48, (size_t)n * 48 * sizeof(float), hipMemcpyHostToDevice,
+                              m->stream));
+        akmc_mlp_launch_forward(m->X, n, m->p, m->w1t, m->w2t, m->w3t, m->Z1, m->H1, m->Z2, m->H2, m->Y, m->stream);
+        HIPCHK(hipGetLastError());
+        y16.resize((size_t)n * 16);
+        HIPCHK(hipMemcpyAsync(y16.data(), m->Y, (size_t)n * 16 * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) Y[(size_t)(r0 + i) * 3 + c] = y16[(size_t)i * 16 + c];
+    }
+    return AKMC_SUCCESS;
+}
+
+int akmc_mlp_loss_grad(akmc_mlp_t m, const float *X, const float *T, int M, double *loss, float *grad)
+{
+    if (!m || !X || !T || M <= 0 || M > m->rows) return fail(AKMC_EINVAL, "M must be in [1, max_rows]");
+    HIPCHK(hipMemcpyAsync(m->X, X, (size_t)M * 48 * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->T, T, (size_t)M * 3 * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    TRY(mlp_loss_grad_dev(m, M, loss));
+    if (grad) {
+        HIPCHK(hipMemcpyAsync(grad, m->g, AKMC_MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+    }
+    return AKMC_SUCCESS;
+}
+
+int akmc_mlp_train(akmc_mlp_t m, const float *X, const float *T, int M, int epochs, int batch, float lr,
+                   unsigned seed, double *loss_history)
+{
+    if (!m || !X || !T || M <= 0 || epochs <= 0 || batch <= 0 || batch > m->rows)
+        return fail(AKMC_EINVAL, "bad training arguments (batch must be <= max_rows)");
+    float *dX = nullptr, *dT = nullptr;
+    int *didx = nullptr;
+    HIPCHK(hipMalloc(&dX, (size_t)M * 48 * sizeof(float)));
+    HIPCHK(hipMalloc(&dT, (size_t)M * 3 * sizeof(float)));
+    HIPCHK(hipMalloc(&didx, (size_t)M * sizeof(int)));
+    auto cleanup = [&]() {
+        (void)hipFree(dX);
+        (void)hipFree(dT);
+        (void)hipFree(didx);
+    };
+    if (hipMemcpy(dX, X, (size_t)M * 48 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dT, T, (size_t)M * 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        return fail(AKMC_EHIP, "training data upload failed");
+    }
+    std::vector<int> perm(M);
+    unsigned long long st = 0xD1B54A32D192ED03ull ^ seed;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        unsigned long long z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (int i = 0; i < M; ++i) perm[i] = i;
+        for (int i = M - 1; i > 0; --i) std::swap(perm[i], perm[next() % (unsigned long long)(i + 1)]);
+        if (hipMemcpyAsync(didx, perm.data(), (size_t)M * sizeof(int), hipMemcpyHostToDevice, m->stream) !=
+            hipSuccess) {
+            cleanup();
+            return fail(AKMC_EHIP, "permutation upload failed");
+        }
+        double lsum = 0.0;
+        int nb = 0;
+        for (int b0 = 0; b0 + batch <= M || (b0 == 0 && M < batch); b0 += batch) {
+            int n = std::min(batch, M - b0);
+            akmc_mlp_launch_gather(dX, 48, didx + b0, n, m->X, m->stream);
+            akmc_mlp_launch_gather(dT, 3, didx + b0, n, m->T, m->stream);
+            double l = 0.0;
+            int rc = mlp_loss_grad_dev(m, n, &l);
+            if (rc != AKMC_SUCCESS) {
+                cleanup();
+                return rc;
+            }
+            akmc_mlp_launch_adam(m->p, m->g, m->m1, m->m2, AKMC_MLP_NPARAM, lr, ++m->adam_t, m->stream);
+            rc = mlp_refresh_weights(m);
+            if (rc != AKMC_SUCCESS) {
+                cleanup();
+                return rc;
+            }
+            lsum += l;
+            ++nb;
+            if (M < batch) break;
+        }
+        if (loss_history) loss_history[ep] = lsum / std::max(nb, 1);
+    }
+    hipError_t e = hipStreamSynchronize(m->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(AKMC_EHIP, "training failed: %s", hipGetErrorString(e));
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_descriptors(akmc_batch_t b, int nq, const int *atom, const double *vac, const double *mover,
+                           float *X)
+{
+    if (!b || nq <= 0 || !atom || !vac || !mover || !X) return fail(AKMC_EINVAL, "bad arguments");
+    for (int q = 0; q < nq; ++q)
+        if (atom[q] < 0 || atom[q] >= b->A) return fail(AKMC_EINVAL, "query %d: atom out of range", q);
+    int *qa;
+    double *qv, *qm;
+    float *qx;
+    HIPCHK(hipMalloc(&qa, nq * sizeof(int)));
+    HIPCHK(hipMalloc(&qv, 3 * (size_t)nq * sizeof(double)));
+    HIPCHK(hipMalloc(&qm, 3 * (size_t)nq * sizeof(double)));
+    HIPCHK(hipMalloc(&qx, 48 * (size_t)nq * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(qa, atom, nq * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(qv, vac, 3 * (size_t)nq * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(qm, mover, 3 * (size_t)nq * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    akmc_mlp_launch_descriptors(b->d, nq, qa, qv, qm, qx, b->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(X, qx, 48 * (size_t)nq * sizeof(float), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(qa);
+    (void)hipFree(qv);
+    (void)hipFree(qm);
+    (void)hipFree(qx);
+    if (e != hipSuccess) return fail(AKMC_EHIP, "descriptors: %s", hipGetErrorString(e));
+    return AKMC_SUCCESS;
+}
+
+}  // extern "C"

@@ -49,6 +49,9 @@ class SaddleRecord:
     disp: np.ndarray
     verified: bool = False
     distinct: bool = False
+    site: np.ndarray | None = None  # vacancy site of the proposal
+    active: np.ndarray | None = None      # free atoms of the search
+    active_disp: np.ndarray | None = None  # their saddle displacement (3, n_active)
 
 
 @dataclass
@@ -93,6 +96,7 @@ class SaddleCampaign:
         self.proposer = proposer  # optional callable(campaign, w) -> (x0, mode, frozen, site)
         self.found = [[] for _ in range(self.W)]
         self.pending = set()
+        self.cur_site = {}
         A = self.batch.A
         self.off = self.batch.offsets
         self._x0 = np.zeros(3 * A)
@@ -134,7 +138,8 @@ class SaddleCampaign:
         A = self.batch.A
         mask = np.zeros(self.W, dtype=np.int32)
         for w in walkers:
-            x0, mode, frozen, _ = (self.proposer or SaddleCampaign.proposal)(self, w)
+            x0, mode, frozen, site = (self.proposer or SaddleCampaign.proposal)(self, w)
+            self.cur_site[w] = site
             a, b = self.off[w], self.off[w + 1]
             for c in range(3):
                 self._x0[c * A + a:c * A + b] = x0[c]
@@ -167,8 +172,10 @@ class SaddleCampaign:
             d = W.min_image(pos - self.basins[w], self.cells[w].lat)
             dn = np.sqrt((d * d).sum(0))
             atoms = np.nonzero(dn > 0.3)[0]
+            act = np.nonzero(self._frozen[self.off[w]:self.off[w + 1]] == 0)[0]
             rec = SaddleRecord(w, s["energy32"], s["energy64"], float(b32), float(b64), s["curvature32"],
-                               (s["lambda1"], s["lambda2"]), s["fmax64"], atoms, d[:, atoms])
+                               (s["lambda1"], s["lambda2"]), s["fmax64"], atoms, d[:, atoms],
+                               site=self.cur_site.get(w), active=act, active_disp=d[:, act])
             rec.verified = (s["fmax64"] < self.fmax_verify and err <= self.barrier_tol
                             and s["lambda1"] < 0.0 < s["lambda2"] and b64 > 0.0)
             if rec.verified:

@@ -207,6 +207,32 @@ int akmc_batch_set_verify(akmc_batch_t b, int nl, double delta);
 int akmc_batch_search_energies(akmc_batch_t b, int bits, double *energies);
 int akmc_batch_restore_basin(akmc_batch_t b, const int *walkers);
 
+/* ---- learned proposal (no counterpart in the reference) ---------------
+ * Per-atom descriptors (32 radial + 16 angular symmetry functions with
+ * vacancy-site and candidate-mover channels) and the MLP 48 -> 128 -> 128
+ * -> 3 with SiLU, BF16 MFMA GEMMs with FP32 accumulation, FP32 master
+ * weights.  Parameters are 23171 floats in the order W1[48][128], b1[128],
+ * W2[128][128], b2[128], W3[128][3], b3[3] (row-major). */
+#define AKMC_MLP_NPARAM 23171
+typedef struct akmc_mlp_s *akmc_mlp_t;
+int akmc_mlp_create(int max_rows, unsigned seed, akmc_mlp_t *out);
+void akmc_mlp_destroy(akmc_mlp_t m);
+int akmc_mlp_set_params(akmc_mlp_t m, const float *params);
+int akmc_mlp_get_params(akmc_mlp_t m, float *params);
+/* Y[M][3] = MLP(X[M][48]) (host arrays, M <= max_rows per call chunk) */
+int akmc_mlp_forward(akmc_mlp_t m, const float *X, int M, float *Y);
+/* mean-squared loss against T[M][3] and its gradient (layout of params) */
+int akmc_mlp_loss_grad(akmc_mlp_t m, const float *X, const float *T, int M, double *loss, float *grad);
+/* Adam training on device-resident data: epochs passes in seeded random
+ * minibatches; loss_history[epochs] receives each epoch's mean loss */
+int akmc_mlp_train(akmc_mlp_t m, const float *X, const float *T, int M, int epochs, int batch, float lr,
+                   unsigned seed, double *loss_history);
+/* descriptors of nq queries (atom = index into the batch's concatenated
+ * atoms; vac, mover = 3 doubles each) at the batch's resident positions,
+ * whose neighbour lists must be current (after akmc_batch_forces) */
+int akmc_batch_descriptors(akmc_batch_t b, int nq, const int *atom, const double *vac, const double *mover,
+                           float *X);
+
 #ifdef __cplusplus
 }
 #endif
