@@ -413,6 +413,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     d.A = A;
     d.W = W;
     d.maxnb = maxnb;
+    d.lpa = 1;
     d.ncells_total = ncells;
     d.nk_total = nk;
     d.G1 = G1;
@@ -943,6 +944,35 @@ int akmc_batch_forces_repeat(akmc_batch_t b, int nrep, double *ms)
     HIPCHK(hipEventElapsedTime(&t, b->ev0, b->ev1));
     if (ms) *ms = t;
     return check_overflow(b);
+}
+
+int akmc_batch_set_lanes(akmc_batch_t b, int lanes)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    if (lanes != 1 && lanes != 2 && lanes != 4) return fail(AKMC_EINVAL, "lanes per atom must be 1, 2 or 4, got %d", lanes);
+    if (b->d.lpa != lanes) drop_graph(b);  // the captured graph holds the old kernels
+    b->d.lpa = lanes;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_autotune(akmc_batch_t b, int nrep, int *best, double ms[3])
+{
+    if (!b || nrep <= 0) return fail(AKMC_EINVAL, "bad arguments");
+    static const int choice[3] = {1, 2, 4};
+    double t[3];
+    TRY(run_forces(b, 0));  // lists current before timing
+    for (int c = 0; c < 3; ++c) {
+        TRY(akmc_batch_set_lanes(b, choice[c]));
+        TRY(run_forces(b, 0));  // first launch of this instantiation
+        TRY(akmc_batch_forces_repeat(b, nrep, &t[c]));
+        if (ms) ms[c] = t[c];
+    }
+    int k = 0;
+    for (int c = 1; c < 3; ++c)
+        if (t[c] < t[k]) k = c;
+    TRY(akmc_batch_set_lanes(b, choice[k]));
+    if (best) *best = choice[k];
+    return AKMC_SUCCESS;
 }
 
 int akmc_batch_total_atoms(akmc_batch_t b) { return b ? b->A : 0; }

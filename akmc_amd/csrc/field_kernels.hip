@@ -334,11 +334,32 @@ __global__ void k_rebuild_done(BatchDev b, unsigned long long *count)
 
 // pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
 // then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2
-template <typename T, bool MINIMG>
+// the global thread index in XCD order (see xcd_atom), -1 past the end
+__device__ inline int xcd_thread(int nblocks)
+{
+    const int per = (nblocks + kXCD - 1) / kXCD;
+    const int logical = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+    return logical < nblocks ? logical * blockDim.x + threadIdx.x : -1;
+}
+
+// sum over the LPA consecutive lanes that share one atom
+template <int LPA>
+__device__ inline double lane_group_sum(double v)
+{
+    for (int o = LPA / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
+// then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2.
+// LPA lanes share one atom's neighbour list (strided) and combine by shuffles.
+template <typename T, bool MINIMG, int LPA>
 __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 {
-    int i = xcd_atom((b.A + kBlock - 1) / kBlock);
-    if (i < 0 || i >= b.A) return;
+    const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
+    if (t < 0) return;
+    const int i = t / LPA, sub = t % LPA;
+    if (i >= b.A) return;
     if (other_precision<T>(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
@@ -347,7 +368,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
     double rho = 0.0;
-    for (int k = 0; k < cnt; ++k) {
+    for (int k = sub; k < cnt; k += LPA) {
         uint32_t e = b.nbr[(size_t)k * b.A + i];
         int j = W.off + int(e & AKMC_NB_JMASK);
         const double4 pj = b.p4[j];
@@ -363,6 +384,8 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
             }
         }
     }
+    rho = lane_group_sum<LPA>(rho);
+    if (sub != 0) return;
     b.rho[i] = rho;
     b.e_a[i] = -sqrt(rho);
     b.inv_sqrt_rho[i] = rho < 1.0e-8 ? -1.0f : float(1.0 / sqrt(rho));
@@ -372,11 +395,13 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 // fm = 1/2 m (eps c)^2 (1/sqrt(rho_i) + 1/sqrt(rho_j)) (a/r)^m
 // (potentials/metal/Field.cpp:587-640, ManyBody.cpp:621-757); each atom
 // also carries half of each pair's V = eps (a/r)^n
-template <typename T, bool MINIMG, bool STRESS>
+template <typename T, bool MINIMG, bool STRESS, int LPA>
 __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
 {
-    int i = xcd_atom((b.A + kBlock - 1) / kBlock);
-    if (i < 0 || i >= b.A) return;
+    const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
+    if (t < 0) return;
+    const int i = t / LPA, sub = t % LPA;
+    if (i >= b.A) return;
     if (other_precision<T>(b, walker_of(b, i))) return;
     const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
@@ -387,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
     const float isi = b.inv_sqrt_rho[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-    for (int k = 0; k < cnt; ++k) {
+    for (int k = sub; k < cnt; k += LPA) {
         uint32_t e = b.nbr[(size_t)k * b.A + i];
         int j = W.off + int(e & AKMC_NB_JMASK);
         const double4 pj = b.p4[j];
@@ -430,6 +455,19 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
             }
         }
     }
+    fx = lane_group_sum<LPA>(fx);
+    fy = lane_group_sum<LPA>(fy);
+    fz = lane_group_sum<LPA>(fz);
+    ep = lane_group_sum<LPA>(ep);
+    if (STRESS) {
+        s0 = lane_group_sum<LPA>(s0);
+        s1 = lane_group_sum<LPA>(s1);
+        s2 = lane_group_sum<LPA>(s2);
+        s3 = lane_group_sum<LPA>(s3);
+        s4 = lane_group_sum<LPA>(s4);
+        s5 = lane_group_sum<LPA>(s5);
+    }
+    if (sub != 0) return;
     b.fx[i] = fx;
     b.fy[i] = fy;
     b.fz[i] = fz;
@@ -771,13 +809,35 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
     k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, akmc_rebuild_counter_ptr());
 }
 
+template <typename T, int LPA>
+static void launch_sc_density(const BatchDev &b, const FieldDev &hf, hipStream_t s)
+{
+    int g = xcd_grid((b.A * LPA + kBlock - 1) / kBlock);
+    if (hf.min_image) k_sc_density<T, true, LPA><<<g, kBlock, 0, s>>>(b);
+    else k_sc_density<T, false, LPA><<<g, kBlock, 0, s>>>(b);
+}
+
+template <typename T, int LPA>
+static void launch_sc_force(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
+{
+    int g = xcd_grid((b.A * LPA + kBlock - 1) / kBlock);
+    if (hf.min_image) {
+        if (want_stress) k_sc_force<T, true, true, LPA><<<g, kBlock, 0, s>>>(b);
+        else k_sc_force<T, true, false, LPA><<<g, kBlock, 0, s>>>(b);
+    } else {
+        if (want_stress) k_sc_force<T, false, true, LPA><<<g, kBlock, 0, s>>>(b);
+        else k_sc_force<T, false, false, LPA><<<g, kBlock, 0, s>>>(b);
+    }
+}
+
 template <typename T>
 static void launch_pass1(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     int gA = xcd_grid(blocks(b.A));
     if (hf.kind == 0) {
-        if (hf.min_image) k_sc_density<T, true><<<gA, kBlock, 0, s>>>(b);
-        else k_sc_density<T, false><<<gA, kBlock, 0, s>>>(b);
+        if (b.lpa == 4) launch_sc_density<T, 4>(b, hf, s);
+        else if (b.lpa == 2) launch_sc_density<T, 2>(b, hf, s);
+        else launch_sc_density<T, 1>(b, hf, s);
     } else {
         if (hf.min_image) {
             if (want_stress) k_ion_real<T, true, true><<<gA, kBlock, 0, s>>>(b);
@@ -793,15 +853,10 @@ template <typename T>
 static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     int gA = blocks(b.A);
-    int gX = xcd_grid(gA);
     if (hf.kind == 0) {
-        if (hf.min_image) {
-            if (want_stress) k_sc_force<T, true, true><<<gX, kBlock, 0, s>>>(b);
-            else k_sc_force<T, true, false><<<gX, kBlock, 0, s>>>(b);
-        } else {
-            if (want_stress) k_sc_force<T, false, true><<<gX, kBlock, 0, s>>>(b);
-            else k_sc_force<T, false, false><<<gX, kBlock, 0, s>>>(b);
-        }
+        if (b.lpa == 4) launch_sc_force<T, 4>(b, hf, want_stress, s);
+        else if (b.lpa == 2) launch_sc_force<T, 2>(b, hf, want_stress, s);
+        else launch_sc_force<T, 1>(b, hf, want_stress, s);
     } else if (hf.do_ewald) {
         k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
         dim3 g((b.maxnk + kBlock - 1) / kBlock, b.W);

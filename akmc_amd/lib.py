@@ -144,6 +144,8 @@ def load() -> C.CDLL:
     L.akmc_batch_get_positions.argtypes = [C.c_void_p, _D, _D, _D]
     L.akmc_batch_get_mode.argtypes = [C.c_void_p, _D]
     L.akmc_batch_time_phases.argtypes = [C.c_void_p, C.c_int, _D]
+    L.akmc_batch_set_lanes.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.akmc_batch_save_basin.argtypes = [C.c_void_p]
     L.akmc_art_defaults.argtypes = [C.POINTER(ARTParams)]
     L.akmc_batch_start_art.argtypes = [C.c_void_p, _I, _D, _D, _I, C.POINTER(ARTParams)]
@@ -390,6 +392,18 @@ class Batch:
         ms = np.zeros(4)
         check(load().akmc_batch_time_phases(self._h, int(nrep), _dp(ms)))
         return ms / nrep
+
+    def set_lanes(self, lanes):
+        check(load().akmc_batch_set_lanes(self._h, int(lanes)))
+
+    def autotune(self, nrep=10):
+        """Time the Sutton-Chen kernels at 1, 2 and 4 lanes per atom on the
+        resident positions, keep the fastest; returns (best, {lanes: ms per
+        force evaluation})."""
+        ms = np.zeros(3)
+        best = C.c_int()
+        check(load().akmc_batch_autotune(self._h, int(nrep), C.byref(best), _dp(ms)))
+        return best.value, {1: ms[0] / nrep, 2: ms[1] / nrep, 4: ms[2] / nrep}
 
     def save_basin(self):
         check(load().akmc_batch_save_basin(self._h))

@@ -63,6 +63,8 @@ def plain_dimer_rate(L, W, text, cells, basins, proposal_fn, params, n_search, b
         w = k % len(cells)
         c = cells[w]
         b = L.Batch(f, [W.Cell(basins[w], c.types, c.lat)])
+        b.forces()
+        b.autotune(5)
         x0, mode, frozen, _ = proposal_fn(w)
         t0 = time.perf_counter()
         b.set_verify(16, 0.01)
@@ -121,7 +123,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the walkers of this path "
-              "would shard across GPUs with no exchange during a search -- see DESIGN.md", file=sys.stderr)
+              "would shard across GPUs with no exchange during a search", file=sys.stderr)
         sys.exit(2)
 
     from akmc_amd import lib as L
@@ -144,6 +146,8 @@ def main():
     # ---- roofline of the force kernels at the basins (lists settled)
     camp.batch.upload([W.Cell(p, c.types, c.lat) for p, c in zip(camp.basins, cells)])
     camp.batch.forces()
+    lanes, lane_ms = camp.batch.autotune(10)
+    print(f"[bench] autotune lanes per atom: {lanes} (ms per evaluation {lane_ms})", file=sys.stderr, flush=True)
     ph = camp.batch.time_phases(20)
     P = ordered_pairs(cells, camp.basins, rc)
     Ptot = int(P.sum())
@@ -209,7 +213,10 @@ def main():
         "force_evals_per_s_batched_cells": round(batched_force_evals, 1),
         "force_eval_ms_all_walkers": round(eval_ms, 4),
         "force_evals_per_s_cells_device": round(len(cells) / (eval_ms * 1e-3), 1),
-        "roofline": {"bound": "valu", "kernel": "k_sc_force (Sutton-Chen pass 2)",
+        "autotune": {"lanes_per_atom": lanes, "ms_per_eval": {str(k): round(v, 4) for k, v in lane_ms.items()}},
+        "roofline": {"bound": "mfma", "bound_note": "compute-bound on the FP32 vector ALU (pair physics has no "
+                     "matrix form); peak is gfx950's FP32 vector figure, equal to its FP32 matrix one",
+                     "kernel": "k_sc_force (Sutton-Chen pass 2)",
                      "achieved": round(achieved_tf, 3), "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tf / FP32_VECTOR_PEAK_TFLOPS, 4), "traffic": None,
                      "flops_per_launch": 40 * Ptot, "ms_per_launch": round(force_ms, 4),

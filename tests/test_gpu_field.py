@@ -152,3 +152,39 @@ def test_skin_reuse_matches_fresh(lib, oracle):
         assert np.abs(frc[0] - ref["force"]).max() < F_TOL
         assert abs(e[0]["total"] - ref["energy"]) / c.n < E_TOL_ATOM
     assert b.rebuilds >= r0
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_lanes_per_atom_and_autotune(lib, oracle, fp64):
+    """Every lanes-per-atom choice of the Sutton-Chen kernels gives the
+    oracle's forces (FP32 within the force budget, FP64 to round-off), and
+    the autotuner keeps one of them without changing the results."""
+    cells = W.batched_sc_cells(n_walkers=5, seed=13)
+    f = lib.Field(lib.sc_potentials_text(_sc_pots(), 2 * CU["a"]))
+    if fp64:
+        f.set_precision(64)
+    b = lib.Batch(f, cells)
+    of = oracle.sc_field(["Cu"], _sc_pots(), 2 * CU["a"])
+    refs = [oracle.sc_forces(of, c.pos, c.types, c.lat) for c in cells]
+    ftol, etol = (1e-9, 1e-11) if fp64 else (F_TOL, E_TOL_ATOM)
+    got = {}
+    for lanes in (1, 2, 4):
+        b.set_lanes(lanes)
+        e = b.forces()
+        _, frc = b.download()
+        got[lanes] = frc
+        for w, c in enumerate(cells):
+            df = np.abs(frc[w] - refs[w]["force"]).max()
+            de = abs(e[w]["total"] - refs[w]["energy"]) / c.n
+            assert df < ftol and de < etol, (lanes, w, df, de)
+        print(f"lanes {lanes} fp{64 if fp64 else 32}: max|dF| "
+              f"{max(np.abs(frc[w] - refs[w]['force']).max() for w in range(len(cells))):.2e}", flush=True)
+    best, ms = b.autotune(nrep=5)
+    print(f"autotune: best {best}, ms per evaluation {ms}", flush=True)
+    assert best in (1, 2, 4)
+    b.forces()
+    _, frc = b.download()
+    for w in range(len(cells)):
+        assert np.array_equal(frc[w], got[best][w])
+    with pytest.raises(Exception):
+        b.set_lanes(3)
