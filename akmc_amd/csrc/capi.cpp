@@ -116,8 +116,13 @@ struct akmc_batch_s {
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
     SearchDev graph_args{};  // kernel arguments baked into the captured graph
+    BatchDev graph_d{};
+    int graph_prec = 0;
     double *basin = nullptr;  // [3][A] saved search positions
     hipEvent_t evp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // pinned status staging: one stream sync per host check of a run
+    WalkerState *hws_pin = nullptr;
+    int *ov_pin = nullptr;
 };
 
 namespace {
@@ -524,6 +529,8 @@ void batch_free(akmc_batch_t b)
         if (e) (void)hipEventDestroy(e);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->hws_pin) (void)hipHostFree(b->hws_pin);
+    if (b->ov_pin) (void)hipHostFree(b->ov_pin);
     delete b;
 }
 
@@ -648,6 +655,8 @@ int search_alloc(akmc_batch_t b)
     HIPCHK(hipMemcpy(b->s.X + A, b->d.y, A * sizeof(double), hipMemcpyDeviceToDevice));
     HIPCHK(hipMemcpy(b->s.X + 2 * A, b->d.z, A * sizeof(double), hipMemcpyDeviceToDevice));
     b->hws.assign(b->W, WalkerState{});
+    HIPCHK(hipHostMalloc((void **)&b->hws_pin, b->W * sizeof(WalkerState), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&b->ov_pin, sizeof(int), hipHostMallocDefault));
     b->search_alloc = true;
     return AKMC_SUCCESS;
 }
@@ -783,6 +792,8 @@ int capture_graph(akmc_batch_t b, int steps)
     HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     b->graph_steps = steps;
     b->graph_args = b->s;
+    b->graph_d = b->d;
+    b->graph_prec = batch_mode(b);
     return AKMC_SUCCESS;
 }
 
@@ -1144,18 +1155,23 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
     TRY(refresh_active(b, &running));
     while (running > 0 && done < max_steps) {
         int k = std::min(sync_every, max_steps - done);
-        if (k > 1) {
-            if (!b->gexec || b->graph_steps != k || std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0)
-                TRY(capture_graph(b, k));
-            HIPCHK(hipGraphLaunch(b->gexec, b->stream));
-        } else {
-            enqueue_micro_step(b);
-            HIPCHK(hipGetLastError());
-        }
+        // k micro-steps as one graph launch; the state kernels keep the
+        // per-walker active flags current (0 when a walker ends, FP64 while
+        // it verifies), so the host only reads states and the overflow flag
+        if (!b->gexec || b->graph_steps != k || b->graph_prec != batch_mode(b) ||
+            std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0 ||
+            std::memcmp(&b->graph_d, &b->d, sizeof(BatchDev)) != 0)
+            TRY(capture_graph(b, k));
+        HIPCHK(hipGraphLaunch(b->gexec, b->stream));
         done += k;
-        TRY(check_overflow(b));
-        TRY(fetch_states(b));
-        TRY(refresh_active(b, &running));
+        HIPCHK(hipMemcpyAsync(b->hws_pin, b->s.ws, b->W * sizeof(WalkerState), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipMemcpyAsync(b->ov_pin, b->d.overflow, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        if (*b->ov_pin) return fail(AKMC_ECAPACITY, "neighbour list capacity %d exceeded", b->d.maxnb);
+        std::memcpy(b->hws.data(), b->hws_pin, b->W * sizeof(WalkerState));
+        running = 0;
+        for (int w = 0; w < b->W; ++w)
+            running += b->hws[w].kind != WK_IDLE && b->hws[w].state != ST_DONE;
     }
     if (steps_done) *steps_done = done;
     return AKMC_SUCCESS;
