@@ -70,6 +70,7 @@ struct WalkerOut {
 struct BatchDev {
     int A, W, maxnb, ncells_total, nk_total, G1, maxnk;
     int lpa;              // lanes per atom of the Sutton-Chen pair kernels (1, 2, 4)
+    int unroll;           // neighbour gathers in flight per lane (1, 4)
     const FieldDev *field;
     const WalkerDev *walkers;
     const int *wk;        // atom -> walker

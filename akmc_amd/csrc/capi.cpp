@@ -419,6 +419,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     d.W = W;
     d.maxnb = maxnb;
     d.lpa = 1;
+    d.unroll = 1;
     d.ncells_total = ncells;
     d.nk_total = nk;
     d.G1 = G1;
@@ -957,32 +958,37 @@ int akmc_batch_forces_repeat(akmc_batch_t b, int nrep, double *ms)
     return check_overflow(b);
 }
 
-int akmc_batch_set_lanes(akmc_batch_t b, int lanes)
+int akmc_batch_set_variant(akmc_batch_t b, int lanes, int unroll)
 {
     if (!b) return fail(AKMC_EINVAL, "null batch");
     if (lanes != 1 && lanes != 2 && lanes != 4) return fail(AKMC_EINVAL, "lanes per atom must be 1, 2 or 4, got %d", lanes);
-    if (b->d.lpa != lanes) drop_graph(b);  // the captured graph holds the old kernels
+    if (unroll != 1 && unroll != 4) return fail(AKMC_EINVAL, "unroll must be 1 or 4, got %d", unroll);
+    if (b->d.lpa != lanes || b->d.unroll != unroll) drop_graph(b);  // the captured graph holds the old kernels
     b->d.lpa = lanes;
+    b->d.unroll = unroll;
     return AKMC_SUCCESS;
 }
 
-int akmc_batch_autotune(akmc_batch_t b, int nrep, int *best, double ms[3])
+int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[2], double ms[6])
 {
     if (!b || nrep <= 0) return fail(AKMC_EINVAL, "bad arguments");
-    static const int choice[3] = {1, 2, 4};
-    double t[3];
+    static const int lanes[3] = {1, 2, 4}, unroll[2] = {1, 4};
+    double t[6];
     TRY(run_forces(b, 0));  // lists current before timing
-    for (int c = 0; c < 3; ++c) {
-        TRY(akmc_batch_set_lanes(b, choice[c]));
+    for (int v = 0; v < 6; ++v) {
+        TRY(akmc_batch_set_variant(b, lanes[v / 2], unroll[v % 2]));
         TRY(run_forces(b, 0));  // first launch of this instantiation
-        TRY(akmc_batch_forces_repeat(b, nrep, &t[c]));
-        if (ms) ms[c] = t[c];
+        TRY(akmc_batch_forces_repeat(b, nrep, &t[v]));
+        if (ms) ms[v] = t[v];
     }
     int k = 0;
-    for (int c = 1; c < 3; ++c)
-        if (t[c] < t[k]) k = c;
-    TRY(akmc_batch_set_lanes(b, choice[k]));
-    if (best) *best = choice[k];
+    for (int v = 1; v < 6; ++v)
+        if (t[v] < t[k]) k = v;
+    TRY(akmc_batch_set_variant(b, lanes[k / 2], unroll[k % 2]));
+    if (best) {
+        best[0] = lanes[k / 2];
+        best[1] = unroll[k % 2];
+    }
     return AKMC_SUCCESS;
 }
 

@@ -332,8 +332,6 @@ __global__ void k_rebuild_done(BatchDev b, unsigned long long *count)
 
 // ---------------------------------------------------------- Sutton-Chen
 
-// pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
-// then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2
 // the global thread index in XCD order (see xcd_atom), -1 past the end
 __device__ inline int xcd_thread(int nblocks)
 {
@@ -350,37 +348,76 @@ __device__ inline double lane_group_sum(double v)
     return v;
 }
 
+// The Sutton-Chen pair table (nspec^2 entries) staged in LDS by every
+// workgroup before any thread leaves, so the coefficient lookup of a pair is
+// an LDS read instead of a third dependent global load.
+__device__ inline void stage_sc_table(const FieldDev &F, SCPairDev *tab)
+{
+    static_assert(sizeof(SCPairDev) % 4 == 0, "table staged in 4-byte words");
+    const int nw = F.nspec * F.nspec * int(sizeof(SCPairDev) / 4);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(F.sc);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(tab);
+    for (int w = threadIdx.x; w < nw; w += blockDim.x) dst[w] = src[w];
+    __syncthreads();
+}
+
+// List entry that stands for "no neighbour": atom i itself with no image
+// shift.  Its separation is exactly 0, which the r^2 > floor test rejects,
+// so padding slots of a U-wide batch of gathers need no other branch.
+__device__ inline uint32_t self_entry(int i, const WalkerDev &W)
+{
+    return uint32_t(i - W.off) | (0x777u << 20);
+}
+
+// The neighbour loops below fetch U list entries, then the U neighbours'
+// positions (and densities), then evaluate the U pairs: U independent
+// gathers in flight per lane instead of one dependent chain per pair.  The
+// per-lane summation order is k ascending for every U, so the U variants
+// give bitwise identical results; LPA variants differ by summation order.
+
 // pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
 // then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2.
 // LPA lanes share one atom's neighbour list (strided) and combine by shuffles.
-template <typename T, bool MINIMG, int LPA>
+template <typename T, bool MINIMG, int LPA, int U>
 __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 {
+    __shared__ SCPairDev tab[AKMC_MAXPAIR];
+    const FieldDev &F = *b.field;
+    stage_sc_table(F, tab);
     const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
     if (t < 0) return;
     const int i = t / LPA, sub = t % LPA;
     if (i >= b.A) return;
     if (other_precision<T>(b, walker_of(b, i))) return;
-    const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
-    const int ti = b.type[i];
+    const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
+    const uint32_t none = self_entry(i, W);
     double rho = 0.0;
-    for (int k = sub; k < cnt; k += LPA) {
-        uint32_t e = b.nbr[(size_t)k * b.A + i];
-        int j = W.off + int(e & AKMC_NB_JMASK);
-        const double4 pj = b.p4[j];
-        double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
-        entry_disp(e, dx, dy, dz, W, MINIMG);
-        double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
-            const SCPairDev &p = F.sc[ti * F.nspec + (int)pj.w];
-            if (p.on) {
-                const SCCoef<T> &c = sc_coef<T>(p);
-                T u2 = c.a * c.a / T(r2);
-                rho += double(c.rho_coef * pow_half<T>(u2, p.m_int, c.m));
+    for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
+        uint32_t e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * LPA;
+            e[u] = k < cnt ? b.nbr[(size_t)k * b.A + i] : none;
+        }
+        double4 pj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) pj[u] = b.p4[W.off + int(e[u] & AKMC_NB_JMASK)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            double dx = xi - pj[u].x, dy = yi - pj[u].y, dz = zi - pj[u].z;
+            entry_disp(e[u], dx, dy, dz, W, MINIMG);
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
+                const SCPairDev &p = tab[row + (int)pj[u].w];
+                if (p.on) {
+                    const SCCoef<T> &c = sc_coef<T>(p);
+                    T u2 = c.a * c.a / T(r2);
+                    rho += double(c.rho_coef * pow_half<T>(u2, p.m_int, c.m));
+                }
             }
         }
     }
@@ -395,63 +432,82 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 // fm = 1/2 m (eps c)^2 (1/sqrt(rho_i) + 1/sqrt(rho_j)) (a/r)^m
 // (potentials/metal/Field.cpp:587-640, ManyBody.cpp:621-757); each atom
 // also carries half of each pair's V = eps (a/r)^n
-template <typename T, bool MINIMG, bool STRESS, int LPA>
+template <typename T, bool MINIMG, bool STRESS, int LPA, int U>
 __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
 {
+    __shared__ SCPairDev tab[AKMC_MAXPAIR];
+    const FieldDev &F = *b.field;
+    stage_sc_table(F, tab);
     const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
     if (t < 0) return;
     const int i = t / LPA, sub = t % LPA;
     if (i >= b.A) return;
     if (other_precision<T>(b, walker_of(b, i))) return;
-    const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
-    const int ti = b.type[i];
+    const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
     const float isi = b.inv_sqrt_rho[i];
+    const uint32_t none = self_entry(i, W);
     double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-    for (int k = sub; k < cnt; k += LPA) {
-        uint32_t e = b.nbr[(size_t)k * b.A + i];
-        int j = W.off + int(e & AKMC_NB_JMASK);
-        const double4 pj = b.p4[j];
-        double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
-        entry_disp(e, dx, dy, dz, W, MINIMG);
-        double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
-            const SCPairDev &p = F.sc[ti * F.nspec + (int)pj.w];
-            if (!p.on) continue;
-            const SCCoef<T> &c = sc_coef<T>(p);
-            T r2t = T(r2);
-            T u2 = c.a * c.a / r2t;
-            T un = pow_half<T>(u2, p.n_int, c.n);
-            T um = pow_half<T>(u2, p.m_int, c.m);
-            T fs = c.fs_coef * un;
-            T fm = T(0);
-            float isj = b.inv_sqrt_rho[j];
-            if (isi >= 0.0f && isj >= 0.0f) {
-                if (sizeof(T) == 8) {
-                    double ri = b.rho[i], rj = b.rho[j];
-                    fm = c.fm_coef * T(1.0 / sqrt(ri) + 1.0 / sqrt(rj)) * um;
-                } else {
-                    fm = c.fm_coef * (T(isi) + T(isj)) * um;
+    for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
+        uint32_t e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * LPA;
+            e[u] = k < cnt ? b.nbr[(size_t)k * b.A + i] : none;
+        }
+        double4 pj[U];
+        float isj[U];
+        double rj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = W.off + int(e[u] & AKMC_NB_JMASK);
+            pj[u] = b.p4[j];
+            isj[u] = b.inv_sqrt_rho[j];
+            if (sizeof(T) == 8) rj[u] = b.rho[j];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            double dx = xi - pj[u].x, dy = yi - pj[u].y, dz = zi - pj[u].z;
+            entry_disp(e[u], dx, dy, dz, W, MINIMG);
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
+                const SCPairDev &p = tab[row + (int)pj[u].w];
+                if (p.on) {
+                    const SCCoef<T> &c = sc_coef<T>(p);
+                    T r2t = T(r2);
+                    T u2 = c.a * c.a / r2t;
+                    T un = pow_half<T>(u2, p.n_int, c.n);
+                    T um = pow_half<T>(u2, p.m_int, c.m);
+                    T fs = c.fs_coef * un;
+                    T fm = T(0);
+                    if (isi >= 0.0f && isj[u] >= 0.0f) {
+                        if (sizeof(T) == 8) {
+                            double ri = b.rho[i];
+                            fm = c.fm_coef * T(1.0 / sqrt(ri) + 1.0 / sqrt(rj[u])) * um;
+                        } else {
+                            fm = c.fm_coef * (T(isi) + T(isj[u])) * um;
+                        }
+                    }
+                    T ft = (fs - fm) / r2t;  // both orderings of the pair
+                    double dft = double(ft);
+                    fx += dft * dx;
+                    fy += dft * dy;
+                    fz += dft * dz;
+                    ep += double(T(0.5) * c.eps * un);
+                    if (STRESS) {
+                        double h = -0.5 * dft;  // one ordered pair: -(fs-fm)/(2 r^2) r r
+                        s0 += h * dx * dx;
+                        s1 += h * dy * dy;
+                        s2 += h * dz * dz;
+                        s3 += h * dy * dz;
+                        s4 += h * dx * dz;
+                        s5 += h * dx * dy;
+                    }
                 }
-            }
-            T ft = (fs - fm) / r2t;  // both orderings of the pair
-            double dft = double(ft);
-            fx += dft * dx;
-            fy += dft * dy;
-            fz += dft * dz;
-            ep += double(T(0.5) * c.eps * un);
-            if (STRESS) {
-                double h = -0.5 * dft;  // one ordered pair: -(fs-fm)/(2 r^2) r r
-                s0 += h * dx * dx;
-                s1 += h * dy * dy;
-                s2 += h * dz * dz;
-                s3 += h * dy * dz;
-                s4 += h * dx * dz;
-                s5 += h * dx * dy;
             }
         }
     }
@@ -809,35 +865,49 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
     k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, akmc_rebuild_counter_ptr());
 }
 
-template <typename T, int LPA>
+template <typename T, int LPA, int U>
 static void launch_sc_density(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     int g = xcd_grid((b.A * LPA + kBlock - 1) / kBlock);
-    if (hf.min_image) k_sc_density<T, true, LPA><<<g, kBlock, 0, s>>>(b);
-    else k_sc_density<T, false, LPA><<<g, kBlock, 0, s>>>(b);
+    if (hf.min_image) k_sc_density<T, true, LPA, U><<<g, kBlock, 0, s>>>(b);
+    else k_sc_density<T, false, LPA, U><<<g, kBlock, 0, s>>>(b);
 }
 
-template <typename T, int LPA>
+template <typename T, int LPA, int U>
 static void launch_sc_force(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     int g = xcd_grid((b.A * LPA + kBlock - 1) / kBlock);
+    // the stress variant exists for U = 1 only: U does not change results
     if (hf.min_image) {
-        if (want_stress) k_sc_force<T, true, true, LPA><<<g, kBlock, 0, s>>>(b);
-        else k_sc_force<T, true, false, LPA><<<g, kBlock, 0, s>>>(b);
+        if (want_stress) k_sc_force<T, true, true, LPA, 1><<<g, kBlock, 0, s>>>(b);
+        else k_sc_force<T, true, false, LPA, U><<<g, kBlock, 0, s>>>(b);
     } else {
-        if (want_stress) k_sc_force<T, false, true, LPA><<<g, kBlock, 0, s>>>(b);
-        else k_sc_force<T, false, false, LPA><<<g, kBlock, 0, s>>>(b);
+        if (want_stress) k_sc_force<T, false, true, LPA, 1><<<g, kBlock, 0, s>>>(b);
+        else k_sc_force<T, false, false, LPA, U><<<g, kBlock, 0, s>>>(b);
     }
 }
+
+// (lanes per atom, gathers in flight per lane) of the batch
+#define AKMC_SC_VARIANT(CALL)                                     \
+    do {                                                          \
+        const bool u4 = b.unroll == 4;                            \
+        if (b.lpa == 4) {                                         \
+            if (u4) CALL(4, 4); else CALL(4, 1);                  \
+        } else if (b.lpa == 2) {                                  \
+            if (u4) CALL(2, 4); else CALL(2, 1);                  \
+        } else {                                                  \
+            if (u4) CALL(1, 4); else CALL(1, 1);                  \
+        }                                                         \
+    } while (0)
 
 template <typename T>
 static void launch_pass1(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     int gA = xcd_grid(blocks(b.A));
     if (hf.kind == 0) {
-        if (b.lpa == 4) launch_sc_density<T, 4>(b, hf, s);
-        else if (b.lpa == 2) launch_sc_density<T, 2>(b, hf, s);
-        else launch_sc_density<T, 1>(b, hf, s);
+#define AKMC_DENS(L_, U_) launch_sc_density<T, L_, U_>(b, hf, s)
+        AKMC_SC_VARIANT(AKMC_DENS);
+#undef AKMC_DENS
     } else {
         if (hf.min_image) {
             if (want_stress) k_ion_real<T, true, true><<<gA, kBlock, 0, s>>>(b);
@@ -854,9 +924,9 @@ static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress,
 {
     int gA = blocks(b.A);
     if (hf.kind == 0) {
-        if (b.lpa == 4) launch_sc_force<T, 4>(b, hf, want_stress, s);
-        else if (b.lpa == 2) launch_sc_force<T, 2>(b, hf, want_stress, s);
-        else launch_sc_force<T, 1>(b, hf, want_stress, s);
+#define AKMC_FORCE(L_, U_) launch_sc_force<T, L_, U_>(b, hf, want_stress, s)
+        AKMC_SC_VARIANT(AKMC_FORCE);
+#undef AKMC_FORCE
     } else if (hf.do_ewald) {
         k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
         dim3 g((b.maxnk + kBlock - 1) / kBlock, b.W);

@@ -144,7 +144,7 @@ def load() -> C.CDLL:
     L.akmc_batch_get_positions.argtypes = [C.c_void_p, _D, _D, _D]
     L.akmc_batch_get_mode.argtypes = [C.c_void_p, _D]
     L.akmc_batch_time_phases.argtypes = [C.c_void_p, C.c_int, _D]
-    L.akmc_batch_set_lanes.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.akmc_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.akmc_batch_save_basin.argtypes = [C.c_void_p]
     L.akmc_art_defaults.argtypes = [C.POINTER(ARTParams)]
@@ -393,17 +393,19 @@ class Batch:
         check(load().akmc_batch_time_phases(self._h, int(nrep), _dp(ms)))
         return ms / nrep
 
-    def set_lanes(self, lanes):
-        check(load().akmc_batch_set_lanes(self._h, int(lanes)))
+    def set_variant(self, lanes, unroll=1):
+        check(load().akmc_batch_set_variant(self._h, int(lanes), int(unroll)))
 
     def autotune(self, nrep=10):
-        """Time the Sutton-Chen kernels at 1, 2 and 4 lanes per atom on the
-        resident positions, keep the fastest; returns (best, {lanes: ms per
-        force evaluation})."""
-        ms = np.zeros(3)
-        best = C.c_int()
-        check(load().akmc_batch_autotune(self._h, int(nrep), C.byref(best), _dp(ms)))
-        return best.value, {1: ms[0] / nrep, 2: ms[1] / nrep, 4: ms[2] / nrep}
+        """Time the six Sutton-Chen kernel variants (lanes per atom 1/2/4 x
+        gathers in flight 1/4) on the resident positions and keep the
+        fastest; returns ((lanes, unroll), {(lanes, unroll): ms per force
+        evaluation})."""
+        ms = np.zeros(6)
+        best = (C.c_int * 2)()
+        check(load().akmc_batch_autotune(self._h, int(nrep), best, _dp(ms)))
+        table = {(l, u): ms[2 * li + ui] / nrep for li, l in enumerate((1, 2, 4)) for ui, u in enumerate((1, 4))}
+        return (best[0], best[1]), table
 
     def save_basin(self):
         check(load().akmc_batch_save_basin(self._h))

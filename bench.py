@@ -110,6 +110,36 @@ def cpu_baseline(cells, basins, proposal_fn, budget_s=20.0, max_searches=3):
                 cores=cores, searches=done, converged=ok, seconds=t_total, sizes=sizes)
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size: reading FETCH_SIZE and
+    WRITE_SIZE for it gives the scale from counter units to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 5}), flush=True)
+
+
+def forces_only(L, W, args):
+    """Force evaluations of every walker of the batched workload at its
+    seeded (unrelaxed) positions, lanes per atom autotuned."""
+    cu = W.SC_CU
+    text = L.sc_potentials_text([("Cu", "Cu", cu["eps"], cu["a"], cu["n"], cu["m"], cu["c"])], 2 * cu["a"])
+    cells = W.batched_sc_cells(n_walkers=args.walkers, seed=args.seed)
+    b = L.Batch(L.Field(text), cells)
+    b.forces()
+    variant, _ = b.autotune(5)
+    ph = b.time_phases(args.forces_only)
+    P = int(ordered_pairs(cells, [c.pos for c in cells], 2 * cu["a"]).sum())
+    print(json.dumps({"forces_only": args.forces_only, "walkers": len(cells), "atoms": b.A,
+                      "pairs": P, "variant": variant, "phase_ms": [round(float(x), 4) for x in ph]}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,16 +150,26 @@ def main():
     ap.add_argument("--plain", type=int, default=4, help="plain GPU dimer searches")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--forces-only", type=int, default=0, metavar="N",
+                    help="only N force evaluations of all walkers (for hardware-counter runs)")
+    ap.add_argument("--copy-mib", type=int, default=0, metavar="MIB",
+                    help="only a plain device copy of MIB MiB, 5 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the walkers of this path "
               "would shard across GPUs with no exchange during a search", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
+        return
     from akmc_amd import lib as L
     from akmc_amd import search as S
     from akmc_amd import workloads as W
     L.load()
+    if args.forces_only:
+        forces_only(L, W, args)
+        return
     cu = W.SC_CU
     rc = 2 * cu["a"]
     text = L.sc_potentials_text([("Cu", "Cu", cu["eps"], cu["a"], cu["n"], cu["m"], cu["c"])], rc)
@@ -146,8 +186,9 @@ def main():
     # ---- roofline of the force kernels at the basins (lists settled)
     camp.batch.upload([W.Cell(p, c.types, c.lat) for p, c in zip(camp.basins, cells)])
     camp.batch.forces()
-    lanes, lane_ms = camp.batch.autotune(10)
-    print(f"[bench] autotune lanes per atom: {lanes} (ms per evaluation {lane_ms})", file=sys.stderr, flush=True)
+    variant, variant_ms = camp.batch.autotune(10)
+    print(f"[bench] autotune (lanes, unroll): {variant} (ms per evaluation "
+          f"{ {k: round(v, 3) for k, v in variant_ms.items()} })", file=sys.stderr, flush=True)
     ph = camp.batch.time_phases(20)
     P = ordered_pairs(cells, camp.basins, rc)
     Ptot = int(P.sum())
@@ -213,7 +254,8 @@ def main():
         "force_evals_per_s_batched_cells": round(batched_force_evals, 1),
         "force_eval_ms_all_walkers": round(eval_ms, 4),
         "force_evals_per_s_cells_device": round(len(cells) / (eval_ms * 1e-3), 1),
-        "autotune": {"lanes_per_atom": lanes, "ms_per_eval": {str(k): round(v, 4) for k, v in lane_ms.items()}},
+        "autotune": {"lanes_per_atom": variant[0], "gathers_in_flight": variant[1],
+                     "ms_per_eval": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in variant_ms.items()}},
         "roofline": {"bound": "mfma", "bound_note": "compute-bound on the FP32 vector ALU (pair physics has no "
                      "matrix form); peak is gfx950's FP32 vector figure, equal to its FP32 matrix one",
                      "kernel": "k_sc_force (Sutton-Chen pass 2)",

@@ -155,10 +155,11 @@ def test_skin_reuse_matches_fresh(lib, oracle):
 
 
 @pytest.mark.parametrize("fp64", [False, True])
-def test_lanes_per_atom_and_autotune(lib, oracle, fp64):
-    """Every lanes-per-atom choice of the Sutton-Chen kernels gives the
-    oracle's forces (FP32 within the force budget, FP64 to round-off), and
-    the autotuner keeps one of them without changing the results."""
+def test_kernel_variants_and_autotune(lib, oracle, fp64):
+    """Every (lanes per atom, gathers in flight) variant of the Sutton-Chen
+    kernels gives the oracle's forces (FP32 within the force budget, FP64 to
+    round-off); variants with the same lanes are bitwise identical; the
+    autotuner keeps one of them without changing its results."""
     cells = W.batched_sc_cells(n_walkers=5, seed=13)
     f = lib.Field(lib.sc_potentials_text(_sc_pots(), 2 * CU["a"]))
     if fp64:
@@ -169,22 +170,28 @@ def test_lanes_per_atom_and_autotune(lib, oracle, fp64):
     ftol, etol = (1e-9, 1e-11) if fp64 else (F_TOL, E_TOL_ATOM)
     got = {}
     for lanes in (1, 2, 4):
-        b.set_lanes(lanes)
-        e = b.forces()
-        _, frc = b.download()
-        got[lanes] = frc
-        for w, c in enumerate(cells):
-            df = np.abs(frc[w] - refs[w]["force"]).max()
-            de = abs(e[w]["total"] - refs[w]["energy"]) / c.n
-            assert df < ftol and de < etol, (lanes, w, df, de)
-        print(f"lanes {lanes} fp{64 if fp64 else 32}: max|dF| "
-              f"{max(np.abs(frc[w] - refs[w]['force']).max() for w in range(len(cells))):.2e}", flush=True)
+        for unroll in (1, 4):
+            b.set_variant(lanes, unroll)
+            e = b.forces()
+            _, frc = b.download()
+            got[(lanes, unroll)] = [f.copy() for f in frc]
+            worst = 0.0
+            for w, c in enumerate(cells):
+                df = np.abs(frc[w] - refs[w]["force"]).max()
+                de = abs(e[w]["total"] - refs[w]["energy"]) / c.n
+                worst = max(worst, df)
+                assert df < ftol and de < etol, (lanes, unroll, w, df, de)
+            print(f"variant {lanes}x{unroll} fp{64 if fp64 else 32}: max|dF| {worst:.2e}", flush=True)
+        for w in range(len(cells)):
+            assert np.array_equal(got[(lanes, 1)][w], got[(lanes, 4)][w])
     best, ms = b.autotune(nrep=5)
     print(f"autotune: best {best}, ms per evaluation {ms}", flush=True)
-    assert best in (1, 2, 4)
+    assert best in got
     b.forces()
     _, frc = b.download()
     for w in range(len(cells)):
         assert np.array_equal(frc[w], got[best][w])
     with pytest.raises(Exception):
-        b.set_lanes(3)
+        b.set_variant(3, 1)
+    with pytest.raises(Exception):
+        b.set_variant(1, 2)
