@@ -372,8 +372,9 @@ __device__ inline uint32_t self_entry(int i, const WalkerDev &W)
 // The neighbour loops below fetch U list entries, then the U neighbours'
 // positions (and densities), then evaluate the U pairs: U independent
 // gathers in flight per lane instead of one dependent chain per pair.  The
-// per-lane summation order is k ascending for every U, so the U variants
-// give bitwise identical results; LPA variants differ by summation order.
+// per-lane summation order is k ascending for every U (the FP32 variants
+// agree bitwise; FP64 ones to the last bit of FMA contraction); LPA variants
+// differ by summation order.
 
 // pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
 // then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2.

@@ -65,9 +65,22 @@ class CampaignStats:
     search_time: float = 0.0
     max_barrier_err: float = 0.0
     records: list = field(default_factory=list)
+    # converged searches that failed one verification test (a search can fail several)
+    fail_unchecked: int = 0   # the device verification did not complete
+    fail_fmax: int = 0        # max FP64 force >= fmax_verify
+    fail_barrier: int = 0     # |FP32 barrier - FP64 barrier| > barrier_tol
+    fail_lambda1: int = 0     # lowest curvature not negative
+    fail_lambda2: int = 0     # second curvature not positive
+    fail_b64: int = 0         # FP64 barrier not positive
+
+    _COUNTS = ("searches", "converged", "verified", "distinct", "force_calls", "micro_steps", "fail_unchecked",
+               "fail_fmax", "fail_barrier", "fail_lambda1", "fail_lambda2", "fail_b64")
+
+    def failures(self):
+        return {k[5:]: getattr(self, k) for k in self._COUNTS if k.startswith("fail_")}
 
     def add(self, o: "CampaignStats"):
-        for k in ("searches", "converged", "verified", "distinct", "force_calls", "micro_steps"):
+        for k in self._COUNTS:
             setattr(self, k, getattr(self, k) + getattr(o, k))
         self.search_time += o.search_time
         self.max_barrier_err = max(self.max_barrier_err, o.max_barrier_err)
@@ -163,6 +176,7 @@ class SaddleCampaign:
                 continue
             stats.converged += 1
             if not s["verified"]:
+                stats.fail_unchecked += 1
                 continue
             b32 = s["energy32"] - self.basin32[w]
             b64 = s["energy64"] - self.basin64[w]
@@ -176,8 +190,12 @@ class SaddleCampaign:
             rec = SaddleRecord(w, s["energy32"], s["energy64"], float(b32), float(b64), s["curvature32"],
                                (s["lambda1"], s["lambda2"]), s["fmax64"], atoms, d[:, atoms],
                                site=self.cur_site.get(w), active=act, active_disp=d[:, act])
-            rec.verified = (s["fmax64"] < self.fmax_verify and err <= self.barrier_tol
-                            and s["lambda1"] < 0.0 < s["lambda2"] and b64 > 0.0)
+            tests = dict(fmax=s["fmax64"] < self.fmax_verify, barrier=err <= self.barrier_tol,
+                         lambda1=s["lambda1"] < 0.0, lambda2=s["lambda2"] > 0.0, b64=b64 > 0.0)
+            for k, ok in tests.items():
+                if not ok:
+                    setattr(stats, "fail_" + k, getattr(stats, "fail_" + k) + 1)
+            rec.verified = all(tests.values())
             if rec.verified:
                 stats.verified += 1
                 rec.distinct = all(not self._same(rec, o) for o in self.found[w])

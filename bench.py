@@ -209,7 +209,7 @@ def main():
         st = camp.advance(args.window, sync_every=25)
         print(f"[bench] window {r}: {st.searches} ended, {st.converged} converged, {st.verified} verified, "
               f"{st.distinct} distinct, {st.micro_steps} micro-steps in {st.search_time:.2f}s, "
-              f"max barrier err {st.max_barrier_err:.2e}", file=sys.stderr, flush=True)
+              f"max barrier err {st.max_barrier_err:.2e}; failed tests {st.failures()}", file=sys.stderr, flush=True)
         if r >= args.warmup:
             t_timed += st.search_time
             tot.add(st)
@@ -249,6 +249,7 @@ def main():
         "verified_saddles_per_s": round(verified_rate, 3),
         "searches": tot.searches, "converged": tot.converged, "verified": tot.verified,
         "distinct": tot.distinct, "max_barrier_error_eV": tot.max_barrier_err,
+        "verification_failures": tot.failures(),
         "plain_gpu_dimer_saddles_per_s": round(plain_rate, 3),
         "batched_over_plain": round(verified_rate / plain_rate, 2) if plain_rate > 0 else None,
         "force_evals_per_s_batched_cells": round(batched_force_evals, 1),

@@ -115,7 +115,8 @@ int akmc_batch_time_phases(akmc_batch_t b, int nrep, double ms[4]);
  * evaluations at the resident positions for each of the six variants,
  * ms[2*l + u] for lanes (1, 2, 4)[l] and unroll (1, 4)[u], keeps the fastest
  * and returns it in best[0] (lanes), best[1] (unroll).  Variants with the
- * same lanes give bitwise identical results; different lanes agree to
+ * same lanes agree bitwise in FP32 (to FMA-contraction round-off in
+ * FP64); different lanes agree to
  * summation-order round-off.  No reference counterpart: the reference
  * evaluates one configuration per call on the host
  * (potentials/metal/Field.cpp:293-490). */

@@ -25,9 +25,11 @@ def _run(tmp_path, action, pos, types, lat, frozen, mode=None):
     (tmp_path / "pot").write_text(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
     n = pos.shape[1]
     lines = [str(n), " ".join(repr(float(v)) for v in lat)]
-    lines += [f"{pos[0, i]!r} {pos[1, i]!r} {pos[2, i]!r} {int(types[i])} {int(frozen[i])}" for i in range(n)]
+    # repr of a Python float round-trips exactly (numpy 2 scalars repr as "np.float64(...)")
+    lines += [f"{float(pos[0, i])!r} {float(pos[1, i])!r} {float(pos[2, i])!r} {int(types[i])} {int(frozen[i])}"
+              for i in range(n)]
     if mode is not None:
-        lines += [f"{mode[0, i]!r} {mode[1, i]!r} {mode[2, i]!r}" for i in range(n)]
+        lines += [f"{float(mode[0, i])!r} {float(mode[1, i])!r} {float(mode[2, i])!r}" for i in range(n)]
     (tmp_path / "cfg").write_text("\n".join(lines) + "\n")
     out = tmp_path / "out"
     r = subprocess.run([DRIVER, action, str(tmp_path / "pot"), str(tmp_path / "cfg"), str(out)],

@@ -183,7 +183,12 @@ def test_kernel_variants_and_autotune(lib, oracle, fp64):
                 assert df < ftol and de < etol, (lanes, unroll, w, df, de)
             print(f"variant {lanes}x{unroll} fp{64 if fp64 else 32}: max|dF| {worst:.2e}", flush=True)
         for w in range(len(cells)):
-            assert np.array_equal(got[(lanes, 1)][w], got[(lanes, 4)][w])
+            # same summation order; the FP32 path is bitwise equal, the FP64
+            # path may differ by the compiler's FMA contraction choices
+            if fp64:
+                assert np.abs(got[(lanes, 1)][w] - got[(lanes, 4)][w]).max() < 1e-12
+            else:
+                assert np.array_equal(got[(lanes, 1)][w], got[(lanes, 4)][w])
     best, ms = b.autotune(nrep=5)
     print(f"autotune: best {best}, ms per evaluation {ms}", flush=True)
     assert best in got
