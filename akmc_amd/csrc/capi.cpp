@@ -118,6 +118,7 @@ struct akmc_batch_s {
     SearchDev graph_args{};  // kernel arguments baked into the captured graph
     BatchDev graph_d{};
     int graph_prec = 0;
+    bool use_graphs = true;
     double *basin = nullptr;  // [3][A] saved search positions
     hipEvent_t evp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // pinned status staging: one stream sync per host check of a run
@@ -1164,11 +1165,16 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
         // k micro-steps as one graph launch; the state kernels keep the
         // per-walker active flags current (0 when a walker ends, FP64 while
         // it verifies), so the host only reads states and the overflow flag
-        if (!b->gexec || b->graph_steps != k || b->graph_prec != batch_mode(b) ||
-            std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0 ||
-            std::memcmp(&b->graph_d, &b->d, sizeof(BatchDev)) != 0)
-            TRY(capture_graph(b, k));
-        HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+        if (b->use_graphs) {
+            if (!b->gexec || b->graph_steps != k || b->graph_prec != batch_mode(b) ||
+                std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0 ||
+                std::memcmp(&b->graph_d, &b->d, sizeof(BatchDev)) != 0)
+                TRY(capture_graph(b, k));
+            HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+        } else {
+            for (int j = 0; j < k; ++j) enqueue_micro_step(b);
+            HIPCHK(hipGetLastError());
+        }
         done += k;
         HIPCHK(hipMemcpyAsync(b->hws_pin, b->s.ws, b->W * sizeof(WalkerState), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipMemcpyAsync(b->ov_pin, b->d.overflow, sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -1180,6 +1186,14 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
             running += b->hws[w].kind != WK_IDLE && b->hws[w].state != ST_DONE;
     }
     if (steps_done) *steps_done = done;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_set_graphs(akmc_batch_t b, int enable)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    b->use_graphs = enable != 0;
+    if (!b->use_graphs) drop_graph(b);
     return AKMC_SUCCESS;
 }
 

@@ -145,6 +145,7 @@ def load() -> C.CDLL:
     L.akmc_batch_get_mode.argtypes = [C.c_void_p, _D]
     L.akmc_batch_time_phases.argtypes = [C.c_void_p, C.c_int, _D]
     L.akmc_batch_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.akmc_batch_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.akmc_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.akmc_batch_save_basin.argtypes = [C.c_void_p]
     L.akmc_art_defaults.argtypes = [C.POINTER(ARTParams)]
@@ -392,6 +393,9 @@ class Batch:
         ms = np.zeros(4)
         check(load().akmc_batch_time_phases(self._h, int(nrep), _dp(ms)))
         return ms / nrep
+
+    def set_graphs(self, enable: bool):
+        check(load().akmc_batch_set_graphs(self._h, int(bool(enable))))
 
     def set_variant(self, lanes, unroll=1):
         check(load().akmc_batch_set_variant(self._h, int(lanes), int(unroll)))

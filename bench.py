@@ -53,7 +53,7 @@ def ordered_pairs(cells, positions, rc):
     return np.array(P)
 
 
-def plain_dimer_rate(L, W, text, cells, basins, proposal_fn, params, n_search, basin32, basin64):
+def plain_dimer_rate(L, W, text, cells, basins, proposal_fn, params, n_search, basin32, basin64, eager=False):
     """The plain GPU dimer: one walker at a time, host-synchronised every
     micro-step (the reference's control flow with device forces), with the
     same FP64 verification."""
@@ -63,6 +63,8 @@ def plain_dimer_rate(L, W, text, cells, basins, proposal_fn, params, n_search, b
         w = k % len(cells)
         c = cells[w]
         b = L.Batch(f, [W.Cell(basins[w], c.types, c.lat)])
+        if eager:
+            b.set_graphs(False)
         b.forces()
         b.autotune(5)
         x0, mode, frozen, _ = proposal_fn(w)
@@ -150,6 +152,9 @@ def main():
     ap.add_argument("--plain", type=int, default=4, help="plain GPU dimer searches")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the micro-step kernels one by one instead of replaying hipGraphs "
+                         "(for kernel tracers that cannot follow graph launches)")
     ap.add_argument("--forces-only", type=int, default=0, metavar="N",
                     help="only N force evaluations of all walkers (for hardware-counter runs)")
     ap.add_argument("--copy-mib", type=int, default=0, metavar="MIB",
@@ -177,6 +182,8 @@ def main():
     natoms = np.array([c.n for c in cells])
     dimer = L.dimer_defaults(maxParIter=400)
     camp = S.SaddleCampaign(text, cells, params=dimer, seed=args.seed)
+    if args.eager:
+        camp.batch.set_graphs(False)
     t0 = time.perf_counter()
     ok = camp.relax_basins()
     t_relax = time.perf_counter() - t0
@@ -219,7 +226,7 @@ def main():
 
     # ---- plain GPU dimer on the same cells
     n_ok, t_plain, plain_calls = plain_dimer_rate(L, W, text, cells, camp.basins, camp.proposal, dimer,
-                                                  args.plain, camp.basin32, camp.basin64)
+                                                  args.plain, camp.basin32, camp.basin64, eager=args.eager)
     plain_rate = n_ok / t_plain if t_plain > 0 else 0.0
     print(f"[bench] plain GPU dimer: {n_ok}/{args.plain} converged in {t_plain:.2f}s "
           f"({plain_calls} force calls)", file=sys.stderr, flush=True)

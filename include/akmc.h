@@ -207,6 +207,10 @@ int akmc_batch_start_art(akmc_batch_t b, const int *walkers, const double *x0, c
  * host looks at the walker states every sync_every steps (1 = the plain,
  * host-driven loop); steps of sync_every > 1 replay a captured hipGraph */
 int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_done);
+/* enable (default) or disable the hipGraph replay of akmc_batch_run; when
+ * disabled the same kernels are launched one by one on the batch's stream
+ * (for tools that cannot trace graph launches).  Results are identical. */
+int akmc_batch_set_graphs(akmc_batch_t b, int enable);
 int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out);
 /* current search positions (the dimer centre / relaxed positions) */
 int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z);
