@@ -122,6 +122,7 @@ enum : int {
     ST_D_F1T = 12,   // dimer: force at the trial end X + D n'
     ST_V_F0 = 20,    // verify: FP64 force at the saddle
     ST_V_LZ = 21,    // verify: FP64 force at X + delta q_k (Lanczos)
+    ST_V_LZM = 22,   // verify: FP64 force at X - delta q_k
     ST_A_L0 = 30,    // ART: force at X, start of a Lanczos evaluation
     ST_A_LZ = 31,    // ART: force at X + deltaX q_i
 };
@@ -165,6 +166,8 @@ struct WalkerState {
 struct SearchDev {
     double *X, *N, *V, *F0, *F1, *T;  // [3][A] each
     const int *frozen;                // [A]
+    const int *act;                   // [A]: walker w's free atoms (global ids) at act[off_w ..)
+    const int *nact;                  // [W]: number of free atoms per walker
     WalkerState *ws;                  // [W]
     RelaxParamsDev rp;
     DimerParamsDev dp;

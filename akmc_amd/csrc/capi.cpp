@@ -121,6 +121,8 @@ struct akmc_batch_s {
     bool use_graphs = true;
     double *basin = nullptr;  // [3][A] saved search positions
     hipEvent_t evp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // host copy of the frozen mask and the per-walker free-atom lists
+    std::vector<int> hfrozen, hact, hnact;
     // pinned status staging: one stream sync per host check of a run
     WalkerState *hws_pin = nullptr;
     int *ov_pin = nullptr;
@@ -635,6 +637,26 @@ int upload(akmc_batch_t b, const double *x, const double *y, const double *z, co
 }
 
 
+// rebuild the free-atom lists of the selected walkers (all when null) from
+// the host copy of the frozen mask and upload them
+int upload_active(akmc_batch_t b, const int *walkers)
+{
+    for (int w = 0; w < b->W; ++w) {
+        if (walkers && !walkers[w]) continue;
+        const int off = b->hw[w].off, n = b->hw[w].n;
+        int m = 0;
+        for (int i = 0; i < n; ++i)
+            if (!b->hfrozen[off + i]) b->hact[off + m++] = off + i;
+        b->hnact[w] = m;
+        if (m > 0)
+            HIPCHK(hipMemcpyAsync((void *)(b->s.act + off), b->hact.data() + off, m * sizeof(int),
+                                  hipMemcpyHostToDevice, b->stream));
+    }
+    HIPCHK(hipMemcpyAsync((void *)b->s.nact, b->hnact.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return AKMC_SUCCESS;
+}
+
 int search_alloc(akmc_batch_t b)
 {
     if (b->search_alloc) return AKMC_SUCCESS;
@@ -649,6 +671,14 @@ int search_alloc(akmc_batch_t b)
     TRY(dalloc(b, &fr, b->A));
     HIPCHK(hipMemset(fr, 0, b->A * sizeof(int)));
     b->s.frozen = fr;
+    int *act, *nact;
+    TRY(dalloc(b, &act, b->A));
+    TRY(dalloc(b, &nact, b->W));
+    b->s.act = act;
+    b->s.nact = nact;
+    b->hfrozen.assign(b->A, 0);
+    b->hact.assign(b->A, 0);
+    b->hnact.assign(b->W, 0);
     TRY(dalloc(b, &b->s.ws, b->W));
     HIPCHK(hipMemset(b->s.ws, 0, b->W * sizeof(WalkerState)));
     // X starts at the resident positions
@@ -657,6 +687,7 @@ int search_alloc(akmc_batch_t b)
     HIPCHK(hipMemcpy(b->s.X + A, b->d.y, A * sizeof(double), hipMemcpyDeviceToDevice));
     HIPCHK(hipMemcpy(b->s.X + 2 * A, b->d.z, A * sizeof(double), hipMemcpyDeviceToDevice));
     b->hws.assign(b->W, WalkerState{});
+    TRY(upload_active(b, nullptr));
     HIPCHK(hipHostMalloc((void **)&b->hws_pin, b->W * sizeof(WalkerState), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&b->ov_pin, sizeof(int), hipHostMallocDefault));
     b->search_alloc = true;
@@ -747,15 +778,12 @@ int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const dou
     for (int w = 0; w < b->W; ++w) {
         if (walkers && !walkers[w]) continue;
         const int off = b->hw[w].off, n = b->hw[w].n;
-        fr.assign(n, 0);
         if (frozen) {
-            std::copy(frozen + off, frozen + off + n, fr.begin());
+            std::copy(frozen + off, frozen + off + n, b->hfrozen.begin() + off);
             HIPCHK(hipMemcpyAsync((void *)(b->s.frozen + off), frozen + off, n * sizeof(int), hipMemcpyHostToDevice,
                                   b->stream));
-        } else {
-            HIPCHK(hipMemcpyAsync(fr.data(), b->s.frozen + off, n * sizeof(int), hipMemcpyDeviceToHost, b->stream));
-            HIPCHK(hipStreamSynchronize(b->stream));
         }
+        fr.assign(b->hfrozen.begin() + off, b->hfrozen.begin() + off + n);
         // zero frozen components and normalise the walker's mode
         nw.assign(3 * (size_t)n, 0.0);
         double nn = 0.0;
@@ -779,7 +807,7 @@ int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const dou
         }
         HIPCHK(hipStreamSynchronize(b->stream));
     }
-    return AKMC_SUCCESS;
+    return upload_active(b, walkers);
 }
 
 int capture_graph(akmc_batch_t b, int steps)
@@ -1068,7 +1096,8 @@ int akmc_batch_set_frozen(akmc_batch_t b, const int *frozen)
     if (!b || !frozen) return fail(AKMC_EINVAL, "bad arguments");
     TRY(search_alloc(b));
     HIPCHK(hipMemcpy((void *)b->s.frozen, frozen, b->A * sizeof(int), hipMemcpyHostToDevice));
-    return AKMC_SUCCESS;
+    std::copy(frozen, frozen + b->A, b->hfrozen.begin());
+    return upload_active(b, nullptr);
 }
 
 int akmc_batch_start_relax(akmc_batch_t b, const int *walkers, const akmc_relax_params *p)

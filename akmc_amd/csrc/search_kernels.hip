@@ -46,6 +46,7 @@ struct Ctx {
     const BatchDev &b;
     const SearchDev &s;
     int off, n, A;
+    int nact;  // free atoms of the walker: s.act[off .. off + nact)
     __device__ double f(int c, int i) const
     {
         if (s.frozen[i]) return 0.0;
@@ -59,13 +60,20 @@ struct Ctx {
     }
 };
 
+// every atom of the walker (FIRE relax keeps the reference's all-atom sums)
 #define FOR_ATOMS(i) for (int i = C_.off + (int)threadIdx.x; i < C_.off + C_.n; i += kBlock)
+// the walker's free atoms only: the vectors of the saddle searches span the
+// active region (optim/SaddleSearch.cpp vectorSize = 3 N_active); frozen
+// components are zero and frozen probe positions stay at X
+#define FOR_ACTIVE(i)                                                                   \
+    for (int t_##i = (int)threadIdx.x, i = 0;                                           \
+         t_##i < C_.nact && ((i = C_.s.act[C_.off + t_##i]), true); t_##i += kBlock)
 #define V3(arr, c, i) (arr)[(size_t)(c) * C_.A + (i)]
 
 // probe = X + a * D (D a [3][A] vector) or X when D is null
 __device__ inline void write_probe(const Ctx &C_, const double *D, double a)
 {
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         double px = V3(C_.s.X, 0, i), py = V3(C_.s.X, 1, i), pz = V3(C_.s.X, 2, i);
         if (D) {
@@ -221,14 +229,14 @@ __device__ void dimer_rot_eval(const Ctx &C_, WalkerState &w, double *sh)
     const DimerParamsDev &p = C_.s.dp;
     const double D = p.dimerDisplacement;
     double acc[1] = {0.0};
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) acc[0] += (V3(C_.s.F0, c, i) - V3(C_.s.F1, c, i)) / D * V3(C_.s.N, c, i);
     }
     bsum<1>(acc, sh);
     double Cv = acc[0];
     double g2[1] = {0.0};
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             double gp = (V3(C_.s.F0, c, i) - V3(C_.s.F1, c, i)) / D - Cv * V3(C_.s.N, c, i);
@@ -250,7 +258,7 @@ __device__ void dimer_rot_eval(const Ctx &C_, WalkerState &w, double *sh)
     }
     w.t1 = t1;
     double ct = cos(t1), st = sin(t1);
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         double pr[3];
         for (int c = 0; c < 3; ++c) {
@@ -271,7 +279,7 @@ __device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
     const double D = p.dimerDisplacement;
     double acc[2] = {0.0, 0.0};  // C*D, F0.n
     double fm2 = 0.0;
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         double q = 0.0;
         for (int c = 0; c < 3; ++c) {
@@ -293,7 +301,7 @@ __device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
     }
     const bool convex = Cv > 0.0;
     double a3[3] = {0, 0, 0};  // Fe.V, |V|^2, |Fe|^2
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             double Nn = V3(C_.s.N, c, i);
@@ -320,7 +328,7 @@ __device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
     }
     const double dt = w.dt, fact = dt * p.timeStepUnit;
     double dm2 = 0.0;
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         double q = 0.0;
         for (int c = 0; c < 3; ++c) {
@@ -336,7 +344,7 @@ __device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
     dm2 = bmax(dm2, sh);
     double dmax = sqrt(dm2);
     double sc = dmax > p.maxStep ? p.maxStep / dmax : 1.0;
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             double v = V3(C_.s.V, c, i) * sc;
@@ -362,7 +370,7 @@ __device__ void dimer_logic(const Ctx &C_, WalkerState &w, double *sh)
     const double D = p.dimerDisplacement;
     w.fcalls += 1;
     if (w.state == ST_D_F0) {
-        FOR_ATOMS(i)
+        FOR_ACTIVE(i)
         {
             for (int c = 0; c < 3; ++c) V3(C_.s.F0, c, i) = C_.f(c, i);
         }
@@ -373,7 +381,7 @@ __device__ void dimer_logic(const Ctx &C_, WalkerState &w, double *sh)
         return;
     }
     if (w.state == ST_D_F1) {
-        FOR_ATOMS(i)
+        FOR_ACTIVE(i)
         {
             for (int c = 0; c < 3; ++c) V3(C_.s.F1, c, i) = C_.f(c, i);
         }
@@ -386,7 +394,7 @@ __device__ void dimer_logic(const Ctx &C_, WalkerState &w, double *sh)
     // extrapolation of the end force to the optimum angle
     const double t1 = w.t1, ct1 = cos(t1), st1 = sin(t1);
     double acc[1] = {0.0};
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             double nt = V3(C_.s.N, c, i) * ct1 + V3(C_.s.T, c, i) * st1;
@@ -405,7 +413,7 @@ __device__ void dimer_logic(const Ctx &C_, WalkerState &w, double *sh)
     double s2 = 1.0 - cos(th) - sin(th) * tan(0.5 * t1);
     double cth = cos(th), sth = sin(th);
     double nn[1] = {0.0};
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             double F1 = V3(C_.s.F1, c, i) * s0 + C_.f(c, i) * s1 + V3(C_.s.F0, c, i) * s2;
@@ -417,7 +425,7 @@ __device__ void dimer_logic(const Ctx &C_, WalkerState &w, double *sh)
     }
     bsum<1>(nn, sh);
     double inv = 1.0 / sqrt(nn[0]);
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) V3(C_.s.N, c, i) *= inv;
     }
@@ -484,7 +492,11 @@ __device__ void tridiag_values(int n, double *d, double *e)
 
 // FP64 check of a converged saddle: max force on the free atoms, energy,
 // and the two lowest Ritz values of the finite-difference Hessian from a
-// Lanczos run started on the dimer mode (full re-orthogonalisation)
+// Lanczos run started on the dimer mode (full re-orthogonalisation).  The
+// Hessian products are central differences (two force calls per vector):
+// the one-sided product of the reference's ART Lanczos carries an
+// O(delta E''') bias that, for vectors concentrated on stiff contacts, can
+// put a Ritz value below the true lowest curvature.
 __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
 {
     const double delta = C_.s.verify_delta;
@@ -493,19 +505,27 @@ __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
     double *lz = C_.s.lz + (size_t)blockIdx.x * 3 * AKMC_LZ_MAX;
     w.fcalls += 1;
     if (w.state == ST_V_F0) {
+        double acc[1] = {0.0};
         double fm2 = 0.0;
-        FOR_ATOMS(i)
+        FOR_ACTIVE(i)
         {
             double q = 0.0;
             for (int c = 0; c < 3; ++c) {
                 double F = C_.f(c, i);
                 V3(C_.s.F0, c, i) = F;
                 q += F * F;
-                V3(C_.s.Q, c, i) = V3(C_.s.N, c, i);
+                const double n = V3(C_.s.N, c, i);
+                acc[0] += n * n;
             }
             fm2 = fmax(fm2, q);
         }
         fm2 = bmax(fm2, sh);
+        bsum<1>(acc, sh);
+        const double inorm = 1.0 / sqrt(acc[0]);
+        FOR_ACTIVE(i)
+        {
+            for (int c = 0; c < 3; ++c) V3(C_.s.Q, c, i) = V3(C_.s.N, c, i) * inorm;
+        }
         w.fmax64 = sqrt(fm2);
         w.e64 = C_.b.out[blockIdx.x].total;
         w.lk = 0;
@@ -514,13 +534,25 @@ __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
         write_probe(C_, C_.s.Q, delta);
         return;
     }
+    if (w.state == ST_V_LZ) {
+        // F(X + delta q_k) kept in F1 while the force at X - delta q_k is taken
+        FOR_ACTIVE(i)
+        {
+            for (int c = 0; c < 3; ++c) V3(C_.s.F1, c, i) = C_.f(c, i);
+        }
+        w.state = ST_V_LZM;
+        __syncthreads();
+        write_probe(C_, C_.s.Q + w.lk * A3, -delta);
+        return;
+    }
     const int k = w.lk;
     const double *Qk = C_.s.Q + k * A3;
     double a1[1] = {0.0};
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
-            double u = -(C_.f(c, i) - V3(C_.s.F0, c, i)) / delta;
+            // central difference: H q = -(F(X + d q) - F(X - d q)) / 2d + O(d^2)
+            double u = -(V3(C_.s.F1, c, i) - C_.f(c, i)) / (2.0 * delta);
             V3(C_.s.T, c, i) = u;
             a1[0] += V3(Qk, c, i) * u;
         }
@@ -529,7 +561,7 @@ __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
     const double alpha = a1[0];
     const double bprev = k > 0 ? lz[AKMC_LZ_MAX + k - 1] : 0.0;
     const double *Qp = k > 0 ? C_.s.Q + (k - 1) * A3 : nullptr;
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             double r = V3(C_.s.T, c, i) - alpha * V3(Qk, c, i);
@@ -540,18 +572,18 @@ __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
     for (int j = 0; j <= k; ++j) {
         const double *Qj = C_.s.Q + j * A3;
         double dj[1] = {0.0};
-        FOR_ATOMS(i)
+        FOR_ACTIVE(i)
         {
             for (int c = 0; c < 3; ++c) dj[0] += V3(C_.s.T, c, i) * V3(Qj, c, i);
         }
         bsum<1>(dj, sh);
-        FOR_ATOMS(i)
+        FOR_ACTIVE(i)
         {
             for (int c = 0; c < 3; ++c) V3(C_.s.T, c, i) -= dj[0] * V3(Qj, c, i);
         }
     }
     double bb[1] = {0.0};
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) bb[0] += V3(C_.s.T, c, i) * V3(C_.s.T, c, i);
     }
@@ -584,7 +616,7 @@ __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
     }
     double *Qn = C_.s.Q + (size_t)(k + 1) * A3;
     const double ib = 1.0 / beta;
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         double pr[3];
         for (int c = 0; c < 3; ++c) {
@@ -594,6 +626,7 @@ __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
         }
         C_.probe(i, pr[0], pr[1], pr[2]);
     }
+    w.state = ST_V_LZ;
 }
 
 // ------------------------------------------------------------------- ART
@@ -636,7 +669,7 @@ __device__ void art_lanczos_begin(const Ctx &C_, WalkerState &w, double *sh)
 {
     const ARTParamsDev &p = C_.s.ap;
     double nn[1] = {0.0};
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             V3(C_.s.F0, c, i) = C_.f(c, i);
@@ -651,7 +684,7 @@ __device__ void art_lanczos_begin(const Ctx &C_, WalkerState &w, double *sh)
     w.lbeta = beta;
     w.lconv = 0;
     w.lam = 1.0e6;  // lowestEigenValue before the first diagonalisation (ART.cpp:157)
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         double pr[3];
         for (int c = 0; c < 3; ++c) {
@@ -677,7 +710,7 @@ __device__ void art_after_lanczos(const Ctx &C_, WalkerState &w, double *sh)
     }
     const bool perp = w.lam >= p.maxEigenvalue;
     const double dot = w.dotfv;
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             double v = V3(C_.s.N, c, i);
@@ -713,7 +746,7 @@ __device__ void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
     const double *Qp = i > 0 ? C_.s.Q + (i - 1) * A3 : nullptr;
     const double beta = w.lbeta;
     double a1[1] = {0.0};
-    FOR_ATOMS(k)
+    FOR_ACTIVE(k)
     {
         for (int c = 0; c < 3; ++c) {
             double u = -(C_.f(c, k) - V3(C_.s.F0, c, k)) / p.deltaX;
@@ -725,7 +758,7 @@ __device__ void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
     bsum<1>(a1, sh);
     const double alpha = a1[0];
     double bb[1] = {0.0};
-    FOR_ATOMS(k)
+    FOR_ACTIVE(k)
     {
         for (int c = 0; c < 3; ++c) {
             double r = V3(C_.s.T, c, k) - alpha * V3(Qi, c, k);
@@ -765,7 +798,7 @@ __device__ void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
     if (next < nl && !w.lconv) {
         double *Qn = C_.s.Q + next * A3;
         const double ib = 1.0 / bnew;
-        FOR_ATOMS(k)
+        FOR_ACTIVE(k)
         {
             double pr[3];
             for (int c = 0; c < 3; ++c) {
@@ -791,7 +824,7 @@ __device__ void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
     __syncthreads();
     const double *y = lz + 2 * AKMC_LZ_MAX;
     double nn[1] = {0.0};
-    FOR_ATOMS(k)
+    FOR_ACTIVE(k)
     {
         for (int c = 0; c < 3; ++c) {
             double v = 0.0;
@@ -803,7 +836,7 @@ __device__ void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
     bsum<1>(nn, sh);
     const double inv = 1.0 / sqrt(nn[0]);
     double dd[1] = {0.0};
-    FOR_ATOMS(k)
+    FOR_ACTIVE(k)
     {
         for (int c = 0; c < 3; ++c) {
             double v = V3(C_.s.N, c, k) * inv;
@@ -832,7 +865,7 @@ __device__ void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh)
     const double dot = w.dotfv;
     double acc[3] = {0, 0, 0};  // |F_par|^2, |F_perp|^2, |v|^2
     double mx[3] = {0, 0, 0};
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             double nv = V3(C_.s.N, c, i);
@@ -856,7 +889,7 @@ __device__ void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh)
     const double vn = sqrt(acc[2]);
     const double fsel = perp ? w.fnperp : w.fnpar;
     double P[1] = {0.0};
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         for (int c = 0; c < 3; ++c) {
             double nv = V3(C_.s.N, c, i);
@@ -879,7 +912,7 @@ __device__ void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh)
         w.alpha = p.alphaStart;
     }
     const double dt = w.dt;
-    FOR_ATOMS(i)
+    FOR_ACTIVE(i)
     {
         const bool fr = C_.s.frozen[i] != 0;
         for (int c = 0; c < 3; ++c) {
@@ -911,7 +944,7 @@ __global__ void __launch_bounds__(kBlock) k_search_logic(BatchDev b, SearchDev s
     __syncthreads();
     if (ws.kind == WK_IDLE || ws.state == ST_DONE) return;
     const WalkerDev W = b.walkers[wid];
-    Ctx C_{b, s, W.off, W.n, b.A};
+    Ctx C_{b, s, W.off, W.n, b.A, s.nact[wid]};
     // every thread keeps a private copy of the walker scalars; all threads
     // take the same branches (the scalars only change through block-wide
     // reductions), thread 0 writes them back
@@ -930,8 +963,8 @@ __global__ void __launch_bounds__(kBlock) k_probe_init(BatchDev b, SearchDev s)
     const WalkerState w = s.ws[wid];
     if (w.kind == WK_IDLE || w.state == ST_DONE) return;
     const WalkerDev W = b.walkers[wid];
-    Ctx C_{b, s, W.off, W.n, b.A};
-    write_probe(C_, nullptr, 0.0);
+    Ctx C_{b, s, W.off, W.n, b.A, s.nact[wid]};
+    FOR_ATOMS(i) C_.probe(i, V3(s.X, 0, i), V3(s.X, 1, i), V3(s.X, 2, i));
 }
 
 }  // namespace

@@ -173,3 +173,48 @@ def test_device_fp64_verification(lib, oracle, c1):
     assert st["lambda1"] < 0 < st["lambda2"]
     assert abs(st["lambda1"] - st["curvature32"]) < 0.05 * abs(st["curvature32"])
     assert abs(st["energy32"] - st["energy64"]) < 0.01
+
+
+def _oracle_hessian_eigs(oracle, of, pos, types, lat, act, d=0.005):
+    """Lowest eigenvalues of the FP64 central-difference Hessian over the
+    free atoms, from the CPU oracle's forces."""
+    dofs = [(k, a) for a in act for k in range(3)]
+    H = np.zeros((len(dofs), len(dofs)))
+    for col, (k, a) in enumerate(dofs):
+        p = pos.copy()
+        p[k, a] += d
+        fp = oracle.sc_forces(of, p, types, lat)["force"]
+        p[k, a] -= 2 * d
+        fm = oracle.sc_forces(of, p, types, lat)["force"]
+        H[:, col] = [-(fp[kk, aa] - fm[kk, aa]) / (2 * d) for (kk, aa) in dofs]
+    return np.linalg.eigvalsh(0.5 * (H + H.T))
+
+
+def test_verification_curvatures_match_exact_hessian(lib, oracle):
+    """Batched dimer searches on the 16 smallest walker cells: nearly every
+    converged saddle passes the device FP64 check, and for sampled saddles
+    the device's two lowest Lanczos curvatures agree with the oracle's exact
+    FP64 Hessian (lambda1 to 1%; lambda2 is a Ritz upper bound, so it may
+    only lie above the exact second eigenvalue)."""
+    from akmc_amd import search as S
+    cells = W.batched_sc_cells(n_walkers=512, seed=3)
+    cells = [cells[i] for i in np.argsort([c.n for c in cells])[:16]]
+    text = lib.sc_potentials_text(_pots(), 2 * CU["a"])
+    camp = S.SaddleCampaign(text, cells, params=lib.dimer_defaults(maxParIter=400), seed=5)
+    camp.relax_basins()
+    camp.begin()
+    st = camp.advance(300, sync_every=25)
+    print(f"{st.converged} converged, {st.verified} verified, failures {st.failures()}", flush=True)
+    assert st.converged >= 16
+    assert st.verified >= 0.9 * st.converged
+    of = oracle.sc_field(["Cu"], _pots(), 2 * CU["a"])
+    for r in [r for r in st.records if r.barrier64 > 0][:2]:
+        c = cells[r.walker]
+        x = camp.basins[r.walker].copy()
+        x[:, r.active] += r.active_disp
+        ev = _oracle_hessian_eigs(oracle, of, x, c.types, c.lat, r.active)
+        l1, l2 = r.lanczos
+        print(f"walker {r.walker}: device {l1:.4f} {l2:.4f}, exact {ev[0]:.4f} {ev[1]:.4f}", flush=True)
+        assert ev[0] < 0 < ev[1]
+        assert abs(l1 - ev[0]) < 0.01 * abs(ev[0])
+        assert l2 > ev[1] - 1e-3 and l2 > 0
