@@ -10,6 +10,9 @@
 #define AKMC_NB_JMASK 0xFFFFFu
 #define AKMC_NB_ZERO_SHIFT (0x777u << 20)
 #define AKMC_MAX_WALKER_ATOMS (1 << 20)
+// cells per axis are capped so a walker's grid fits one workgroup's LDS
+#define AKMC_MAX_CELLS_AXIS 16
+#define AKMC_MAX_WALKER_CELLS (AKMC_MAX_CELLS_AXIS * AKMC_MAX_CELLS_AXIS * AKMC_MAX_CELLS_AXIS)
 
 // Sutton-Chen pair parameters, FP32 for the fast path and FP64 for the
 // verification path
@@ -85,7 +88,8 @@ struct BatchDev {
     int *nnear;           // leading list entries inside the near shell
     int *rebuild;         // per walker flag
     int *overflow;        // sticky capacity flag
-    int *cell_count, *cell_start, *cell_atoms, *atom_cell, *atom_slot;
+    int *cell_count, *cell_start, *cell_atoms, *atom_cell;
+    const int *cell_wk;   // cell -> walker
     double *rho;          // SC density
     float *inv_sqrt_rho;  // SC 1/sqrt(rho), -1 when rho < 1e-8
     double *e_a, *e_b;    // per-atom energies (SC: many, pair; ionic: real, vdw)

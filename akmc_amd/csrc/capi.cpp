@@ -384,7 +384,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
         h.L[2] = lw[8];
         for (int a = 0; a < 3; ++a) {
             h.invL[a] = 1.0 / h.L[a];
-            h.nc[a] = std::max(1, (int)std::floor(h.L[a] / rl));
+            h.nc[a] = std::min(AKMC_MAX_CELLS_AXIS, std::max(1, (int)std::floor(h.L[a] / rl)));
             h.shells[a] = (int)(rl / h.L[a]) + 1;
             if (h.shells[a] > 7) return fail(AKMC_ECAPACITY, "walker %d: more than 7 image shells", w);
         }
@@ -460,7 +460,6 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     TRY(dalloc(b, &d.cell_start, ncells));
     TRY(dalloc(b, &d.cell_atoms, A));
     TRY(dalloc(b, &d.atom_cell, A));
-    TRY(dalloc(b, &d.atom_slot, A));
     TRY(dalloc(b, &d.rho, A));
     TRY(dalloc(b, &d.inv_sqrt_rho, A));
     TRY(dalloc(b, &d.e_a, A));
@@ -501,6 +500,17 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
         for (int i = 0; i < b->hw[w].n; ++i) hwk[b->hw[w].off + i] = w;
     HIPCHK(hipMemcpy(wd, b->hw.data(), W * sizeof(WalkerDev), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(wk, hwk.data(), A * sizeof(int), hipMemcpyHostToDevice));
+    {
+        std::vector<int> hcw(ncells);
+        for (int w = 0; w < W; ++w) {
+            const WalkerDev &h = b->hw[w];
+            for (int c = 0; c < h.nc[0] * h.nc[1] * h.nc[2]; ++c) hcw[h.cell_off + c] = w;
+        }
+        int *cw;
+        TRY(dalloc(b, &cw, ncells));
+        HIPCHK(hipMemcpy(cw, hcw.data(), ncells * sizeof(int), hipMemcpyHostToDevice));
+        d.cell_wk = cw;
+    }
     HIPCHK(hipMemset(d.rebuild, 0, W * sizeof(int)));
     {
         std::vector<int> one(W, f->fp64 ? 2 : 1);

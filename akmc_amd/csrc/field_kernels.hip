@@ -118,15 +118,6 @@ __global__ void k_ref_positions(BatchDev b)
     b.zr[i] = b.z[i];
 }
 
-__global__ void k_cell_clear(BatchDev b)
-{
-    int w = blockIdx.x;
-    if (!b.rebuild[w]) return;
-    const WalkerDev W = b.walkers[w];
-    int nc = W.nc[0] * W.nc[1] * W.nc[2];
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) b.cell_count[W.cell_off + c] = 0;
-}
-
 __device__ inline int cell_coord(double x, double invL, int nc)
 {
     double s = x * invL;
@@ -135,78 +126,75 @@ __device__ inline int cell_coord(double x, double invL, int nc)
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
-__global__ void k_cell_assign(BatchDev b)
+// Cell grid of one walker per workgroup (walkers whose list is rebuilt):
+// counts, exclusive scan and a stable placement of the atoms in index order,
+// so every cell lists its atoms by ascending index and the neighbour lists
+// (and the FP64 sums that follow them) do not depend on scheduling.  The
+// placement takes the atoms kBlock at a time; an atom's slot is the cell's
+// running fill plus its rank among earlier atoms of the same cell in the
+// chunk (an LDS count).
+__global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.A) return;
-    int w = walker_of(b, i);
-    if (!b.rebuild[w]) return;
-    const WalkerDev W = b.walkers[w];
-    int ca = cell_coord(b.x[i], W.invL[0], W.nc[0]);
-    int cb = cell_coord(b.y[i], W.invL[1], W.nc[1]);
-    int cc = cell_coord(b.z[i], W.invL[2], W.nc[2]);
-    int cid = W.cell_off + (ca * W.nc[1] + cb) * W.nc[2] + cc;
-    b.atom_cell[i] = cid;
-    b.atom_slot[i] = atomicAdd(&b.cell_count[cid], 1);
-}
-
-// exclusive scan of a walker's cell counts, one workgroup per walker
-__global__ void k_cell_scan(BatchDev b)
-{
-    __shared__ int part[kBlock];
+    __shared__ int fill[AKMC_MAX_WALKER_CELLS];
+    __shared__ int chunk[kBlock];
     __shared__ int carry;
-    int w = blockIdx.x;
+    const int w = blockIdx.x;
     if (!b.rebuild[w]) return;
     const WalkerDev W = b.walkers[w];
-    int nc = W.nc[0] * W.nc[1] * W.nc[2];
-    if (threadIdx.x == 0) carry = W.off;
+    const int nc = W.nc[0] * W.nc[1] * W.nc[2];
+    const int tid = threadIdx.x;
+    for (int c = tid; c < nc; c += kBlock) fill[c] = 0;
     __syncthreads();
-    for (int base = 0; base < nc; base += blockDim.x) {
-        int c = base + threadIdx.x;
-        int v = (c < nc) ? b.cell_count[W.cell_off + c] : 0;
-        part[threadIdx.x] = v;
+    for (int l = tid; l < W.n; l += kBlock) {
+        const int i = W.off + l;
+        const int ca = cell_coord(b.x[i], W.invL[0], W.nc[0]);
+        const int cb = cell_coord(b.y[i], W.invL[1], W.nc[1]);
+        const int cc = cell_coord(b.z[i], W.invL[2], W.nc[2]);
+        const int lc = (ca * W.nc[1] + cb) * W.nc[2] + cc;
+        b.atom_cell[i] = lc;
+        atomicAdd(&fill[lc], 1);
+    }
+    if (tid == 0) carry = W.off;
+    __syncthreads();
+    // exclusive scan of the counts; fill[c] becomes the cell's next free slot
+    for (int base = 0; base < nc; base += kBlock) {
+        const int c = base + tid;
+        const int v = c < nc ? fill[c] : 0;
+        chunk[tid] = v;
         __syncthreads();
-        for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-            int t = (threadIdx.x >= (unsigned)o) ? part[threadIdx.x - o] : 0;
+        for (int o = 1; o < kBlock; o <<= 1) {
+            const int t = tid >= o ? chunk[tid - o] : 0;
             __syncthreads();
-            part[threadIdx.x] += t;
+            chunk[tid] += t;
             __syncthreads();
         }
-        if (c < nc) b.cell_start[W.cell_off + c] = carry + part[threadIdx.x] - v;
+        if (c < nc) {
+            const int start = carry + chunk[tid] - v;
+            b.cell_count[W.cell_off + c] = v;
+            b.cell_start[W.cell_off + c] = start;
+            fill[c] = start;
+        }
         __syncthreads();
-        if (threadIdx.x == blockDim.x - 1) carry += part[threadIdx.x];
+        if (tid == kBlock - 1) carry += chunk[tid];
         __syncthreads();
     }
-}
-
-__global__ void k_cell_scatter(BatchDev b)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.A) return;
-    int w = walker_of(b, i);
-    if (!b.rebuild[w]) return;
-    b.cell_atoms[b.cell_start[b.atom_cell[i]] + b.atom_slot[i]] = i;
-}
-
-// order each cell's atoms by index so the lists (and the FP64 sums that
-// follow them) do not depend on the atomic slot order
-__global__ void k_cell_sort(BatchDev b)
-{
-    int w = blockIdx.x;
-    if (!b.rebuild[w]) return;
-    const WalkerDev W = b.walkers[w];
-    int nc = W.nc[0] * W.nc[1] * W.nc[2];
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-        int s = b.cell_start[W.cell_off + c], n = b.cell_count[W.cell_off + c];
-        int *a = b.cell_atoms + s;
-        for (int k = 1; k < n; ++k) {
-            int v = a[k], m = k - 1;
-            while (m >= 0 && a[m] > v) {
-                a[m + 1] = a[m];
-                --m;
-            }
-            a[m + 1] = v;
+    for (int base = 0; base < W.n; base += kBlock) {
+        const int l = base + tid;
+        const int c = l < W.n ? b.atom_cell[W.off + l] : -1;
+        chunk[tid] = c;
+        __syncthreads();
+        int slot = 0;
+        if (c >= 0) {
+            int rank = 0;
+            for (int t = 0; t < tid; ++t) rank += chunk[t] == c;
+            slot = fill[c] + rank;
         }
+        __syncthreads();  // every slot is read before the fills move
+        if (c >= 0) {
+            b.cell_atoms[slot] = W.off + l;
+            atomicAdd(&fill[c], 1);
+        }
+        __syncthreads();
     }
 }
 
@@ -217,22 +205,24 @@ __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
     ++cnt;
 }
 
-// minimum-image list from the cell grid: all j != i with min-image
-// distance <= rc + skin; axes with fewer than 3 cells visit every cell once
-__global__ void k_nbr_build_cells(BatchDev b)
+// Minimum-image Verlet list, one workgroup per cell of a walker whose list
+// is rebuilt: the home cell's atoms (one per thread) scan the neighbouring
+// cells (c-1..c+1 per axis; every cell once along axes with fewer than 3
+// cells), whose atoms are staged through LDS kBlock at a time and read by
+// all threads alike.  Entries are every j != i with min-image distance <=
+// rc + skin, in cell order then ascending index; with a near shell (ionic
+// fields) the entries inside it come first.
+__global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.A) return;
-    int w = walker_of(b, i);
+    __shared__ double sx[kBlock], sy[kBlock], sz[kBlock];
+    __shared__ int sj[kBlock];
+    const int cg = blockIdx.x;
+    const int w = b.cell_wk[cg];
     if (!b.rebuild[w]) return;
     const WalkerDev W = b.walkers[w];
-    const double rl2 = b.field->rlist2;
-    double xi = b.x[i], yi = b.y[i], zi = b.z[i];
-    int ca = cell_coord(xi, W.invL[0], W.nc[0]);
-    int cb = cell_coord(yi, W.invL[1], W.nc[1]);
-    int cc = cell_coord(zi, W.invL[2], W.nc[2]);
+    const int lc = cg - W.cell_off;
+    const int cen[3] = {lc / (W.nc[1] * W.nc[2]), (lc / W.nc[2]) % W.nc[1], lc % W.nc[2]};
     int lo[3], hi[3];
-    const int cen[3] = {ca, cb, cc};
     for (int a = 0; a < 3; ++a) {
         if (W.nc[a] >= 3) {
             lo[a] = cen[a] - 1;
@@ -242,35 +232,57 @@ __global__ void k_nbr_build_cells(BatchDev b)
             hi[a] = W.nc[a] - 1;
         }
     }
-    int cnt = 0, nnear = 0;
+    const double rl2 = b.field->rlist2;
     const double rn2 = b.field->rnear2;
-    // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
-    for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
-        for (int pa = lo[0]; pa <= hi[0]; ++pa) {
-            int qa = (pa + W.nc[0]) % W.nc[0];
-            for (int pb = lo[1]; pb <= hi[1]; ++pb) {
-                int qb = (pb + W.nc[1]) % W.nc[1];
-                for (int pc = lo[2]; pc <= hi[2]; ++pc) {
-                    int qc = (pc + W.nc[2]) % W.nc[2];
-                    int cid = W.cell_off + (qa * W.nc[1] + qb) * W.nc[2] + qc;
-                    int s = b.cell_start[cid], n = b.cell_count[cid];
-                    for (int t = 0; t < n; ++t) {
-                        int j = b.cell_atoms[s + t];
-                        if (j == i) continue;
-                        double dx = xi - b.x[j], dy = yi - b.y[j], dz = zi - b.z[j];
-                        min_image(dx, dy, dz, W);
-                        double r2 = dx * dx + dy * dy + dz * dz;
-                        bool near = r2 <= rn2;
-                        if (r2 <= rl2 && (pass == 0 ? near : !near))
-                            push_nbr(b, i, cnt, uint32_t(j - W.off) | AKMC_NB_ZERO_SHIFT);
+    const int s0 = b.cell_start[cg], n0 = b.cell_count[cg];
+    for (int hb = 0; hb < n0; hb += kBlock) {
+        const bool have = hb + (int)threadIdx.x < n0;
+        const int i = have ? b.cell_atoms[s0 + hb + threadIdx.x] : 0;
+        const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
+        int cnt = 0, nnear = 0;
+        // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
+        for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
+            for (int pa = lo[0]; pa <= hi[0]; ++pa) {
+                const int qa = (pa + W.nc[0]) % W.nc[0];
+                for (int pb = lo[1]; pb <= hi[1]; ++pb) {
+                    const int qb = (pb + W.nc[1]) % W.nc[1];
+                    for (int pc = lo[2]; pc <= hi[2]; ++pc) {
+                        const int qc = (pc + W.nc[2]) % W.nc[2];
+                        const int cid = W.cell_off + (qa * W.nc[1] + qb) * W.nc[2] + qc;
+                        const int s = b.cell_start[cid], n = b.cell_count[cid];
+                        for (int tb = 0; tb < n; tb += kBlock) {
+                            const int m = min(kBlock, n - tb);
+                            __syncthreads();
+                            if ((int)threadIdx.x < m) {
+                                const int j = b.cell_atoms[s + tb + threadIdx.x];
+                                sj[threadIdx.x] = j;
+                                sx[threadIdx.x] = b.x[j];
+                                sy[threadIdx.x] = b.y[j];
+                                sz[threadIdx.x] = b.z[j];
+                            }
+                            __syncthreads();
+                            if (!have) continue;
+                            for (int t = 0; t < m; ++t) {
+                                const int j = sj[t];
+                                if (j == i) continue;
+                                double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
+                                min_image(dx, dy, dz, W);
+                                const double r2 = dx * dx + dy * dy + dz * dz;
+                                const bool near = r2 <= rn2;
+                                if (r2 <= rl2 && (pass == 0 ? near : !near))
+                                    push_nbr(b, i, cnt, uint32_t(j - W.off) | AKMC_NB_ZERO_SHIFT);
+                            }
+                        }
                     }
                 }
             }
+            if (pass == 0) nnear = cnt;
         }
-        if (pass == 0) nnear = cnt;
+        if (have) {
+            b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
+            b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
+        }
     }
-    b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
-    b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
 }
 
 // explicit-image list (fields without the image convention): every (j, S)
@@ -854,12 +866,8 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
     k_nbr_check<<<gA, kBlock, 0, s>>>(b);
     k_ref_positions<<<gA, kBlock, 0, s>>>(b);
     if (hf.min_image) {
-        k_cell_clear<<<b.W, kBlock, 0, s>>>(b);
-        k_cell_assign<<<gA, kBlock, 0, s>>>(b);
-        k_cell_scan<<<b.W, kBlock, 0, s>>>(b);
-        k_cell_scatter<<<gA, kBlock, 0, s>>>(b);
-        k_cell_sort<<<b.W, kBlock, 0, s>>>(b);
-        k_nbr_build_cells<<<gA, kBlock, 0, s>>>(b);
+        k_cell_bin<<<b.W, kBlock, 0, s>>>(b);
+        k_nbr_build_cells<<<b.ncells_total, kBlock, 0, s>>>(b);
     } else {
         k_nbr_build_images<<<gA, kBlock, 0, s>>>(b);
     }

@@ -63,6 +63,9 @@ class CampaignStats:
     force_calls: int = 0
     micro_steps: int = 0
     search_time: float = 0.0
+    device_time: float = 0.0   # inside akmc_batch_run (device work + its host checks)
+    harvest_time: float = 0.0  # reading ended searches
+    refill_time: float = 0.0   # proposals and restarts
     max_barrier_err: float = 0.0
     records: list = field(default_factory=list)
     # converged searches that failed one verification test (a search can fail several)
@@ -83,6 +86,9 @@ class CampaignStats:
         for k in self._COUNTS:
             setattr(self, k, getattr(self, k) + getattr(o, k))
         self.search_time += o.search_time
+        self.device_time += o.device_time
+        self.harvest_time += o.harvest_time
+        self.refill_time += o.refill_time
         self.max_barrier_err = max(self.max_barrier_err, o.max_barrier_err)
         self.records.extend(o.records)
 
@@ -226,10 +232,16 @@ class SaddleCampaign:
         steps = 0
         while steps < micro_steps and self.pending:
             k = min(sync_every, micro_steps - steps)
+            t1 = time.perf_counter()
             steps += self.batch.run(k, sync_every=k)
+            t2 = time.perf_counter()
             done = self.harvest(stats)
+            t3 = time.perf_counter()
             if refill and done:
                 self.start(done)
+            stats.device_time += t2 - t1
+            stats.harvest_time += t3 - t2
+            stats.refill_time += time.perf_counter() - t3
         stats.search_time = time.perf_counter() - t0
         stats.micro_steps = steps
         return stats

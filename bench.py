@@ -184,6 +184,11 @@ def main():
     camp = S.SaddleCampaign(text, cells, params=dimer, seed=args.seed)
     if args.eager:
         camp.batch.set_graphs(False)
+    # kernel variant chosen on the seeded cells, before the relaxation that uses it
+    camp.batch.forces()
+    variant, variant_ms = camp.batch.autotune(10)
+    print(f"[bench] autotune (lanes, unroll): {variant} (ms per evaluation "
+          f"{ {k: round(v, 3) for k, v in variant_ms.items()} })", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     ok = camp.relax_basins()
     t_relax = time.perf_counter() - t0
@@ -193,9 +198,6 @@ def main():
     # ---- roofline of the force kernels at the basins (lists settled)
     camp.batch.upload([W.Cell(p, c.types, c.lat) for p, c in zip(camp.basins, cells)])
     camp.batch.forces()
-    variant, variant_ms = camp.batch.autotune(10)
-    print(f"[bench] autotune (lanes, unroll): {variant} (ms per evaluation "
-          f"{ {k: round(v, 3) for k, v in variant_ms.items()} })", file=sys.stderr, flush=True)
     ph = camp.batch.time_phases(20)
     P = ordered_pairs(cells, camp.basins, rc)
     Ptot = int(P.sum())
@@ -216,7 +218,8 @@ def main():
         st = camp.advance(args.window, sync_every=25)
         print(f"[bench] window {r}: {st.searches} ended, {st.converged} converged, {st.verified} verified, "
               f"{st.distinct} distinct, {st.micro_steps} micro-steps in {st.search_time:.2f}s, "
-              f"max barrier err {st.max_barrier_err:.2e}; failed tests {st.failures()}", file=sys.stderr, flush=True)
+              f"max barrier err {st.max_barrier_err:.2e}; failed tests {st.failures()}; device {st.device_time:.2f}s "
+              f"harvest {st.harvest_time:.2f}s refill {st.refill_time:.2f}s", file=sys.stderr, flush=True)
         if r >= args.warmup:
             t_timed += st.search_time
             tot.add(st)

@@ -59,8 +59,11 @@ def test_cpp_field_relax_dimer_art(tmp_path, oracle):
     dv = W.min_image(basin - c.vacancies[0][:, None], c.lat)
     frozen = (np.sqrt((dv * dv).sum(0)) >= 6.0).astype(np.int32)
     nb = W.neighbours_of_site(W.Cell(basin, c.types, c.lat), c.vacancies[0], 12)
-    disp, mode = W.local_displacement(W.Cell(basin, c.types, c.lat), nb, 0.1, 3)
-    for action in ("dimer", "art"):
+    # the ART start is the one the batched ART test uses: from the dimer's
+    # start (seed 3) the oracle's own ART needs 436 iterations and the FP32
+    # device trajectory can tip either way on summation-order round-off
+    for action, seed in (("dimer", 3), ("art", 5)):
+        disp, mode = W.local_displacement(W.Cell(basin, c.types, c.lat), nb, 0.1, seed)
         st, es, curv, sad, _ = _run(tmp_path, action, basin + disp, c.types, c.lat, frozen, mode)
         e64 = oracle.sc_energy(of, sad, c.types, c.lat)[0]
         print(f"C++ {action}: status {st} Ea {es - e1:.5f} (FP64 {e64 - e1:.5f}) curvature {curv:.4f}", flush=True)
