@@ -381,12 +381,25 @@ __device__ inline uint32_t self_entry(int i, const WalkerDev &W)
     return uint32_t(i - W.off) | (0x777u << 20);
 }
 
-// The neighbour loops below fetch U list entries, then the U neighbours'
-// positions (and densities), then evaluate the U pairs: U independent
-// gathers in flight per lane instead of one dependent chain per pair.  The
+// The neighbour loops below hold U list entries, fetch the next U entries
+// while gathering the current U neighbours' positions (and densities), then
+// evaluate the U pairs: U independent gathers per lane in flight together
+// with the next batch of the (streamed) list, instead of one dependent
+// chain per pair.  The
 // per-lane summation order is k ascending for every U (the FP32 variants
 // agree bitwise; FP64 ones to the last bit of FMA contraction); LPA variants
 // differ by summation order.
+
+// list entries k, k + LPA, ..., k + (U-1) LPA of atom i ("none" past the end)
+template <int LPA, int U>
+__device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, uint32_t none, uint32_t (&e)[U])
+{
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int kk = k + u * LPA;
+        e[u] = kk < cnt ? b.nbr[(size_t)kk * b.A + i] : none;
+    }
+}
 
 // pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
 // then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2.
@@ -409,13 +422,13 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
     const int cnt = b.nnb[i];
     const uint32_t none = self_entry(i, W);
     double rho = 0.0;
+    uint32_t en[U];
+    fetch_entries<LPA, U>(b, i, sub, cnt, none, en);
     for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
         uint32_t e[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = k0 + u * LPA;
-            e[u] = k < cnt ? b.nbr[(size_t)k * b.A + i] : none;
-        }
+        for (int u = 0; u < U; ++u) e[u] = en[u];
+        fetch_entries<LPA, U>(b, i, k0 + LPA * U, cnt, none, en);  // next batch in flight
         double4 pj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) pj[u] = b.p4[W.off + int(e[u] & AKMC_NB_JMASK)];
@@ -465,13 +478,13 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
     const uint32_t none = self_entry(i, W);
     double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    uint32_t en[U];
+    fetch_entries<LPA, U>(b, i, sub, cnt, none, en);
     for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
         uint32_t e[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = k0 + u * LPA;
-            e[u] = k < cnt ? b.nbr[(size_t)k * b.A + i] : none;
-        }
+        for (int u = 0; u < U; ++u) e[u] = en[u];
+        fetch_entries<LPA, U>(b, i, k0 + LPA * U, cnt, none, en);  // next batch in flight
         double4 pj[U];
         float isj[U];
         double rj[U];
