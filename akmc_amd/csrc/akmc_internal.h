@@ -73,6 +73,8 @@ struct WalkerOut {
 struct BatchDev {
     int A, W, maxnb, ncells_total, nk_total, G1, maxnk;
     int lpa;              // lanes per atom of the Sutton-Chen pair kernels (1, 2, 4)
+    int nb16;             // list entries are 16-bit local indices (minimum-image
+                          // many-body lists of walkers below 65536 atoms)
     int unroll;           // neighbour gathers in flight per lane (1, 4)
     const FieldDev *field;
     const WalkerDev *walkers;
@@ -104,6 +106,14 @@ struct BatchDev {
     WalkerOut *out;
     int *active;          // per walker: 0 skip, 1 FP32 pair terms, 2 FP64
 };
+
+// entry k of atom i's neighbour list in the 32-bit form (local index and
+// image shift), whichever width the batch stores
+__device__ inline uint32_t nbr_entry(const BatchDev &b, int k, int i)
+{
+    const size_t o = (size_t)k * b.A + i;
+    return b.nb16 ? (uint32_t(reinterpret_cast<const uint16_t *>(b.nbr)[o]) | AKMC_NB_ZERO_SHIFT) : b.nbr[o];
+}
 
 // launcher (field_kernels.hip): list maintenance + one force evaluation of
 // every walker; hf is the host copy of *b.field
@@ -173,6 +183,7 @@ struct SearchDev {
     const int *act;                   // [A]: walker w's free atoms (global ids) at act[off_w ..)
     const int *nact;                  // [W]: number of free atoms per walker
     WalkerState *ws;                  // [W]
+    unsigned long long *evals32;      // [W]: FP32 force evaluations consumed by the state machines
     RelaxParamsDev rp;
     DimerParamsDev dp;
     ARTParamsDev ap;

@@ -119,6 +119,13 @@ struct akmc_batch_s {
     BatchDev graph_d{};
     int graph_prec = 0;
     bool use_graphs = true;
+    // HIP events around the FP32 Sutton-Chen pass-2 launch of every
+    // micro-step of akmc_batch_run (kernel timer)
+    static constexpr int kTimerSteps = 64;
+    bool ktimer = false, graph_ktimer = false;
+    hipEvent_t kev[2][kTimerSteps] = {};
+    double kms = 0.0;
+    long long klaunches = 0;
     double *basin = nullptr;  // [3][A] saved search positions
     hipEvent_t evp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // host copy of the frozen mask and the per-walker free-atom lists
@@ -423,6 +430,11 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     d.maxnb = maxnb;
     d.lpa = 1;
     d.unroll = 1;
+    {
+        int nmax = 0;
+        for (int w = 0; w < W; ++w) nmax = std::max(nmax, natoms[w]);
+        d.nb16 = (f->kind == 0 && f->min_image && nmax <= 65535) ? 1 : 0;
+    }
     d.ncells_total = ncells;
     d.nk_total = nk;
     d.G1 = G1;
@@ -543,6 +555,9 @@ void batch_free(akmc_batch_t b)
         if (e) (void)hipEventDestroy(e);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (auto &row : b->kev)
+        for (auto &e : row)
+            if (e) (void)hipEventDestroy(e);
     if (b->hws_pin) (void)hipHostFree(b->hws_pin);
     if (b->ov_pin) (void)hipHostFree(b->ov_pin);
     delete b;
@@ -684,6 +699,8 @@ int search_alloc(akmc_batch_t b)
     int *act, *nact;
     TRY(dalloc(b, &act, b->A));
     TRY(dalloc(b, &nact, b->W));
+    TRY(dalloc(b, &b->s.evals32, b->W));
+    HIPCHK(hipMemset(b->s.evals32, 0, b->W * sizeof(unsigned long long)));
     b->s.act = act;
     b->s.nact = nact;
     b->hfrozen.assign(b->A, 0);
@@ -713,10 +730,20 @@ void drop_graph(akmc_batch_t b)
     b->graph_steps = 0;
 }
 
-void enqueue_micro_step(akmc_batch_t b)
+// one micro-step; step j of a run chunk records the kernel-timer events
+void enqueue_micro_step(akmc_batch_t b, int j)
 {
     int prec = batch_mode(b) | (b->s.verify_nl > 0 ? 2 : 0);
-    akmc_launch_forces_impl(b->d, b->f->host, prec, 0, b->stream);
+    if (b->ktimer && (prec & 1) && b->f->host.kind == 0) {
+        for (int ph = 0; ph < 2; ++ph) akmc_launch_phase(b->d, b->f->host, prec, 0, ph, b->stream);
+        (void)hipEventRecord(b->kev[0][j], b->stream);
+        akmc_launch_phase(b->d, b->f->host, 1, 0, 2, b->stream);
+        (void)hipEventRecord(b->kev[1][j], b->stream);
+        if (prec & 2) akmc_launch_phase(b->d, b->f->host, 2, 0, 2, b->stream);
+        akmc_launch_phase(b->d, b->f->host, prec, 0, 3, b->stream);
+    } else {
+        akmc_launch_forces_impl(b->d, b->f->host, prec, 0, b->stream);
+    }
     akmc_launch_search_logic(b->d, b->s, b->stream);
 }
 
@@ -824,7 +851,7 @@ int capture_graph(akmc_batch_t b, int steps)
 {
     drop_graph(b);
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < steps; ++k) enqueue_micro_step(b);
+    for (int k = 0; k < steps; ++k) enqueue_micro_step(b, k);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess) return fail(AKMC_EHIP, "graph capture failed: %s", hipGetErrorString(e));
@@ -834,6 +861,7 @@ int capture_graph(akmc_batch_t b, int steps)
     b->graph_args = b->s;
     b->graph_d = b->d;
     b->graph_prec = batch_mode(b);
+    b->graph_ktimer = b->ktimer;
     return AKMC_SUCCESS;
 }
 
@@ -1204,14 +1232,16 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
         // k micro-steps as one graph launch; the state kernels keep the
         // per-walker active flags current (0 when a walker ends, FP64 while
         // it verifies), so the host only reads states and the overflow flag
+        if (b->ktimer && k > akmc_batch_s::kTimerSteps)
+            return fail(AKMC_EINVAL, "sync_every is limited to %d while the kernel timer runs", akmc_batch_s::kTimerSteps);
         if (b->use_graphs) {
-            if (!b->gexec || b->graph_steps != k || b->graph_prec != batch_mode(b) ||
+            if (!b->gexec || b->graph_steps != k || b->graph_prec != batch_mode(b) || b->graph_ktimer != b->ktimer ||
                 std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0 ||
                 std::memcmp(&b->graph_d, &b->d, sizeof(BatchDev)) != 0)
                 TRY(capture_graph(b, k));
             HIPCHK(hipGraphLaunch(b->gexec, b->stream));
         } else {
-            for (int j = 0; j < k; ++j) enqueue_micro_step(b);
+            for (int j = 0; j < k; ++j) enqueue_micro_step(b, j);
             HIPCHK(hipGetLastError());
         }
         done += k;
@@ -1219,12 +1249,51 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
         HIPCHK(hipMemcpyAsync(b->ov_pin, b->d.overflow, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
         if (*b->ov_pin) return fail(AKMC_ECAPACITY, "neighbour list capacity %d exceeded", b->d.maxnb);
+        if (b->ktimer && batch_mode(b) == 1 && b->f->host.kind == 0)
+            for (int j = 0; j < k; ++j) {
+                float t = 0.0f;
+                HIPCHK(hipEventElapsedTime(&t, b->kev[0][j], b->kev[1][j]));
+                b->kms += t;
+                b->klaunches += 1;
+            }
         std::memcpy(b->hws.data(), b->hws_pin, b->W * sizeof(WalkerState));
         running = 0;
         for (int w = 0; w < b->W; ++w)
             running += b->hws[w].kind != WK_IDLE && b->hws[w].state != ST_DONE;
     }
     if (steps_done) *steps_done = done;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_kernel_timer(akmc_batch_t b, int enable)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    if (enable)
+        for (auto &row : b->kev)
+            for (auto &e : row)
+                if (!e) HIPCHK(hipEventCreate(&e));
+    b->ktimer = enable != 0;
+    b->kms = 0.0;
+    b->klaunches = 0;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_eval_counts(akmc_batch_t b, long long *counts)
+{
+    if (!b || !counts) return fail(AKMC_EINVAL, "bad arguments");
+    if (!b->search_alloc) {
+        for (int w = 0; w < b->W; ++w) counts[w] = 0;
+        return AKMC_SUCCESS;
+    }
+    HIPCHK(hipMemcpy(counts, b->s.evals32, b->W * sizeof(long long), hipMemcpyDeviceToHost));
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_kernel_time(akmc_batch_t b, double *ms, long long *launches)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    if (ms) *ms = b->kms;
+    if (launches) *launches = b->klaunches;
     return AKMC_SUCCESS;
 }
 

@@ -200,8 +200,13 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
 
 __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
 {
-    if (cnt < b.maxnb) b.nbr[(size_t)cnt * b.A + i] = e;
-    else *b.overflow = 1;
+    if (cnt < b.maxnb) {
+        const size_t o = (size_t)cnt * b.A + i;
+        if (b.nb16) reinterpret_cast<uint16_t *>(b.nbr)[o] = uint16_t(e & 0xFFFFu);
+        else b.nbr[o] = e;
+    } else {
+        *b.overflow = 1;
+    }
     ++cnt;
 }
 
@@ -397,7 +402,7 @@ __device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, u
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int kk = k + u * LPA;
-        e[u] = kk < cnt ? b.nbr[(size_t)kk * b.A + i] : none;
+        e[u] = kk < cnt ? nbr_entry(b, kk, i) : none;
     }
 }
 
@@ -629,7 +634,7 @@ __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
     double fx = 0.0, fy = 0.0, fz = 0.0, er = 0.0, ev = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
     for (int k = 0; k < cnt; ++k) {
-        uint32_t e = b.nbr[(size_t)k * b.A + i];
+        uint32_t e = nbr_entry(b, k, i);
         int jl = int(e & AKMC_NB_JMASK);
         int j = W.off + jl;
         const double4 pj = b.p4[j];

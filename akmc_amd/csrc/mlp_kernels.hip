@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(kB) k_descriptors(BatchDev b, int nq, const in
     for (int t = 0; t < 48; ++t) g[t] = 0.0f;
     const int cnt = b.nnb[i];
     for (int k = 0; k < cnt; ++k) {
-        uint32_t e = b.nbr[(size_t)k * b.A + i];
+        uint32_t e = nbr_entry(b, k, i);
         const double4 pj = b.p4[W.off + int(e & AKMC_NB_JMASK)];
         double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
         dx -= W.L[0] * rint(dx * W.invL[0]);
@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(kB) k_descriptors(BatchDev b, int nq, const in
         float ax = (float)dx, ay = (float)dy, az = (float)dz;
         // pairs (j, k) with k later in the list
         for (int k2 = k + 1; k2 < cnt; ++k2) {
-            uint32_t e2 = b.nbr[(size_t)k2 * b.A + i];
+            uint32_t e2 = nbr_entry(b, k2, i);
             const double4 pk = b.p4[W.off + int(e2 & AKMC_NB_JMASK)];
             double ex = pk.x - pi.x, ey = pk.y - pi.y, ez = pk.z - pi.z;
             ex -= W.L[0] * rint(ex * W.invL[0]);

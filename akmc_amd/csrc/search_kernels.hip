@@ -940,7 +940,12 @@ __global__ void __launch_bounds__(kBlock) k_search_logic(BatchDev b, SearchDev s
     __shared__ double sh[4 * kBlock];
     __shared__ WalkerState ws;
     const int wid = blockIdx.x;
-    if (threadIdx.x == 0) ws = s.ws[wid];
+    if (threadIdx.x == 0) {
+        ws = s.ws[wid];
+        // this step's FP32 pair kernels evaluated the walker iff it is
+        // still marked FP32 here (the logic below may change the mark)
+        if (b.active[wid] == 1) s.evals32[wid] += 1;
+    }
     __syncthreads();
     if (ws.kind == WK_IDLE || ws.state == ST_DONE) return;
     const WalkerDev W = b.walkers[wid];

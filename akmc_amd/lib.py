@@ -146,6 +146,9 @@ def load() -> C.CDLL:
     L.akmc_batch_time_phases.argtypes = [C.c_void_p, C.c_int, _D]
     L.akmc_batch_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.akmc_batch_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_kernel_timer.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_kernel_time.argtypes = [C.c_void_p, _D, C.POINTER(C.c_longlong)]
+    L.akmc_batch_eval_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.akmc_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.akmc_batch_save_basin.argtypes = [C.c_void_p]
     L.akmc_art_defaults.argtypes = [C.POINTER(ARTParams)]
@@ -396,6 +399,23 @@ class Batch:
 
     def set_graphs(self, enable: bool):
         check(load().akmc_batch_set_graphs(self._h, int(bool(enable))))
+
+    def kernel_timer(self, enable: bool):
+        check(load().akmc_batch_kernel_timer(self._h, int(bool(enable))))
+
+    def kernel_time(self):
+        """(total ms, launches) of the FP32 Sutton-Chen pass-2 kernel timed
+        inside run() since the timer was started."""
+        ms = C.c_double()
+        n = C.c_longlong()
+        check(load().akmc_batch_kernel_time(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def eval_counts(self):
+        """FP32 force evaluations consumed per walker (int64 [W])."""
+        c = np.zeros(self.W, dtype=np.int64)
+        check(load().akmc_batch_eval_counts(self._h, c.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return c
 
     def set_variant(self, lanes, unroll=1):
         check(load().akmc_batch_set_variant(self._h, int(lanes), int(unroll)))

@@ -38,6 +38,11 @@ sys.path.insert(0, ROOT)
 METRIC = "FP64-verified saddles/sec at ≤0.01 eV barrier error; force evals/sec vs CPU"
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per ordered pair of one all-walker k_sc_force launch, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on `bench.py --forces-only 5`,
+# calibrated on a 1 GiB device copy (FETCH_SIZE reads half the copied bytes,
+# WRITE_SIZE all of them); the summary is under profiles/
+TRAFFIC_BYTES_PER_PAIR = None
 
 
 def ordered_pairs(cells, positions, rc):
@@ -215,6 +220,11 @@ def main():
     tot = S.CampaignStats()
     t_timed = 0.0
     for r in range(args.warmup + args.steps):
+        if r == args.warmup:
+            # the dominant kernel timed with HIP events inside the timed windows,
+            # and the FP32 evaluations each walker consumed there
+            counts0 = camp.batch.eval_counts()
+            camp.batch.kernel_timer(True)
         st = camp.advance(args.window, sync_every=25)
         print(f"[bench] window {r}: {st.searches} ended, {st.converged} converged, {st.verified} verified, "
               f"{st.distinct} distinct, {st.micro_steps} micro-steps in {st.search_time:.2f}s, "
@@ -223,6 +233,14 @@ def main():
         if r >= args.warmup:
             t_timed += st.search_time
             tot.add(st)
+    k_ms, k_launches = camp.batch.kernel_time()
+    evals = camp.batch.eval_counts() - counts0
+    camp.batch.kernel_timer(False)
+    timed_flops = 40.0 * float((evals * P).sum())  # SC pass 2: 40 FLOP per ordered pair of each evaluated walker
+    timed_ms = k_ms / max(k_launches, 1)
+    timed_tf = timed_flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
+    print(f"[bench] k_sc_force in the timed windows: {k_launches} launches, {timed_ms:.4f} ms per launch, "
+          f"{timed_tf:.3f} TFLOP/s ({int(evals.sum())} walker evaluations)", file=sys.stderr, flush=True)
     value = tot.distinct / t_timed
     verified_rate = tot.verified / t_timed
     batched_force_evals = tot.force_calls / t_timed
@@ -269,12 +287,18 @@ def main():
                      "ms_per_eval": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in variant_ms.items()}},
         "roofline": {"bound": "mfma", "bound_note": "compute-bound on the FP32 vector ALU (pair physics has no "
                      "matrix form); peak is gfx950's FP32 vector figure, equal to its FP32 matrix one",
-                     "kernel": "k_sc_force (Sutton-Chen pass 2)",
-                     "achieved": round(achieved_tf, 3), "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved_tf / FP32_VECTOR_PEAK_TFLOPS, 4), "traffic": None,
-                     "flops_per_launch": 40 * Ptot, "ms_per_launch": round(force_ms, 4),
-                     "density_pass_tflops": round(dens_tf, 3),
-                     "hbm_algorithmic_GBs": round(hbm_gbs, 1), "hbm_peak_GBs": HBM_PEAK_GBS},
+                     "kernel": "k_sc_force (Sutton-Chen pass 2, autotuned variant)",
+                     "achieved": round(timed_tf, 3), "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(timed_tf / FP32_VECTOR_PEAK_TFLOPS, 4),
+                     "traffic": TRAFFIC_BYTES_PER_PAIR * Ptot if TRAFFIC_BYTES_PER_PAIR else None,
+                     "traffic_note": "HBM bytes of one all-walker launch at the basins (FETCH_SIZE x 2.0 + "
+                                     "WRITE_SIZE x 1.0, scales from a 1 GiB copy), from the hardware-counter run",
+                     "timed_launches": k_launches, "ms_per_launch": round(timed_ms, 4),
+                     "flops_per_launch": round(timed_flops / max(k_launches, 1)),
+                     "basin_all_walkers": {"ms_per_launch": round(force_ms, 4), "flops_per_launch": 40 * Ptot,
+                                           "tflops": round(achieved_tf, 3), "density_pass_tflops": round(dens_tf, 3),
+                                           "hbm_algorithmic_GBs": round(hbm_gbs, 1)},
+                     "hbm_peak_GBs": HBM_PEAK_GBS},
     }
     if cpu:
         out["cpu_baseline"] = {"value": round(cpu["saddles_per_s"], 5), "unit": "saddles/s",

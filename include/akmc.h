@@ -211,6 +211,17 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
  * disabled the same kernels are launched one by one on the batch's stream
  * (for tools that cannot trace graph launches).  Results are identical. */
 int akmc_batch_set_graphs(akmc_batch_t b, int enable);
+/* Kernel timer of akmc_batch_run: HIP events on the batch's stream around
+ * the FP32 Sutton-Chen pass-2 launch (k_sc_force) of every micro-step, also
+ * inside replayed graphs.  akmc_batch_kernel_timer(b, 1) clears and starts
+ * it (sync_every <= 64 while it runs), (b, 0) stops it;
+ * akmc_batch_kernel_time returns the accumulated milliseconds and the
+ * number of timed launches.  No reference counterpart (measurement). */
+int akmc_batch_kernel_timer(akmc_batch_t b, int enable);
+int akmc_batch_kernel_time(akmc_batch_t b, double *ms, long long *launches);
+/* per walker: FP32 force evaluations the walker's state machines have
+ * consumed since the search state was allocated (counts[W]) */
+int akmc_batch_eval_counts(akmc_batch_t b, long long *counts);
 int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out);
 /* current search positions (the dimer centre / relaxed positions) */
 int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z);
