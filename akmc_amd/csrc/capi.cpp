@@ -122,7 +122,7 @@ struct akmc_batch_s {
     // HIP events around the FP32 Sutton-Chen pass-2 launch of every
     // micro-step of akmc_batch_run (kernel timer)
     static constexpr int kTimerSteps = 64;
-    bool ktimer = false, graph_ktimer = false;
+    bool ktimer = false;
     hipEvent_t kev[2][kTimerSteps] = {};
     double kms = 0.0;
     long long klaunches = 0;
@@ -861,7 +861,6 @@ int capture_graph(akmc_batch_t b, int steps)
     b->graph_args = b->s;
     b->graph_d = b->d;
     b->graph_prec = batch_mode(b);
-    b->graph_ktimer = b->ktimer;
     return AKMC_SUCCESS;
 }
 
@@ -1234,8 +1233,10 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
         // it verifies), so the host only reads states and the overflow flag
         if (b->ktimer && k > akmc_batch_s::kTimerSteps)
             return fail(AKMC_EINVAL, "sync_every is limited to %d while the kernel timer runs", akmc_batch_s::kTimerSteps);
-        if (b->use_graphs) {
-            if (!b->gexec || b->graph_steps != k || b->graph_prec != batch_mode(b) || b->graph_ktimer != b->ktimer ||
+        // HIP events recorded inside a replayed graph cannot be timed, so the
+        // kernel timer launches the micro-steps one by one
+        if (b->use_graphs && !b->ktimer) {
+            if (!b->gexec || b->graph_steps != k || b->graph_prec != batch_mode(b) ||
                 std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0 ||
                 std::memcmp(&b->graph_d, &b->d, sizeof(BatchDev)) != 0)
                 TRY(capture_graph(b, k));

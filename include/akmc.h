@@ -212,9 +212,10 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
  * (for tools that cannot trace graph launches).  Results are identical. */
 int akmc_batch_set_graphs(akmc_batch_t b, int enable);
 /* Kernel timer of akmc_batch_run: HIP events on the batch's stream around
- * the FP32 Sutton-Chen pass-2 launch (k_sc_force) of every micro-step, also
- * inside replayed graphs.  akmc_batch_kernel_timer(b, 1) clears and starts
- * it (sync_every <= 64 while it runs), (b, 0) stops it;
+ * the FP32 Sutton-Chen pass-2 launch (k_sc_force) of every micro-step.
+ * Events inside a replayed graph cannot be timed, so while the timer runs
+ * the micro-steps are launched one by one.  akmc_batch_kernel_timer(b, 1)
+ * clears and starts it (sync_every <= 64 while it runs), (b, 0) stops it;
  * akmc_batch_kernel_time returns the accumulated milliseconds and the
  * number of timed launches.  No reference counterpart (measurement). */
 int akmc_batch_kernel_timer(akmc_batch_t b, int enable);
