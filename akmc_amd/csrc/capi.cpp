@@ -810,8 +810,13 @@ int ensure_lanczos(akmc_batch_t b)
 int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const double *mode, const int *frozen)
 {
     size_t A = b->A;
-    std::vector<double> nw, zero;
-    std::vector<int> fr;
+    // normalised modes of every selected walker, staged together and kept
+    // until the one stream sync at the end
+    size_t total = 0;
+    for (int w = 0; w < b->W; ++w)
+        if (!walkers || walkers[w]) total += 3 * (size_t)b->hw[w].n;
+    std::vector<double> stage(total);
+    size_t at = 0;
     for (int w = 0; w < b->W; ++w) {
         if (walkers && !walkers[w]) continue;
         const int off = b->hw[w].off, n = b->hw[w].n;
@@ -820,9 +825,9 @@ int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const dou
             HIPCHK(hipMemcpyAsync((void *)(b->s.frozen + off), frozen + off, n * sizeof(int), hipMemcpyHostToDevice,
                                   b->stream));
         }
-        fr.assign(b->hfrozen.begin() + off, b->hfrozen.begin() + off + n);
         // zero frozen components and normalise the walker's mode
-        nw.assign(3 * (size_t)n, 0.0);
+        double *nw = stage.data() + at;
+        const int *fr = b->hfrozen.data() + off;
         double nn = 0.0;
         for (int c = 0; c < 3; ++c)
             for (int i = 0; i < n; ++i) {
@@ -830,20 +835,23 @@ int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const dou
                 nw[(size_t)c * n + i] = v;
                 nn += v * v;
             }
-        if (nn <= 0.0) return fail(AKMC_EINVAL, "walker %d: zero start mode", w);
+        if (nn <= 0.0) {
+            (void)hipStreamSynchronize(b->stream);
+            return fail(AKMC_EINVAL, "walker %d: zero start mode", w);
+        }
         nn = 1.0 / std::sqrt(nn);
-        for (auto &v : nw) v *= nn;
-        zero.assign(n, 0.0);
+        for (size_t k = 0; k < 3 * (size_t)n; ++k) nw[k] *= nn;
         for (int c = 0; c < 3; ++c) {
             size_t o = c * A + off;
-            HIPCHK(hipMemcpyAsync(b->s.N + o, nw.data() + (size_t)c * n, n * sizeof(double), hipMemcpyHostToDevice,
+            HIPCHK(hipMemcpyAsync(b->s.N + o, nw + (size_t)c * n, n * sizeof(double), hipMemcpyHostToDevice,
                                   b->stream));
-            HIPCHK(hipMemcpyAsync(b->s.V + o, zero.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+            HIPCHK(hipMemsetAsync(b->s.V + o, 0, n * sizeof(double), b->stream));
             if (x0)
                 HIPCHK(hipMemcpyAsync(b->s.X + o, x0 + o, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
         }
-        HIPCHK(hipStreamSynchronize(b->stream));
+        at += 3 * (size_t)n;
     }
+    HIPCHK(hipStreamSynchronize(b->stream));
     return upload_active(b, walkers);
 }
 

@@ -114,6 +114,7 @@ class SaddleCampaign:
         self.verify_delta = verify_delta
         self.proposer = proposer  # optional callable(campaign, w) -> (x0, mode, frozen, site)
         self.found = [[] for _ in range(self.W)]
+        self._regions = {}
         self.pending = set()
         self.cur_site = {}
         A = self.batch.A
@@ -130,6 +131,7 @@ class SaddleCampaign:
         st = self.batch.status()
         self.basin_ok = np.array([s["status"] == 0 for s in st])
         self.basins = self.batch.positions()
+        self._regions = {}
         self.batch.save_basin()
         self.basin32 = self.batch.search_energies(32)
         self.basin64 = self.batch.search_energies(64)
@@ -137,18 +139,31 @@ class SaddleCampaign:
         return self.basin_ok
 
     # ---------------------------------------------------------- searches
+    def _site_region(self, w, v):
+        """(frozen mask, the 12 neighbours) of vacancy v of walker w at its
+        basin; fixed per basin, so computed once."""
+        key = (w, v)
+        hit = self._regions.get(key)
+        if hit is None:
+            c = self.cells[w]
+            site = c.vacancies[v]
+            basin = self.basins[w]
+            frozen = active_region(c, basin, site, self.region_radius)
+            nb = W.neighbours_of_site(W.Cell(basin, c.types, c.lat), site, 12)
+            hit = self._regions[key] = (frozen, nb)
+        return hit
+
     def proposal(self, w):
         """Seeded random displacement (d0, centre-of-mass free) of the 12
         neighbours of one of the walker's vacancies; the displacement
         direction is the start mode."""
         c = self.cells[w]
-        site = c.vacancies[int(self.rng.integers(0, len(c.vacancies)))]
+        v = int(self.rng.integers(0, len(c.vacancies)))
+        frozen, nb = self._site_region(w, v)
         basin = self.basins[w]
-        frozen = active_region(c, basin, site, self.region_radius)
-        cb = W.Cell(basin, c.types, c.lat)
-        nb = W.neighbours_of_site(cb, site, 12)
-        disp, mode = W.local_displacement(cb, nb, self.d0, int(self.rng.integers(0, 2**31 - 1)))
-        return basin + disp, mode, frozen, site
+        disp, mode = W.local_displacement(W.Cell(basin, c.types, c.lat), nb, self.d0,
+                                          int(self.rng.integers(0, 2**31 - 1)))
+        return basin + disp, mode, frozen, c.vacancies[v]
 
     def start(self, walkers):
         walkers = [int(w) for w in walkers if self.basin_ok[w]]

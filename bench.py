@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on `bench.py --forces-only 5`,
 # calibrated on a 1 GiB device copy (FETCH_SIZE reads half the copied bytes,
 # WRITE_SIZE all of them); the summary is under profiles/
-TRAFFIC_BYTES_PER_PAIR = None
+TRAFFIC_BYTES_PER_PAIR = 3.318
 
 
 def ordered_pairs(cells, positions, rc):
@@ -294,7 +294,7 @@ def main():
                      "kernel": "k_sc_force (Sutton-Chen pass 2, autotuned variant)",
                      "achieved": round(timed_tf, 3), "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(timed_tf / FP32_VECTOR_PEAK_TFLOPS, 4),
-                     "traffic": TRAFFIC_BYTES_PER_PAIR * Ptot if TRAFFIC_BYTES_PER_PAIR else None,
+                     "traffic": round(TRAFFIC_BYTES_PER_PAIR * Ptot) if TRAFFIC_BYTES_PER_PAIR else None,
                      "traffic_note": "HBM bytes of one all-walker launch at the basins (FETCH_SIZE x 2.0 + "
                                      "WRITE_SIZE x 1.0, scales from a 1 GiB copy), from the hardware-counter run",
                      "measured_over": f"{k_launches} launches in a {args.roofline_window}-micro-step window of the "
