@@ -45,6 +45,21 @@ __device__ inline const SCCoef<float> &sc_coef<float>(const SCPairDev &p) { retu
 template <>
 __device__ inline const SCCoef<double> &sc_coef<double>(const SCPairDev &p) { return p.d; }
 
+// 1/x: the hardware reciprocal (1 ulp) on the FP32 path, an exact division
+// on the FP64 verification path
+template <typename T>
+__device__ inline T recip(T x) { return T(1) / x; }
+template <>
+__device__ inline float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// u^x for non-integer exponents: libm pow on the FP64 path, the hardware
+// log2/exp2 (about 1 ulp each) on the FP32 path
+__device__ inline double pow_general(double u, double x) { return pow(u, x); }
+__device__ inline float pow_general(float u, float x)
+{
+    return __builtin_amdgcn_exp2f(x * __builtin_amdgcn_logf(u));
+}
+
 template <typename T>
 __device__ inline T pow_half(T u2, int k, T e)
 {
@@ -60,7 +75,7 @@ __device__ inline T pow_half(T u2, int k, T e)
         if (k & 1) r *= sqrt(u2);
         return r;
     }
-    return pow(u2, e * T(0.5));
+    return pow_general(u2, e * T(0.5));
 }
 
 __device__ inline void min_image(double &dx, double &dy, double &dz, const WalkerDev &w)
@@ -446,7 +461,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
                 const SCPairDev &p = tab[row + (int)pj[u].w];
                 if (p.on) {
                     const SCCoef<T> &c = sc_coef<T>(p);
-                    T u2 = c.a * c.a / T(r2);
+                    T u2 = c.a * c.a * recip<T>(T(r2));
                     rho += double(c.rho_coef * pow_half<T>(u2, p.m_int, c.m));
                 }
             }
@@ -509,8 +524,8 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
                 const SCPairDev &p = tab[row + (int)pj[u].w];
                 if (p.on) {
                     const SCCoef<T> &c = sc_coef<T>(p);
-                    T r2t = T(r2);
-                    T u2 = c.a * c.a / r2t;
+                    const T ir2 = recip<T>(T(r2));
+                    T u2 = c.a * c.a * ir2;
                     T un = pow_half<T>(u2, p.n_int, c.n);
                     T um = pow_half<T>(u2, p.m_int, c.m);
                     T fs = c.fs_coef * un;
@@ -523,7 +538,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
                             fm = c.fm_coef * (T(isi) + T(isj[u])) * um;
                         }
                     }
-                    T ft = (fs - fm) / r2t;  // both orderings of the pair
+                    T ft = (fs - fm) * ir2;  // both orderings of the pair
                     double dft = double(ft);
                     fx += dft * dx;
                     fy += dft * dy;

@@ -200,3 +200,18 @@ def test_kernel_variants_and_autotune(lib, oracle, fp64):
         b.set_variant(3, 1)
     with pytest.raises(Exception):
         b.set_variant(1, 2)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_sc_non_integer_exponents(lib, oracle, fp64):
+    """Sutton-Chen with non-integer n and m (the general power path: libm
+    pow in FP64, hardware log2/exp2 in FP32) against the oracle."""
+    pots = [("Cu", "Cu", CU["eps"], CU["a"], 8.7, 5.6, CU["c"])]
+    c = W.plain_sc_workload(seed=4)
+    f = lib.Field(lib.sc_potentials_text(pots, 2 * CU["a"]))
+    if fp64:
+        f.set_precision(64)
+    ref = oracle.sc_forces(oracle.sc_field(["Cu"], pots, 2 * CU["a"]), c.pos, c.types, c.lat)
+    got = f.calculate_forces(c.pos, c.types, c.lat)
+    tol = (1e-9, 1e-11) if fp64 else (F_TOL, E_TOL_ATOM)
+    _compare(f"SC n=8.7 m=5.6 fp{64 if fp64 else 32}", ref, got, c.n, *tol)
