@@ -1619,6 +1619,30 @@ int akmc_mlp_loss_grad(akmc_mlp_t m, const float *X, const float *T, int M, doub
     return AKMC_SUCCESS;
 }
 
+int akmc_mlp_time(akmc_mlp_t m, int M, int nrep, double ms[2])
+{
+    if (!m || !ms || M <= 0 || M > m->rows || nrep <= 0) return fail(AKMC_EINVAL, "M must be in [1, max_rows]");
+    HIPCHK(hipMemsetAsync(m->X, 0, (size_t)M * 48 * sizeof(float), m->stream));
+    HIPCHK(hipMemsetAsync(m->T, 0, (size_t)M * 3 * sizeof(float), m->stream));
+    hipEvent_t e[3];
+    for (auto &v : e) HIPCHK(hipEventCreate(&v));
+    TRY(mlp_loss_grad_dev(m, M, nullptr));  // first launches outside the timing
+    HIPCHK(hipEventRecord(e[0], m->stream));
+    for (int r = 0; r < nrep; ++r)
+        akmc_mlp_launch_forward(m->X, M, m->p, m->w1t, m->w2t, m->w3t, m->Z1, m->H1, m->Z2, m->H2, m->Y, m->stream);
+    HIPCHK(hipEventRecord(e[1], m->stream));
+    for (int r = 0; r < nrep; ++r) TRY(mlp_loss_grad_dev(m, M, nullptr));
+    HIPCHK(hipEventRecord(e[2], m->stream));
+    HIPCHK(hipEventSynchronize(e[2]));
+    float t0 = 0.0f, t1 = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t0, e[0], e[1]));
+    HIPCHK(hipEventElapsedTime(&t1, e[1], e[2]));
+    for (auto &v : e) (void)hipEventDestroy(v);
+    ms[0] = t0 / nrep;
+    ms[1] = t1 / nrep;
+    return AKMC_SUCCESS;
+}
+
 int akmc_mlp_train(akmc_mlp_t m, const float *X, const float *T, int M, int epochs, int batch, float lr,
                    unsigned seed, double *loss_history)
 {

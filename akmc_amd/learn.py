@@ -35,6 +35,7 @@ def _setup(lib):
     lib.akmc_mlp_loss_grad.argtypes = [C.c_void_p, _F, _F, C.c_int, C.POINTER(C.c_double), _F]
     lib.akmc_mlp_train.argtypes = [C.c_void_p, _F, _F, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint,
                                    C.POINTER(C.c_double)]
+    lib.akmc_mlp_time.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
     lib.akmc_batch_descriptors.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                            C.POINTER(C.c_double), _F]
 
@@ -129,6 +130,12 @@ class MLP:
                                         int(min(batch, self.max_rows)), float(lr), int(seed),
                                         hist.ctypes.data_as(C.POINTER(C.c_double))))
         return hist
+
+    def time(self, rows, nrep=20):
+        """(ms per forward, ms per forward + backward) over `rows` rows."""
+        ms = (C.c_double * 2)()
+        L.check(L.load().akmc_mlp_time(self._h, int(rows), int(nrep), ms))
+        return ms[0], ms[1]
 
     def predict(self, X):
         return self.forward(self.standardize(X))

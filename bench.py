@@ -147,6 +147,55 @@ def forces_only(L, W, args):
           flush=True)
 
 
+def drain(camp, stats):
+    """Let every running search end (no refill)."""
+    while camp.pending:
+        camp.harvest(stats)
+        if camp.pending:
+            camp.batch.run(25, sync_every=25)
+
+
+def learned_ab(camp, records, window, epochs):
+    """Config 4 on the bench's own walkers: train the MLP on the timed
+    windows' FP64-verified saddles, time its MFMA kernels, then run the same
+    cells from random and from learned proposals (fresh de-duplication, one
+    warm-up and one timed window each)."""
+    from akmc_amd import learn as LR
+    from akmc_amd import search as S
+    drain(camp, S.CampaignStats())
+    X, T = LR.dataset_from_records(camp, [r for r in records if r.verified])
+    net = LR.MLP(max_rows=8192, seed=1)
+    net.fit_scaling(X)
+    t0 = time.perf_counter()
+    hist = net.train(X, T, epochs=epochs, batch=4096, lr=2e-3, seed=3)
+    t_train = time.perf_counter() - t0
+    rows = 8192
+    fwd_ms, fb_ms = net.time(rows, 20)
+    fwd_flop = 2.0 * 22912 * rows               # 48x128 + 128x128 + 128x3 MACs per row
+    fb_flop = fwd_flop + 2.0 * 39680 * rows     # + dZ2, dZ1 and the three weight gradients
+    out = {"training_rows": int(X.shape[0]), "training_saddles": int(sum(r.verified for r in records)),
+           "epochs": epochs, "train_seconds": round(t_train, 2),
+           "loss_first_last": [round(float(hist[0]), 5), round(float(hist[-1]), 5)],
+           "mlp_mfma": {"rows": rows, "forward_ms": round(fwd_ms, 4), "forward_backward_ms": round(fb_ms, 4),
+                        "forward_tflops": round(fwd_flop / (fwd_ms * 1e-3) / 1e12, 2),
+                        "forward_backward_tflops": round(fb_flop / (fb_ms * 1e-3) / 1e12, 2),
+                        "peak_bf16_dense_tflops": 2500.0}}
+    for name, prop in (("random", None), ("learned", LR.LearnedProposer(camp, net, seed=4))):
+        drain(camp, S.CampaignStats())
+        camp.proposer = prop
+        camp.found = [[] for _ in range(camp.W)]
+        camp.begin()
+        camp.advance(window // 2, sync_every=25)
+        st = camp.advance(window, sync_every=25)
+        out[name] = {"verified_saddles_per_s": round(st.verified / st.search_time, 2),
+                     "distinct_saddles_per_s": round(st.distinct / st.search_time, 2),
+                     "searches": st.searches, "verified": st.verified, "distinct": st.distinct,
+                     "force_calls_per_verified": round(st.force_calls / max(st.verified, 1), 1)}
+        print(f"[bench] {name} proposals: {out[name]}", file=sys.stderr, flush=True)
+    camp.proposer = None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +207,8 @@ def main():
                     help="micro-steps of the eagerly launched window that times the dominant kernel")
     ap.add_argument("--plain", type=int, default=4, help="plain GPU dimer searches")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal section")
+    ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--eager", action="store_true",
                     help="launch the micro-step kernels one by one instead of replaying hipGraphs "
@@ -256,6 +307,12 @@ def main():
     print(f"[bench] plain GPU dimer: {n_ok}/{args.plain} converged in {t_plain:.2f}s "
           f"({plain_calls} force calls)", file=sys.stderr, flush=True)
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_ab(camp, tot.records, args.window, args.train_epochs)
+        print(f"[bench] learned section: { {k: v for k, v in learned.items() if k not in ('random', 'learned')} }",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cells, camp.basins, camp.proposal)
@@ -306,6 +363,8 @@ def main():
                                            "hbm_algorithmic_GBs": round(hbm_gbs, 1)},
                      "hbm_peak_GBs": HBM_PEAK_GBS},
     }
+    if learned:
+        out["learned_proposal"] = learned
     if cpu:
         out["cpu_baseline"] = {"value": round(cpu["saddles_per_s"], 5), "unit": "saddles/s",
                                "cores": cpu["cores"], "kind": "port",

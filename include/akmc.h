@@ -259,6 +259,10 @@ int akmc_mlp_forward(akmc_mlp_t m, const float *X, int M, float *Y);
 int akmc_mlp_loss_grad(akmc_mlp_t m, const float *X, const float *T, int M, double *loss, float *grad);
 /* Adam training on device-resident data: epochs passes in seeded random
  * minibatches; loss_history[epochs] receives each epoch's mean loss */
+/* device time of one forward (ms[0]) and of one forward + backward
+ * (loss gradient, ms[1]) over M <= max_rows zero rows, HIP events on the
+ * network's stream, averaged over nrep launches (measurement) */
+int akmc_mlp_time(akmc_mlp_t m, int M, int nrep, double ms[2]);
 int akmc_mlp_train(akmc_mlp_t m, const float *X, const float *T, int M, int epochs, int batch, float lr,
                    unsigned seed, double *loss_history);
 /* descriptors of nq queries (atom = index into the batch's concatenated
