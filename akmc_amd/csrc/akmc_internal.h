@@ -93,7 +93,8 @@ struct BatchDev {
     int *cell_count, *cell_start, *cell_atoms, *atom_cell;
     const int *cell_wk;   // cell -> walker
     double *rho;          // SC density
-    float *inv_sqrt_rho;  // SC 1/sqrt(rho), -1 when rho < 1e-8
+    float *inv_sqrt_rho;  // SC 1/sqrt(rho), -1 when rho < 1e-8 (FP32 walkers)
+    double *isr64;        // the same in FP64 (FP64 walkers)
     double *e_a, *e_b;    // per-atom energies (SC: many, pair; ionic: real, vdw)
     double *e_c;          // ionic: reciprocal per atom
     double *st_atom;      // [6][A] per-atom stress when requested

@@ -474,6 +474,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     TRY(dalloc(b, &d.atom_cell, A));
     TRY(dalloc(b, &d.rho, A));
     TRY(dalloc(b, &d.inv_sqrt_rho, A));
+    TRY(dalloc(b, &d.isr64, A));
     TRY(dalloc(b, &d.e_a, A));
     TRY(dalloc(b, &d.e_b, A));
     TRY(dalloc(b, &d.e_c, A));

@@ -472,6 +472,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
     b.rho[i] = rho;
     b.e_a[i] = -sqrt(rho);
     b.inv_sqrt_rho[i] = rho < 1.0e-8 ? -1.0f : float(1.0 / sqrt(rho));
+    if (sizeof(T) == 8) b.isr64[i] = rho < 1.0e-8 ? -1.0 : 1.0 / sqrt(rho);
 }
 
 // pass 2: F_i = sum_j r_ij (fs - fm)/r^2, fs = n eps (a/r)^n,
@@ -494,7 +495,8 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
     const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
-    const float isi = b.inv_sqrt_rho[i];
+    const float isi = sizeof(T) == 8 ? 0.0f : b.inv_sqrt_rho[i];
+    const double isi64 = sizeof(T) == 8 ? b.isr64[i] : 0.0;
     const uint32_t none = self_entry(i, W);
     double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
@@ -507,13 +509,13 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
         fetch_entries<LPA, U>(b, i, k0 + LPA * U, cnt, none, en);  // next batch in flight
         double4 pj[U];
         float isj[U];
-        double rj[U];
+        double isj64[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int j = W.off + int(e[u] & AKMC_NB_JMASK);
             pj[u] = b.p4[j];
-            isj[u] = b.inv_sqrt_rho[j];
-            if (sizeof(T) == 8) rj[u] = b.rho[j];
+            if (sizeof(T) == 8) isj64[u] = b.isr64[j];
+            else isj[u] = b.inv_sqrt_rho[j];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -530,13 +532,10 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
                     T um = pow_half<T>(u2, p.m_int, c.m);
                     T fs = c.fs_coef * un;
                     T fm = T(0);
-                    if (isi >= 0.0f && isj[u] >= 0.0f) {
-                        if (sizeof(T) == 8) {
-                            double ri = b.rho[i];
-                            fm = c.fm_coef * T(1.0 / sqrt(ri) + 1.0 / sqrt(rj[u])) * um;
-                        } else {
-                            fm = c.fm_coef * (T(isi) + T(isj[u])) * um;
-                        }
+                    if (sizeof(T) == 8) {
+                        if (isi64 >= 0.0 && isj64[u] >= 0.0) fm = c.fm_coef * T(isi64 + isj64[u]) * um;
+                    } else if (isi >= 0.0f && isj[u] >= 0.0f) {
+                        fm = c.fm_coef * (T(isi) + T(isj[u])) * um;
                     }
                     T ft = (fs - fm) * ir2;  // both orderings of the pair
                     double dft = double(ft);

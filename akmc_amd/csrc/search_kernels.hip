@@ -1,12 +1,15 @@
-// Walker state machines for gfx950: FIRE relax (optim/Relax.cpp:44-315)
-// and the dimer saddle search (rotation + FIRE translation), advanced by
-// one logic kernel after every batched force evaluation.
+// Walker state machines for gfx950: FIRE relax (optim/Relax.cpp:44-315),
+// the dimer saddle search (rotation + FIRE translation), ART
+// (optim/ART.cpp) and the FP64 verification of converged saddles, advanced
+// by one logic kernel after every batched force evaluation.
 //
 // One workgroup owns one walker.  Every vector operation of the walker is a
-// strided loop of its 256 threads over the walker's atoms followed by a
-// fixed-order LDS tree reduction, so a walker's trajectory is bitwise the
-// same whether it runs alone or in a batch of hundreds.  Vectors are FP64
-// [3][A] arrays (component c of atom i at c*A + i).
+// strided loop of its 256 threads over the walker's atoms (relax) or free
+// atoms (saddle searches) followed by a fixed-order reduction (wavefront
+// butterfly, then the four wavefront partials in order), so a walker's
+// trajectory is bitwise the same whether it runs alone or in a batch of
+// hundreds.  Vectors are FP64 [3][A] arrays (component c of atom i at
+// c*A + i).
 #include "akmc_internal.h"
 
 #include <math.h>
@@ -15,31 +18,39 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Block-wide sums and maxima in a fixed order: a butterfly over the 64
+// lanes of each wavefront, then the four wavefront partials in order, so
+// every thread gets the same bits and a walker's result does not depend on
+// the rest of the batch.
+constexpr int kWaves = kBlock / 64;
+
 template <int NV>
 __device__ inline void bsum(double (&v)[NV], double *sh)
 {
-    for (int c = 0; c < NV; ++c) sh[c * kBlock + threadIdx.x] = v[c];
+    for (int c = 0; c < NV; ++c)
+        for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < NV; ++c) sh[c * kWaves + wave] = v[c];
     __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < (unsigned)o)
-            for (int c = 0; c < NV; ++c) sh[c * kBlock + threadIdx.x] += sh[c * kBlock + threadIdx.x + o];
-        __syncthreads();
+    for (int c = 0; c < NV; ++c) {
+        double t = sh[c * kWaves];
+        for (int k = 1; k < kWaves; ++k) t += sh[c * kWaves + k];
+        v[c] = t;
     }
-    for (int c = 0; c < NV; ++c) v[c] = sh[c * kBlock];
     __syncthreads();
 }
 
 __device__ inline double bmax(double v, double *sh)
 {
-    sh[threadIdx.x] = v;
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[wave] = v;
     __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < (unsigned)o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
-        __syncthreads();
-    }
-    v = sh[0];
+    double t = sh[0];
+    for (int k = 1; k < kWaves; ++k) t = fmax(t, sh[k]);
     __syncthreads();
-    return v;
+    return t;
 }
 
 struct Ctx {

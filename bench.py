@@ -273,8 +273,10 @@ def main():
     tot = S.CampaignStats()
     t_timed = 0.0
     for r in range(args.warmup + args.steps):
+        rb0 = camp.batch.rebuilds
         st = camp.advance(args.window, sync_every=25)
-        print(f"[bench] window {r}: {st.searches} ended, {st.converged} converged, {st.verified} verified, "
+        print(f"[bench] window {r}: {camp.batch.rebuilds - rb0} walker list rebuilds; "
+              f"{st.searches} ended, {st.converged} converged, {st.verified} verified, "
               f"{st.distinct} distinct, {st.micro_steps} micro-steps in {st.search_time:.2f}s, "
               f"max barrier err {st.max_barrier_err:.2e}; failed tests {st.failures()}; device {st.device_time:.2f}s "
               f"harvest {st.harvest_time:.2f}s refill {st.refill_time:.2f}s", file=sys.stderr, flush=True)
