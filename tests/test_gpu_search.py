@@ -219,3 +219,64 @@ def test_verification_curvatures_match_exact_hessian(lib, oracle):
         assert ev[0] < 0 < ev[1]
         assert abs(l1 - ev[0]) < 0.01 * abs(ev[0]) + 0.005
         assert l1 > ev[0] - 1e-3 and l2 > ev[1] - 1e-3 and l2 > 0
+
+
+def test_ionic_art_batch_rocksalt(lib, oracle):
+    """Config 2: 16 batched ART searches on the rock-salt cell (510 ions,
+    charges +-2, Buckingham + Ewald with eta = 0.35 1/A, rc = 10 A over
+    explicit images, |n| < 8), FP32 pair terms and FP64 reciprocal sums.
+    Starts are seeded 0.1 A displacements of the 12 cations around the
+    cation vacancy, atoms beyond 6 A frozen.  Every converged search has a
+    negative lowest curvature, the oracle's FP64 forces at the device saddle
+    are small and its FP64 barrier is within 0.01 eV of the device's."""
+    c = W.rocksalt_cell(seed=5)
+    pots = [("M", "X") + W.ROCKSALT_BUCK[("M", "X")], ("X", "X") + W.ROCKSALT_BUCK[("X", "X")],
+            ("M", "M") + W.ROCKSALT_BUCK[("M", "M")]]
+    f = lib.Field(lib.buck_potentials_text(["M", "X"], [2.0, -2.0], pots, 10.0, noimage=True))
+    f.set_ewald(0.35, 8)
+    nw = 16
+    b = lib.Batch(f, [c] * nw)
+    b.set_frozen([np.zeros(c.n, np.int32)] * nw)
+    b.start_relax()
+    b.run(20000, sync_every=50)
+    st = b.status()
+    assert all(s["status"] == 0 for s in st)
+    basin = b.positions()[0]
+    e1_dev = st[0]["energy"]
+    ew = oracle.Ewald(c.lat, 0.35, gcell=8)
+    of = oracle.ion_field(["M", "X"], pots, rc=10.0, ewald=ew, min_image=False, lat=c.lat)
+    e1 = oracle.ion_forces(of, basin, c.types, c.charges, c.lat)["energy"]
+    site = c.vacancies[0]  # the cation vacancy
+    d = W.min_image(basin - site[:, None], c.lat)
+    r = np.sqrt((d * d).sum(0))
+    frozen = (r >= 6.0).astype(np.int32)
+    cations = np.nonzero(c.types == 0)[0]
+    movers = cations[np.argsort(r[cations])[:12]]
+    cb = W.Cell(basin, c.types, c.lat)
+    x0s, modes = [], []
+    for w in range(nw):
+        disp, mode = W.local_displacement(cb, movers, 0.1, 100 + w)
+        x0s.append(basin + disp)
+        modes.append(mode)
+    p = lib.art_defaults(initTimeStep=0.0005, maxTimeStep=0.002, forceTol=1e-2, parDamp=0.25,
+                         numLanczosVectors=25)
+    b.start_art(modes, params=p, x0_per_walker=x0s, frozen_per_walker=[frozen] * nw)
+    b.run(200000, sync_every=50)
+    st = b.status()
+    sad = b.positions()
+    ok = 0
+    for w in range(nw):
+        s = st[w]
+        if s["status"] != 0:
+            print(f"walker {w}: status {s['status']} after {s['iterations']} iterations", flush=True)
+            continue
+        ok += 1
+        ref = oracle.ion_forces(of, sad[w], c.types, c.charges, c.lat)
+        fmax = np.sqrt((ref["force"][:, frozen == 0] ** 2).sum(0).max())
+        b_dev, b64 = s["energy"] - e1_dev, ref["energy"] - e1
+        print(f"walker {w}: Ea {b_dev:.4f} (FP64 {b64:.4f}) curvature {s['curvature']:.4f} "
+              f"FP64 max force {fmax:.2e} calls {s['force_calls']}", flush=True)
+        assert s["curvature"] < 0
+        assert abs(b_dev - b64) < 0.01
+        assert fmax < 0.05 and b64 > 0
+    assert ok >= nw // 2
