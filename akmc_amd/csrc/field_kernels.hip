@@ -60,6 +60,22 @@ __device__ inline float pow_general(float u, float x)
     return __builtin_amdgcn_exp2f(x * __builtin_amdgcn_logf(u));
 }
 
+// u2^(k/2) for an integer k >= 0 by repeated squaring; with k uniform across
+// the wavefront the loop is scalar control flow around the multiplications
+template <typename T>
+__device__ inline T pow_int(T u2, int k)
+{
+    T r = T(1), base = u2;
+    int p = k >> 1;
+    while (p) {
+        if (p & 1) r *= base;
+        base *= base;
+        p >>= 1;
+    }
+    if (k & 1) r *= sqrt(u2);
+    return r;
+}
+
 template <typename T>
 __device__ inline T pow_half(T u2, int k, T e)
 {
@@ -424,12 +440,16 @@ __device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, u
 // pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
 // then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2.
 // LPA lanes share one atom's neighbour list (strided) and combine by shuffles.
-template <typename T, bool MINIMG, int LPA, int U>
+// ONE: the field has a single species pair with integer exponents, whose
+// coefficients stay in registers and whose powers take uniform control flow.
+template <typename T, bool MINIMG, int LPA, int U, bool ONE>
 __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 {
-    __shared__ SCPairDev tab[AKMC_MAXPAIR];
+    __shared__ SCPairDev tab[ONE ? 1 : AKMC_MAXPAIR];
     const FieldDev &F = *b.field;
-    stage_sc_table(F, tab);
+    if (!ONE) stage_sc_table(F, tab);
+    const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
+    const int m1 = F.sc[0].m_int;
     const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
     if (t < 0) return;
     const int i = t / LPA, sub = t % LPA;
@@ -458,11 +478,16 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
             entry_disp(e[u], dx, dy, dz, W, MINIMG);
             double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
-                const SCPairDev &p = tab[row + (int)pj[u].w];
-                if (p.on) {
-                    const SCCoef<T> &c = sc_coef<T>(p);
-                    T u2 = c.a * c.a * recip<T>(T(r2));
-                    rho += double(c.rho_coef * pow_half<T>(u2, p.m_int, c.m));
+                if constexpr (ONE) {
+                    T u2 = c1.a * c1.a * recip<T>(T(r2));
+                    rho += double(c1.rho_coef * pow_int<T>(u2, m1));
+                } else {
+                    const SCPairDev &p = tab[row + (int)pj[u].w];
+                    if (p.on) {
+                        const SCCoef<T> &c = sc_coef<T>(p);
+                        T u2 = c.a * c.a * recip<T>(T(r2));
+                        rho += double(c.rho_coef * pow_half<T>(u2, p.m_int, c.m));
+                    }
                 }
             }
         }
@@ -479,12 +504,14 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
 // fm = 1/2 m (eps c)^2 (1/sqrt(rho_i) + 1/sqrt(rho_j)) (a/r)^m
 // (potentials/metal/Field.cpp:587-640, ManyBody.cpp:621-757); each atom
 // also carries half of each pair's V = eps (a/r)^n
-template <typename T, bool MINIMG, bool STRESS, int LPA, int U>
+template <typename T, bool MINIMG, bool STRESS, int LPA, int U, bool ONE>
 __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
 {
-    __shared__ SCPairDev tab[AKMC_MAXPAIR];
+    __shared__ SCPairDev tab[ONE ? 1 : AKMC_MAXPAIR];
     const FieldDev &F = *b.field;
-    stage_sc_table(F, tab);
+    if (!ONE) stage_sc_table(F, tab);
+    const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
+    const int n1 = F.sc[0].n_int, m1 = F.sc[0].m_int;
     const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
     if (t < 0) return;
     const int i = t / LPA, sub = t % LPA;
@@ -523,13 +550,13 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
             entry_disp(e[u], dx, dy, dz, W, MINIMG);
             double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
-                const SCPairDev &p = tab[row + (int)pj[u].w];
-                if (p.on) {
-                    const SCCoef<T> &c = sc_coef<T>(p);
+                const SCPairDev &p = ONE ? F.sc[0] : tab[row + (int)pj[u].w];
+                if (ONE || p.on) {
+                    const SCCoef<T> &c = ONE ? c1 : sc_coef<T>(p);
                     const T ir2 = recip<T>(T(r2));
                     T u2 = c.a * c.a * ir2;
-                    T un = pow_half<T>(u2, p.n_int, c.n);
-                    T um = pow_half<T>(u2, p.m_int, c.m);
+                    T un = ONE ? pow_int<T>(u2, n1) : pow_half<T>(u2, p.n_int, c.n);
+                    T um = ONE ? pow_int<T>(u2, m1) : pow_half<T>(u2, p.m_int, c.m);
                     T fs = c.fs_coef * un;
                     T fm = T(0);
                     if (sizeof(T) == 8) {
@@ -906,26 +933,44 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
     k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, akmc_rebuild_counter_ptr());
 }
 
+// a single species pair with integer exponents (the ONE kernels)
+static bool sc_one(const FieldDev &hf)
+{
+    return hf.nspec == 1 && hf.sc[0].on && hf.sc[0].n_int >= 0 && hf.sc[0].m_int >= 0;
+}
+
 template <typename T, int LPA, int U>
 static void launch_sc_density(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     int g = xcd_grid((b.A * LPA + kBlock - 1) / kBlock);
-    if (hf.min_image) k_sc_density<T, true, LPA, U><<<g, kBlock, 0, s>>>(b);
-    else k_sc_density<T, false, LPA, U><<<g, kBlock, 0, s>>>(b);
+    if (sc_one(hf)) {
+        if (hf.min_image) k_sc_density<T, true, LPA, U, true><<<g, kBlock, 0, s>>>(b);
+        else k_sc_density<T, false, LPA, U, true><<<g, kBlock, 0, s>>>(b);
+    } else {
+        if (hf.min_image) k_sc_density<T, true, LPA, U, false><<<g, kBlock, 0, s>>>(b);
+        else k_sc_density<T, false, LPA, U, false><<<g, kBlock, 0, s>>>(b);
+    }
+}
+
+template <typename T, int LPA, int U, bool ONE>
+static void launch_sc_force_one(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
+{
+    int g = xcd_grid((b.A * LPA + kBlock - 1) / kBlock);
+    // the stress variant exists for U = 1 only: U does not change results
+    if (hf.min_image) {
+        if (want_stress) k_sc_force<T, true, true, LPA, 1, ONE><<<g, kBlock, 0, s>>>(b);
+        else k_sc_force<T, true, false, LPA, U, ONE><<<g, kBlock, 0, s>>>(b);
+    } else {
+        if (want_stress) k_sc_force<T, false, true, LPA, 1, ONE><<<g, kBlock, 0, s>>>(b);
+        else k_sc_force<T, false, false, LPA, U, ONE><<<g, kBlock, 0, s>>>(b);
+    }
 }
 
 template <typename T, int LPA, int U>
 static void launch_sc_force(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
-    int g = xcd_grid((b.A * LPA + kBlock - 1) / kBlock);
-    // the stress variant exists for U = 1 only: U does not change results
-    if (hf.min_image) {
-        if (want_stress) k_sc_force<T, true, true, LPA, 1><<<g, kBlock, 0, s>>>(b);
-        else k_sc_force<T, true, false, LPA, U><<<g, kBlock, 0, s>>>(b);
-    } else {
-        if (want_stress) k_sc_force<T, false, true, LPA, 1><<<g, kBlock, 0, s>>>(b);
-        else k_sc_force<T, false, false, LPA, U><<<g, kBlock, 0, s>>>(b);
-    }
+    if (sc_one(hf)) launch_sc_force_one<T, LPA, U, true>(b, hf, want_stress, s);
+    else launch_sc_force_one<T, LPA, U, false>(b, hf, want_stress, s);
 }
 
 // (lanes per atom, gathers in flight per lane) of the batch
