@@ -52,12 +52,21 @@ __device__ inline T recip(T x) { return T(1) / x; }
 template <>
 __device__ inline float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// u^x for non-integer exponents: libm pow on the FP64 path, the hardware
-// log2/exp2 (about 1 ulp each) on the FP32 path
+// u^x for non-integer exponents: libm pow on the FP64 path; on the FP32
+// path the integer part of x by squaring and the fraction by libm log2f /
+// exp2f (relative-accurate near u = 1, where (a/r)^2 lives), which keeps
+// the error of u^x at a few ulp without the cost of a full powf
 __device__ inline double pow_general(double u, double x) { return pow(u, x); }
 __device__ inline float pow_general(float u, float x)
 {
-    return __builtin_amdgcn_exp2f(x * __builtin_amdgcn_logf(u));
+    if (!(x >= 0.0f && x < 64.0f)) return exp2f(x * log2f(u));  // outside the split's range
+    const float xi = floorf(x);
+    float r = 1.0f, base = u;
+    for (int p = int(xi); p > 0; p >>= 1) {
+        if (p & 1) r *= base;
+        base *= base;
+    }
+    return r * exp2f((x - xi) * log2f(u));
 }
 
 // u2^(k/2) for an integer k >= 0 by repeated squaring; with k uniform across
@@ -74,6 +83,31 @@ __device__ inline T pow_int(T u2, int k)
     }
     if (k & 1) r *= sqrt(u2);
     return r;
+}
+
+// pow_int for U values at once: the uniform loop control runs once per
+// batch of pairs instead of once per pair (same operations per value)
+template <typename T, int U>
+__device__ inline void pow_int_batch(const T (&u2)[U], int k, T (&r)[U])
+{
+    T base[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        r[u] = T(1);
+        base[u] = u2[u];
+    }
+    for (int p = k >> 1; p; p >>= 1) {
+        if (p & 1) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) r[u] *= base[u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) base[u] *= base[u];
+    }
+    if (k & 1) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) r[u] *= sqrt(u2[u]);
+    }
 }
 
 template <typename T>
@@ -472,22 +506,34 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
         double4 pj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) pj[u] = b.p4[W.off + int(e[u] & AKMC_NB_JMASK)];
+        double r2[U];
+        bool in[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             double dx = xi - pj[u].x, dy = yi - pj[u].y, dz = zi - pj[u].z;
             entry_disp(e[u], dx, dy, dz, W, MINIMG);
-            double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
-                if constexpr (ONE) {
-                    T u2 = c1.a * c1.a * recip<T>(T(r2));
-                    rho += double(c1.rho_coef * pow_int<T>(u2, m1));
-                } else {
-                    const SCPairDev &p = tab[row + (int)pj[u].w];
-                    if (p.on) {
-                        const SCCoef<T> &c = sc_coef<T>(p);
-                        T u2 = c.a * c.a * recip<T>(T(r2));
-                        rho += double(c.rho_coef * pow_half<T>(u2, p.m_int, c.m));
-                    }
+            r2[u] = dx * dx + dy * dy + dz * dz;
+            in[u] = r2[u] <= rc2 && r2[u] > (MINIMG ? 1.0e-16 : 1.0e-3);
+        }
+        if constexpr (ONE) {
+            // every slot is evaluated (padding gives r2 = 0 and infinities),
+            // only the pairs inside the cutoff are summed
+            T u2[U], um[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) u2[u] = c1.a * c1.a * recip<T>(T(r2[u]));
+            pow_int_batch<T, U>(u2, m1, um);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (in[u]) rho += double(c1.rho_coef * um[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!in[u]) continue;
+                const SCPairDev &p = tab[row + (int)pj[u].w];
+                if (p.on) {
+                    const SCCoef<T> &c = sc_coef<T>(p);
+                    T u2 = c.a * c.a * recip<T>(T(r2[u]));
+                    rho += double(c.rho_coef * pow_half<T>(u2, p.m_int, c.m));
                 }
             }
         }
@@ -544,42 +590,63 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
             if (sizeof(T) == 8) isj64[u] = b.isr64[j];
             else isj[u] = b.inv_sqrt_rho[j];
         }
+        double dx[U], dy[U], dz[U], r2[U];
+        bool in[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            double dx = xi - pj[u].x, dy = yi - pj[u].y, dz = zi - pj[u].z;
-            entry_disp(e[u], dx, dy, dz, W, MINIMG);
-            double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3)) {
-                const SCPairDev &p = ONE ? F.sc[0] : tab[row + (int)pj[u].w];
-                if (ONE || p.on) {
-                    const SCCoef<T> &c = ONE ? c1 : sc_coef<T>(p);
-                    const T ir2 = recip<T>(T(r2));
-                    T u2 = c.a * c.a * ir2;
-                    T un = ONE ? pow_int<T>(u2, n1) : pow_half<T>(u2, p.n_int, c.n);
-                    T um = ONE ? pow_int<T>(u2, m1) : pow_half<T>(u2, p.m_int, c.m);
-                    T fs = c.fs_coef * un;
-                    T fm = T(0);
-                    if (sizeof(T) == 8) {
-                        if (isi64 >= 0.0 && isj64[u] >= 0.0) fm = c.fm_coef * T(isi64 + isj64[u]) * um;
-                    } else if (isi >= 0.0f && isj[u] >= 0.0f) {
-                        fm = c.fm_coef * (T(isi) + T(isj[u])) * um;
-                    }
-                    T ft = (fs - fm) * ir2;  // both orderings of the pair
-                    double dft = double(ft);
-                    fx += dft * dx;
-                    fy += dft * dy;
-                    fz += dft * dz;
-                    ep += double(T(0.5) * c.eps * un);
-                    if (STRESS) {
-                        double h = -0.5 * dft;  // one ordered pair: -(fs-fm)/(2 r^2) r r
-                        s0 += h * dx * dx;
-                        s1 += h * dy * dy;
-                        s2 += h * dz * dz;
-                        s3 += h * dy * dz;
-                        s4 += h * dx * dz;
-                        s5 += h * dx * dy;
-                    }
-                }
+            dx[u] = xi - pj[u].x;
+            dy[u] = yi - pj[u].y;
+            dz[u] = zi - pj[u].z;
+            entry_disp(e[u], dx[u], dy[u], dz[u], W, MINIMG);
+            r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+            in[u] = r2[u] <= rc2 && r2[u] > (MINIMG ? 1.0e-16 : 1.0e-3);
+        }
+        T ir2[U], un[U], um[U];
+        if constexpr (ONE) {
+            // every slot is evaluated (padding gives r2 = 0 and infinities),
+            // only the pairs inside the cutoff are summed
+            T u2[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ir2[u] = recip<T>(T(r2[u]));
+                u2[u] = c1.a * c1.a * ir2[u];
+            }
+            pow_int_batch<T, U>(u2, n1, un);
+            pow_int_batch<T, U>(u2, m1, um);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!in[u]) continue;
+            const SCPairDev &p = ONE ? F.sc[0] : tab[row + (int)pj[u].w];
+            if (!(ONE || p.on)) continue;
+            const SCCoef<T> &c = ONE ? c1 : sc_coef<T>(p);
+            if constexpr (!ONE) {
+                ir2[u] = recip<T>(T(r2[u]));
+                const T u2 = c.a * c.a * ir2[u];
+                un[u] = pow_half<T>(u2, p.n_int, c.n);
+                um[u] = pow_half<T>(u2, p.m_int, c.m);
+            }
+            T fs = c.fs_coef * un[u];
+            T fm = T(0);
+            if (sizeof(T) == 8) {
+                if (isi64 >= 0.0 && isj64[u] >= 0.0) fm = c.fm_coef * T(isi64 + isj64[u]) * um[u];
+            } else if (isi >= 0.0f && isj[u] >= 0.0f) {
+                fm = c.fm_coef * (T(isi) + T(isj[u])) * um[u];
+            }
+            T ft = (fs - fm) * ir2[u];  // both orderings of the pair
+            double dft = double(ft);
+            fx += dft * dx[u];
+            fy += dft * dy[u];
+            fz += dft * dz[u];
+            ep += double(T(0.5) * c.eps * un[u]);
+            if (STRESS) {
+                double h = -0.5 * dft;  // one ordered pair: -(fs-fm)/(2 r^2) r r
+                s0 += h * dx[u] * dx[u];
+                s1 += h * dy[u] * dy[u];
+                s2 += h * dz[u] * dz[u];
+                s3 += h * dy[u] * dz[u];
+                s4 += h * dx[u] * dz[u];
+                s5 += h * dx[u] * dy[u];
             }
         }
     }
