@@ -276,12 +276,28 @@ __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
 }
 
 // Minimum-image Verlet list, one workgroup per cell of a walker whose list
-// is rebuilt: the home cell's atoms (one per thread) scan the neighbouring
-// cells (c-1..c+1 per axis; every cell once along axes with fewer than 3
-// cells), whose atoms are staged through LDS kBlock at a time and read by
-// all threads alike.  Entries are every j != i with min-image distance <=
-// rc + skin, in cell order then ascending index; with a near shell (ionic
-// fields) the entries inside it come first.
+// is rebuilt.  The home cell's atoms are spread over the workgroup, four
+// lanes per atom; the neighbouring cells (c-1..c+1 per axis, every cell
+// once along axes with fewer than 3 cells) are staged through LDS kBlock
+// atoms at a time and read by all threads alike.  The four lanes of an atom
+// test four consecutive candidates at once and place their hits by a prefix
+// count over the four, so entries come out exactly in candidate order: cell
+// order, then ascending index; with a near shell (ionic fields) the entries
+// inside it come first.  Entries are every j != i with min-image distance
+// <= rc + skin.
+constexpr int kNbLanes = 4;
+
+__device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
+{
+    if (k < b.maxnb) {
+        const size_t o = (size_t)k * b.A + i;
+        if (b.nb16) reinterpret_cast<uint16_t *>(b.nbr)[o] = uint16_t(e & 0xFFFFu);
+        else b.nbr[o] = e;
+    } else {
+        *b.overflow = 1;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
 {
     __shared__ double sx[kBlock], sy[kBlock], sz[kBlock];
@@ -305,9 +321,11 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
     const double rl2 = b.field->rlist2;
     const double rn2 = b.field->rnear2;
     const int s0 = b.cell_start[cg], n0 = b.cell_count[cg];
-    for (int hb = 0; hb < n0; hb += kBlock) {
-        const bool have = hb + (int)threadIdx.x < n0;
-        const int i = have ? b.cell_atoms[s0 + hb + threadIdx.x] : 0;
+    const int lane = threadIdx.x % kNbLanes, slot = threadIdx.x / kNbLanes;
+    constexpr int kSlots = kBlock / kNbLanes;
+    for (int hb = 0; hb < n0; hb += kSlots) {
+        const bool have = hb + slot < n0;
+        const int i = have ? b.cell_atoms[s0 + hb + slot] : 0;
         const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
         int cnt = 0, nnear = 0;
         // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
@@ -331,16 +349,27 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
                                 sz[threadIdx.x] = b.z[j];
                             }
                             __syncthreads();
-                            if (!have) continue;
-                            for (int t = 0; t < m; ++t) {
-                                const int j = sj[t];
-                                if (j == i) continue;
-                                double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
-                                min_image(dx, dy, dz, W);
-                                const double r2 = dx * dx + dy * dy + dz * dz;
-                                const bool near = r2 <= rn2;
-                                if (r2 <= rl2 && (pass == 0 ? near : !near))
-                                    push_nbr(b, i, cnt, uint32_t(j - W.off) | AKMC_NB_ZERO_SHIFT);
+                            for (int g = 0; g < m; g += kNbLanes) {
+                                const int t = g + lane;
+                                int hit = 0;
+                                int j = 0;
+                                if (have && t < m) {
+                                    j = sj[t];
+                                    double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
+                                    min_image(dx, dy, dz, W);
+                                    const double r2 = dx * dx + dy * dy + dz * dz;
+                                    const bool near = r2 <= rn2;
+                                    hit = (j != i && r2 <= rl2 && (pass == 0 ? near : !near)) ? 1 : 0;
+                                }
+                                // inclusive prefix of the hits over the atom's four lanes
+                                int inc = hit;
+                                for (int o = 1; o < kNbLanes; o <<= 1) {
+                                    const int v = __shfl_up(inc, o, kNbLanes);
+                                    if (lane >= o) inc += v;
+                                }
+                                const int total = __shfl(inc, kNbLanes - 1, kNbLanes);
+                                if (hit) put_nbr(b, i, cnt + inc - 1, uint32_t(j - W.off) | AKMC_NB_ZERO_SHIFT);
+                                cnt += total;
                             }
                         }
                     }
@@ -348,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
             }
             if (pass == 0) nnear = cnt;
         }
-        if (have) {
+        if (have && lane == 0) {
             b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
             b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
         }

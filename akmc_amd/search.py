@@ -96,12 +96,14 @@ class CampaignStats:
 class SaddleCampaign:
     def __init__(self, potentials_text, cells, method="dimer", region_radius=6.0, d0=0.1, params=None,
                  relax=None, seed=0, fmax_verify=0.05, barrier_tol=0.01, verify_nl=16, verify_delta=0.01,
-                 proposer=None):
+                 proposer=None, skin=None):
         self.text = potentials_text
         self.cells = cells
         self.W = len(cells)
         self.method = method
         self.field = L.Field(potentials_text)
+        if skin is not None:
+            self.field.set_skin(skin)  # Verlet skin of the device lists (A)
         self.batch = L.Batch(self.field, cells)
         self.region_radius = region_radius
         self.d0 = d0

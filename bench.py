@@ -207,6 +207,7 @@ def main():
                     help="micro-steps of the eagerly launched window that times the dominant kernel")
     ap.add_argument("--plain", type=int, default=4, help="plain GPU dimer searches")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skin", type=float, default=None, help="Verlet skin of the device lists (A)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal section")
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--seed", type=int, default=3)
@@ -239,7 +240,7 @@ def main():
     cells = W.batched_sc_cells(n_walkers=args.walkers, seed=args.seed)
     natoms = np.array([c.n for c in cells])
     dimer = L.dimer_defaults(maxParIter=400)
-    camp = S.SaddleCampaign(text, cells, params=dimer, seed=args.seed)
+    camp = S.SaddleCampaign(text, cells, params=dimer, seed=args.seed, skin=args.skin)
     if args.eager:
         camp.batch.set_graphs(False)
     # kernel variant chosen on the seeded cells, before the relaxation that uses it
