@@ -1036,33 +1036,34 @@ int akmc_batch_forces_repeat(akmc_batch_t b, int nrep, double *ms)
 int akmc_batch_set_variant(akmc_batch_t b, int lanes, int unroll)
 {
     if (!b) return fail(AKMC_EINVAL, "null batch");
-    if (lanes != 1 && lanes != 2 && lanes != 4) return fail(AKMC_EINVAL, "lanes per atom must be 1, 2 or 4, got %d", lanes);
-    if (unroll != 1 && unroll != 4) return fail(AKMC_EINVAL, "unroll must be 1 or 4, got %d", unroll);
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
+        return fail(AKMC_EINVAL, "lanes per atom must be 1, 2, 4 or 8, got %d", lanes);
+    if (unroll != 1 && unroll != 2 && unroll != 4) return fail(AKMC_EINVAL, "unroll must be 1, 2 or 4, got %d", unroll);
     if (b->d.lpa != lanes || b->d.unroll != unroll) drop_graph(b);  // the captured graph holds the old kernels
     b->d.lpa = lanes;
     b->d.unroll = unroll;
     return AKMC_SUCCESS;
 }
 
-int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[2], double ms[6])
+int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[2], double ms[AKMC_NVARIANTS])
 {
     if (!b || nrep <= 0) return fail(AKMC_EINVAL, "bad arguments");
-    static const int lanes[3] = {1, 2, 4}, unroll[2] = {1, 4};
-    double t[6];
+    static const int lanes[4] = {1, 2, 4, 8}, unroll[3] = {1, 2, 4};
+    double t[AKMC_NVARIANTS];
     TRY(run_forces(b, 0));  // lists current before timing
-    for (int v = 0; v < 6; ++v) {
-        TRY(akmc_batch_set_variant(b, lanes[v / 2], unroll[v % 2]));
+    for (int v = 0; v < AKMC_NVARIANTS; ++v) {
+        TRY(akmc_batch_set_variant(b, lanes[v / 3], unroll[v % 3]));
         TRY(run_forces(b, 0));  // first launch of this instantiation
         TRY(akmc_batch_forces_repeat(b, nrep, &t[v]));
         if (ms) ms[v] = t[v];
     }
     int k = 0;
-    for (int v = 1; v < 6; ++v)
+    for (int v = 1; v < AKMC_NVARIANTS; ++v)
         if (t[v] < t[k]) k = v;
-    TRY(akmc_batch_set_variant(b, lanes[k / 2], unroll[k % 2]));
+    TRY(akmc_batch_set_variant(b, lanes[k / 3], unroll[k % 3]));
     if (best) {
-        best[0] = lanes[k / 2];
-        best[1] = unroll[k % 2];
+        best[0] = lanes[k / 3];
+        best[1] = unroll[k % 3];
     }
     return AKMC_SUCCESS;
 }

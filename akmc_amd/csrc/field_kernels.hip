@@ -1070,16 +1070,18 @@ static void launch_sc_force(const BatchDev &b, const FieldDev &hf, int want_stre
 }
 
 // (lanes per atom, gathers in flight per lane) of the batch
+#define AKMC_SC_UNROLL(CALL, L_)                                  \
+    do {                                                          \
+        if (b.unroll == 4) CALL(L_, 4);                           \
+        else if (b.unroll == 2) CALL(L_, 2);                      \
+        else CALL(L_, 1);                                         \
+    } while (0)
 #define AKMC_SC_VARIANT(CALL)                                     \
     do {                                                          \
-        const bool u4 = b.unroll == 4;                            \
-        if (b.lpa == 4) {                                         \
-            if (u4) CALL(4, 4); else CALL(4, 1);                  \
-        } else if (b.lpa == 2) {                                  \
-            if (u4) CALL(2, 4); else CALL(2, 1);                  \
-        } else {                                                  \
-            if (u4) CALL(1, 4); else CALL(1, 1);                  \
-        }                                                         \
+        if (b.lpa == 8) AKMC_SC_UNROLL(CALL, 8);                  \
+        else if (b.lpa == 4) AKMC_SC_UNROLL(CALL, 4);             \
+        else if (b.lpa == 2) AKMC_SC_UNROLL(CALL, 2);             \
+        else AKMC_SC_UNROLL(CALL, 1);                             \
     } while (0)
 
 template <typename T>

@@ -169,8 +169,8 @@ def test_kernel_variants_and_autotune(lib, oracle, fp64):
     refs = [oracle.sc_forces(of, c.pos, c.types, c.lat) for c in cells]
     ftol, etol = (1e-9, 1e-11) if fp64 else (F_TOL, E_TOL_ATOM)
     got = {}
-    for lanes in (1, 2, 4):
-        for unroll in (1, 4):
+    for lanes in (1, 2, 4, 8):
+        for unroll in (1, 2, 4):
             b.set_variant(lanes, unroll)
             e = b.forces()
             _, frc = b.download()
@@ -183,12 +183,13 @@ def test_kernel_variants_and_autotune(lib, oracle, fp64):
                 assert df < ftol and de < etol, (lanes, unroll, w, df, de)
             print(f"variant {lanes}x{unroll} fp{64 if fp64 else 32}: max|dF| {worst:.2e}", flush=True)
         for w in range(len(cells)):
-            # same summation order; the FP32 path is bitwise equal, the FP64
-            # path may differ by the compiler's FMA contraction choices
-            if fp64:
-                assert np.abs(got[(lanes, 1)][w] - got[(lanes, 4)][w]).max() < 1e-12
-            else:
-                assert np.array_equal(got[(lanes, 1)][w], got[(lanes, 4)][w])
+            for unroll in (2, 4):
+                # same summation order; the FP32 path is bitwise equal, the FP64
+                # path may differ by the compiler's FMA contraction choices
+                if fp64:
+                    assert np.abs(got[(lanes, 1)][w] - got[(lanes, unroll)][w]).max() < 1e-12
+                else:
+                    assert np.array_equal(got[(lanes, 1)][w], got[(lanes, unroll)][w])
     best, ms = b.autotune(nrep=5)
     print(f"autotune: best {best}, ms per evaluation {ms}", flush=True)
     assert best in got
@@ -199,7 +200,7 @@ def test_kernel_variants_and_autotune(lib, oracle, fp64):
     with pytest.raises(Exception):
         b.set_variant(3, 1)
     with pytest.raises(Exception):
-        b.set_variant(1, 2)
+        b.set_variant(1, 3)
 
 
 @pytest.mark.parametrize("fp64", [False, True])
