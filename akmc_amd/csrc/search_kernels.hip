@@ -448,57 +448,47 @@ __device__ void dimer_logic(const Ctx &C_, WalkerState &w, double *sh)
 
 // ------------------------------------------------------------ verification
 
-// eigenvalues of the k x k symmetric tridiagonal (d, e) by implicit QL
-// (values only), ascending in d; single thread
-__device__ void tridiag_values(int n, double *d, double *e)
+// number of eigenvalues of the symmetric tridiagonal (d, e) below x (Sturm
+// sequence of the LDL^T pivots)
+__device__ inline int sturm_count(int n, const double *d, const double *e, double x)
 {
-    for (int i = 0; i < n - 1; ++i) e[i] = e[i];
-    e[n - 1] = 0.0;
-    for (int l = 0; l < n; ++l) {
-        int iter = 0, m;
-        for (;;) {
-            for (m = l; m < n - 1; ++m) {
-                double dd = fabs(d[m]) + fabs(d[m + 1]);
-                if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
-            }
-            if (m == l || ++iter > 60) break;
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = hypot(g, 1.0);
-            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
-            double sn = 1.0, c = 1.0, p = 0.0;
-            int i;
-            for (i = m - 1; i >= l; --i) {
-                double f = sn * e[i], bb = c * e[i];
-                r = hypot(f, g);
-                e[i + 1] = r;
-                if (r == 0.0) {
-                    d[i + 1] -= p;
-                    e[m] = 0.0;
-                    break;
-                }
-                sn = f / r;
-                c = g / r;
-                g = d[i + 1] - p;
-                r = (d[i] - g) * sn + 2.0 * c * bb;
-                p = sn * r;
-                d[i + 1] = g + p;
-                g = c * r - bb;
-            }
-            if (r == 0.0 && i >= l) continue;
-            d[l] -= p;
-            e[l] = g;
-            e[m] = 0.0;
-        }
+    int c = 0;
+    double q = d[0] - x;
+    for (int i = 0;;) {
+        if (q == 0.0) q = 1.0e-300;
+        c += q < 0.0;
+        if (++i >= n) break;
+        q = d[i] - x - e[i - 1] * e[i - 1] / q;
     }
-    for (int i = 1; i < n; ++i) {  // insertion sort
-        double v = d[i];
-        int j = i - 1;
-        while (j >= 0 && d[j] > v) {
-            d[j + 1] = d[j];
-            --j;
-        }
-        d[j + 1] = v;
+    return c;
+}
+
+// k-th smallest eigenvalue (k = 0, 1, ...) of the n x n symmetric
+// tridiagonal (d, e) by multisection: the 64 lanes of the calling wavefront
+// evaluate Sturm counts at 64 interior points of the bracket, which shrinks
+// 65-fold per round, from the Gershgorin interval down to the last bits.
+// Every lane of the wavefront must call it; all return the same value.
+__device__ double tridiag_kth(int n, const double *d, const double *e, int k)
+{
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; ++i) {
+        const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
+        lo = fmin(lo, d[i] - r);
+        hi = fmax(hi, d[i] + r);
     }
+    const int lane = threadIdx.x & 63;
+    for (int round = 0; round < 16; ++round) {
+        const double w = hi - lo;
+        if (!(w > 4.0e-16 * fmax(fabs(lo), fabs(hi)) + 1.0e-300)) break;
+        const double x = lo + w * (double)(lane + 1) / 65.0;
+        const unsigned long long above = __ballot(sturm_count(n, d, e, x) > k);
+        const int first = above ? __ffsll((long long)above) - 1 : 64;  // bracket (x_first-1, x_first]
+        const double nlo = first == 0 ? lo : lo + w * (double)first / 65.0;
+        const double nhi = first == 64 ? hi : lo + w * (double)(first + 1) / 65.0;
+        lo = nlo;
+        hi = nhi;
+    }
+    return 0.5 * (lo + hi);
 }
 
 // FP64 check of a converged saddle: max force on the free atoms, energy,
@@ -607,16 +597,21 @@ __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
     w.lk = k + 1;
     if (w.lk >= nl || beta < 1.0e-12) {
         __shared__ double ev[2];
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < 64) {  // one wavefront: the two lowest Ritz values
             double d[AKMC_LZ_MAX], e[AKMC_LZ_MAX];
-            int n = w.lk;
-            for (int t = 0; t < n; ++t) {
+            const int n = w.lk;
+            for (int t = 0; t < k; ++t) {
                 d[t] = lz[t];
                 e[t] = lz[AKMC_LZ_MAX + t];
             }
-            tridiag_values(n, d, e);
-            ev[0] = d[0];
-            ev[1] = n > 1 ? d[1] : 1.0e300;
+            d[k] = alpha;
+            e[k] = 0.0;
+            const double l1 = tridiag_kth(n, d, e, 0);
+            const double l2 = n > 1 ? tridiag_kth(n, d, e, 1) : 1.0e300;
+            if (threadIdx.x == 0) {
+                ev[0] = l1;
+                ev[1] = l2;
+            }
         }
         __syncthreads();
         w.lam1 = ev[0];
@@ -783,17 +778,19 @@ __device__ void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
     if (threadIdx.x == 0) {
         lz[i] = alpha;
         if (i > 0) lz[AKMC_LZ_MAX + i - 1] = beta;
-        if (i > 0) {
-            double d[AKMC_LZ_MAX], e[AKMC_LZ_MAX];
-            for (int t = 0; t <= i; ++t) {
-                d[t] = lz[t];
-                e[t] = t < i ? lz[AKMC_LZ_MAX + t] : 0.0;
-            }
-            tridiag_values(i + 1, d, e);
-            evs[0] = d[0];
-        } else {
-            evs[0] = alpha;
+    }
+    if (i == 0) {
+        if (threadIdx.x == 0) evs[0] = alpha;
+    } else if (threadIdx.x < 64) {  // one wavefront: the lowest Ritz value
+        double d[AKMC_LZ_MAX], e[AKMC_LZ_MAX];
+        for (int t = 0; t < i; ++t) {
+            d[t] = lz[t];
+            e[t] = t < i - 1 ? lz[AKMC_LZ_MAX + t] : beta;
         }
+        d[i] = alpha;
+        e[i] = 0.0;
+        const double l1 = tridiag_kth(i + 1, d, e, 0);
+        if (threadIdx.x == 0) evs[0] = l1;
     }
     __syncthreads();
     const double evnew = evs[0];
