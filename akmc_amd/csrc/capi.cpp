@@ -1038,7 +1038,8 @@ int akmc_batch_set_variant(akmc_batch_t b, int lanes, int unroll)
     if (!b) return fail(AKMC_EINVAL, "null batch");
     if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
         return fail(AKMC_EINVAL, "lanes per atom must be 1, 2, 4 or 8, got %d", lanes);
-    if (unroll != 1 && unroll != 2 && unroll != 4) return fail(AKMC_EINVAL, "unroll must be 1, 2 or 4, got %d", unroll);
+    if (unroll != 1 && unroll != 2 && unroll != 4 && unroll != 8)
+        return fail(AKMC_EINVAL, "unroll must be 1, 2, 4 or 8, got %d", unroll);
     if (b->d.lpa != lanes || b->d.unroll != unroll) drop_graph(b);  // the captured graph holds the old kernels
     b->d.lpa = lanes;
     b->d.unroll = unroll;
@@ -1048,11 +1049,11 @@ int akmc_batch_set_variant(akmc_batch_t b, int lanes, int unroll)
 int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[2], double ms[AKMC_NVARIANTS])
 {
     if (!b || nrep <= 0) return fail(AKMC_EINVAL, "bad arguments");
-    static const int lanes[4] = {1, 2, 4, 8}, unroll[3] = {1, 2, 4};
+    static const int lanes[4] = {1, 2, 4, 8}, unroll[4] = {1, 2, 4, 8};
     double t[AKMC_NVARIANTS];
     TRY(run_forces(b, 0));  // lists current before timing
     for (int v = 0; v < AKMC_NVARIANTS; ++v) {
-        TRY(akmc_batch_set_variant(b, lanes[v / 3], unroll[v % 3]));
+        TRY(akmc_batch_set_variant(b, lanes[v / 4], unroll[v % 4]));
         TRY(run_forces(b, 0));  // first launch of this instantiation
         TRY(akmc_batch_forces_repeat(b, nrep, &t[v]));
         if (ms) ms[v] = t[v];
@@ -1060,10 +1061,10 @@ int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[2], double ms[AKMC_NV
     int k = 0;
     for (int v = 1; v < AKMC_NVARIANTS; ++v)
         if (t[v] < t[k]) k = v;
-    TRY(akmc_batch_set_variant(b, lanes[k / 3], unroll[k % 3]));
+    TRY(akmc_batch_set_variant(b, lanes[k / 4], unroll[k % 4]));
     if (best) {
-        best[0] = lanes[k / 3];
-        best[1] = unroll[k % 3];
+        best[0] = lanes[k / 4];
+        best[1] = unroll[k % 4];
     }
     return AKMC_SUCCESS;
 }

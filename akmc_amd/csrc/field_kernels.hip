@@ -1072,7 +1072,8 @@ static void launch_sc_force(const BatchDev &b, const FieldDev &hf, int want_stre
 // (lanes per atom, gathers in flight per lane) of the batch
 #define AKMC_SC_UNROLL(CALL, L_)                                  \
     do {                                                          \
-        if (b.unroll == 4) CALL(L_, 4);                           \
+        if (b.unroll == 8) CALL(L_, 8);                           \
+        else if (b.unroll == 4) CALL(L_, 4);                      \
         else if (b.unroll == 2) CALL(L_, 2);                      \
         else CALL(L_, 1);                                         \
     } while (0)

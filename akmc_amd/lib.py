@@ -422,13 +422,13 @@ class Batch:
 
     def autotune(self, nrep=10):
         """Time the Sutton-Chen kernel variants (lanes per atom 1/2/4/8 x
-        gathers in flight 1/2/4) on the resident positions and keep the
+        gathers in flight 1/2/4/8) on the resident positions and keep the
         fastest; returns ((lanes, unroll), {(lanes, unroll): ms per force
         evaluation})."""
-        ms = np.zeros(12)
+        ms = np.zeros(16)
         best = (C.c_int * 2)()
         check(load().akmc_batch_autotune(self._h, int(nrep), best, _dp(ms)))
-        table = {(l, u): ms[3 * li + ui] / nrep for li, l in enumerate((1, 2, 4, 8)) for ui, u in enumerate((1, 2, 4))}
+        table = {(l, u): ms[4 * li + ui] / nrep for li, l in enumerate((1, 2, 4, 8)) for ui, u in enumerate((1, 2, 4, 8))}
         return (best[0], best[1]), table
 
     def save_basin(self):

@@ -205,7 +205,7 @@ def main():
     ap.add_argument("--walkers", type=int, default=512)
     ap.add_argument("--roofline-window", type=int, default=100,
                     help="micro-steps of the eagerly launched window that times the dominant kernel")
-    ap.add_argument("--plain", type=int, default=4, help="plain GPU dimer searches")
+    ap.add_argument("--plain", type=int, default=12, help="plain GPU dimer searches")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skin", type=float, default=None, help="Verlet skin of the device lists (A)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal section")

@@ -107,19 +107,19 @@ int akmc_batch_time_phases(akmc_batch_t b, int nrep, double ms[4]);
 /* Autotuner of the Sutton-Chen pair kernels.  A kernel variant is
  * (lanes, unroll): each atom's neighbour list is split over `lanes`
  * consecutive lanes of a wavefront (1, 2, 4 or 8), combined by cross-lane
- * sums, and every lane keeps `unroll` neighbour gathers in flight (1, 2 or
- * 4).  Small walkers fill the chip better with more lanes per atom; more
+ * sums, and every lane keeps `unroll` neighbour gathers in flight (1, 2, 4
+ * or 8).  Small walkers fill the chip better with more lanes per atom; more
  * gathers in flight hide memory latency at the cost of registers.
  * akmc_batch_set_variant fixes the choice (default 1, 1; the graph of a
  * running search is recaptured).  akmc_batch_autotune times nrep force
  * evaluations at the resident positions for each of the AKMC_NVARIANTS
- * variants, ms[3*l + u] for lanes (1, 2, 4, 8)[l] and unroll (1, 2, 4)[u],
+ * variants, ms[4*l + u] for lanes (1, 2, 4, 8)[l] and unroll (1, 2, 4, 8)[u],
  * keeps the fastest and returns it in best[0] (lanes), best[1] (unroll).
  * Variants with the same lanes agree bitwise in FP32 (to FMA-contraction
  * round-off in FP64); different lanes agree to summation-order round-off.
  * No reference counterpart: the reference evaluates one configuration per
  * call on the host (potentials/metal/Field.cpp:293-490). */
-#define AKMC_NVARIANTS 12
+#define AKMC_NVARIANTS 16
 int akmc_batch_set_variant(akmc_batch_t b, int lanes, int unroll);
 int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[2], double ms[AKMC_NVARIANTS]);
 int akmc_batch_total_atoms(akmc_batch_t b);

@@ -170,7 +170,7 @@ def test_kernel_variants_and_autotune(lib, oracle, fp64):
     ftol, etol = (1e-9, 1e-11) if fp64 else (F_TOL, E_TOL_ATOM)
     got = {}
     for lanes in (1, 2, 4, 8):
-        for unroll in (1, 2, 4):
+        for unroll in (1, 2, 4, 8):
             b.set_variant(lanes, unroll)
             e = b.forces()
             _, frc = b.download()
@@ -183,7 +183,7 @@ def test_kernel_variants_and_autotune(lib, oracle, fp64):
                 assert df < ftol and de < etol, (lanes, unroll, w, df, de)
             print(f"variant {lanes}x{unroll} fp{64 if fp64 else 32}: max|dF| {worst:.2e}", flush=True)
         for w in range(len(cells)):
-            for unroll in (2, 4):
+            for unroll in (2, 4, 8):
                 # same summation order; the FP32 path is bitwise equal, the FP64
                 # path may differ by the compiler's FMA contraction choices
                 if fp64:
