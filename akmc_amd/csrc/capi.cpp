@@ -133,6 +133,8 @@ struct akmc_batch_s {
     // pinned status staging: one stream sync per host check of a run
     WalkerState *hws_pin = nullptr;
     int *ov_pin = nullptr;
+    double *pos_pin = nullptr;  // pinned staging of walker positions
+    size_t pos_pin_n = 0;
 };
 
 namespace {
@@ -561,6 +563,7 @@ void batch_free(akmc_batch_t b)
             if (e) (void)hipEventDestroy(e);
     if (b->hws_pin) (void)hipHostFree(b->hws_pin);
     if (b->ov_pin) (void)hipHostFree(b->ov_pin);
+    if (b->pos_pin) (void)hipHostFree(b->pos_pin);
     delete b;
 }
 
@@ -1349,6 +1352,35 @@ int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z)
     if (x) HIPCHK(hipMemcpy(x, b->s.X, A * sizeof(double), hipMemcpyDeviceToHost));
     if (y) HIPCHK(hipMemcpy(y, b->s.X + A, A * sizeof(double), hipMemcpyDeviceToHost));
     if (z) HIPCHK(hipMemcpy(z, b->s.X + 2 * A, A * sizeof(double), hipMemcpyDeviceToHost));
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_get_walkers_positions(akmc_batch_t b, int nw, const int *walkers, double *out)
+{
+    if (!b || !b->search_alloc || nw < 0 || (nw && (!walkers || !out))) return fail(AKMC_EINVAL, "bad arguments");
+    size_t total = 0;
+    for (int k = 0; k < nw; ++k) {
+        if (walkers[k] < 0 || walkers[k] >= b->W) return fail(AKMC_EINVAL, "walker %d out of range", walkers[k]);
+        total += 3 * (size_t)b->hw[walkers[k]].n;
+    }
+    if (total > b->pos_pin_n) {
+        if (b->pos_pin) HIPCHK(hipHostFree(b->pos_pin));
+        b->pos_pin = nullptr;
+        b->pos_pin_n = 0;
+        HIPCHK(hipHostMalloc((void **)&b->pos_pin, total * sizeof(double), hipHostMallocDefault));
+        b->pos_pin_n = total;
+    }
+    size_t o = 0;
+    const size_t A = b->A;
+    for (int k = 0; k < nw; ++k) {
+        const WalkerDev &h = b->hw[walkers[k]];
+        for (int c = 0; c < 3; ++c)
+            HIPCHK(hipMemcpyAsync(b->pos_pin + o + (size_t)c * h.n, b->s.X + c * A + h.off, h.n * sizeof(double),
+                                  hipMemcpyDeviceToHost, b->stream));
+        o += 3 * (size_t)h.n;
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (total) std::memcpy(out, b->pos_pin, total * sizeof(double));
     return AKMC_SUCCESS;
 }
 

@@ -146,6 +146,7 @@ def load() -> C.CDLL:
     L.akmc_batch_time_phases.argtypes = [C.c_void_p, C.c_int, _D]
     L.akmc_batch_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.akmc_batch_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_get_walkers_positions.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.akmc_batch_kernel_timer.argtypes = [C.c_void_p, C.c_int]
     L.akmc_batch_kernel_time.argtypes = [C.c_void_p, _D, C.POINTER(C.c_longlong)]
     L.akmc_batch_eval_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
@@ -349,6 +350,24 @@ class Batch:
         m, x0, mode, fr = self._start_args(mode_per_walker, disp_per_walker, walkers, x0_per_walker,
                                            frozen_per_walker)
         check(load().akmc_batch_start_art(self._h, _ip(m), _dp(x0), _dp(mode), _ip(fr), C.byref(p)))
+
+    def walkers_positions(self, walkers):
+        """{w: (3, n_w) search positions} for several walkers, one transfer."""
+        ws = np.ascontiguousarray(walkers, dtype=np.int32)
+        n = self.natoms[ws]
+        buf = np.zeros(int(3 * n.sum()))
+        check(load().akmc_batch_get_walkers_positions(self._h, int(ws.size), _ip(ws), _dp(buf)))
+        out, o = {}, 0
+        for w, k in zip(ws, n):
+            out[int(w)] = buf[o:o + 3 * k].reshape(3, k)
+            o += 3 * k
+        return out
+
+    def status_array(self):
+        """Walker statuses as a numpy structured array (fields of WalkerStatus)."""
+        st = (WalkerStatus * self.W)()
+        check(load().akmc_batch_status(self._h, st))
+        return np.ctypeslib.as_array(st)
 
     def walker_positions(self, w):
         n = int(self.natoms[w])

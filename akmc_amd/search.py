@@ -188,13 +188,15 @@ class SaddleCampaign:
     def harvest(self, stats: CampaignStats):
         """Collect ended searches (their FP64 verification already ran on
         the device); returns the walkers that ended."""
-        st = self.batch.status()
-        done = [w for w in sorted(self.pending) if st[w]["done"]]
+        st = self.batch.status_array()
+        done = [w for w in sorted(self.pending) if st["done"][w]]
+        good = [w for w in done if st["status"][w] == 0 and st["verified"][w]]
+        positions = self.batch.walkers_positions(good) if good else {}
         for w in done:
             s = st[w]
             self.pending.discard(w)
             stats.searches += 1
-            stats.force_calls += s["force_calls"]
+            stats.force_calls += int(s["force_calls"])
             if s["status"] != 0:
                 continue
             stats.converged += 1
@@ -205,7 +207,7 @@ class SaddleCampaign:
             b64 = s["energy64"] - self.basin64[w]
             err = abs(b32 - b64)
             stats.max_barrier_err = max(stats.max_barrier_err, err)
-            pos = self.batch.walker_positions(w)
+            pos = positions[w]
             d = W.min_image(pos - self.basins[w], self.cells[w].lat)
             dn = np.sqrt((d * d).sum(0))
             atoms = np.nonzero(dn > 0.3)[0]

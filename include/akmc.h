@@ -229,6 +229,9 @@ int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z);
 int akmc_batch_get_mode(akmc_batch_t b, double *mode);
 /* search positions of one walker (x, y, z of its n atoms) */
 int akmc_batch_get_walker_positions(akmc_batch_t b, int walker, double *x, double *y, double *z);
+/* search positions of several walkers in one transfer: out holds, for each
+ * listed walker in order, its x[n], y[n], z[n] */
+int akmc_batch_get_walkers_positions(akmc_batch_t b, int nw, const int *walkers, double *out);
 /* keep the current search positions of every walker as its basin, and put
  * the walkers flagged in walkers (all when NULL) back on their basin */
 int akmc_batch_save_basin(akmc_batch_t b);
