@@ -72,6 +72,13 @@ class WalkerStatus(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+# numpy view of WalkerStatus with the C layout's offsets
+_STATUS_DTYPE = np.dtype({"names": [k for k, _ in WalkerStatus._fields_],
+                          "formats": [np.int32 if t is C.c_int else np.float64 for _, t in WalkerStatus._fields_],
+                          "offsets": [getattr(WalkerStatus, k).offset for k, _ in WalkerStatus._fields_],
+                          "itemsize": C.sizeof(WalkerStatus)})
+
+
 def relax_defaults(**kw) -> RelaxParams:
     p = RelaxParams()
     load().akmc_relax_defaults(C.byref(p))
@@ -367,7 +374,7 @@ class Batch:
         """Walker statuses as a numpy structured array (fields of WalkerStatus)."""
         st = (WalkerStatus * self.W)()
         check(load().akmc_batch_status(self._h, st))
-        return np.ctypeslib.as_array(st)
+        return np.frombuffer(st, dtype=_STATUS_DTYPE).copy()
 
     def walker_positions(self, w):
         n = int(self.natoms[w])

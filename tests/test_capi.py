@@ -70,3 +70,16 @@ def test_python_binding_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(lib, "_lib", None)
     with pytest.raises(ImportError):
         lib.load()
+
+
+def test_status_array_layout():
+    """The numpy view of akmc_walker_status reads every field at its C offset."""
+    import ctypes as C
+    import numpy as np
+    from akmc_amd import lib as L
+    st = (L.WalkerStatus * 3)()
+    st[1].done, st[1].energy, st[2].lambda2, st[2].force_calls, st[0].verified = 1, -3.5, 1.25, 77, 1
+    a = np.frombuffer(st, dtype=L._STATUS_DTYPE)
+    assert L._STATUS_DTYPE.itemsize == C.sizeof(L.WalkerStatus)
+    assert list(a["done"]) == [0, 1, 0] and list(a["force_calls"]) == [0, 0, 77]
+    assert a["energy"][1] == -3.5 and a["lambda2"][2] == 1.25 and a["verified"][0] == 1
