@@ -201,7 +201,7 @@ __device__ inline int cell_coord(double x, double invL, int nc)
 __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
 {
     __shared__ int fill[AKMC_MAX_WALKER_CELLS];
-    __shared__ int chunk[kBlock];
+    __shared__ __align__(16) int chunk[kBlock];
     __shared__ int carry;
     const int w = blockIdx.x;
     if (!b.rebuild[w]) return;
@@ -250,8 +250,17 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
         __syncthreads();
         int slot = 0;
         if (c >= 0) {
+            // rank among the same-cell atoms earlier in the chunk, read four
+            // cells per LDS load
             int rank = 0;
-            for (int t = 0; t < tid; ++t) rank += chunk[t] == c;
+            const int4 *c4 = reinterpret_cast<const int4 *>(chunk);
+            const int full = tid >> 2;
+#pragma unroll 4
+            for (int q = 0; q < full; ++q) {
+                const int4 v = c4[q];
+                rank += (v.x == c) + (v.y == c) + (v.z == c) + (v.w == c);
+            }
+            for (int t = full * 4; t < tid; ++t) rank += chunk[t] == c;
             slot = fill[c] + rank;
         }
         __syncthreads();  // every slot is read before the fills move
@@ -280,12 +289,13 @@ __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
 // lanes per atom; the neighbouring cells (c-1..c+1 per axis, every cell
 // once along axes with fewer than 3 cells) are staged through LDS kBlock
 // atoms at a time and read by all threads alike.  The four lanes of an atom
-// test four consecutive candidates at once and place their hits by a prefix
-// count over the four, so entries come out exactly in candidate order: cell
-// order, then ascending index; with a near shell (ionic fields) the entries
-// inside it come first.  Entries are every j != i with min-image distance
+// test sixteen consecutive candidates per round, four each, and place their
+// hits by prefix counts over the four lanes, so entries come out exactly in
+// candidate order: cell order, then ascending index; with a near shell
+// (ionic fields) the entries inside it come first.  Entries are every j != i with min-image distance
 // <= rc + skin.
 constexpr int kNbLanes = 4;
+constexpr int kNbIlp = 4;  // candidates per lane per round
 
 __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
 {
@@ -349,27 +359,37 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
                                 sz[threadIdx.x] = b.z[j];
                             }
                             __syncthreads();
-                            for (int g = 0; g < m; g += kNbLanes) {
-                                const int t = g + lane;
-                                int hit = 0;
-                                int j = 0;
-                                if (have && t < m) {
-                                    j = sj[t];
-                                    double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
-                                    min_image(dx, dy, dz, W);
-                                    const double r2 = dx * dx + dy * dy + dz * dz;
-                                    const bool near = r2 <= rn2;
-                                    hit = (j != i && r2 <= rl2 && (pass == 0 ? near : !near)) ? 1 : 0;
+                            // each lane tests kNbIlp candidates per round (independent
+                            // chains); round candidates g + kNbLanes*u + lane keep
+                            // their order through the per-u prefix counts
+                            for (int g = 0; g < m; g += kNbLanes * kNbIlp) {
+                                int hit[kNbIlp], jj[kNbIlp];
+#pragma unroll
+                                for (int u = 0; u < kNbIlp; ++u) {
+                                    const int t = g + kNbLanes * u + lane;
+                                    hit[u] = 0;
+                                    jj[u] = 0;
+                                    if (have && t < m) {
+                                        jj[u] = sj[t];
+                                        double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
+                                        min_image(dx, dy, dz, W);
+                                        const double r2 = dx * dx + dy * dy + dz * dz;
+                                        const bool near = r2 <= rn2;
+                                        hit[u] = (jj[u] != i && r2 <= rl2 && (pass == 0 ? near : !near)) ? 1 : 0;
+                                    }
                                 }
-                                // inclusive prefix of the hits over the atom's four lanes
-                                int inc = hit;
-                                for (int o = 1; o < kNbLanes; o <<= 1) {
-                                    const int v = __shfl_up(inc, o, kNbLanes);
-                                    if (lane >= o) inc += v;
+#pragma unroll
+                                for (int u = 0; u < kNbIlp; ++u) {
+                                    // inclusive prefix of the hits over the atom's four lanes
+                                    int inc = hit[u];
+                                    for (int o = 1; o < kNbLanes; o <<= 1) {
+                                        const int v = __shfl_up(inc, o, kNbLanes);
+                                        if (lane >= o) inc += v;
+                                    }
+                                    const int total = __shfl(inc, kNbLanes - 1, kNbLanes);
+                                    if (hit[u]) put_nbr(b, i, cnt + inc - 1, uint32_t(jj[u] - W.off) | AKMC_NB_ZERO_SHIFT);
+                                    cnt += total;
                                 }
-                                const int total = __shfl(inc, kNbLanes - 1, kNbLanes);
-                                if (hit) put_nbr(b, i, cnt + inc - 1, uint32_t(j - W.off) | AKMC_NB_ZERO_SHIFT);
-                                cnt += total;
                             }
                         }
                     }

@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on `bench.py --forces-only 5`,
 # calibrated on a 1 GiB device copy (FETCH_SIZE reads half the copied bytes,
 # WRITE_SIZE all of them); the summary is under profiles/
-TRAFFIC_BYTES_PER_PAIR = 3.318
+TRAFFIC_BYTES_PER_PAIR = 3.288
 
 
 def ordered_pairs(cells, positions, rc):
