@@ -92,6 +92,8 @@ struct BatchDev {
     int *overflow;        // sticky capacity flag
     int *cell_count, *cell_start, *cell_atoms, *atom_cell;
     const int *cell_wk;   // cell -> walker
+    int *cell_work;       // cells whose lists are rebuilt this step (queued by k_cell_bin)
+    int *cell_work_n;
     double *rho;          // SC density
     float *inv_sqrt_rho;  // SC 1/sqrt(rho), -1 when rho < 1e-8 (FP32 walkers)
     double *isr64;        // the same in FP64 (FP64 walkers)

@@ -525,6 +525,9 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
         TRY(dalloc(b, &cw, ncells));
         HIPCHK(hipMemcpy(cw, hcw.data(), ncells * sizeof(int), hipMemcpyHostToDevice));
         d.cell_wk = cw;
+        TRY(dalloc(b, &d.cell_work, ncells));
+        TRY(dalloc(b, &d.cell_work_n, 1));
+        HIPCHK(hipMemset(d.cell_work_n, 0, sizeof(int)));
     }
     HIPCHK(hipMemset(d.rebuild, 0, W * sizeof(int)));
     {

@@ -152,6 +152,7 @@ __device__ inline void entry_disp(uint32_t e, double &dx, double &dy, double &dz
 
 __global__ void k_nbr_check(BatchDev b)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *b.cell_work_n = 0;  // k_cell_bin queues this step's cells
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
     int w = walker_of(b, i);
@@ -270,6 +271,10 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
         }
         __syncthreads();
     }
+    // queue this walker's cells for the list build
+    if (tid == 0) carry = atomicAdd(b.cell_work_n, nc);
+    __syncthreads();
+    for (int c = tid; c < nc; c += kBlock) b.cell_work[carry + c] = W.cell_off + c;
 }
 
 __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
@@ -285,17 +290,21 @@ __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
 }
 
 // Minimum-image Verlet list, one workgroup per cell of a walker whose list
-// is rebuilt.  The home cell's atoms are spread over the workgroup, four
-// lanes per atom; the neighbouring cells (c-1..c+1 per axis, every cell
-// once along axes with fewer than 3 cells) are staged through LDS kBlock
-// atoms at a time and read by all threads alike.  The four lanes of an atom
-// test sixteen consecutive candidates per round, four each, and place their
-// hits by prefix counts over the four lanes, so entries come out exactly in
-// candidate order: cell order, then ascending index; with a near shell
-// (ionic fields) the entries inside it come first.  Entries are every j != i with min-image distance
-// <= rc + skin.
+// is rebuilt.  The candidates -- the atoms of the neighbouring cells
+// (c-1..c+1 per axis, every cell once along axes with fewer than 3 cells),
+// in cell order and ascending index within a cell -- are staged into LDS in
+// one cooperative pass (every thread's loads in flight together), up to
+// kNbCap at a time.  The home cell's atoms are spread over the workgroup,
+// four lanes per atom and up to kNbHome chunks of 64 atoms per thread; the
+// four lanes of an atom test sixteen consecutive candidates per round and
+// place their hits by prefix counts over the four lanes, so entries come
+// out exactly in candidate order; with a near shell (ionic fields) all
+// entries inside it come first.  Entries are every j != i with min-image
+// distance <= rc + skin.
 constexpr int kNbLanes = 4;
-constexpr int kNbIlp = 4;  // candidates per lane per round
+constexpr int kNbIlp = 4;     // candidates per lane per round
+constexpr int kNbCap = 2048;  // candidates staged in LDS at a time
+constexpr int kNbHome = 4;    // home-atom chunks held per thread
 
 __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
 {
@@ -308,99 +317,145 @@ __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
+// one cell of the work list
+__device__ void nbr_build_cell(const BatchDev &b, int cg, double *sx, double *sy, double *sz, int *sj, int *c_start,
+                               int *c_pref, int &n_cells)
 {
-    __shared__ double sx[kBlock], sy[kBlock], sz[kBlock];
-    __shared__ int sj[kBlock];
-    const int cg = blockIdx.x;
     const int w = b.cell_wk[cg];
-    if (!b.rebuild[w]) return;
     const WalkerDev W = b.walkers[w];
-    const int lc = cg - W.cell_off;
-    const int cen[3] = {lc / (W.nc[1] * W.nc[2]), (lc / W.nc[2]) % W.nc[1], lc % W.nc[2]};
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-        if (W.nc[a] >= 3) {
-            lo[a] = cen[a] - 1;
-            hi[a] = cen[a] + 1;
-        } else {
-            lo[a] = 0;
-            hi[a] = W.nc[a] - 1;
+    if (threadIdx.x == 0) {
+        const int lc = cg - W.cell_off;
+        const int cen[3] = {lc / (W.nc[1] * W.nc[2]), (lc / W.nc[2]) % W.nc[1], lc % W.nc[2]};
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            if (W.nc[a] >= 3) {
+                lo[a] = cen[a] - 1;
+                hi[a] = cen[a] + 1;
+            } else {
+                lo[a] = 0;
+                hi[a] = W.nc[a] - 1;
+            }
         }
+        int nc = 0;
+        c_pref[0] = 0;
+        for (int pa = lo[0]; pa <= hi[0]; ++pa)
+            for (int pb = lo[1]; pb <= hi[1]; ++pb)
+                for (int pc = lo[2]; pc <= hi[2]; ++pc) {
+                    const int qa = (pa + W.nc[0]) % W.nc[0], qb = (pb + W.nc[1]) % W.nc[1];
+                    const int qc = (pc + W.nc[2]) % W.nc[2];
+                    const int cid = W.cell_off + (qa * W.nc[1] + qb) * W.nc[2] + qc;
+                    c_start[nc] = b.cell_start[cid];
+                    c_pref[nc + 1] = c_pref[nc] + b.cell_count[cid];
+                    ++nc;
+                }
+        n_cells = nc;
     }
+    __syncthreads();
+    const int ncand = c_pref[n_cells];
     const double rl2 = b.field->rlist2;
     const double rn2 = b.field->rnear2;
     const int s0 = b.cell_start[cg], n0 = b.cell_count[cg];
     const int lane = threadIdx.x % kNbLanes, slot = threadIdx.x / kNbLanes;
     constexpr int kSlots = kBlock / kNbLanes;
-    for (int hb = 0; hb < n0; hb += kSlots) {
-        const bool have = hb + slot < n0;
-        const int i = have ? b.cell_atoms[s0 + hb + slot] : 0;
-        const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
-        int cnt = 0, nnear = 0;
+    for (int hg = 0; hg < n0; hg += kSlots * kNbHome) {  // groups of home chunks
+        int hi_[kNbHome], cnt[kNbHome], nnear[kNbHome];
+        double hx[kNbHome], hy[kNbHome], hz[kNbHome];
+        bool have[kNbHome];
+#pragma unroll
+        for (int h = 0; h < kNbHome; ++h) {
+            const int q = hg + h * kSlots + slot;
+            have[h] = q < n0;
+            hi_[h] = have[h] ? b.cell_atoms[s0 + q] : 0;
+            hx[h] = have[h] ? b.x[hi_[h]] : 0.0;
+            hy[h] = have[h] ? b.y[hi_[h]] : 0.0;
+            hz[h] = have[h] ? b.z[hi_[h]] : 0.0;
+            cnt[h] = 0;
+            nnear[h] = 0;
+        }
         // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
         for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
-            for (int pa = lo[0]; pa <= hi[0]; ++pa) {
-                const int qa = (pa + W.nc[0]) % W.nc[0];
-                for (int pb = lo[1]; pb <= hi[1]; ++pb) {
-                    const int qb = (pb + W.nc[1]) % W.nc[1];
-                    for (int pc = lo[2]; pc <= hi[2]; ++pc) {
-                        const int qc = (pc + W.nc[2]) % W.nc[2];
-                        const int cid = W.cell_off + (qa * W.nc[1] + qb) * W.nc[2] + qc;
-                        const int s = b.cell_start[cid], n = b.cell_count[cid];
-                        for (int tb = 0; tb < n; tb += kBlock) {
-                            const int m = min(kBlock, n - tb);
-                            __syncthreads();
-                            if ((int)threadIdx.x < m) {
-                                const int j = b.cell_atoms[s + tb + threadIdx.x];
-                                sj[threadIdx.x] = j;
-                                sx[threadIdx.x] = b.x[j];
-                                sy[threadIdx.x] = b.y[j];
-                                sz[threadIdx.x] = b.z[j];
-                            }
-                            __syncthreads();
-                            // each lane tests kNbIlp candidates per round (independent
-                            // chains); round candidates g + kNbLanes*u + lane keep
-                            // their order through the per-u prefix counts
-                            for (int g = 0; g < m; g += kNbLanes * kNbIlp) {
-                                int hit[kNbIlp], jj[kNbIlp];
+            for (int seg = 0; seg < ncand; seg += kNbCap) {
+                const int m = min(kNbCap, ncand - seg);
+                if (!(pass == 1 && m == ncand && rn2 > 0.0)) {  // a single segment stays staged
+                    __syncthreads();
+                    for (int t = threadIdx.x; t < m; t += kBlock) {
+                        const int q = seg + t;
+                        int c = 0;
+                        while (c_pref[c + 1] <= q) ++c;
+                        const int j = b.cell_atoms[c_start[c] + q - c_pref[c]];
+                        sj[t] = j;
+                        sx[t] = b.x[j];
+                        sy[t] = b.y[j];
+                        sz[t] = b.z[j];
+                    }
+                    __syncthreads();
+                }
 #pragma unroll
-                                for (int u = 0; u < kNbIlp; ++u) {
-                                    const int t = g + kNbLanes * u + lane;
-                                    hit[u] = 0;
-                                    jj[u] = 0;
-                                    if (have && t < m) {
-                                        jj[u] = sj[t];
-                                        double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
-                                        min_image(dx, dy, dz, W);
-                                        const double r2 = dx * dx + dy * dy + dz * dz;
-                                        const bool near = r2 <= rn2;
-                                        hit[u] = (jj[u] != i && r2 <= rl2 && (pass == 0 ? near : !near)) ? 1 : 0;
-                                    }
-                                }
+                for (int h = 0; h < kNbHome; ++h) {
+                    if (hg + h * kSlots >= n0) break;  // uniform: no home atoms left in this chunk
+                    const int i = hi_[h];
+                    for (int g = 0; g < m; g += kNbLanes * kNbIlp) {
+                        int hit[kNbIlp], jj[kNbIlp];
 #pragma unroll
-                                for (int u = 0; u < kNbIlp; ++u) {
-                                    // inclusive prefix of the hits over the atom's four lanes
-                                    int inc = hit[u];
-                                    for (int o = 1; o < kNbLanes; o <<= 1) {
-                                        const int v = __shfl_up(inc, o, kNbLanes);
-                                        if (lane >= o) inc += v;
-                                    }
-                                    const int total = __shfl(inc, kNbLanes - 1, kNbLanes);
-                                    if (hit[u]) put_nbr(b, i, cnt + inc - 1, uint32_t(jj[u] - W.off) | AKMC_NB_ZERO_SHIFT);
-                                    cnt += total;
-                                }
+                        for (int u = 0; u < kNbIlp; ++u) {
+                            const int t = g + kNbLanes * u + lane;
+                            hit[u] = 0;
+                            jj[u] = 0;
+                            if (have[h] && t < m) {
+                                jj[u] = sj[t];
+                                double dx = hx[h] - sx[t], dy = hy[h] - sy[t], dz = hz[h] - sz[t];
+                                min_image(dx, dy, dz, W);
+                                const double r2 = dx * dx + dy * dy + dz * dz;
+                                const bool near = r2 <= rn2;
+                                hit[u] = (jj[u] != i && r2 <= rl2 && (pass == 0 ? near : !near)) ? 1 : 0;
                             }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kNbIlp; ++u) {
+                            // inclusive prefix of the hits over the atom's four lanes
+                            int inc = hit[u];
+                            for (int o = 1; o < kNbLanes; o <<= 1) {
+                                const int v = __shfl_up(inc, o, kNbLanes);
+                                if (lane >= o) inc += v;
+                            }
+                            const int total = __shfl(inc, kNbLanes - 1, kNbLanes);
+                            if (hit[u]) put_nbr(b, i, cnt[h] + inc - 1, uint32_t(jj[u] - W.off) | AKMC_NB_ZERO_SHIFT);
+                            cnt[h] += total;
                         }
                     }
                 }
             }
-            if (pass == 0) nnear = cnt;
+            if (pass == 0) {
+#pragma unroll
+                for (int h = 0; h < kNbHome; ++h) nnear[h] = cnt[h];
+            }
         }
-        if (have && lane == 0) {
-            b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
-            b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
-        }
+#pragma unroll
+        for (int h = 0; h < kNbHome; ++h)
+            if (have[h] && lane == 0) {
+                b.nnb[hi_[h]] = cnt[h] < b.maxnb ? cnt[h] : b.maxnb;
+                b.nnear[hi_[h]] = nnear[h] < b.maxnb ? nnear[h] : b.maxnb;
+            }
+        __syncthreads();  // the next home group restages the candidates
+    }
+}
+
+// A fixed grid works through the cells that k_cell_bin queued this step
+// (the cells of rebuilding walkers), so steps without rebuilds cost one
+// short launch and the LDS of the staging buffers is claimed only by
+// workgroups with work.
+constexpr int kNbGrid = 512;
+
+__global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
+{
+    __shared__ double sx[kNbCap], sy[kNbCap], sz[kNbCap];
+    __shared__ int sj[kNbCap];
+    __shared__ int c_start[27], c_pref[28];
+    __shared__ int n_cells;
+    const int items = *b.cell_work_n;
+    for (int it = blockIdx.x; it < items; it += gridDim.x) {
+        nbr_build_cell(b, b.cell_work[it], sx, sy, sz, sj, c_start, c_pref, n_cells);
+        __syncthreads();
     }
 }
 
@@ -1042,7 +1097,7 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
     k_ref_positions<<<gA, kBlock, 0, s>>>(b);
     if (hf.min_image) {
         k_cell_bin<<<b.W, kBlock, 0, s>>>(b);
-        k_nbr_build_cells<<<b.ncells_total, kBlock, 0, s>>>(b);
+        k_nbr_build_cells<<<kNbGrid, kBlock, 0, s>>>(b);
     } else {
         k_nbr_build_images<<<gA, kBlock, 0, s>>>(b);
     }
