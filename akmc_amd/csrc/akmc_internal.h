@@ -90,7 +90,7 @@ struct BatchDev {
     int *nnear;           // leading list entries inside the near shell
     int *rebuild;         // per walker flag
     int *overflow;        // sticky capacity flag
-    int *cell_count, *cell_start, *cell_atoms, *atom_cell;
+    int *cell_count, *cell_start, *cell_atoms;
     const int *cell_wk;   // cell -> walker
     int *cell_work;       // cells whose lists are rebuilt this step (queued by k_cell_bin)
     int *cell_work_n;

@@ -473,7 +473,6 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     TRY(dalloc(b, &d.cell_count, ncells));
     TRY(dalloc(b, &d.cell_start, ncells));
     TRY(dalloc(b, &d.cell_atoms, A));
-    TRY(dalloc(b, &d.atom_cell, A));
     TRY(dalloc(b, &d.rho, A));
     TRY(dalloc(b, &d.inv_sqrt_rho, A));
     TRY(dalloc(b, &d.isr64, A));
@@ -525,7 +524,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
         TRY(dalloc(b, &cw, ncells));
         HIPCHK(hipMemcpy(cw, hcw.data(), ncells * sizeof(int), hipMemcpyHostToDevice));
         d.cell_wk = cw;
-        TRY(dalloc(b, &d.cell_work, ncells));
+        TRY(dalloc(b, &d.cell_work, (size_t)ncells + A / 16 + 1));  // list-build items (cell, home chunk)
         TRY(dalloc(b, &d.cell_work_n, 1));
         HIPCHK(hipMemset(d.cell_work_n, 0, sizeof(int)));
     }

@@ -192,17 +192,26 @@ __device__ inline int cell_coord(double x, double invL, int nc)
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
+// walker-local cell of atom i
+__device__ inline int atom_cell_of(const BatchDev &b, const WalkerDev &W, int i)
+{
+    const int ca = cell_coord(b.x[i], W.invL[0], W.nc[0]);
+    const int cb = cell_coord(b.y[i], W.invL[1], W.nc[1]);
+    const int cc = cell_coord(b.z[i], W.invL[2], W.nc[2]);
+    return (ca * W.nc[1] + cb) * W.nc[2] + cc;
+}
+
+// home atoms per list-build work item (k_nbr_build_cells)
+constexpr int kNbSlotsQ = 16;
+
 // Cell grid of one walker per workgroup (walkers whose list is rebuilt):
 // counts, exclusive scan and a stable placement of the atoms in index order,
 // so every cell lists its atoms by ascending index and the neighbour lists
-// (and the FP64 sums that follow them) do not depend on scheduling.  The
-// placement takes the atoms kBlock at a time; an atom's slot is the cell's
-// running fill plus its rank among earlier atoms of the same cell in the
-// chunk (an LDS count).
+// (and the FP64 sums that follow them) do not depend on scheduling.
 __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
 {
     __shared__ int fill[AKMC_MAX_WALKER_CELLS];
-    __shared__ __align__(16) int chunk[kBlock];
+    __shared__ int chunk[kBlock], rank_[kBlock], gsize[kBlock];
     __shared__ int carry;
     const int w = blockIdx.x;
     if (!b.rebuild[w]) return;
@@ -211,15 +220,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
     const int tid = threadIdx.x;
     for (int c = tid; c < nc; c += kBlock) fill[c] = 0;
     __syncthreads();
-    for (int l = tid; l < W.n; l += kBlock) {
-        const int i = W.off + l;
-        const int ca = cell_coord(b.x[i], W.invL[0], W.nc[0]);
-        const int cb = cell_coord(b.y[i], W.invL[1], W.nc[1]);
-        const int cc = cell_coord(b.z[i], W.invL[2], W.nc[2]);
-        const int lc = (ca * W.nc[1] + cb) * W.nc[2] + cc;
-        b.atom_cell[i] = lc;
-        atomicAdd(&fill[lc], 1);
-    }
+    for (int l = tid; l < W.n; l += kBlock) atomicAdd(&fill[atom_cell_of(b, W, W.off + l)], 1);
     if (tid == 0) carry = W.off;
     __syncthreads();
     // exclusive scan of the counts; fill[c] becomes the cell's next free slot
@@ -244,37 +245,54 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
         if (tid == kBlock - 1) carry += chunk[tid];
         __syncthreads();
     }
+    // stable placement in index order, kBlock atoms at a time: each
+    // wavefront finds, by ballots, the lanes of its 64 atoms that hold the
+    // same cell (rank below, group size, highest lane); then one wavefront
+    // walks the four groups of 64 in order: an atom's slot is the cell's fill
+    // plus its rank, and the highest lane of each group advances the fill
     for (int base = 0; base < W.n; base += kBlock) {
         const int l = base + tid;
-        const int c = l < W.n ? b.atom_cell[W.off + l] : -1;
-        chunk[tid] = c;
-        __syncthreads();
-        int slot = 0;
-        if (c >= 0) {
-            // rank among the same-cell atoms earlier in the chunk, read four
-            // cells per LDS load
-            int rank = 0;
-            const int4 *c4 = reinterpret_cast<const int4 *>(chunk);
-            const int full = tid >> 2;
-#pragma unroll 4
-            for (int q = 0; q < full; ++q) {
-                const int4 v = c4[q];
-                rank += (v.x == c) + (v.y == c) + (v.z == c) + (v.w == c);
-            }
-            for (int t = full * 4; t < tid; ++t) rank += chunk[t] == c;
-            slot = fill[c] + rank;
+        const int c = l < W.n ? atom_cell_of(b, W, W.off + l) : -1;
+        unsigned long long todo = __ballot(c >= 0);
+        unsigned long long same = 0;
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int cl = __shfl(c, leader);
+            const unsigned long long grp = __ballot(c == cl);
+            if (c == cl) same = grp;
+            todo &= ~grp;
         }
-        __syncthreads();  // every slot is read before the fills move
-        if (c >= 0) {
-            b.cell_atoms[slot] = W.off + l;
-            atomicAdd(&fill[c], 1);
+        const int wl = tid & 63;
+        chunk[tid] = c;
+        rank_[tid] = __popcll(same & ((1ull << wl) - 1ull));
+        gsize[tid] = (c >= 0 && (same >> wl) == 1ull) ? __popcll(same) : 0;  // set on the group's highest lane
+        __syncthreads();
+        if (tid < 64) {
+            for (int wv = 0; wv < kBlock / 64; ++wv) {
+                const int k = wv * 64 + tid;
+                const int ck = chunk[k];
+                if (ck >= 0) {
+                    b.cell_atoms[fill[ck] + rank_[k]] = W.off + base + k;
+                    if (gsize[k]) fill[ck] += gsize[k];
+                }
+            }
         }
         __syncthreads();
     }
-    // queue this walker's cells for the list build
-    if (tid == 0) carry = atomicAdd(b.cell_work_n, nc);
-    __syncthreads();
-    for (int c = tid; c < nc; c += kBlock) b.cell_work[carry + c] = W.cell_off + c;
+    // queue this walker's list-build items: home chunks of kNbSlots atoms per
+    // cell (chunk indices above 255 are served by the item of chunk % 256)
+    if (tid == 0) {
+        int items = 0;
+        for (int c = 0; c < nc; ++c) {
+            const int k = (b.cell_count[W.cell_off + c] + kNbSlotsQ - 1) / kNbSlotsQ;
+            items += k < 256 ? k : 256;
+        }
+        int at = atomicAdd(b.cell_work_n, items);
+        for (int c = 0; c < nc; ++c) {
+            const int k = (b.cell_count[W.cell_off + c] + kNbSlotsQ - 1) / kNbSlotsQ;
+            for (int ch = 0; ch < (k < 256 ? k : 256); ++ch) b.cell_work[at++] = ((W.cell_off + c) << 8) | ch;
+        }
+    }
 }
 
 __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
@@ -289,22 +307,22 @@ __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
     ++cnt;
 }
 
-// Minimum-image Verlet list, one workgroup per cell of a walker whose list
-// is rebuilt.  The candidates -- the atoms of the neighbouring cells
-// (c-1..c+1 per axis, every cell once along axes with fewer than 3 cells),
-// in cell order and ascending index within a cell -- are staged into LDS in
+// Minimum-image Verlet list.  A work item is up to kNbSlots home atoms of
+// one cell of a walker whose list is rebuilt (queued by k_cell_bin); a
+// workgroup stages the item's candidates -- the atoms of the neighbouring
+// cells (c-1..c+1 per axis, every cell once along axes with fewer than 3
+// cells), in cell order and ascending index within a cell -- into LDS in
 // one cooperative pass (every thread's loads in flight together), up to
-// kNbCap at a time.  The home cell's atoms are spread over the workgroup,
-// four lanes per atom and up to kNbHome chunks of 64 atoms per thread; the
-// four lanes of an atom test sixteen consecutive candidates per round and
-// place their hits by prefix counts over the four lanes, so entries come
-// out exactly in candidate order; with a near shell (ionic fields) all
-// entries inside it come first.  Entries are every j != i with min-image
-// distance <= rc + skin.
-constexpr int kNbLanes = 4;
-constexpr int kNbIlp = 4;     // candidates per lane per round
-constexpr int kNbCap = 2048;  // candidates staged in LDS at a time
-constexpr int kNbHome = 4;    // home-atom chunks held per thread
+// kNbCap at a time.  kNbLanes lanes serve one home atom: they test
+// kNbLanes * kNbIlp consecutive candidates per round and place their hits
+// by counts in the wavefront's hit ballot, so entries come out exactly in candidate
+// order; with a near shell (ionic fields) all entries inside it come first.
+// Entries are every j != i with min-image distance <= rc + skin.
+constexpr int kNbLanes = 16;
+constexpr int kNbSlots = kBlock / kNbLanes;  // home atoms per work item
+static_assert(kNbSlots == kNbSlotsQ, "queued items and the list build agree on the chunk size");
+constexpr int kNbIlp = 4;                    // candidates per lane per round
+constexpr int kNbCap = 2048;                 // candidates staged in LDS at a time
 
 __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
 {
@@ -317,9 +335,9 @@ __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
     }
 }
 
-// one cell of the work list
-__device__ void nbr_build_cell(const BatchDev &b, int cg, double *sx, double *sy, double *sz, int *sj, int *c_start,
-                               int *c_pref, int &n_cells)
+// one work item: home atoms [chunk * kNbSlots, +kNbSlots) of cell cg
+__device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx, double *sy, double *sz, int *sj,
+                               int *c_start, int *c_pref, int &n_cells)
 {
     const int w = b.cell_wk[cg];
     const WalkerDev W = b.walkers[w];
@@ -356,94 +374,72 @@ __device__ void nbr_build_cell(const BatchDev &b, int cg, double *sx, double *sy
     const double rn2 = b.field->rnear2;
     const int s0 = b.cell_start[cg], n0 = b.cell_count[cg];
     const int lane = threadIdx.x % kNbLanes, slot = threadIdx.x / kNbLanes;
-    constexpr int kSlots = kBlock / kNbLanes;
-    for (int hg = 0; hg < n0; hg += kSlots * kNbHome) {  // groups of home chunks
-        int hi_[kNbHome], cnt[kNbHome], nnear[kNbHome];
-        double hx[kNbHome], hy[kNbHome], hz[kNbHome];
-        bool have[kNbHome];
-#pragma unroll
-        for (int h = 0; h < kNbHome; ++h) {
-            const int q = hg + h * kSlots + slot;
-            have[h] = q < n0;
-            hi_[h] = have[h] ? b.cell_atoms[s0 + q] : 0;
-            hx[h] = have[h] ? b.x[hi_[h]] : 0.0;
-            hy[h] = have[h] ? b.y[hi_[h]] : 0.0;
-            hz[h] = have[h] ? b.z[hi_[h]] : 0.0;
-            cnt[h] = 0;
-            nnear[h] = 0;
-        }
-        // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
-        for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
-            for (int seg = 0; seg < ncand; seg += kNbCap) {
-                const int m = min(kNbCap, ncand - seg);
-                if (!(pass == 1 && m == ncand && rn2 > 0.0)) {  // a single segment stays staged
-                    __syncthreads();
-                    for (int t = threadIdx.x; t < m; t += kBlock) {
-                        const int q = seg + t;
-                        int c = 0;
-                        while (c_pref[c + 1] <= q) ++c;
-                        const int j = b.cell_atoms[c_start[c] + q - c_pref[c]];
-                        sj[t] = j;
-                        sx[t] = b.x[j];
-                        sy[t] = b.y[j];
-                        sz[t] = b.z[j];
-                    }
-                    __syncthreads();
+    const int wlane = threadIdx.x & 63;
+    static_assert(kNbLanes == 16, "the hit ballot is split in 16-bit fields");
+    const int q = chunk * kNbSlots + slot;
+    const bool have = q < n0;
+    const int i = have ? b.cell_atoms[s0 + q] : 0;
+    const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
+    int cnt = 0, nnear = 0;
+    // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
+    for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
+        for (int seg = 0; seg < ncand; seg += kNbCap) {
+            const int m = min(kNbCap, ncand - seg);
+            if (!(pass == 1 && m == ncand && rn2 > 0.0)) {  // a single segment stays staged
+                __syncthreads();
+                for (int t = threadIdx.x; t < m; t += kBlock) {
+                    const int qq = seg + t;
+                    int c = 0;
+                    while (c_pref[c + 1] <= qq) ++c;
+                    const int j = b.cell_atoms[c_start[c] + qq - c_pref[c]];
+                    sj[t] = j;
+                    sx[t] = b.x[j];
+                    sy[t] = b.y[j];
+                    sz[t] = b.z[j];
                 }
+                __syncthreads();
+            }
+            for (int g = 0; g < m; g += kNbLanes * kNbIlp) {
+                int hit[kNbIlp], jj[kNbIlp];
 #pragma unroll
-                for (int h = 0; h < kNbHome; ++h) {
-                    if (hg + h * kSlots >= n0) break;  // uniform: no home atoms left in this chunk
-                    const int i = hi_[h];
-                    for (int g = 0; g < m; g += kNbLanes * kNbIlp) {
-                        int hit[kNbIlp], jj[kNbIlp];
-#pragma unroll
-                        for (int u = 0; u < kNbIlp; ++u) {
-                            const int t = g + kNbLanes * u + lane;
-                            hit[u] = 0;
-                            jj[u] = 0;
-                            if (have[h] && t < m) {
-                                jj[u] = sj[t];
-                                double dx = hx[h] - sx[t], dy = hy[h] - sy[t], dz = hz[h] - sz[t];
-                                min_image(dx, dy, dz, W);
-                                const double r2 = dx * dx + dy * dy + dz * dz;
-                                const bool near = r2 <= rn2;
-                                hit[u] = (jj[u] != i && r2 <= rl2 && (pass == 0 ? near : !near)) ? 1 : 0;
-                            }
-                        }
-#pragma unroll
-                        for (int u = 0; u < kNbIlp; ++u) {
-                            // inclusive prefix of the hits over the atom's four lanes
-                            int inc = hit[u];
-                            for (int o = 1; o < kNbLanes; o <<= 1) {
-                                const int v = __shfl_up(inc, o, kNbLanes);
-                                if (lane >= o) inc += v;
-                            }
-                            const int total = __shfl(inc, kNbLanes - 1, kNbLanes);
-                            if (hit[u]) put_nbr(b, i, cnt[h] + inc - 1, uint32_t(jj[u] - W.off) | AKMC_NB_ZERO_SHIFT);
-                            cnt[h] += total;
-                        }
+                for (int u = 0; u < kNbIlp; ++u) {
+                    const int t = g + kNbLanes * u + lane;
+                    hit[u] = 0;
+                    jj[u] = 0;
+                    if (have && t < m) {
+                        jj[u] = sj[t];
+                        double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
+                        min_image(dx, dy, dz, W);
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        const bool near = r2 <= rn2;
+                        hit[u] = (jj[u] != i && r2 <= rl2 && (pass == 0 ? near : !near)) ? 1 : 0;
                     }
                 }
-            }
-            if (pass == 0) {
 #pragma unroll
-                for (int h = 0; h < kNbHome; ++h) nnear[h] = cnt[h];
+                for (int u = 0; u < kNbIlp; ++u) {
+                    // the atom's lanes are a 16-bit field of the wavefront's hit
+                    // ballot: a lane's slot is the count of hits below it
+                    const unsigned long long bal = __ballot(hit[u]);
+                    const unsigned grp = unsigned(bal >> (wlane & ~(kNbLanes - 1))) & 0xFFFFu;
+                    if (hit[u])
+                        put_nbr(b, i, cnt + __popc(grp & ((1u << lane) - 1u)), uint32_t(jj[u] - W.off) | AKMC_NB_ZERO_SHIFT);
+                    cnt += __popc(grp);
+                }
             }
         }
-#pragma unroll
-        for (int h = 0; h < kNbHome; ++h)
-            if (have[h] && lane == 0) {
-                b.nnb[hi_[h]] = cnt[h] < b.maxnb ? cnt[h] : b.maxnb;
-                b.nnear[hi_[h]] = nnear[h] < b.maxnb ? nnear[h] : b.maxnb;
-            }
-        __syncthreads();  // the next home group restages the candidates
+        if (pass == 0) nnear = cnt;
+    }
+    if (have && lane == 0) {
+        b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
+        b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
     }
 }
 
-// A fixed grid works through the cells that k_cell_bin queued this step
-// (the cells of rebuilding walkers), so steps without rebuilds cost one
-// short launch and the LDS of the staging buffers is claimed only by
-// workgroups with work.
+// A fixed grid works through the items that k_cell_bin queued this step, so
+// steps without rebuilds cost one short launch and the LDS of the staging
+// buffers is claimed only by workgroups with work.  An item is encoded as
+// (cell << 8) | chunk; chunks beyond 255 of very full cells are taken by the
+// same item in strides of 256.
 constexpr int kNbGrid = 512;
 
 __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
@@ -454,8 +450,13 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
     __shared__ int n_cells;
     const int items = *b.cell_work_n;
     for (int it = blockIdx.x; it < items; it += gridDim.x) {
-        nbr_build_cell(b, b.cell_work[it], sx, sy, sz, sj, c_start, c_pref, n_cells);
-        __syncthreads();
+        const int code = b.cell_work[it];
+        const int cg = code >> 8, chunk0 = code & 255;
+        const int n0 = b.cell_count[cg];
+        for (int chunk = chunk0; chunk * kNbSlots < n0; chunk += 256) {
+            nbr_build_item(b, cg, chunk, sx, sy, sz, sj, c_start, c_pref, n_cells);
+            __syncthreads();
+        }
     }
 }
 
