@@ -647,8 +647,13 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
     if (sub != 0) return;
     b.rho[i] = rho;
     b.e_a[i] = -sqrt(rho);
-    b.inv_sqrt_rho[i] = rho < 1.0e-8 ? -1.0f : float(1.0 / sqrt(rho));
-    if (sizeof(T) == 8) b.isr64[i] = rho < 1.0e-8 ? -1.0 : 1.0 / sqrt(rho);
+    const float isr32 = rho < 1.0e-8 ? -1.0f : float(1.0 / sqrt(rho));
+    const double isr64 = rho < 1.0e-8 ? -1.0 : 1.0 / sqrt(rho);
+    b.inv_sqrt_rho[i] = isr32;
+    if (sizeof(T) == 8) b.isr64[i] = isr64;
+    // single pair type: the packed gather record carries 1/sqrt(rho) in place
+    // of the species for pass 2 (k_pack restores the species every step)
+    if (ONE) reinterpret_cast<double *>(&b.p4[i])[3] = sizeof(T) == 8 ? isr64 : double(isr32);
 }
 
 // pass 2: F_i = sum_j r_ij (fs - fm)/r^2, fs = n eps (a/r)^n,
@@ -692,8 +697,13 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
         for (int u = 0; u < U; ++u) {
             const int j = W.off + int(e[u] & AKMC_NB_JMASK);
             pj[u] = b.p4[j];
-            if (sizeof(T) == 8) isj64[u] = b.isr64[j];
-            else isj[u] = b.inv_sqrt_rho[j];
+            if (ONE) {  // 1/sqrt(rho_j) rides in the gather record (pass 1)
+                if (sizeof(T) == 8) isj64[u] = pj[u].w;
+                else isj[u] = float(pj[u].w);
+            } else {
+                if (sizeof(T) == 8) isj64[u] = b.isr64[j];
+                else isj[u] = b.inv_sqrt_rho[j];
+            }
         }
         double dx[U], dy[U], dz[U], r2[U];
         bool in[U];
