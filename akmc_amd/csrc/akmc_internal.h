@@ -73,9 +73,8 @@ struct WalkerOut {
 struct BatchDev {
     int A, W, maxnb, ncells_total, nk_total, G1, maxnk;
     int lpa;              // lanes per atom of the Sutton-Chen pair kernels (1, 2, 4)
-    int nb16;             // list entries are 16-bit local indices in blocks of 8
-                          // per atom (minimum-image many-body lists of walkers
-                          // below 65536 atoms)
+    int nb16;             // list entries are 16-bit local indices (minimum-image
+                          // many-body lists of walkers below 65536 atoms)
     int unroll;           // neighbour gathers in flight per lane (1, 4)
     const FieldDev *field;
     const WalkerDev *walkers;
@@ -111,19 +110,11 @@ struct BatchDev {
     int *active;          // per walker: 0 skip, 1 FP32 pair terms, 2 FP64
 };
 
-// position of entry k of atom i's neighbour list: 32-bit lists are
-// column-major [k][A]; 16-bit lists hold blocks of 8 entries per atom,
-// [k / 8][A][k % 8], so one 16-byte load reads 8 entries of one atom
-__host__ __device__ inline size_t nbr_slot(const BatchDev &b, int k, int i)
-{
-    return b.nb16 ? ((size_t)(k >> 3) * b.A + i) * 8 + (k & 7) : (size_t)k * b.A + i;
-}
-
 // entry k of atom i's neighbour list in the 32-bit form (local index and
 // image shift), whichever width the batch stores
 __device__ inline uint32_t nbr_entry(const BatchDev &b, int k, int i)
 {
-    const size_t o = nbr_slot(b, k, i);
+    const size_t o = (size_t)k * b.A + i;
     return b.nb16 ? (uint32_t(reinterpret_cast<const uint16_t *>(b.nbr)[o]) | AKMC_NB_ZERO_SHIFT) : b.nbr[o];
 }
 

@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
 __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
 {
     if (cnt < b.maxnb) {
-        const size_t o = nbr_slot(b, cnt, i);
+        const size_t o = (size_t)cnt * b.A + i;
         if (b.nb16) reinterpret_cast<uint16_t *>(b.nbr)[o] = uint16_t(e & 0xFFFFu);
         else b.nbr[o] = e;
     } else {
@@ -327,7 +327,7 @@ constexpr int kNbCap = 2048;                 // candidates staged in LDS at a ti
 __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
 {
     if (k < b.maxnb) {
-        const size_t o = nbr_slot(b, k, i);
+        const size_t o = (size_t)k * b.A + i;
         if (b.nb16) reinterpret_cast<uint16_t *>(b.nbr)[o] = uint16_t(e & 0xFFFFu);
         else b.nbr[o] = e;
     } else {
@@ -576,50 +576,6 @@ __device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, u
     }
 }
 
-// Calls body(e) for successive batches of U list entries of atom i served
-// by lane sub of its LPA lanes ("none" past the end).  16-bit lists are
-// stored in blocks of 8 entries per atom (16 bytes): the lane reads whole
-// blocks sub, sub + LPA, ... with one load each, the next block in flight
-// while the current one is evaluated; 32-bit lists are read entry by entry
-// (k = sub, sub + LPA, ...), the next batch in flight likewise.  The order
-// of the batches within a lane does not depend on U.
-template <int LPA, int U, typename Body>
-__device__ inline void for_entries(const BatchDev &b, int i, int sub, int cnt, uint32_t none, Body &&body)
-{
-    static_assert(8 % U == 0, "batches tile a block of 8 entries");
-    if (b.nb16) {
-        const uint4 *blk = reinterpret_cast<const uint4 *>(b.nbr);
-        uint4 nx = make_uint4(0, 0, 0, 0);
-        if (sub * 8 < cnt) nx = blk[(size_t)sub * b.A + i];
-        for (int kb = sub; kb * 8 < cnt; kb += LPA) {
-            const uint4 cur = nx;
-            if ((kb + LPA) * 8 < cnt) nx = blk[(size_t)(kb + LPA) * b.A + i];
-            const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
-#pragma unroll
-            for (int h = 0; h < 8; h += U) {
-                uint32_t e[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int q = h + u;
-                    const uint32_t v = (w4[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
-                    e[u] = kb * 8 + q < cnt ? (v | AKMC_NB_ZERO_SHIFT) : none;
-                }
-                body(e);
-            }
-        }
-    } else {
-        uint32_t en[U];
-        fetch_entries<LPA, U>(b, i, sub, cnt, none, en);
-        for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
-            uint32_t e[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) e[u] = en[u];
-            fetch_entries<LPA, U>(b, i, k0 + LPA * U, cnt, none, en);  // next batch in flight
-            body(e);
-        }
-    }
-}
-
 // pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
 // then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2.
 // LPA lanes share one atom's neighbour list (strided) and combine by shuffles.
@@ -645,7 +601,13 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
     const int cnt = b.nnb[i];
     const uint32_t none = self_entry(i, W);
     double rho = 0.0;
-    for_entries<LPA, U>(b, i, sub, cnt, none, [&](const uint32_t (&e)[U]) {
+    uint32_t en[U];
+    fetch_entries<LPA, U>(b, i, sub, cnt, none, en);
+    for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
+        uint32_t e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) e[u] = en[u];
+        fetch_entries<LPA, U>(b, i, k0 + LPA * U, cnt, none, en);  // next batch in flight
         double4 pj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) pj[u] = b.p4[W.off + int(e[u] & AKMC_NB_JMASK)];
@@ -680,7 +642,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
                 }
             }
         }
-    });
+    }
     rho = lane_group_sum<LPA>(rho);
     if (sub != 0) return;
     b.rho[i] = rho;
@@ -721,7 +683,13 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
     const uint32_t none = self_entry(i, W);
     double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-    for_entries<LPA, U>(b, i, sub, cnt, none, [&](const uint32_t (&e)[U]) {
+    uint32_t en[U];
+    fetch_entries<LPA, U>(b, i, sub, cnt, none, en);
+    for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
+        uint32_t e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) e[u] = en[u];
+        fetch_entries<LPA, U>(b, i, k0 + LPA * U, cnt, none, en);  // next batch in flight
         double4 pj[U];
         float isj[U];
         double isj64[U];
@@ -796,7 +764,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
                 s5 += h * dx[u] * dy[u];
             }
         }
-    });
+    }
     fx = lane_group_sum<LPA>(fx);
     fy = lane_group_sum<LPA>(fy);
     fz = lane_group_sum<LPA>(fz);
