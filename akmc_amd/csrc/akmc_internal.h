@@ -129,11 +129,13 @@ extern "C" unsigned long long *akmc_rebuild_counter_ptr();
 // ------------------------------------------------------------ searches
 
 // walker kinds and states of the device state machines
-enum : int { WK_IDLE = 0, WK_RELAX = 1, WK_DIMER = 2, WK_VERIFY = 3, WK_ART = 4 };
+// WK_BASIN2: FIRE relax of a kicked converged saddle into the basin beyond it
+enum : int { WK_IDLE = 0, WK_RELAX = 1, WK_DIMER = 2, WK_VERIFY = 3, WK_ART = 4, WK_BASIN2 = 5 };
 enum : int {
     ST_DONE = 0,
     ST_R_INIT = 1,   // relax: force at X (before the loop)
     ST_R_STEP = 2,   // relax: force at the updated X
+    ST_R_E64 = 3,    // relax: FP64 energy at the relaxed X
     ST_D_F0 = 10,    // dimer: force at the centre
     ST_D_F1 = 11,    // dimer: force at the end X + D n
     ST_D_F1T = 12,   // dimer: force at the trial end X + D n'
@@ -178,6 +180,11 @@ struct WalkerState {
     // last and previous Lanczos eigenvalue, Lanczos beta
     double fnpar, fnperp, fmpar, fmperp, dotfv, half, lam, lamold, lbeta;
     int lconv, pad3;
+    // basin 2 of a converged saddle: energy (search precision, FP64),
+    // largest squared displacement from the basin, new-basin flag, FIRE
+    // status, force calls and iterations of its relax
+    double eb2, eb2_64, b2_maxd2;
+    int b2_new, b2_status, b2_calls, b2_iters;
 };
 
 struct SearchDev {
@@ -197,6 +204,13 @@ struct SearchDev {
     int verify_nl;
     double verify_delta;
     int default_mode;                 // active mode of searching walkers (1 FP32, 2 FP64)
+    // basin 2 after every converged search (checkForNewBasin,
+    // kmc/KineticMonteCarlo.cpp:1626-1713): on when basin2_on and basin set
+    const double *basin;              // [3][A] the walkers' basin positions
+    double *Xs;                       // [3][A] saddle positions kept for the host
+    double basin2_delta, basin_r2, max_r2;
+    int basin2_on;
+    int final64;                      // relaxes end with one FP64 energy evaluation
 };
 
 void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st);

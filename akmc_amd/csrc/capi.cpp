@@ -739,7 +739,9 @@ void drop_graph(akmc_batch_t b)
 // one micro-step; step j of a run chunk records the kernel-timer events
 void enqueue_micro_step(akmc_batch_t b, int j)
 {
-    int prec = batch_mode(b) | (b->s.verify_nl > 0 ? 2 : 0);
+    // FP64 pair terms run when a walker may ask for them: verification or
+    // the closing FP64 energy of a relax
+    int prec = batch_mode(b) | ((b->s.verify_nl > 0 || b->s.final64) ? 2 : 0);
     if (b->ktimer && (prec & 1) && b->f->host.kind == 0) {
         for (int ph = 0; ph < 2; ++ph) akmc_launch_phase(b->d, b->f->host, prec, 0, ph, b->stream);
         (void)hipEventRecord(b->kev[0][j], b->stream);
@@ -767,7 +769,8 @@ int refresh_active(akmc_batch_t b, int *n_running)
     int run = 0;
     for (int w = 0; w < b->W; ++w) {
         const WalkerState &ws = b->hws[w];
-        act[w] = (ws.kind != WK_IDLE && ws.state != ST_DONE) ? (ws.kind == WK_VERIFY ? 2 : batch_mode(b)) : 0;
+        const bool fp64 = ws.kind == WK_VERIFY || ws.state == ST_R_E64;
+        act[w] = (ws.kind != WK_IDLE && ws.state != ST_DONE) ? (fp64 ? 2 : batch_mode(b)) : 0;
         run += act[w];
     }
     HIPCHK(hipMemcpyAsync(b->d.active, act.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -860,6 +863,8 @@ int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const dou
     HIPCHK(hipStreamSynchronize(b->stream));
     return upload_active(b, walkers);
 }
+
+int copy_walkers_out(akmc_batch_t b, const double *src, int nw, const int *walkers, double *out);
 
 int capture_graph(akmc_batch_t b, int steps)
 {
@@ -1240,6 +1245,7 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
 {
     if (!b || !b->search_alloc || max_steps < 0 || sync_every < 1) return fail(AKMC_EINVAL, "bad arguments");
     int done = 0, running = 0;
+    b->s.default_mode = batch_mode(b);  // the mode the state kernels give walkers they move on
     TRY(fetch_states(b));
     TRY(refresh_active(b, &running));
     while (running > 0 && done < max_steps) {
@@ -1343,6 +1349,13 @@ int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out)
         out[w].fmax64 = ws.fmax64;
         out[w].lambda1 = ws.lam1;
         out[w].lambda2 = ws.lam2;
+        out[w].new_basin = ws.b2_new;
+        out[w].basin2_status = ws.b2_status;
+        out[w].basin2_force_calls = ws.b2_calls;
+        out[w].basin2_iterations = ws.b2_iters;
+        out[w].energy_basin2 = ws.eb2;
+        out[w].energy_basin2_64 = ws.eb2_64;
+        out[w].basin2_max_disp = std::sqrt(ws.b2_maxd2);
     }
     return AKMC_SUCCESS;
 }
@@ -1359,7 +1372,25 @@ int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z)
 
 int akmc_batch_get_walkers_positions(akmc_batch_t b, int nw, const int *walkers, double *out)
 {
-    if (!b || !b->search_alloc || nw < 0 || (nw && (!walkers || !out))) return fail(AKMC_EINVAL, "bad arguments");
+    if (!b || !b->search_alloc) return fail(AKMC_EINVAL, "no search state");
+    return copy_walkers_out(b, b->s.X, nw, walkers, out);
+}
+
+int akmc_batch_get_walkers_saddles(akmc_batch_t b, int nw, const int *walkers, double *out)
+{
+    if (!b || !b->search_alloc || !b->s.Xs) return fail(AKMC_EINVAL, "the basin-2 stage has not been set up");
+    return copy_walkers_out(b, b->s.Xs, nw, walkers, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+// the [3][n] slices of src ([3][A]) of the listed walkers, one pinned
+// transfer
+int copy_walkers_out(akmc_batch_t b, const double *src, int nw, const int *walkers, double *out)
+{
+    if (nw < 0 || (nw && (!walkers || !out))) return fail(AKMC_EINVAL, "bad arguments");
     size_t total = 0;
     for (int k = 0; k < nw; ++k) {
         if (walkers[k] < 0 || walkers[k] >= b->W) return fail(AKMC_EINVAL, "walker %d out of range", walkers[k]);
@@ -1377,7 +1408,7 @@ int akmc_batch_get_walkers_positions(akmc_batch_t b, int nw, const int *walkers,
     for (int k = 0; k < nw; ++k) {
         const WalkerDev &h = b->hw[walkers[k]];
         for (int c = 0; c < 3; ++c)
-            HIPCHK(hipMemcpyAsync(b->pos_pin + o + (size_t)c * h.n, b->s.X + c * A + h.off, h.n * sizeof(double),
+            HIPCHK(hipMemcpyAsync(b->pos_pin + o + (size_t)c * h.n, src + c * A + h.off, h.n * sizeof(double),
                                   hipMemcpyDeviceToHost, b->stream));
         o += 3 * (size_t)h.n;
     }
@@ -1385,6 +1416,10 @@ int akmc_batch_get_walkers_positions(akmc_batch_t b, int nw, const int *walkers,
     if (total) std::memcpy(out, b->pos_pin, total * sizeof(double));
     return AKMC_SUCCESS;
 }
+
+}  // namespace
+
+extern "C" {
 
 int akmc_batch_get_walker_positions(akmc_batch_t b, int walker, double *x, double *y, double *z)
 {
@@ -1428,12 +1463,95 @@ int akmc_batch_time_phases(akmc_batch_t b, int nrep, double ms[4])
     return check_overflow(b);
 }
 
-int akmc_batch_save_basin(akmc_batch_t b)
+int akmc_batch_save_basin(akmc_batch_t b) { return akmc_batch_set_basins(b, nullptr, nullptr); }
+
+int akmc_batch_set_basins(akmc_batch_t b, const int *walkers, const double *x)
 {
     if (!b || !b->search_alloc) return fail(AKMC_EINVAL, "no search state");
     if (!b->basin) TRY(dalloc(b, &b->basin, 3 * (size_t)b->A));
-    HIPCHK(hipMemcpy(b->basin, b->s.X, 3 * (size_t)b->A * sizeof(double), hipMemcpyDeviceToDevice));
+    b->s.basin = b->basin;
+    const size_t A = b->A;
+    if (!walkers && !x) {
+        HIPCHK(hipMemcpyAsync(b->basin, b->s.X, 3 * A * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    } else {
+        for (int w = 0; w < b->W; ++w) {
+            if (walkers && !walkers[w]) continue;
+            for (int c = 0; c < 3; ++c) {
+                const size_t o = c * A + b->hw[w].off;
+                if (x)
+                    HIPCHK(hipMemcpyAsync(b->basin + o, x + o, b->hw[w].n * sizeof(double), hipMemcpyHostToDevice,
+                                          b->stream));
+                else
+                    HIPCHK(hipMemcpyAsync(b->basin + o, b->s.X + o, b->hw[w].n * sizeof(double),
+                                          hipMemcpyDeviceToDevice, b->stream));
+            }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
     return AKMC_SUCCESS;
+}
+
+void akmc_basin2_defaults(akmc_basin2_params *p)
+{
+    if (!p) return;
+    p->basin2Delta = 0.2;
+    p->kmcBasinRadius = 0.5;  // docs/examples/gdc/control:11
+    p->kmcMaxRadius = 5.0;
+}
+
+int akmc_batch_set_basin2(akmc_batch_t b, const akmc_basin2_params *p)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    TRY(search_alloc(b));
+    if (!p) {
+        b->s.basin2_on = 0;
+        b->s.final64 = 0;
+        return AKMC_SUCCESS;
+    }
+    if (!(p->basin2Delta >= 0.0) || !(p->kmcBasinRadius >= 0.0) || !(p->kmcMaxRadius >= 0.0))
+        return fail(AKMC_EINVAL, "basin2Delta, kmcBasinRadius and kmcMaxRadius must be >= 0");
+    if (!b->s.Xs) TRY(dalloc(b, &b->s.Xs, 3 * (size_t)b->A));
+    b->s.basin2_delta = p->basin2Delta;
+    b->s.basin_r2 = p->kmcBasinRadius * p->kmcBasinRadius;
+    b->s.max_r2 = p->kmcMaxRadius * p->kmcMaxRadius;
+    b->s.basin2_on = 1;
+    b->s.final64 = 1;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_start_relax_at(akmc_batch_t b, const int *walkers, const double *x0, const int *frozen,
+                              const akmc_relax_params *p)
+{
+    if (!b || !p) return fail(AKMC_EINVAL, "bad arguments");
+    TRY(search_alloc(b));
+    const size_t A = b->A;
+    for (int w = 0; w < b->W; ++w) {
+        if (walkers && !walkers[w]) continue;
+        const int off = b->hw[w].off, n = b->hw[w].n;
+        if (frozen) {
+            std::copy(frozen + off, frozen + off + n, b->hfrozen.begin() + off);
+            HIPCHK(hipMemcpyAsync((void *)(b->s.frozen + off), frozen + off, n * sizeof(int), hipMemcpyHostToDevice,
+                                  b->stream));
+        }
+        if (x0)
+            for (int c = 0; c < 3; ++c)
+                HIPCHK(hipMemcpyAsync(b->s.X + c * A + off, x0 + c * A + off, n * sizeof(double),
+                                      hipMemcpyHostToDevice, b->stream));
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (frozen) TRY(upload_active(b, walkers));
+    return akmc_batch_start_relax(b, walkers, p);
+}
+
+int akmc_batch_cancel(akmc_batch_t b, const int *walkers)
+{
+    if (!b || !b->search_alloc) return fail(AKMC_EINVAL, "no search state");
+    TRY(fetch_states(b));
+    for (int w = 0; w < b->W; ++w)
+        if (!walkers || walkers[w]) b->hws[w].kind = WK_IDLE;
+    HIPCHK(hipMemcpyAsync(b->s.ws, b->hws.data(), b->W * sizeof(WalkerState), hipMemcpyHostToDevice, b->stream));
+    int run;
+    return refresh_active(b, &run);
 }
 
 int akmc_batch_restore_basin(akmc_batch_t b, const int *walkers)

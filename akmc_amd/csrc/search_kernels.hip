@@ -104,8 +104,62 @@ __device__ inline void finish(const Ctx &C_, WalkerState &w)
     write_probe(C_, nullptr, 0.0);
 }
 
+// ------------------------------------------------------------------ basin 2
+
+__device__ inline double wrap(double d, double L, double iL) { return d - L * rint(d * iL); }
+
+// the kick of checkForNewBasin (kmc/KineticMonteCarlo.cpp:1646-1683): with
+// Delta the minimum-image displacement of every atom from the basin to the
+// saddle, free atoms move from the saddle by norm * Delta, where
+// norm = basin2Delta * |Delta|; the saddle is kept in Xs and the walker
+// FIRE-relaxes from there with the search's frozen mask
+// (KineticMonteCarlo.cpp:1686)
+__device__ void enter_basin2(const Ctx &C_, WalkerState &w, double *sh)
+{
+    const WalkerDev &W = C_.b.walkers[blockIdx.x];
+    double acc[1] = {0.0};
+    FOR_ATOMS(i)
+    {
+        for (int c = 0; c < 3; ++c) {
+            const double x = V3(C_.s.X, c, i);
+            const double d = wrap(x - V3(C_.s.basin, c, i), W.L[c], W.invL[c]);
+            V3(C_.s.Xs, c, i) = x;
+            V3(C_.s.T, c, i) = d;
+            acc[0] += d * d;
+        }
+    }
+    bsum<1>(acc, sh);
+    const double norm = sqrt(acc[0]) * C_.s.basin2_delta;
+    FOR_ATOMS(i)
+    {
+        const bool fr = C_.s.frozen[i] != 0;
+        for (int c = 0; c < 3; ++c) {
+            if (!fr) V3(C_.s.X, c, i) += norm * V3(C_.s.T, c, i);
+            V3(C_.s.V, c, i) = 0.0;
+        }
+    }
+    w.kind = WK_BASIN2;
+    w.state = ST_R_INIT;
+    w.it = 0;
+    w.nstep = 1;  // Relax.cpp:49
+    w.dt = C_.s.rp.initTimeStep;
+    w.alpha = C_.s.rp.alphaStart;
+    w.b2_calls = 0;
+    if (threadIdx.x == 0) C_.b.active[blockIdx.x] = C_.s.default_mode;
+    __syncthreads();
+    write_probe(C_, nullptr, 0.0);
+}
+
+// a converged search that is not verified (or whose verification ended)
+// goes on to its basin 2 when that stage is on
+__device__ void end_converged(const Ctx &C_, WalkerState &w, double *sh)
+{
+    if (C_.s.basin2_on && C_.s.basin) enter_basin2(C_, w, sh);
+    else finish(C_, w);
+}
+
 // a converged saddle is re-verified in FP64 when verification is enabled
-__device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature)
+__device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature, double *sh)
 {
     w.e32 = w.E0;
     w.c32 = curvature;
@@ -116,7 +170,7 @@ __device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double cur
         if (threadIdx.x == 0) C_.b.active[blockIdx.x] = 2;
         write_probe(C_, nullptr, 0.0);
     } else {
-        finish(C_, w);
+        end_converged(C_, w, sh);
     }
 }
 
@@ -175,17 +229,65 @@ __device__ void relax_pre(const Ctx &C_, WalkerState &w, double *sh)
     write_probe(C_, nullptr, 0.0);
 }
 
+// the end of a FIRE relax.  A basin-2 relax is then compared with the
+// basin atom by atom (Basis::checkBasin, Common/Basis.cpp:731-781): a new
+// basin when some atom moved by at least basinRadius and none by more than
+// maxRadius, i.e. when the largest minimum-image displacement lies in
+// [basinRadius, maxRadius].  Its relax status is kept apart from the
+// search's, and, as KineticMonteCarlo.cpp:1692-1710 does, does not enter
+// the decision.  With final64 on, one FP64 energy evaluation at the relaxed
+// positions follows (their forces were just evaluated, so the probes
+// already sit at X).
+__device__ void relax_end(const Ctx &C_, WalkerState &w, double *sh, int status)
+{
+    if (w.kind == WK_BASIN2) {
+        const WalkerDev &W = C_.b.walkers[blockIdx.x];
+        double m = 0.0;
+        FOR_ATOMS(i)
+        {
+            double r2 = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double d = wrap(V3(C_.s.basin, c, i) - V3(C_.s.X, c, i), W.L[c], W.invL[c]);
+                r2 += d * d;
+            }
+            m = fmax(m, r2);
+        }
+        m = bmax(m, sh);
+        w.b2_status = status;
+        w.b2_iters = w.it;
+        w.eb2 = w.E_last;
+        w.b2_maxd2 = m;
+        w.b2_new = (m >= C_.s.basin_r2 && m <= C_.s.max_r2) ? 1 : 0;
+    } else {
+        w.status = status;
+    }
+    if (C_.s.final64) {
+        w.state = ST_R_E64;
+        if (threadIdx.x == 0) C_.b.active[blockIdx.x] = 2;
+        return;
+    }
+    finish(C_, w);
+}
+
 __device__ void relax_logic(const Ctx &C_, WalkerState &w, double *sh)
 {
     const RelaxParamsDev &p = C_.s.rp;
+    if (w.state == ST_R_E64) {
+        const double e = C_.b.out[blockIdx.x].total;
+        if (w.kind == WK_BASIN2) w.eb2_64 = e;
+        else w.e64 = e;
+        finish(C_, w);
+        return;
+    }
     // forces at X (frozen atoms zeroed, Relax.cpp:230)
     FOR_ATOMS(i)
     {
         for (int c = 0; c < 3; ++c) V3(C_.s.F0, c, i) = C_.f(c, i);
     }
-    w.fcalls += 1;
+    if (w.kind == WK_BASIN2) w.b2_calls += 1;
+    else w.fcalls += 1;
     w.E_last = C_.b.out[blockIdx.x].total;
-    w.E0 = w.E_last;
+    if (w.kind != WK_BASIN2) w.E0 = w.E_last;  // a basin-2 relax keeps the saddle energy in E0
     __syncthreads();
     if (w.state == ST_R_INIT) {
         w.it = 1;
@@ -216,13 +318,11 @@ __device__ void relax_logic(const Ctx &C_, WalkerState &w, double *sh)
     bsum<1>(t, sh);
     fm2 = bmax(fm2, sh);
     if (sqrt(t[0]) < p.normTol || sqrt(fm2) < p.forceTol) {
-        w.status = 0;
-        finish(C_, w);
+        relax_end(C_, w, sh, 0);
         return;
     }
     if (w.it >= p.maxIter) {
-        w.status = 1;
-        finish(C_, w);
+        relax_end(C_, w, sh, 1);
         return;
     }
     w.it += 1;
@@ -307,7 +407,7 @@ __device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
     w.C = Cv;
     if (sqrt(fm2) < p.forceTol && Cv < p.maxEigenvalue) {
         w.status = 0;
-        enter_verify_or_finish(C_, w, Cv);
+        enter_verify_or_finish(C_, w, Cv, sh);
         return;
     }
     const bool convex = Cv > 0.0;
@@ -617,7 +717,7 @@ __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
         w.lam1 = ev[0];
         w.lam2 = ev[1];
         w.verified_state = 1;
-        finish(C_, w);
+        end_converged(C_, w, sh);
         return;
     }
     double *Qn = C_.s.Q + (size_t)(k + 1) * A3;
@@ -667,7 +767,7 @@ __device__ void tridiag_lowest_vector(int n, const double *d, const double *e, d
 }
 
 __device__ void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh);
-__device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature);
+__device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature, double *sh);
 
 // start of ART::calculateLanczosForce (optim/ART.cpp:159-190): force at x,
 // q_0 = r / |r| with r the current lowest-mode estimate
@@ -726,7 +826,7 @@ __device__ void art_after_lanczos(const Ctx &C_, WalkerState &w, double *sh)
     }
     if (w.it > 5 && ((w.fnpar + w.fnperp) < p.normTol || (w.fmpar + w.fmperp) < p.forceTol)) {
         w.status = (w.lam <= p.maxEigenvalue) ? 0 : 1;
-        if (w.status == 0) enter_verify_or_finish(C_, w, w.lam);
+        if (w.status == 0) enter_verify_or_finish(C_, w, w.lam, sh);
         else finish(C_, w);
         return;
     }
@@ -962,19 +1062,21 @@ __global__ void __launch_bounds__(kBlock) k_search_logic(BatchDev b, SearchDev s
     // take the same branches (the scalars only change through block-wide
     // reductions), thread 0 writes them back
     WalkerState w = ws;
-    if (w.kind == WK_RELAX) relax_logic(C_, w, sh);
+    if (w.kind == WK_RELAX || w.kind == WK_BASIN2) relax_logic(C_, w, sh);
     else if (w.kind == WK_DIMER) dimer_logic(C_, w, sh);
     else if (w.kind == WK_VERIFY) verify_logic(C_, w, sh);
     else if (w.kind == WK_ART) art_logic(C_, w, sh);
     if (threadIdx.x == 0) s.ws[wid] = w;
 }
 
-// probes of freshly started walkers: X for relax and the dimer centre
+// probes of freshly started walkers (no force call consumed yet): X for
+// relax and the dimer centre.  Walkers already under way keep the probe
+// their last logic step wrote (a dimer end, a Lanczos displacement).
 __global__ void __launch_bounds__(kBlock) k_probe_init(BatchDev b, SearchDev s)
 {
     const int wid = blockIdx.x;
     const WalkerState w = s.ws[wid];
-    if (w.kind == WK_IDLE || w.state == ST_DONE) return;
+    if (w.kind == WK_IDLE || w.state == ST_DONE || w.fcalls != 0) return;
     const WalkerDev W = b.walkers[wid];
     Ctx C_{b, s, W.off, W.n, b.A, s.nact[wid]};
     FOR_ATOMS(i) C_.probe(i, V3(s.X, 0, i), V3(s.X, 1, i), V3(s.X, 2, i));

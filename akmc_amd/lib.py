@@ -66,7 +66,10 @@ class WalkerStatus(C.Structure):
                 ("force_calls", C.c_int), ("energy", C.c_double), ("curvature", C.c_double),
                 ("verified", C.c_int), ("energy32", C.c_double), ("curvature32", C.c_double),
                 ("energy64", C.c_double), ("fmax64", C.c_double), ("lambda1", C.c_double),
-                ("lambda2", C.c_double)]
+                ("lambda2", C.c_double), ("new_basin", C.c_int), ("basin2_status", C.c_int),
+                ("basin2_force_calls", C.c_int), ("basin2_iterations", C.c_int),
+                ("energy_basin2", C.c_double), ("energy_basin2_64", C.c_double),
+                ("basin2_max_disp", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -77,6 +80,20 @@ _STATUS_DTYPE = np.dtype({"names": [k for k, _ in WalkerStatus._fields_],
                           "formats": [np.int32 if t is C.c_int else np.float64 for _, t in WalkerStatus._fields_],
                           "offsets": [getattr(WalkerStatus, k).offset for k, _ in WalkerStatus._fields_],
                           "itemsize": C.sizeof(WalkerStatus)})
+
+
+class Basin2Params(C.Structure):
+    """akmc_basin2_params: the basin-2 keys of the reference's kmc{} block
+    (basin2Delta, kmcBasinRadius, kmcMaxRadius; kmc/KineticMonteCarlo.cpp:1672,1690)."""
+    _fields_ = [("basin2Delta", C.c_double), ("kmcBasinRadius", C.c_double), ("kmcMaxRadius", C.c_double)]
+
+
+def basin2_defaults(**kw) -> Basin2Params:
+    p = Basin2Params()
+    load().akmc_basin2_defaults(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
 
 
 def relax_defaults(**kw) -> RelaxParams:
@@ -165,6 +182,24 @@ def load() -> C.CDLL:
     L.akmc_batch_search_energies.argtypes = [C.c_void_p, C.c_int, _D]
     L.akmc_batch_get_walker_positions.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
     L.akmc_batch_restore_basin.argtypes = [C.c_void_p, _I]
+    L.akmc_basin2_defaults.argtypes = [C.POINTER(Basin2Params)]
+    L.akmc_batch_set_basin2.argtypes = [C.c_void_p, C.POINTER(Basin2Params)]
+    L.akmc_batch_get_walkers_saddles.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
+    L.akmc_batch_set_basins.argtypes = [C.c_void_p, _I, _D]
+    L.akmc_batch_start_relax_at.argtypes = [C.c_void_p, _I, _D, _I, C.POINTER(RelaxParams)]
+    L.akmc_batch_cancel.argtypes = [C.c_void_p, _I]
+    L.akmc_kmc_rate.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.akmc_kmc_rate.restype = C.c_double
+    L.akmc_kmc_max_cap.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.akmc_kmc_max_cap.restype = C.c_double
+    L.akmc_kmc_select.argtypes = [C.c_int, _D, C.c_double, _D]
+    L.akmc_kmc_time_increase.argtypes = [C.c_double, C.c_double]
+    L.akmc_kmc_time_increase.restype = C.c_double
+    L.akmc_kmc_check_basin.argtypes = [C.c_int, _D, _D, _D, C.c_double, C.c_double]
+    L.akmc_kmc_displaced_atoms.argtypes = [C.c_int, _D, _D, _D, C.c_double, _I]
+    L.akmc_kmc_recycle.argtypes = [C.c_int, _D, C.c_int, _D, C.c_double, C.c_int, C.c_int, C.c_double, _D, _I,
+                                   _I]
+    L.akmc_kmc_update_excluded.argtypes = [C.c_int, _I, C.c_int, _I, C.c_int]
     _lib = L
     return L
 
@@ -360,10 +395,13 @@ class Batch:
 
     def walkers_positions(self, walkers):
         """{w: (3, n_w) search positions} for several walkers, one transfer."""
+        return self._walkers_out(load().akmc_batch_get_walkers_positions, walkers)
+
+    def _walkers_out(self, fn, walkers):
         ws = np.ascontiguousarray(walkers, dtype=np.int32)
         n = self.natoms[ws]
         buf = np.zeros(int(3 * n.sum()))
-        check(load().akmc_batch_get_walkers_positions(self._h, int(ws.size), _ip(ws), _dp(buf)))
+        check(fn(self._h, int(ws.size), _ip(ws), _dp(buf)))
         out, o = {}, 0
         for w, k in zip(ws, n):
             out[int(w)] = buf[o:o + 3 * k].reshape(3, k)
@@ -459,6 +497,28 @@ class Batch:
 
     def save_basin(self):
         check(load().akmc_batch_save_basin(self._h))
+
+    def set_basins(self, mask=None, x=None):
+        """basins of the flagged walkers (mask: int32 [W], None = all) from the
+        flat [3][A] host array x, or from their current positions when x is None"""
+        check(load().akmc_batch_set_basins(self._h, _ip(mask), _dp(x)))
+
+    def set_basin2(self, params: Basin2Params | None):
+        """basin-2 stage after every converged search (None: off)"""
+        check(load().akmc_batch_set_basin2(self._h, C.byref(params) if params is not None else None))
+
+    def walkers_saddles(self, walkers):
+        """{w: (3, n_w) saddle positions kept by the basin-2 stage}"""
+        return self._walkers_out(load().akmc_batch_get_walkers_saddles, walkers)
+
+    def start_relax_at(self, mask, x0=None, frozen=None, params: RelaxParams | None = None):
+        """relax the flagged walkers from flat [3][A] positions x0 with the
+        flat [A] frozen mask (None: keep current ones)"""
+        p = params or relax_defaults()
+        check(load().akmc_batch_start_relax_at(self._h, _ip(mask), _dp(x0), _ip(frozen), C.byref(p)))
+
+    def cancel(self, walkers=None):
+        check(load().akmc_batch_cancel(self._h, _ip(self._mask(walkers))))
 
     def restore_basin(self, walkers=None):
         check(load().akmc_batch_restore_basin(self._h, _ip(self._mask(walkers))))

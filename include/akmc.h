@@ -168,7 +168,7 @@ typedef struct {
 } akmc_art_params;
 
 typedef struct {
-    int kind;        /* 0 idle, 1 relax, 2 dimer, 3 FP64 verification, 4 ART */
+    int kind;        /* 0 idle, 1 relax, 2 dimer, 3 FP64 verification, 4 ART, 5 basin-2 relax */
     int status;      /* AKMC_SUCCESS / AKMC_FAILED once done */
     int done;
     int iterations;
@@ -182,7 +182,27 @@ typedef struct {
     double energy64;     /* FP64 energy at the saddle */
     double fmax64;       /* largest FP64 atomic force on the free atoms */
     double lambda1, lambda2; /* two lowest FP64 Lanczos curvatures */
+    /* basin 2 of a converged search (akmc_batch_set_basin2); with the stage
+     * on, a relax walker's energy64 is its FP64 energy at the minimum */
+    int new_basin;            /* 1: Basis::checkBasin accepted basin 2 */
+    int basin2_status;        /* AKMC_SUCCESS / AKMC_FAILED of its FIRE relax */
+    int basin2_force_calls;
+    int basin2_iterations;
+    double energy_basin2;     /* at basin 2, search precision */
+    double energy_basin2_64;  /* at basin 2, FP64 pair terms */
+    double basin2_max_disp;   /* largest minimum-image displacement from the basin (A) */
 } akmc_walker_status;
+
+/* Basin 2 of every converged search: the part of
+ * KineticMonteCarlo::checkForNewBasin (kmc/KineticMonteCarlo.cpp:1626-1713)
+ * that follows findSaddlePoint.  Names of the reference's kmc{} keys
+ * basin2delta, kmcbasinradius (docs/docs/source/ACDCUserGuide.rst:150-180)
+ * and of the KMCCntrl members read at kmc/KineticMonteCarlo.cpp:1672,1690. */
+typedef struct {
+    double basin2Delta;     /* kick: free atoms move by basin2Delta |D| D from the saddle */
+    double kmcBasinRadius;  /* a new basin has an atom displaced by at least this (A) */
+    double kmcMaxRadius;    /* ... and none by more than this (A) */
+} akmc_basin2_params;
 
 void akmc_relax_defaults(akmc_relax_params *p);
 void akmc_dimer_defaults(akmc_dimer_params *p);
@@ -243,6 +263,77 @@ int akmc_batch_set_verify(akmc_batch_t b, int nl, double delta);
  * FP64 (bits = 64) pair terms */
 int akmc_batch_search_energies(akmc_batch_t b, int bits, double *energies);
 int akmc_batch_restore_basin(akmc_batch_t b, const int *walkers);
+
+/* ---- KMC step on the device ------------------------------------------
+ * checkForNewBasin (kmc/KineticMonteCarlo.cpp:1626-1713) after every
+ * converged search, in the same walker stream (after its FP64
+ * verification when that is on): the saddle is kept, the free atoms are
+ * kicked by basin2Delta |D| D (D = minimum-image saddle - basin over all
+ * atoms), FIRE-relaxed with the search's frozen mask (Relax::RelaxStructure
+ * at :1686) and compared with the basin (Basis::checkBasin,
+ * Common/Basis.cpp:731-781).  The basin is the one set by
+ * akmc_batch_save_basin / akmc_batch_set_basins.  While the stage is on,
+ * every relax (basin and basin 2) ends with one FP64 energy evaluation.
+ * p = NULL turns the stage off. */
+void akmc_basin2_defaults(akmc_basin2_params *p);
+int akmc_batch_set_basin2(akmc_batch_t b, const akmc_basin2_params *p);
+/* the saddle positions kept by the basin-2 stage, laid out as in
+ * akmc_batch_get_walkers_positions (which then returns basin 2) */
+int akmc_batch_get_walkers_saddles(akmc_batch_t b, int nw, const int *walkers, double *out);
+/* set the basin of the walkers flagged in walkers (all when NULL) to x
+ * ([3][A] host array, only the flagged slices are read) or, when x is
+ * NULL, to their current search positions */
+int akmc_batch_set_basins(akmc_batch_t b, const int *walkers, const double *x);
+/* Relax::RelaxStructure (optim/Relax.h:116-117) of the flagged walkers
+ * from x0 ([3][A], NULL: current positions) with the frozen mask
+ * ([A], NULL: current mask); only the flagged walkers' slices are read.
+ * The KMC loop relaxes each new basin 1 this way
+ * (kmc/KineticMonteCarlo.cpp:110-115). */
+int akmc_batch_start_relax_at(akmc_batch_t b, const int *walkers, const double *x0, const int *frozen,
+                              const akmc_relax_params *p);
+/* stop the flagged walkers (they become idle and are not evaluated), the
+ * TERMINATED command the reference sends to search groups once enough
+ * events are in (kmc/KineticMonteCarlo.cpp:446-455) */
+int akmc_batch_cancel(akmc_batch_t b, const int *walkers);
+
+/* ---- KMC step bookkeeping on the host (no device work) ----------------
+ * Energies in eV, temperatures in K, rates in the prefactor's unit.  The
+ * Boltzmann constant lives in the reference's missing Constants.h; this is
+ * the CODATA 2018 value. */
+#define AKMC_BOLTZMANN 8.617333262e-5
+/* Event rate prefactor exp(-Ea / (kB T)) (kmc/Event.cpp:66-105) */
+double akmc_kmc_rate(double activation_energy, double prefactor, double temperature);
+/* largest accepted activation energy minCap + window kB T (kmc/kmc.cpp:160) */
+double akmc_kmc_max_cap(double min_cap, double window, double temperature);
+/* the event chosen by u in [0, 1): the first i whose cumulative rate
+ * exceeds u * total (kmc/KineticMonteCarlo.cpp:144-169); total_rate may be
+ * NULL; -1 when no event qualifies */
+int akmc_kmc_select(int n, const double *rates, double u, double *total_rate);
+/* KMC time step -(1/R) ln u, 0 when |R| <= 1e-18 (calculateTime,
+ * kmc/KineticMonteCarlo.cpp:1151-1161, negated at :171) */
+double akmc_kmc_time_increase(double total_rate, double u);
+/* Basis::checkBasin (Common/Basis.cpp:731-781) for n atoms, positions SoA
+ * [3][n], any cell (rows of lat are the lattice vectors): 1 when some atom
+ * moved by >= radius and none by > max_radius */
+int akmc_kmc_check_basin(int n, const double *basin1, const double *basin2, const double lat[9], double radius,
+                         double max_radius);
+/* Event::getDisplacedAtoms (kmc/Event.cpp:303-341): the atoms of pos
+ * displaced from basin by more than radius, written to atoms[]; returns
+ * their count */
+int akmc_kmc_displaced_atoms(int n, const double *pos, const double *basin, const double lat[9], double radius,
+                             int *atoms);
+/* clearEventList (kmc/KineticMonteCarlo.cpp:1163-1265): keep[i] = 1 for
+ * the events that are recycled after event `chosen` ran: recycle on, centre
+ * at least radius from the chosen centre (minimum image), for mode 2 a
+ * draw u[i] below percent / 100, and blocked[i] == 0 (blocked: the event
+ * displaces an atom of the excluded list; NULL when superBasin is off).
+ * centres[3 * i + k]; returns the number kept */
+int akmc_kmc_recycle(int n, const double *centres, int chosen, const double lat[9], double radius, int recycle,
+                     int mode, double percent, const double *u, const int *blocked, int *keep);
+/* updateExcludedList (kmc/KineticMonteCarlo.cpp:1291-1313): count every
+ * positive entry of excluded[natoms] down by one, then add history to the
+ * displaced atoms */
+int akmc_kmc_update_excluded(int natoms, int *excluded, int nd, const int *displaced, int history);
 
 /* ---- learned proposal (no counterpart in the reference) ---------------
  * Per-atom descriptors (32 radial + 16 angular symmetry functions with

@@ -1,0 +1,137 @@
+"""GPU parity of the KMC step: the device basin-2 stage (kick, FIRE relax,
+checkBasin after a converged, FP64-verified dimer) against the oracle's
+restatement of checkForNewBasin (oracle/kmc_oracle.py kick + the C
+oracle's FIRE relax + checkBasin) from the device's own saddle, and a
+short multi-system KMC run whose every step is re-checked on the CPU.
+
+Bars: basin-2 energies within 1e-6 eV/atom of the oracle's relax from the
+same kicked start (FP64 pair terms on the device: positions within 1e-6 A),
+the same checkBasin verdict and largest displacement; KMC rates, total
+rates and activation energies equal to the oracle's from the device's
+FP64 energies to 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+from akmc_amd import workloads as W
+from oracle import kmc_oracle as K
+
+pytestmark = pytest.mark.gpu
+CU = W.SC_CU
+
+
+def _pots():
+    return [("Cu", "Cu", CU["eps"], CU["a"], CU["n"], CU["m"], CU["c"])]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from akmc_amd import lib as L
+    L.load()
+    return L
+
+
+@pytest.fixture(scope="module")
+def c1(oracle):
+    c = W.plain_sc_workload(seed=1)
+    of = oracle.sc_field(["Cu"], _pots(), 2 * CU["a"])
+    h = oracle.FieldHandle(of, 0, c.types, c.lat)
+    r = oracle.fire_relax(h, c.pos)
+    return c, of, h, r
+
+
+def _start(c, basin, seed):
+    dv = W.min_image(basin - c.vacancies[0][:, None], c.lat)
+    frozen = (np.sqrt((dv * dv).sum(0)) >= 6.0).astype(np.int32)
+    nb = W.neighbours_of_site(W.Cell(basin, c.types, c.lat), c.vacancies[0], 12)
+    disp, mode = W.local_displacement(W.Cell(basin, c.types, c.lat), nb, 0.1, seed)
+    return frozen, disp, mode
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_basin2_stage_matches_oracle(lib, oracle, c1, bits):
+    c, of, h, r = c1
+    f = lib.Field(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
+    f.set_precision(bits)
+    b = lib.Batch(f, [c])
+    b.set_frozen([np.zeros(c.n, np.int32)])
+    b.set_basin2(lib.basin2_defaults())
+    b.start_relax()
+    b.run(5000, sync_every=25)
+    st = b.status()[0]
+    basin = b.positions()[0]
+    e1 = oracle.sc_energy(of, basin, c.types, c.lat)[0]
+    print(f"[{bits}] basin relax: status {st['status']} E {st['energy']:.9f} E64 {st['energy64']:.9f} "
+          f"(oracle at the same positions {e1:.9f})", flush=True)
+    assert st["status"] == 0
+    assert abs(st["energy64"] - e1) < 1e-9 * abs(e1)      # the relax's closing FP64 evaluation
+    b.save_basin()
+    b.set_verify(16, 0.01)
+    n_new = 0
+    for seed in (0, 3, 6):
+        frozen, disp, mode = _start(c, basin, seed)
+        b.start_dimer([mode], params=lib.dimer_defaults(), x0_per_walker=[basin + disp], frozen_per_walker=[frozen])
+        b.run(40000, sync_every=25)
+        s = b.status()[0]
+        sad = b.walkers_saddles([0])[0]
+        b2 = b.walkers_positions([0])[0]
+        print(f"[{bits}] seed {seed}: kind {s['kind']} status {s['status']} verified {s['verified']} "
+              f"Ea64 {s['energy64'] - e1:.5f} | basin2 status {s['basin2_status']} new {s['new_basin']} "
+              f"calls {s['basin2_force_calls']} iters {s['basin2_iterations']} E {s['energy_basin2']:.8f} "
+              f"E64 {s['energy_basin2_64']:.8f} maxdisp {s['basin2_max_disp']:.4f}", flush=True)
+        assert s["status"] == 0 and s["verified"] == 1 and s["kind"] == 5 and s["done"] == 1
+        # the saddle kept by the stage is where the FP64 check ran
+        e_sad = oracle.sc_energy(of, sad, c.types, c.lat)[0]
+        assert abs(e_sad - s["energy64"]) < 1e-9 * abs(e_sad)
+        start = K.basin2_start(sad, basin, frozen, c.lat, 0.2)
+        o = oracle.fire_relax(h, start, frozen)
+        new_o = K.check_basin(basin, o["pos"], c.lat, 0.5, 5.0)
+        d = W.min_image(o["pos"] - basin, c.lat)
+        md = float(np.sqrt((d * d).sum(0).max()))
+        e2 = oracle.sc_energy(of, b2, c.types, c.lat)[0]
+        print(f"      oracle: status {o['status']} iters {o['iters']} E {o['energy']:.8f} new {new_o} "
+              f"maxdisp {md:.4f}; |dx| {np.abs(W.min_image(b2 - o['pos'], c.lat)).max():.2e}", flush=True)
+        assert s["basin2_status"] == 0 and o["status"] == 0
+        assert bool(s["new_basin"]) == new_o
+        assert abs(s["energy_basin2_64"] - e2) < 1e-9 * abs(e2)
+        assert abs(s["energy_basin2_64"] - o["energy"]) < 1e-6 * c.n
+        assert abs(s["basin2_max_disp"] - md) < (1e-6 if bits == 64 else 1e-2)
+        if bits == 64:
+            assert np.abs(W.min_image(b2 - o["pos"], c.lat)).max() < 1e-6
+        n_new += int(s["new_basin"])
+    assert n_new >= 1
+
+
+def test_kmc_steps_rechecked_on_cpu(lib, oracle):
+    """Two systems x 4 walkers, 2 KMC steps of 3 events each: every step
+    ends in a new basin whose FP64 energy is the oracle's at the chosen
+    event's basin-2 positions; rates and the total rate are the oracle's
+    Arrhenius rates of the events' FP64 barriers; the clock advances."""
+    from akmc_amd import kmc as M
+    cells = [W.sc_cell(4, 4, 4, n_vac=1, jitter=0.01, seed=11), W.sc_cell(4, 4, 5, n_vac=2, jitter=0.01, seed=12)]
+    text = lib.sc_potentials_text(_pots(), 2 * CU["a"])
+    p = M.KMCParams(kmcSteps=2, kmcImages=3)
+    run = M.KineticMonteCarlo(text, cells, walkers_per_system=4, params=p,
+                              search_params=lib.dimer_defaults(maxParIter=400), seed=2)
+    st = run.run(max_micro_steps=40000)
+    print(f"KMC: {st.kmc_steps} steps, {st.searches} searches, {st.converged} converged, {st.verified} verified, "
+          f"{st.new_basin} new basins, {st.valid_events} valid, {st.out_of_window} out of window, "
+          f"{st.micro_steps} micro-steps in {st.seconds:.1f} s", flush=True)
+    assert st.kmc_steps == 4
+    of = oracle.sc_field(["Cu"], _pots(), 2 * CU["a"])
+    for s in run.systems:
+        assert s.iteration == 2 and s.phase == "done" and s.time > 0
+    by_sys = {}
+    for rec in st.steps:
+        by_sys.setdefault(rec.system, []).append(rec)
+        print(rec, flush=True)
+        assert rec.events >= 3 and 0 <= rec.chosen < rec.events
+        assert p.kmcMinCap <= rec.activation_energy <= p.kmcMaxCap
+        assert rec.total_rate == pytest.approx(sum(K.rate(e, p.kmcPreFactor, p.kmcTemperature)
+                                                   for e in rec.barriers), rel=1e-12)
+        assert rec.dt > 0
+    for si, s in enumerate(run.systems):
+        c = cells[si]
+        e_final = oracle.sc_energy(of, s.basin, c.types, c.lat)[0]
+        assert abs(e_final - s.e_basin64) < 1e-9 * abs(e_final)
+        assert len(by_sys[si]) == 2

@@ -115,6 +115,35 @@ def test_batched_walkers_equal_plain(lib, oracle, c1):
     assert np.array_equal(bb.mode()[1], b1.mode()[0])
 
 
+def test_refill_leaves_running_walkers_alone(lib, oracle, c1):
+    """Starting a search on one walker while another is in the middle of
+    its dimer rotation (its probe sits at the dimer end) does not touch the
+    running walker: its trajectory stays bitwise that of the walker alone."""
+    c, of, h, r = c1
+    basin = r["pos"]
+    f = lib.Field(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
+    frozen, disp, mode = _dimer_start(c, basin, 1)
+    frozen2, disp2, mode2 = _dimer_start(c, basin, 4)
+    cell = W.Cell(basin, c.types, c.lat)
+    p = lib.dimer_defaults(maxParIter=60)
+    bb = lib.Batch(f, [cell, cell])
+    bb.set_frozen([frozen, frozen2])
+    bb.start_dimer([mode, mode2], [disp, disp2], p, walkers=[0])
+    bb.run(7, sync_every=7)
+    assert bb.status()[0]["done"] == 0
+    x0 = [basin + disp, basin + disp2]
+    bb.start_dimer([mode, mode2], params=p, walkers=[1], x0_per_walker=x0, frozen_per_walker=[frozen, frozen2])
+    bb.run(400, sync_every=5)
+    b1 = lib.Batch(f, [cell])
+    b1.set_frozen([frozen])
+    b1.start_dimer([mode], [disp], p)
+    b1.run(400, sync_every=1)
+    s_b, s_1 = bb.status()[0], b1.status()[0]
+    print("refilled batch", s_b, "\nplain         ", s_1, flush=True)
+    assert s_b == s_1
+    assert np.array_equal(bb.positions()[0], b1.positions()[0])
+
+
 def _art_params_ref(oracle):
     """ART settings of the reference's own example (docs/examples/gdc/control:16-23)."""
     c = oracle.art_defaults()
