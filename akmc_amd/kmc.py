@@ -265,7 +265,7 @@ class KineticMonteCarlo:
         blocked = None
         if self.p.superBasin:
             # the reference compares the excluded list's entries with the
-            # event's displaced atom numbers (KineticMonteCarlo.cpp:1226-1234)
+            # event's displaced atom numbers (KineticMonteCarlo.cpp:1228-1234)
             vals = set(int(v) for v in s.excluded)
             blocked = np.ascontiguousarray(
                 [int(any(int(a) in vals for a in self._displaced(e.basin2, s.basin, lat))) for e in s.events],

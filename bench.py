@@ -17,8 +17,10 @@ wall time.
 Also reported: the plain unbatched GPU dimer (one walker, host-driven
 micro-steps) on the same cells, force evaluations per second, the roofline
 of the dominant kernel (Sutton-Chen pass-2 force) timed with HIP events on
-the batch's stream, and the CPU baseline (the FP64 C restatement of the
-reference, OpenMP) on a bounded sample of the same searches.
+the batch's stream, the learned-proposal A/B test, the KMC step (64
+systems x 8 walkers taking kmcSteps steps of 5 valid events each) and the
+CPU baseline (the FP64 C restatement of the reference, OpenMP) on a
+bounded sample of the same searches.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 2]
 """
@@ -196,6 +198,37 @@ def learned_ab(camp, records, window, epochs):
     return out
 
 
+def kmc_section(L, W, text, args):
+    """The KMC step around the search path: 64 independent systems (walker
+    cells of the batched workload) x 8 walkers each, kmcSteps 3 of
+    kmcevents 5 (docs/examples/gdc/control's count) per system.  Each
+    search is FP64-verified and followed on the device by its basin-2 kick,
+    FIRE relax and checkBasin; each system takes its KMC step (rates,
+    choice, clock, move) as soon as it holds 5 valid events, then re-relaxes
+    its new basin 1 and searches again."""
+    from akmc_amd import kmc as M
+    systems = W.batched_sc_cells(n_walkers=args.kmc_systems, seed=args.seed + 1)
+    p = M.KMCParams(kmcSteps=args.kmc_steps, kmcImages=5)
+    run = M.KineticMonteCarlo(text, systems, walkers_per_system=8, params=p,
+                              search_params=L.dimer_defaults(maxParIter=400), seed=args.seed, skin=args.skin)
+    if args.eager:
+        run.batch.set_graphs(False)
+    st = run.run()
+    ea = [r.activation_energy for r in st.steps]
+    out = {"systems": len(systems), "walkers": run.W, "atoms": run.A, "kmc_steps": st.kmc_steps,
+           "seconds": round(st.seconds, 2), "kmc_steps_per_s": round(st.kmc_steps / st.seconds, 2),
+           "valid_events_per_s": round(st.valid_events / st.seconds, 1),
+           "searches": st.searches, "converged": st.converged, "verified": st.verified,
+           "new_basin": st.new_basin, "valid_events": st.valid_events, "out_of_window": st.out_of_window,
+           "force_calls_per_valid_event": round(st.force_calls / max(st.valid_events, 1), 1),
+           "basin2_force_calls_per_new_basin": round(st.basin2_force_calls / max(st.new_basin, 1), 1),
+           "basin1_relax_force_calls": st.relax_force_calls, "micro_steps": st.micro_steps,
+           "mean_chosen_barrier_eV": round(float(np.mean(ea)), 4) if ea else None,
+           "simulated_time_s_mean": float(np.mean([s.time for s in run.systems]))}
+    run.batch.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +243,9 @@ def main():
     ap.add_argument("--skin", type=float, default=None, help="Verlet skin of the device lists (A)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal section")
     ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--no-kmc", action="store_true", help="skip the KMC-step section")
+    ap.add_argument("--kmc-systems", type=int, default=64)
+    ap.add_argument("--kmc-steps", type=int, default=3)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--eager", action="store_true",
                     help="launch the micro-step kernels one by one instead of replaying hipGraphs "
@@ -316,6 +352,11 @@ def main():
         print(f"[bench] learned section: { {k: v for k, v in learned.items() if k not in ('random', 'learned')} }",
               file=sys.stderr, flush=True)
 
+    kmc = None
+    if not args.no_kmc:
+        kmc = kmc_section(L, W, text, args)
+        print(f"[bench] KMC steps: {kmc}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cells, camp.basins, camp.proposal)
@@ -368,6 +409,8 @@ def main():
     }
     if learned:
         out["learned_proposal"] = learned
+    if kmc:
+        out["kmc"] = kmc
     if cpu:
         out["cpu_baseline"] = {"value": round(cpu["saddles_per_s"], 5), "unit": "saddles/s",
                                "cores": cpu["cores"], "kind": "port",

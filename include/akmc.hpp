@@ -11,11 +11,20 @@
 //   DimerCntrl Common/DimerCntrl.h:16-94 (field names and defaults)
 //   Relax      optim/Relax.h:116-117 (RelaxStructure)
 //   ART, Dimer optim/ART.h:136-138, optim/Dimer.h:68-70 (findSaddlePoint)
+//   KMCCntrl   the kmc{} keys (docs/docs/source/ACDCUserGuide.rst:150-180;
+//              member names as read in kmc/KineticMonteCarlo.cpp)
+//   Basis::checkBasin  Common/Basis.cpp:731-781
+//   Event      kmc/Event.cpp (rate, setPositions, getDisplacedAtoms)
+//   KineticMonteCarlo  kmc/KineticMonteCarlo.cpp: checkForNewBasin,
+//              calculateTime, the event choice of runKMC, clearEventList,
+//              updateExcludedList
 // The MPIComms/Species/Basis/JobControl arguments of the reference become the
 // plain arrays they carry (positions SoA, lattice, labels, charges, frozen).
 #pragma once
 
+#include <cmath>
 #include <fstream>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -212,6 +221,78 @@ struct Basis {
     std::vector<double> posX, posY, posZ, charge;
     std::vector<int> atmLabel, frozen;
     double latVector[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+    // positions as one SoA array [3][n]
+    std::vector<double> soa() const
+    {
+        std::vector<double> p(posX);
+        p.insert(p.end(), posY.begin(), posY.end());
+        p.insert(p.end(), posZ.begin(), posZ.end());
+        return p;
+    }
+    // Basis::checkBasin: some atom moved by >= detectBasinRadius and none
+    // by more than maxBasinRadius
+    bool checkBasin(const Basis &basin2, double detectBasinRadius, double maxBasinRadius) const
+    {
+        std::vector<double> a = soa(), b = basin2.soa();
+        int r = akmc_kmc_check_basin(numberOfAtoms, a.data(), b.data(), latVector, detectBasinRadius, maxBasinRadius);
+        check(r);
+        return r == 1;
+    }
+};
+
+// KMC control keys; the reference's KMCCntrl.h is missing, so defaults the
+// example control file does not set are this library's choice
+struct KMCCntrl {
+    std::string kmcMethod = "dimer";
+    int kmcSteps = 2, kmcImages = 5, recycleType = 1, historySize = 10;
+    double kmcTemperature = 1000.0, kmcPreFactor = 1.0e13, kmcMinCap = 0.1, kmcWindow = 50.0, kmcMaxCap = 0.0;
+    double basin2Delta = 0.2, kmcBasinRadius = 0.5, kmcMaxRadius = 5.0, recycleRadius = 10.0,
+           recyclePercentage = 100.0;
+    bool recycle = false, superBasin = false;
+    // kmc/kmc.cpp:160
+    void setMaxCap() { kmcMaxCap = akmc_kmc_max_cap(kmcMinCap, kmcWindow, kmcTemperature); }
+};
+
+// One KMC event: basin-2 and saddle positions of every atom (FP64),
+// energies, centre and rate
+class Event {
+public:
+    std::vector<double> bas2PosX, bas2PosY, bas2PosZ, sadPosX, sadPosY, sadPosZ;
+    double centre[3] = {0, 0, 0};
+    double basin1Energy = 0, basin2Energy = 0, saddleEnergy = 0, actEnergy = 0, rate = 0;
+
+    Event(const Basis &basin2, const Basis &saddle, const double *c, double e1, double e2, double es, double eact,
+          double prefactor, double temperature)
+        : bas2PosX(basin2.posX), bas2PosY(basin2.posY), bas2PosZ(basin2.posZ), sadPosX(saddle.posX),
+          sadPosY(saddle.posY), sadPosZ(saddle.posZ), basin1Energy(e1), basin2Energy(e2), saddleEnergy(es),
+          actEnergy(eact), rate(akmc_kmc_rate(eact, prefactor, temperature))
+    {
+        for (int k = 0; k < 3; ++k) centre[k] = c[k];
+    }
+    double getRate() const { return rate; }
+    double getActivationEnergy() const { return actEnergy; }
+    double getBasin2Energy() const { return basin2Energy; }
+    const double *getCentre() const { return centre; }
+    void setPositions(Basis &bas) const
+    {
+        bas.posX = bas2PosX;
+        bas.posY = bas2PosY;
+        bas.posZ = bas2PosZ;
+    }
+    std::vector<int> getDisplacedAtoms(const Basis &basin, double basinRadius) const
+    {
+        std::vector<double> p(bas2PosX);
+        p.insert(p.end(), bas2PosY.begin(), bas2PosY.end());
+        p.insert(p.end(), bas2PosZ.begin(), bas2PosZ.end());
+        std::vector<double> b = basin.soa();
+        std::vector<int> atoms(basin.numberOfAtoms);
+        int m = akmc_kmc_displaced_atoms(basin.numberOfAtoms, p.data(), b.data(), basin.latVector, basinRadius,
+                                         atoms.data());
+        check(m);
+        atoms.resize(m);
+        return atoms;
+    }
 };
 
 namespace detail {
@@ -283,6 +364,109 @@ public:
         int steps = 0;
         check(akmc_batch_run(w.b, 1 << 30, 1, &steps));
         w.read_back(saddle, saddleEnergy, status, &eigenValue);
+    }
+};
+
+// The host steps of kmc/KineticMonteCarlo.cpp around the device searches.
+// Random numbers come from a seeded mt19937_64 through a uniform [0, 1)
+// distribution, the generator of kmc/kmc.cpp:27-28,250-253.
+class KineticMonteCarlo {
+    std::mt19937_64 generator;
+    std::uniform_real_distribution<double> dist{0.0, 1.0};
+
+public:
+    explicit KineticMonteCarlo(unsigned long long seed = 0) : generator(seed) {}
+    double getRandomNumber() { return dist(generator); }
+
+    // calculateTime (:1151-1161): (1/R) ln u, <= 0; runKMC negates it (:173)
+    double calculateTime(double totalRate) { return -akmc_kmc_time_increase(totalRate, getRandomNumber()); }
+
+    // the event choice of runKMC (:144-169); -1 when none qualifies
+    int selectEvent(const std::vector<Event> &events, double &totalRate)
+    {
+        std::vector<double> r;
+        for (const Event &e : events) r.push_back(e.getRate());
+        return akmc_kmc_select((int)r.size(), r.data(), getRandomNumber(), &totalRate);
+    }
+
+    // checkForNewBasin (:1626-1713): basin2 starts at the saddle with its free
+    // atoms kicked by basin2Delta |D| D (D the minimum-image saddle - basin1
+    // displacement of every atom), is relaxed on the device with the
+    // saddle's frozen mask, and is a new basin when basin1.checkBasin says so
+    bool checkForNewBasin(Field *fld, const KMCCntrl &kmc, const RelaxCntrl &relax, const Basis &basin1,
+                          Basis &basin2, const Basis &saddle, Status &relStatus, Energy &basin2Energy)
+    {
+        const int n = basin1.numberOfAtoms;
+        basin2 = saddle;
+        const double *L = basin1.latVector;
+        // fractional coordinates through the cofactors of the lattice
+        const double c0 = L[4] * L[8] - L[5] * L[7], c1 = L[2] * L[7] - L[1] * L[8], c2 = L[1] * L[5] - L[2] * L[4];
+        const double c3 = L[5] * L[6] - L[3] * L[8], c4 = L[0] * L[8] - L[2] * L[6], c5 = L[2] * L[3] - L[0] * L[5];
+        const double c6 = L[3] * L[7] - L[4] * L[6], c7 = L[1] * L[6] - L[0] * L[7], c8 = L[0] * L[4] - L[1] * L[3];
+        const double det = L[0] * c0 + L[1] * c3 + L[2] * c6;
+        const double R[9] = {c0 / det, c1 / det, c2 / det, c3 / det, c4 / det, c5 / det, c6 / det, c7 / det, c8 / det};
+        std::vector<double> d(3 * (size_t)n);
+        double norm = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double r[3] = {saddle.posX[i] - basin1.posX[i], saddle.posY[i] - basin1.posY[i],
+                                 saddle.posZ[i] - basin1.posZ[i]};
+            double f[3];
+            for (int j = 0; j < 3; ++j) {
+                f[j] = r[0] * R[j] + r[1] * R[3 + j] + r[2] * R[6 + j];
+                f[j] -= std::rint(f[j]);
+            }
+            for (int j = 0; j < 3; ++j) {
+                d[j * (size_t)n + i] = f[0] * L[j] + f[1] * L[3 + j] + f[2] * L[6 + j];
+                norm += d[j * (size_t)n + i] * d[j * (size_t)n + i];
+            }
+        }
+        norm = std::sqrt(norm) * kmc.basin2Delta;
+        for (int i = 0; i < n; ++i) {
+            if (!saddle.frozen.empty() && saddle.frozen[i]) continue;
+            basin2.posX[i] = saddle.posX[i] + norm * d[i];
+            basin2.posY[i] = saddle.posY[i] + norm * d[(size_t)n + i];
+            basin2.posZ[i] = saddle.posZ[i] + norm * d[2 * (size_t)n + i];
+        }
+        Relax rel;
+        Status st;
+        rel.RelaxStructure(basin2, fld, relax, basin2Energy, st);
+        const bool found = basin1.checkBasin(basin2, kmc.kmcBasinRadius, kmc.kmcMaxRadius);
+        if (found) relStatus.setStatusSuccess();
+        else relStatus.setStatusFailed();
+        return found;
+    }
+
+    // updateExcludedList (:1291-1313)
+    void updateExcludedList(const Basis &basin, const Event &ev, std::vector<int> &excludedList, const KMCCntrl &kmc)
+    {
+        std::vector<int> atoms = ev.getDisplacedAtoms(basin, kmc.kmcBasinRadius);
+        check(akmc_kmc_update_excluded((int)excludedList.size(), excludedList.data(), (int)atoms.size(),
+                                       atoms.data(), kmc.historySize));
+    }
+
+    // clearEventList (:1163-1265); with superBasin the reference compares the
+    // excluded list's entries with the displaced atoms' numbers (:1228-1234)
+    void clearEventList(std::vector<Event> &events, const Basis &basin, int chosen, const KMCCntrl &kmc,
+                        const std::vector<int> &excludedList)
+    {
+        const int n = (int)events.size();
+        std::vector<double> cen, u(n, 0.0);
+        std::vector<int> blocked(n, 0), keep(n, 0);
+        for (const Event &e : events) cen.insert(cen.end(), e.centre, e.centre + 3);
+        if (kmc.recycle && kmc.recycleType == 2)
+            for (int i = 0; i < n; ++i)
+                if (i != chosen) u[i] = getRandomNumber();
+        if (kmc.superBasin)
+            for (int i = 0; i < n; ++i)
+                for (int a : events[i].getDisplacedAtoms(basin, kmc.kmcBasinRadius))
+                    for (int ex : excludedList)
+                        if (ex == a) blocked[i] = 1;
+        check(akmc_kmc_recycle(n, cen.data(), chosen, basin.latVector, kmc.recycleRadius, kmc.recycle ? 1 : 0,
+                               kmc.recycleType, kmc.recyclePercentage, u.data(), blocked.data(), keep.data()));
+        std::vector<Event> kept;
+        for (int i = 0; i < n; ++i)
+            if (keep[i]) kept.push_back(events[i]);
+        events.swap(kept);
     }
 };
 

@@ -3,11 +3,13 @@
 // Relax::RelaxStructure, Dimer/ART::findSaddlePoint).
 //
 //   host_driver <action> <potentials file> <config file> <out file>
-//   action: forces | relax | dimer | art
+//   action: forces | relax | dimer | art | kmc
 // config: N, 9 lattice numbers, N lines "x y z type frozen", then for dimer
-// and art N lines "mx my mz" (start direction).
+// and art N lines "mx my mz" (start direction), for kmc N lines of saddle
+// positions (the first block is basin 1, its frozen column the saddle's).
 // out: "status energy curvature" then N lines of x y z and, for forces,
-// N lines of fx fy fz.
+// N lines of fx fy fz.  kmc: KineticMonteCarlo::checkForNewBasin; status 0
+// for a new basin, energy and positions of basin 2.
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -38,7 +40,7 @@ int main(int argc, char **argv)
     bas.frozen.resize(n);
     for (int i = 0; i < n; ++i) cin_ >> bas.posX[i] >> bas.posY[i] >> bas.posZ[i] >> bas.atmLabel[i] >> bas.frozen[i];
     std::vector<double> mode;
-    if (action == "dimer" || action == "art") {
+    if (action == "dimer" || action == "art" || action == "kmc") {
         mode.assign(3 * (size_t)n, 0.0);
         for (int i = 0; i < n; ++i) cin_ >> mode[i] >> mode[n + i] >> mode[2 * (size_t)n + i];
     }
@@ -72,6 +74,17 @@ int main(int argc, char **argv)
             c.numLanczosVectors = 25;
             a.findSaddlePoint(&fld, bas, mode, c, st, eng);
             curv = a.eigenValue;
+        } else if (action == "kmc") {
+            akmc::Basis saddle = bas;
+            saddle.posX.assign(mode.begin(), mode.begin() + n);
+            saddle.posY.assign(mode.begin() + n, mode.begin() + 2 * n);
+            saddle.posZ.assign(mode.begin() + 2 * n, mode.end());
+            akmc::Basis basin2;
+            akmc::KMCCntrl kc;
+            akmc::RelaxCntrl rc;
+            akmc::KineticMonteCarlo kmc(7);
+            kmc.checkForNewBasin(&fld, kc, rc, bas, basin2, saddle, st, eng);
+            bas = basin2;
         } else {
             std::fprintf(stderr, "unknown action %s\n", action.c_str());
             return 2;

@@ -1,6 +1,7 @@
 """The C++ host classes (include/akmc.hpp) driven like the reference's
-callers: Field::calculateForces, Relax::RelaxStructure and
-Dimer/ART::findSaddlePoint through the compiled driver tests/cpp/host_driver.cpp,
+callers: Field::calculateForces, Relax::RelaxStructure,
+Dimer/ART::findSaddlePoint and KineticMonteCarlo::checkForNewBasin through
+the compiled driver tests/cpp/host_driver.cpp,
 checked against the FP64 oracle."""
 import os
 import subprocess
@@ -68,3 +69,14 @@ def test_cpp_field_relax_dimer_art(tmp_path, oracle):
         e64 = oracle.sc_energy(of, sad, c.types, c.lat)[0]
         print(f"C++ {action}: status {st} Ea {es - e1:.5f} (FP64 {e64 - e1:.5f}) curvature {curv:.4f}", flush=True)
         assert st == 0 and curv < 0 and 0.3 < e64 - e1 < 0.9 and abs(es - e64) < 0.01
+        if action == "dimer":
+            # KineticMonteCarlo::checkForNewBasin from this saddle against the
+            # oracle's kick + FIRE relax + checkBasin
+            from oracle import kmc_oracle as K
+            st2, e2, _, b2, _ = _run(tmp_path, "kmc", basin, c.types, c.lat, frozen, sad)
+            o = oracle.fire_relax(h, K.basin2_start(sad, basin, frozen, c.lat, 0.2), frozen)
+            new_o = K.check_basin(basin, o["pos"], c.lat, 0.5, 5.0)
+            print(f"C++ checkForNewBasin: status {st2} E2 {e2:.8f} (oracle {o['energy']:.8f}, new {new_o}); "
+                  f"max|dx| {np.abs(W.min_image(b2 - o['pos'], c.lat)).max():.2e}", flush=True)
+            assert (st2 == 0) == new_o and new_o
+            assert abs(e2 - o["energy"]) / c.n < 1e-6
