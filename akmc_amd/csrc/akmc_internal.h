@@ -31,6 +31,22 @@ struct SCPairDev {
     int on;      // 1 when the species pair has a potential
 };
 
+// One tabulated EAM potential (MetalPotential, LAMMPS setfl layout read by
+// readLEAM, potentials/metal/MetalPotential.cpp:276-361).  Every table is
+// stored as (t_i, t_{i+1} - t_i) pairs so one 16-byte load feeds the linear
+// interpolation t_i + ((t_{i+1} - t_i) (x - i h)) / h of eamMeshEnergy
+// (potentials/metal/ManyBody.cpp:455-460) with the reference's operation
+// order.  The derivative tables are the forward differences of
+// calculateDerivativeMesh (MetalPotential.cpp:388-392), last entry 0.
+#define AKMC_MAXEAM 8
+struct EamTabDev {
+    int nr, nrho;          // points of the r tables and of the embedding tables
+    double dr, drho, cut;  // steps and the table's cutoff (potCutoff)
+    const double2 *dens;   // [nr] rho(r)
+    const double2 *pair;   // [nr][3] phi(r), phi'(r), rho'(r)
+    const double2 *emb;    // [nrho][2] F(rho), F'(rho)
+};
+
 struct IonPairDev {
     float A, inv_rho, C;
     double Ad, inv_rhod, Cd;
@@ -52,7 +68,15 @@ struct FieldDev {
     double cfact;
     SCPairDev sc[AKMC_MAXPAIR];
     IonPairDev ion[AKMC_MAXPAIR];
+    // tabulated EAM (kind 0 with eam = 1): species pair -> table (-1 none)
+    int eam;
+    int eam_one;           // one table serves every species pair
+    int eam_tab[AKMC_MAXPAIR];
+    EamTabDev eamt[AKMC_MAXEAM];
 };
+// bits of BatchDev::overflow
+#define AKMC_OV_LIST 1     // a neighbour list ran out of slots
+#define AKMC_OV_EAM_RHO 2  // a density fell outside its embedding table
 
 struct WalkerDev {
     int off, n;          // first atom, atom count

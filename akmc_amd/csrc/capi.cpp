@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -67,6 +68,14 @@ struct BuckPot {
     std::string A, B;
     double Ab, rho, C;
 };
+// a tabulated EAM potential as readLEAM leaves it
+// (potentials/metal/MetalPotential.cpp:276-361)
+struct EamPot {
+    std::string A, B, file;
+    int nrho = 0, nr = 0;
+    double drho = 0, dr = 0, cut = 0;
+    std::vector<double> F, rho, phi;  // embedding, density, pair (the file's r phi divided by r)
+};
 
 }  // namespace
 
@@ -85,6 +94,8 @@ struct akmc_field_s {
     std::vector<SpeciesRec> species;
     std::vector<SCPot> sc;
     std::vector<BuckPot> buck;
+    std::vector<EamPot> eam;
+    double2 *eam_dev = nullptr;  // device copy of every EAM table (one allocation)
     FieldDev host{};
     FieldDev *dev = nullptr;
     int table[AKMC_MAXPAIR];
@@ -139,6 +150,54 @@ struct akmc_batch_s {
 
 namespace {
 
+// readParameters (MetalPotential.cpp:363-386): whole lines of numbers until
+// n values are in; numbers past n on the last line are dropped (the
+// reference writes them past its array)
+bool read_values(std::istream &in, int n, std::vector<double> &out)
+{
+    out.clear();
+    std::string line;
+    while ((int)out.size() < n) {
+        if (!std::getline(in, line)) return false;
+        for (const std::string &w : split_ws(line)) {
+            if ((int)out.size() >= n) break;
+            out.push_back(std::stod(w));
+        }
+    }
+    return true;
+}
+
+// readLEAM (MetalPotential.cpp:276-361): four comment/element lines,
+// "Nrho drho Nr dr cutoff", one element line, then F(rho) on the rho mesh,
+// rho(r) and r phi(r) on the r mesh; phi_i = (r phi)_i / (i dr), so the
+// first entry is the file's value / 0 as in the reference.
+int read_lammps_eam(EamPot &p)
+{
+    std::ifstream in(p.file);
+    if (!in) return fail(AKMC_EINVAL, "could not find LAMMPS style EAM file %s", p.file.c_str());
+    std::string line;
+    try {
+        for (int k = 0; k < 4; ++k)
+            if (!std::getline(in, line)) throw std::runtime_error("header");
+        if (!std::getline(in, line)) throw std::runtime_error("mesh line");
+        auto w = split_ws(line);
+        p.nrho = std::stoi(w.at(0));
+        p.drho = std::stod(w.at(1));
+        p.nr = std::stoi(w.at(2));
+        p.dr = std::stod(w.at(3));
+        p.cut = std::stod(w.at(4));
+        if (p.nrho < 2 || p.nr < 2 || !(p.drho > 0) || !(p.dr > 0))
+            return fail(AKMC_EINVAL, "EAM file %s: bad mesh line", p.file.c_str());
+        if (!std::getline(in, line)) throw std::runtime_error("element line");
+        if (!read_values(in, p.nrho, p.F) || !read_values(in, p.nr, p.rho) || !read_values(in, p.nr, p.phi))
+            throw std::runtime_error("tables");
+    } catch (const std::exception &) {
+        return fail(AKMC_EINVAL, "EAM file %s is truncated or malformed", p.file.c_str());
+    }
+    for (int i = 0; i < p.nr; ++i) p.phi[i] = p.phi[i] / (i * p.dr);
+    return AKMC_SUCCESS;
+}
+
 int parse_potentials(akmc_field_t f, const char *text)
 {
     std::istringstream in(text ? text : "");
@@ -186,12 +245,30 @@ int parse_potentials(akmc_field_t f, const char *text)
                     std::string kl, pl;
                     if (!next_line(kl) || !next_line(pl)) return fail(AKMC_EINVAL, "manybody: missing lines");
                     std::string key = lower(split_ws(kl).at(0));
-                    if (key != "suttonchen")
+                    auto pw = split_ws(pl);
+                    if (key == "eam") {
+                        // A B file [lammps|dlpoly] (ManyBody.cpp:269-307)
+                        EamPot e;
+                        e.A = pw.at(0);
+                        e.B = pw.at(1);
+                        e.file = pw.at(2);
+                        std::string fmt = pw.size() > 3 ? pw[3] : "lammps";
+                        if (fmt == "dlpoly")
+                            return fail(AKMC_EUNSUPPORTED, "the DL_POLY tabulated EAM format is not supported");
+                        if (fmt != "lammps") return fail(AKMC_EINVAL, "unrecognised EAM format");
+                        const int rc = read_lammps_eam(e);
+                        if (rc != AKMC_SUCCESS) return rc;
+                        f->eam.push_back(std::move(e));
+                    } else if (key == "suttonchen") {
+                        // A B eps a n m c (ManyBody.cpp:148-163)
+                        f->sc.push_back({pw.at(0), pw.at(1), std::stod(pw.at(2)), std::stod(pw.at(3)),
+                                         std::stod(pw.at(4)), std::stod(pw.at(5)), std::stod(pw.at(6))});
+                    } else {
                         return fail(AKMC_EUNSUPPORTED, "many-body form '%s' is not supported on the device",
                                     key.c_str());
-                    auto pw = split_ws(pl);  // A B eps a n m c (ManyBody.cpp:148-163)
-                    f->sc.push_back({pw.at(0), pw.at(1), std::stod(pw.at(2)), std::stod(pw.at(3)),
-                                     std::stod(pw.at(4)), std::stod(pw.at(5)), std::stod(pw.at(6))});
+                    }
+                    if (!f->eam.empty() && !f->sc.empty())
+                        return fail(AKMC_EINVAL, "many-body potential types can not be mixed");
                 }
             } else if (k == "twobody") {
                 f->kind = 1;
@@ -223,6 +300,7 @@ int parse_potentials(akmc_field_t f, const char *text)
     if (f->kind < 0) return fail(AKMC_EINVAL, "potentials text has no manybody or twobody section");
     if (f->species.empty()) return fail(AKMC_EINVAL, "potentials text has no species");
     if ((int)f->species.size() > AKMC_MAXSPEC) return fail(AKMC_ECAPACITY, "more than %d species", AKMC_MAXSPEC);
+    if ((int)f->eam.size() > AKMC_MAXEAM) return fail(AKMC_ECAPACITY, "more than %d EAM tables", AKMC_MAXEAM);
     return AKMC_SUCCESS;
 }
 
@@ -247,6 +325,32 @@ void build_tables(akmc_field_t f)
     h.rnear2 = f->kind == 1 ? (3.5 + f->skin) * (3.5 + f->skin) : 0.0;
     h.cfact = kCTOEV;
     for (int i = 0; i < AKMC_MAXPAIR; ++i) f->table[i] = f->kind == 0 ? -1 : 0;
+    // EAM: species pair -> table, both orders (calculateEAMEnergyMesh,
+    // ManyBody.cpp:937-1000); the device table pointers are set when the
+    // tables are uploaded
+    h.eam = f->eam.empty() ? 0 : 1;
+    for (int i = 0; i < AKMC_MAXPAIR; ++i) h.eam_tab[i] = -1;
+    for (int k = 0; k < (int)f->eam.size(); ++k)
+        for (int i = 0; i < ns; ++i)
+            for (int j = 0; j < ns; ++j)
+                if (f->species[i].name == f->eam[k].A && f->species[j].name == f->eam[k].B) {
+                    h.eam_tab[j * ns + i] = k;
+                    h.eam_tab[i * ns + j] = k;
+                }
+    h.eam_one = 0;
+    if (h.eam) {
+        h.eam_one = 1;
+        for (int i = 0; i < ns * ns; ++i) h.eam_one &= h.eam_tab[i] == h.eam_tab[0] && h.eam_tab[0] >= 0;
+        for (int k = 0; k < (int)f->eam.size(); ++k) {
+            const EamPot &e = f->eam[k];
+            EamTabDev &t = h.eamt[k];
+            t.nr = e.nr;
+            t.nrho = e.nrho;
+            t.dr = e.dr;
+            t.drho = e.drho;
+            t.cut = e.cut;
+        }
+    }
     for (int i = 0; i < ns; ++i)
         for (int j = 0; j < ns; ++j) {
             int np = f->kind == 0 ? (int)f->sc.size() : (int)f->buck.size();
@@ -369,6 +473,58 @@ int dalloc(akmc_batch_t b, T **p, size_t n)
         int rc_ = (expr);              \
         if (rc_ != AKMC_SUCCESS) return rc_; \
     } while (0)
+
+// device copy of the EAM tables (once per field) and their addresses in the
+// host FieldDev (again after every build_tables)
+int ensure_eam_tables(akmc_field_t f)
+{
+    if (f->eam.empty()) return AKMC_SUCCESS;
+    std::vector<size_t> off_d, off_p, off_e;
+    size_t total = 0;
+    for (const EamPot &e : f->eam) {
+        off_d.push_back(total);
+        total += e.nr;
+        off_p.push_back(total);
+        total += 3 * (size_t)e.nr;
+        off_e.push_back(total);
+        total += 2 * (size_t)e.nrho;
+    }
+    if (!f->eam_dev) {
+        // value tables as (t_i, t_{i+1} - t_i); derivative tables are the
+        // forward differences (MetalPotential.cpp:388-392), last entry 0
+        auto deriv = [](const std::vector<double> &t, double h) {
+            std::vector<double> d(t.size(), 0.0);
+            for (size_t i = 0; i + 1 < t.size(); ++i) d[i] = (t[i + 1] - t[i]) / h;
+            return d;
+        };
+        auto pairs = [](const std::vector<double> &t, size_t i) {
+            return make_double2(t[i], i + 1 < t.size() ? t[i + 1] - t[i] : 0.0);
+        };
+        std::vector<double2> h(total);
+        for (size_t k = 0; k < f->eam.size(); ++k) {
+            const EamPot &e = f->eam[k];
+            const std::vector<double> dphi = deriv(e.phi, e.dr), drho = deriv(e.rho, e.dr), dF = deriv(e.F, e.drho);
+            for (int i = 0; i < e.nr; ++i) {
+                h[off_d[k] + i] = pairs(e.rho, i);
+                h[off_p[k] + 3 * (size_t)i + 0] = pairs(e.phi, i);
+                h[off_p[k] + 3 * (size_t)i + 1] = pairs(dphi, i);
+                h[off_p[k] + 3 * (size_t)i + 2] = pairs(drho, i);
+            }
+            for (int i = 0; i < e.nrho; ++i) {
+                h[off_e[k] + 2 * (size_t)i + 0] = pairs(e.F, i);
+                h[off_e[k] + 2 * (size_t)i + 1] = pairs(dF, i);
+            }
+        }
+        HIPCHK(hipMalloc((void **)&f->eam_dev, total * sizeof(double2)));
+        HIPCHK(hipMemcpy(f->eam_dev, h.data(), total * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    for (size_t k = 0; k < f->eam.size(); ++k) {
+        f->host.eamt[k].dens = f->eam_dev + off_d[k];
+        f->host.eamt[k].pair = f->eam_dev + off_p[k];
+        f->host.eamt[k].emb = f->eam_dev + off_e[k];
+    }
+    return AKMC_SUCCESS;
+}
 
 int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akmc_batch_t b)
 {
@@ -539,6 +695,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     // device copy of the field (the eta of the last walker; all walkers of a
     // batch share the field's eta rule)
     if (!f->dev) HIPCHK(hipMalloc(&f->dev, sizeof(FieldDev)));
+    TRY(ensure_eam_tables(f));
     HIPCHK(hipMemcpy(f->dev, &f->host, sizeof(FieldDev), hipMemcpyHostToDevice));
     d.field = f->dev;
     b->hout.resize(W);
@@ -578,13 +735,20 @@ int run_forces(akmc_batch_t b, int want_stress)
     return AKMC_SUCCESS;
 }
 
+// the error of a nonzero overflow word
+int overflow_error(akmc_batch_t b, int ov)
+{
+    if (ov & AKMC_OV_LIST) return fail(AKMC_ECAPACITY, "neighbour list capacity %d exceeded", b->d.maxnb);
+    if (ov & AKMC_OV_EAM_RHO) return fail(AKMC_EINVAL, "the rho for embedding is out of bounds of its EAM table");
+    return AKMC_SUCCESS;
+}
+
 int check_overflow(akmc_batch_t b)
 {
     int ov = 0;
     HIPCHK(hipMemcpyAsync(&ov, b->d.overflow, sizeof(int), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    if (ov) return fail(AKMC_ECAPACITY, "neighbour list capacity %d exceeded", b->d.maxnb);
-    return AKMC_SUCCESS;
+    return overflow_error(b, ov);
 }
 
 int fetch_out(akmc_batch_t b)
@@ -911,6 +1075,7 @@ void akmc_field_destroy(akmc_field_t f)
     if (!f) return;
     if (f->single) batch_free(f->single);
     if (f->dev) (void)hipFree(f->dev);
+    if (f->eam_dev) (void)hipFree(f->eam_dev);
     delete f;
 }
 
@@ -979,7 +1144,14 @@ int akmc_field_energy(akmc_field_t f, double *x, double *y, double *z, const dou
 {
     if (!f || n <= 0) return fail(AKMC_EINVAL, "bad arguments");
     std::vector<double> fx(n), fy(n), fz(n);
-    return akmc_field_forces(f, x, y, z, fx.data(), fy.data(), fz.data(), nullptr, lat, charge, type, nullptr, n, e);
+    TRY(akmc_field_forces(f, x, y, z, fx.data(), fy.data(), fz.data(), nullptr, lat, charge, type, nullptr, n, e));
+    // the energy path of a tabulated EAM field doubles the pair term
+    // (calculateManyBodyPairEnergy, potentials/metal/ManyBody.cpp:444-450)
+    if (e && f->kind == 0 && !f->eam.empty()) {
+        e->vdw *= 2.0;
+        e->total = e->many + e->vdw;
+    }
+    return AKMC_SUCCESS;
 }
 
 int akmc_batch_create(akmc_field_t f, int nwalkers, const int *natoms, const double *lat, akmc_batch_t *out)
@@ -1271,7 +1443,7 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
         HIPCHK(hipMemcpyAsync(b->hws_pin, b->s.ws, b->W * sizeof(WalkerState), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipMemcpyAsync(b->ov_pin, b->d.overflow, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
-        if (*b->ov_pin) return fail(AKMC_ECAPACITY, "neighbour list capacity %d exceeded", b->d.maxnb);
+        if (*b->ov_pin) return overflow_error(b, *b->ov_pin);
         if (b->ktimer && batch_mode(b) == 1 && b->f->host.kind == 0)
             for (int j = 0; j < k; ++j) {
                 float t = 0.0f;

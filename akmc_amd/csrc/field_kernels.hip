@@ -792,6 +792,184 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
     }
 }
 
+// ------------------------------------------------------- tabulated EAM
+
+// eamMeshEnergy (potentials/metal/ManyBody.cpp:455-460) on a table stored as
+// (t_i, t_{i+1} - t_i): t_i + ((t_{i+1} - t_i) (x - i h)) / h.  The index
+// is clamped to the last interval; a density past its table is flagged by
+// the caller (the reference exits there, and reads one entry past the table
+// for the last index).
+__device__ inline double eam_interp(const double2 *t, int stride, int n, double x, double h, int &idx)
+{
+    idx = int(x / h);
+    const int k = idx < 0 ? 0 : (idx > n - 2 ? n - 2 : idx);
+    const double2 v = t[(size_t)k * stride];
+    return v.x + (v.y * (x - k * h)) / h;
+}
+
+constexpr double kEamTiny = 1.0e-8;  // ManyBody.cpp:728
+constexpr double kEamNoDeriv = -1.0e300;  // F'(rho) of an atom whose rho < tiny
+
+// pass 1: rho_i = sum_j rho_{t_j t_j}(r) over the pairs inside both the
+// field cutoff and the table's (Field.cpp:1782-1853 with
+// calculateManyBodyDensEnergy, ManyBody.cpp:462-545), then
+// F_{t_i t_i}(rho_i) (embed, ManyBody.cpp:546-619) and F'(rho_i).  The
+// interpolation runs in FP64 on both precision paths: a table lookup is a
+// gather, not arithmetic, so the FP32 path gains nothing from FP32 here.
+// ONE: one table for every pair; F'(rho_i) then rides in the packed gather
+// record for pass 2 (k_pack restores the species every step).
+template <typename T, bool MINIMG, int LPA, bool ONE>
+__global__ void __launch_bounds__(kBlock) k_eam_density(BatchDev b)
+{
+    const FieldDev &F = *b.field;
+    const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
+    if (t < 0) return;
+    const int i = t / LPA, sub = t % LPA;
+    if (i >= b.A) return;
+    if (other_precision<T>(b, walker_of(b, i))) return;
+    const WalkerDev W = b.walkers[walker_of(b, i)];
+    const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    const double rc2 = F.rc2;
+    const int cnt = b.nnb[i];
+    const int ns = F.nspec;
+    double rho = 0.0;
+    for (int k = sub; k < cnt; k += LPA) {
+        const uint32_t e = nbr_entry(b, k, i);
+        const double4 pj = b.p4[W.off + int(e & AKMC_NB_JMASK)];
+        double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+        entry_disp(e, dx, dy, dz, W, MINIMG);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3))) continue;
+        const int tj = ONE ? 0 : (int)pj.w;
+        const int tb = ONE ? F.eam_tab[0] : F.eam_tab[tj * ns + tj];  // density of species b
+        if (tb < 0) continue;
+        const EamTabDev &T_ = F.eamt[tb];
+        const double r = sqrt(r2);
+        if (r < kEamTiny || !(r < T_.cut)) continue;
+        int idx;
+        rho += eam_interp(T_.dens, 1, T_.nr, r, T_.dr, idx);
+    }
+    rho = lane_group_sum<LPA>(rho);
+    if (sub != 0) return;
+    b.rho[i] = rho;
+    const int ti = ONE ? 0 : b.type[i];
+    const int ta = ONE ? F.eam_tab[0] : F.eam_tab[ti * ns + ti];
+    double Fi = 0.0, Fp = kEamNoDeriv;
+    if (ta >= 0) {
+        const EamTabDev &T_ = F.eamt[ta];
+        int idx;
+        Fi = eam_interp(T_.emb, 2, T_.nrho, rho, T_.drho, idx);
+        if (idx < 0 || idx >= T_.nrho) atomicOr(b.overflow, AKMC_OV_EAM_RHO);
+        if (rho >= kEamTiny) Fp = eam_interp(T_.emb + 1, 2, T_.nrho, rho, T_.drho, idx);
+    }
+    b.e_a[i] = Fi;
+    if (ONE) reinterpret_cast<double *>(&b.p4[i])[3] = Fp;
+}
+
+// pass 2: per ordered pair (Field.cpp:587-640 with
+// calculateManyBodyPairForce and calculateManyBodyForce,
+// ManyBody.cpp:621-835): fs = -r phi'(r), fm = (F'(rho_i) + F'(rho_j))
+// rho'(r) r (zero when either density is below 1e-8), F_i += r_ij
+// (fs - fm) / r^2 over both orderings of each pair, and half of phi(r) to
+// the pair energy.  phi, phi' and rho' are those of the (t_i, t_j) table,
+// F' too, evaluated at rho_i and rho_j, as the reference does.
+template <typename T, bool MINIMG, bool STRESS, int LPA, bool ONE>
+__global__ void __launch_bounds__(kBlock) k_eam_force(BatchDev b)
+{
+    const FieldDev &F = *b.field;
+    const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
+    if (t < 0) return;
+    const int i = t / LPA, sub = t % LPA;
+    if (i >= b.A) return;
+    if (other_precision<T>(b, walker_of(b, i))) return;
+    const WalkerDev W = b.walkers[walker_of(b, i)];
+    const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    const double rc2 = F.rc2;
+    const int cnt = b.nnb[i];
+    const int ns = F.nspec;
+    const int ti = ONE ? 0 : b.type[i];
+    const double rhoi = b.rho[i];
+    const double fpi = ONE ? b.p4[i].w : 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    for (int k = sub; k < cnt; k += LPA) {
+        const uint32_t e = nbr_entry(b, k, i);
+        const int j = W.off + int(e & AKMC_NB_JMASK);
+        const double4 pj = b.p4[j];
+        double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+        entry_disp(e, dx, dy, dz, W, MINIMG);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 <= rc2 && r2 > (MINIMG ? 1.0e-16 : 1.0e-3))) continue;
+        const int tab = ONE ? F.eam_tab[0] : F.eam_tab[ti * ns + (int)pj.w];
+        if (tab < 0) continue;
+        const EamTabDev &T_ = F.eamt[tab];
+        const double r = sqrt(r2);
+        if (r < kEamTiny || !(r < T_.cut)) continue;
+        int idx;
+        const double phi = eam_interp(T_.pair, 3, T_.nr, r, T_.dr, idx);
+        const double dphi = eam_interp(T_.pair + 1, 3, T_.nr, r, T_.dr, idx);
+        const double fs = -r * dphi;
+        double fm = 0.0;
+        double fpa, fpb;
+        bool dens_ok;
+        if (ONE) {
+            fpa = fpi;
+            fpb = pj.w;
+            dens_ok = fpa > 0.5 * kEamNoDeriv && fpb > 0.5 * kEamNoDeriv;
+        } else {
+            const double rhoj = b.rho[j];
+            dens_ok = rhoi >= kEamTiny && rhoj >= kEamTiny;
+            fpa = dens_ok ? eam_interp(T_.emb + 1, 2, T_.nrho, rhoi, T_.drho, idx) : 0.0;
+            if (dens_ok && (idx < 0 || idx >= T_.nrho)) atomicOr(b.overflow, AKMC_OV_EAM_RHO);
+            fpb = dens_ok ? eam_interp(T_.emb + 1, 2, T_.nrho, rhoj, T_.drho, idx) : 0.0;
+            if (dens_ok && (idx < 0 || idx >= T_.nrho)) atomicOr(b.overflow, AKMC_OV_EAM_RHO);
+        }
+        if (dens_ok) {
+            const double drho = eam_interp(T_.pair + 2, 3, T_.nr, r, T_.dr, idx);
+            fm = ((fpa + fpb) * drho) * r;
+        }
+        const double ft = (fs - fm) / r2;  // both orderings of the pair
+        fx += ft * dx;
+        fy += ft * dy;
+        fz += ft * dz;
+        ep += 0.5 * phi;
+        if (STRESS) {
+            const double h = -0.5 * ft;
+            s0 += h * dx * dx;
+            s1 += h * dy * dy;
+            s2 += h * dz * dz;
+            s3 += h * dy * dz;
+            s4 += h * dx * dz;
+            s5 += h * dx * dy;
+        }
+    }
+    fx = lane_group_sum<LPA>(fx);
+    fy = lane_group_sum<LPA>(fy);
+    fz = lane_group_sum<LPA>(fz);
+    ep = lane_group_sum<LPA>(ep);
+    if (STRESS) {
+        s0 = lane_group_sum<LPA>(s0);
+        s1 = lane_group_sum<LPA>(s1);
+        s2 = lane_group_sum<LPA>(s2);
+        s3 = lane_group_sum<LPA>(s3);
+        s4 = lane_group_sum<LPA>(s4);
+        s5 = lane_group_sum<LPA>(s5);
+    }
+    if (sub != 0) return;
+    b.fx[i] = fx;
+    b.fy[i] = fy;
+    b.fz[i] = fz;
+    b.e_b[i] = ep;
+    if (STRESS) {
+        b.st_atom[0 * (size_t)b.A + i] = s0;
+        b.st_atom[1 * (size_t)b.A + i] = s1;
+        b.st_atom[2 * (size_t)b.A + i] = s2;
+        b.st_atom[3 * (size_t)b.A + i] = s3;
+        b.st_atom[4 * (size_t)b.A + i] = s4;
+        b.st_atom[5 * (size_t)b.A + i] = s5;
+    }
+}
+
 // ------------------------------------------------ Buckingham + Ewald real
 
 // real-space Ewald and Buckingham over the full list
@@ -1171,11 +1349,49 @@ static void launch_sc_force(const BatchDev &b, const FieldDev &hf, int want_stre
         else AKMC_SC_UNROLL(CALL, 1);                             \
     } while (0)
 
+// the EAM kernels: 4 lanes per atom
+constexpr int kEamLanes = 4;
+
+template <typename T>
+static void launch_eam_density(const BatchDev &b, const FieldDev &hf, hipStream_t s)
+{
+    int g = xcd_grid((b.A * kEamLanes + kBlock - 1) / kBlock);
+    if (hf.eam_one) {
+        if (hf.min_image) k_eam_density<T, true, kEamLanes, true><<<g, kBlock, 0, s>>>(b);
+        else k_eam_density<T, false, kEamLanes, true><<<g, kBlock, 0, s>>>(b);
+    } else {
+        if (hf.min_image) k_eam_density<T, true, kEamLanes, false><<<g, kBlock, 0, s>>>(b);
+        else k_eam_density<T, false, kEamLanes, false><<<g, kBlock, 0, s>>>(b);
+    }
+}
+
+template <typename T, bool ONE>
+static void launch_eam_force_one(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
+{
+    int g = xcd_grid((b.A * kEamLanes + kBlock - 1) / kBlock);
+    if (hf.min_image) {
+        if (want_stress) k_eam_force<T, true, true, kEamLanes, ONE><<<g, kBlock, 0, s>>>(b);
+        else k_eam_force<T, true, false, kEamLanes, ONE><<<g, kBlock, 0, s>>>(b);
+    } else {
+        if (want_stress) k_eam_force<T, false, true, kEamLanes, ONE><<<g, kBlock, 0, s>>>(b);
+        else k_eam_force<T, false, false, kEamLanes, ONE><<<g, kBlock, 0, s>>>(b);
+    }
+}
+
+template <typename T>
+static void launch_eam_force(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
+{
+    if (hf.eam_one) launch_eam_force_one<T, true>(b, hf, want_stress, s);
+    else launch_eam_force_one<T, false>(b, hf, want_stress, s);
+}
+
 template <typename T>
 static void launch_pass1(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     int gA = xcd_grid(blocks(b.A));
-    if (hf.kind == 0) {
+    if (hf.kind == 0 && hf.eam) {
+        launch_eam_density<T>(b, hf, s);
+    } else if (hf.kind == 0) {
 #define AKMC_DENS(L_, U_) launch_sc_density<T, L_, U_>(b, hf, s)
         AKMC_SC_VARIANT(AKMC_DENS);
 #undef AKMC_DENS
@@ -1194,7 +1410,9 @@ template <typename T>
 static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     int gA = blocks(b.A);
-    if (hf.kind == 0) {
+    if (hf.kind == 0 && hf.eam) {
+        launch_eam_force<T>(b, hf, want_stress, s);
+    } else if (hf.kind == 0) {
 #define AKMC_FORCE(L_, U_) launch_sc_force<T, L_, U_>(b, hf, want_stress, s)
         AKMC_SC_VARIANT(AKMC_FORCE);
 #undef AKMC_FORCE

@@ -284,7 +284,16 @@ class Field:
         return dict(pos=np.stack([x, y, z]), force=np.stack([fx, fy, fz]), stress=st, energy=e.as_dict())
 
     def calculate_energy(self, pos, types, lat, charges=None):
-        return self.calculate_forces(pos, types, lat, charges, stress=False)["energy"]
+        """Field::calculateEnergy (akmc_field_energy): the energy record only."""
+        pos = np.array(pos, dtype=np.float64, copy=True).reshape(3, -1)
+        n = pos.shape[1]
+        x, y, z = [np.ascontiguousarray(pos[k]) for k in range(3)]
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        q = _f64(charges) if charges is not None else None
+        e = Energy()
+        check(load().akmc_field_energy(self._h, _dp(x), _dp(y), _dp(z), _dp(_f64(lat)), _dp(q), _ip(t), n,
+                                       C.byref(e)))
+        return e.as_dict()
 
 
 class Batch:
@@ -550,6 +559,22 @@ def sc_potentials_text(species_pots, rc, species=None, noimage=False):
     lines.append(f"manybody {len(species_pots)} ev")
     for (A, B, eps, a, n, m, c) in species_pots:
         lines += ["suttonchen", f"{A} {B} {eps!r} {a!r} {n!r} {m!r} {c!r}"]
+    lines.append("close")
+    return "\n".join(lines) + "\n"
+
+
+def eam_potentials_text(entries, rc, species=None, noimage=False):
+    """Reference 'potentials' text for tabulated EAM: entries are
+    (A, B, setfl file path) triples, read as LAMMPS tables
+    (potentials/metal/ManyBody.cpp:269-307)."""
+    species = species or sorted({e[0] for e in entries} | {e[1] for e in entries})
+    lines = [f"cutoff {rc!r}", f"species {len(species)}"]
+    lines += [f"{s} 63.546 0.0 29" for s in species]
+    if noimage:
+        lines.append("noimage")
+    lines.append(f"manybody {len(entries)} ev")
+    for (A, B, path) in entries:
+        lines += ["eam", f"{A} {B} {path} lammps"]
     lines.append("close")
     return "\n".join(lines) + "\n"
 
