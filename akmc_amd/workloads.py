@@ -143,3 +143,26 @@ def batched_sc_cells(n_walkers=512, n_min=500, n_max=4000, seed=3, jitter=0.01):
         nv = int(rng.integers(1, 4))
         cells.append(sc_cell(*s, n_vac=nv, jitter=jitter, seed=int(rng.integers(0, 2**31 - 1))))
     return cells
+
+
+def write_sc_setfl(path, eps, a, n, m, c, rc, dr=1.0e-3, drho=1.0e-3, rho_max=40.0):
+    """A LAMMPS setfl file (the layout of the reference's readLEAM) whose
+    tables sample the Sutton-Chen functions: F(rho) = -sqrt(rho),
+    rho(r) = (eps c)^2 (a/r)^m and r phi(r) = r eps (a/r)^n, with the r = 0
+    end held at r = 0.5 A (never evaluated).  The table cutoff sits 0.05 A
+    past rc so the field's cutoff decides which pairs count."""
+    import math
+    cut = rc + 0.05
+    nr = int(math.ceil((cut + 2 * dr) / dr)) + 1
+    nrho = int(math.ceil(rho_max / drho)) + 1
+    rr = [max(i * dr, 0.5) for i in range(nr)]
+    cols = [[-math.sqrt(i * drho) for i in range(nrho)],
+            [(eps * c) ** 2 * (a / r) ** m for r in rr],
+            [i * dr * eps * (a / r) ** n for i, r in enumerate(rr)]]
+    with open(path, "w") as fh:
+        fh.write("# Sutton-Chen functions on a mesh\n#\n#\n1 Cu\n")
+        fh.write(f"{nrho} {drho!r} {nr} {dr!r} {cut!r}\n29 63.546 {a!r} fcc\n")
+        for col in cols:
+            for k in range(0, len(col), 5):
+                fh.write(" ".join(repr(float(v)) for v in col[k:k + 5]) + "\n")
+    return path

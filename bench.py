@@ -18,7 +18,8 @@ Also reported: the plain unbatched GPU dimer (one walker, host-driven
 micro-steps) on the same cells, force evaluations per second, the roofline
 of the dominant kernel (Sutton-Chen pass-2 force) timed with HIP events on
 the batch's stream, the learned-proposal A/B test, the KMC step (64
-systems x 8 walkers taking kmcSteps steps of 5 valid events each) and the
+systems x 8 walkers taking kmcSteps steps of 5 valid events each), the
+tabulated-EAM field's force evaluation on the same walkers, and the
 CPU baseline (the FP64 C restatement of the reference, OpenMP) on a
 bounded sample of the same searches.
 
@@ -198,6 +199,33 @@ def learned_ab(camp, records, window, epochs):
     return out
 
 
+def eam_section(L, W, cells, basins, P, args):
+    """The tabulated-EAM field (LAMMPS setfl tables sampling the Cu
+    Sutton-Chen functions on a 1e-3 mesh) on the same 512 walkers at their
+    basins: one force evaluation of every walker, per-pass device times
+    (HIP events) and ordered pairs per second."""
+    import tempfile
+    cu = W.SC_CU
+    rc = 2 * cu["a"]
+    path = os.path.join(tempfile.mkdtemp(), "cu_sc.eam")
+    W.write_sc_setfl(path, cu["eps"], cu["a"], cu["n"], cu["m"], cu["c"], rc)
+    f = L.Field(L.eam_potentials_text([("Cu", "Cu", path)], rc))
+    b = L.Batch(f, [W.Cell(p, c.types, c.lat) for p, c in zip(basins, cells)])
+    b.forces()
+    ph = b.time_phases(20)
+    Ptot = int(P.sum())
+    out = {"table": "LAMMPS setfl, Cu Sutton-Chen functions sampled at dr = drho = 1e-3",
+           "ms_lists": round(float(ph[0]), 4), "ms_density": round(float(ph[1]), 4),
+           "ms_force": round(float(ph[2]), 4), "ms_reduce": round(float(ph[3]), 4),
+           "force_eval_ms_all_walkers": round(float(ph.sum()), 4),
+           "ordered_pairs": Ptot,
+           "pairs_per_s_force_pass": round(Ptot / (ph[2] * 1e-3), 1),
+           "force_evals_per_s_cells_device": round(len(cells) / (float(ph.sum()) * 1e-3), 1)}
+    b.close()
+    f.close()
+    return out
+
+
 def kmc_section(L, W, text, args):
     """The KMC step around the search path: 64 independent systems (walker
     cells of the batched workload) x 8 walkers each, kmcSteps 3 of
@@ -244,6 +272,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal section")
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--no-kmc", action="store_true", help="skip the KMC-step section")
+    ap.add_argument("--no-eam", action="store_true", help="skip the tabulated-EAM section")
     ap.add_argument("--kmc-systems", type=int, default=64)
     ap.add_argument("--kmc-steps", type=int, default=3)
     ap.add_argument("--seed", type=int, default=3)
@@ -352,6 +381,11 @@ def main():
         print(f"[bench] learned section: { {k: v for k, v in learned.items() if k not in ('random', 'learned')} }",
               file=sys.stderr, flush=True)
 
+    eam = None
+    if not args.no_eam:
+        eam = eam_section(L, W, cells, camp.basins, P, args)
+        print(f"[bench] tabulated EAM: {eam}", file=sys.stderr, flush=True)
+
     kmc = None
     if not args.no_kmc:
         kmc = kmc_section(L, W, text, args)
@@ -411,6 +445,8 @@ def main():
         out["learned_proposal"] = learned
     if kmc:
         out["kmc"] = kmc
+    if eam:
+        out["tabulated_eam"] = eam
     if cpu:
         out["cpu_baseline"] = {"value": round(cpu["saddles_per_s"], 5), "unit": "saddles/s",
                                "cores": cpu["cores"], "kind": "port",
