@@ -149,6 +149,8 @@ void akmc_launch_forces_impl(const BatchDev &b, const FieldDev &hf, int fp64, in
 void akmc_launch_phase(const BatchDev &b, const FieldDev &hf, int fp64, int want_stress, int phase,
                        hipStream_t s);
 extern "C" unsigned long long *akmc_rebuild_counter_ptr();
+// records the message akmc_last_error() returns and passes code through
+extern "C" int akmc_fail(int code, const char *fmt, ...);
 
 // ------------------------------------------------------------ searches
 

@@ -1054,6 +1054,17 @@ extern "C" {
 
 const char *akmc_last_error(void) { return g_err.c_str(); }
 
+int akmc_fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
 int akmc_version(void) { return 1; }
 
 int akmc_field_create(const char *text, akmc_field_t *out)

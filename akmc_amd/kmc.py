@@ -137,7 +137,8 @@ class KineticMonteCarlo:
 
     def __init__(self, potentials_text, systems, walkers_per_system=8, params: KMCParams | None = None,
                  method="dimer", search_params=None, relax=None, region_radius=6.0, d0=0.1, seed=0,
-                 fmax_verify=0.05, barrier_tol=0.01, verify_nl=16, verify_delta=0.01, skin=None, sync_every=25):
+                 fmax_verify=0.05, barrier_tol=0.01, verify_nl=16, verify_delta=0.01, skin=None, sync_every=25,
+                 output_dir=None, write_events=False):
         if method not in ("dimer", "art"):
             raise ValueError("the kmc method is not recognised")  # KineticMonteCarlo.cpp:1034-1046
         self.p = params or KMCParams()
@@ -169,6 +170,14 @@ class KineticMonteCarlo:
         self.pending = set()
         self._L = L.load()
         self._lat = [np.ascontiguousarray(s.cell.lat, dtype=np.float64) for s in self.systems]
+        # files of runKMC (kmc/KineticMonteCarlo.cpp:181-241), one folder per
+        # system: events.<iteration> (writeEvents), restart and archive
+        self.output_dir = output_dir
+        self.write_events = write_events
+        if output_dir:
+            import os
+            for si in range(len(self.systems)):
+                os.makedirs(os.path.join(output_dir, f"system{si}"), exist_ok=True)
 
     # ------------------------------------------------------------ helpers
     def _sys(self, w):
@@ -239,14 +248,48 @@ class KineticMonteCarlo:
         s.basin = np.array(ev.basin2, copy=True)            # Event::setPositions
         self._move_vacancy(s, old, s.basin)
         e1 = s.e_basin64
+        if self.output_dir and self.write_events:
+            self._write_events(s, old, e1, chosen, s.iteration + 1)
         s.e_basin64 = ev.basin2_energy
         kept = self._recycle(s, chosen, lat)
+        if self.output_dir:
+            self._write_basis(s, s.iteration + 1)
         s.iteration += 1
         stats.kmc_steps += 1
         stats.steps.append(KMCStepRecord(s.walkers[0] // self.G, s.iteration, n, chosen, total.value, dt, s.time,
                                          ev.activation_energy, e1, ev.basin2_energy, len(kept),
                                          [e.activation_energy for e in s.events]))
         s.events = kept
+
+    def _folder(self, s):
+        import os
+        return os.path.join(self.output_dir, f"system{self.systems.index(s)}")
+
+    def _names(self, s):
+        return list(s.cell.species) or ["X"]
+
+    def _write_events(self, s, basin1, e1, chosen, iteration):
+        """writeEventList (:1561-1616) before the move; the time written is
+        kmcTime x prefactor as at :181"""
+        import os
+        evs = [dict(act=e.activation_energy, centre=e.centre, saddle_energy=e.saddle_energy, saddle=e.saddle,
+                    basin2_energy=e.basin2_energy, basin2=e.basin2) for e in s.events]
+        L.write_events(os.path.join(self._folder(s), f"events.{iteration}"), self._names(s), s.cell.types, basin1,
+                       s.cell.lat, e1, evs, chosen, s.time * self.p.kmcPreFactor)
+
+    def _write_basis(self, s, iteration):
+        """the restart file and the archive after a step (:210-236): basis
+        dumps of the new basin 1 with time kmcTime x prefactor; the dump
+        header goes to the first file written, as the reference's Basis
+        writes it once"""
+        import os
+        d = self._folder(s)
+        first = not getattr(s, "_dumped", False)
+        args = ("kmc", self._names(s), s.cell.types, s.basin, s.cell.lat, s.time * self.p.kmcPreFactor,
+                s.e_basin64, iteration)
+        L.write_restart(os.path.join(d, "restart"), *args, header=first)
+        L.dump_basis(os.path.join(d, "archive"), *args, header=False, append=not first)
+        s._dumped = True
 
     def _displaced(self, pos, basin, lat):
         n = pos.shape[1]

@@ -200,6 +200,18 @@ def load() -> C.CDLL:
     L.akmc_kmc_recycle.argtypes = [C.c_int, _D, C.c_int, _D, C.c_double, C.c_int, C.c_int, C.c_double, _D, _I,
                                    _I]
     L.akmc_kmc_update_excluded.argtypes = [C.c_int, _I, C.c_int, _I, C.c_int]
+    _F = C.POINTER(C.c_float)
+    L.akmc_kmc_event_pack.argtypes = [C.c_int, _I, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double, _D,
+                                      _I, _F]
+    L.akmc_kmc_event_unpack.argtypes = [C.c_int, _I, _F, C.c_double, C.c_double, _I, _D, _D, _D, _D, _D]
+    L.akmc_io_dump_basis.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int, _I, _D, _D,
+                                     C.c_double, C.c_double, C.c_int]
+    L.akmc_io_write_restart.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_char_p, C.c_int, _I, _D, _D,
+                                        C.c_double, C.c_double, C.c_int]
+    L.akmc_io_basis_size.argtypes = [C.c_char_p, _I, _I]
+    L.akmc_io_read_basis.argtypes = [C.c_char_p, C.c_char_p, C.c_int, _I, _D, _D, _I, _D, C.c_char_p, C.c_int]
+    L.akmc_io_write_events.argtypes = [C.c_char_p, C.c_char_p, C.c_int, _I, _D, _D, C.c_double, C.c_int, _D, _D,
+                                       _D, _D, _D, _D, C.c_int, C.c_double, C.c_int]
     _lib = L
     return L
 
@@ -593,3 +605,88 @@ def buck_potentials_text(species, charges, pots, rc, noimage=False, ewald_precis
         lines += ["buck", f"{A} {B} {a!r} {rho!r} {c!r}"]
     lines.append("close")
     return "\n".join(lines) + "\n"
+
+
+# ---------------------------------------------------------------- text formats
+# (host only; see include/akmc.h for the reference formats each follows)
+
+def _b(s):
+    return str(s).encode()
+
+
+def _soa(pos):
+    return _f64(np.asarray(pos, dtype=np.float64).reshape(3, -1).reshape(-1))
+
+
+def dump_basis(path, title, species, labels, pos, lat, time, energy, iteration, header=True, append=False):
+    """Basis::dumpBasis: the basis / archive format (species: list of names)."""
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    check(load().akmc_io_dump_basis(_b(path), int(append), int(header), _b(title), _b(" ".join(species)),
+                                    int(lab.size), _ip(lab), _dp(_soa(pos)), _dp(_f64(lat)), float(time),
+                                    float(energy), int(iteration)))
+
+
+def write_restart(path, title, species, labels, pos, lat, time, energy, iteration, header=False):
+    """runKMC's restart file: 'restart title', ' 0 3 N N', then dumpBasis."""
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    check(load().akmc_io_write_restart(_b(path), int(header), _b(title), _b(" ".join(species)), int(lab.size),
+                                       _ip(lab), _dp(_soa(pos)), _dp(_f64(lat)), float(time), float(energy),
+                                       int(iteration)))
+
+
+def read_basis(path, species):
+    """Basis::inputBasis: dict(title, labels, pos (3, n), lat (9,), iteration, time, levcfg)."""
+    L = load()
+    n, lev = C.c_int(), C.c_int()
+    check(L.akmc_io_basis_size(_b(path), C.byref(n), C.byref(lev)))
+    lab = np.zeros(n.value, dtype=np.int32)
+    pos = np.zeros(3 * n.value)
+    lat = np.zeros(9)
+    it, t = C.c_int(), C.c_double()
+    title = C.create_string_buffer(4096)
+    check(L.akmc_io_read_basis(_b(path), _b(" ".join(species)), n.value, _ip(lab), _dp(pos), _dp(lat),
+                               C.byref(it), C.byref(t), title, 4096))
+    return dict(title=title.value.decode(), labels=lab, pos=pos.reshape(3, -1), lat=lat, iteration=it.value,
+                time=t.value, levcfg=lev.value)
+
+
+def write_events(path, species, labels, basin1, lat, basin1_energy, events, chosen, time, fp32=True):
+    """writeEventList: events is a list of dicts with act, centre, saddle_energy,
+    saddle (3, n), basin2_energy, basin2 (3, n)."""
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    n, m = int(lab.size), len(events)
+    arr = lambda k: _f64([float(e[k]) for e in events]) if m else None
+    cen = _f64(np.array([e["centre"] for e in events], dtype=np.float64).reshape(-1)) if m else None
+    sad = _f64(np.stack([np.asarray(e["saddle"], dtype=np.float64) for e in events]).reshape(-1)) if m else None
+    b2 = _f64(np.stack([np.asarray(e["basin2"], dtype=np.float64) for e in events]).reshape(-1)) if m else None
+    check(load().akmc_io_write_events(_b(path), _b(" ".join(species)), n, _ip(lab), _dp(_soa(basin1)),
+                                      _dp(_f64(lat)), float(basin1_energy), m, _dp(arr("act")), _dp(cen),
+                                      _dp(arr("saddle_energy")), _dp(sad), _dp(arr("basin2_energy")), _dp(b2),
+                                      int(chosen), float(time), int(bool(fp32))))
+
+
+def event_pack(global_no, basin2, saddle, e1, e2, es, ea, centre):
+    """sendEvent's packet: (int32 [n], float32 [6n + 7])."""
+    g = np.ascontiguousarray(global_no, dtype=np.int32)
+    n = int(g.size)
+    veci = np.zeros(n, dtype=np.int32)
+    vecf = np.zeros(6 * n + 7, dtype=np.float32)
+    check(load().akmc_kmc_event_pack(n, _ip(g), _dp(_soa(basin2)), _dp(_soa(saddle)), float(e1), float(e2),
+                                     float(es), float(ea), _dp(_f64(centre)), _ip(veci),
+                                     vecf.ctypes.data_as(C.POINTER(C.c_float))))
+    return veci, vecf
+
+
+def event_unpack(veci, vecf, prefactor, temperature):
+    """Event(int*, float*, natoms, prefactor, temperature)."""
+    vi = np.ascontiguousarray(veci, dtype=np.int32)
+    vf = np.ascontiguousarray(vecf, dtype=np.float32)
+    n = int(vi.size)
+    g = np.zeros(n, dtype=np.int32)
+    b2, sd = np.zeros(3 * n), np.zeros(3 * n)
+    en, cen, rate = np.zeros(4), np.zeros(3), C.c_double()
+    check(load().akmc_kmc_event_unpack(n, _ip(vi), vf.ctypes.data_as(C.POINTER(C.c_float)), float(prefactor),
+                                       float(temperature), _ip(g), _dp(b2), _dp(sd), _dp(en), _dp(cen),
+                                       C.byref(rate)))
+    return dict(global_no=g, basin2=b2.reshape(3, -1), saddle=sd.reshape(3, -1), basin1_energy=en[0],
+                basin2_energy=en[1], saddle_energy=en[2], activation_energy=en[3], centre=cen, rate=rate.value)

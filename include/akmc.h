@@ -335,6 +335,53 @@ int akmc_kmc_recycle(int n, const double *centres, int chosen, const double lat[
  * displaced atoms */
 int akmc_kmc_update_excluded(int natoms, int *excluded, int nd, const int *displaced, int history);
 
+/* The KMC event packet of sendEvent / receiveEvent
+ * (kmc/KineticMonteCarlo.cpp:1376-1461) and Event(int*, float*, natoms,
+ * prefactor, temperature) (kmc/Event.cpp:66-105): veci[n] the atoms'
+ * global numbers; vecf[6n + 7] = per atom basin-2 x y z, saddle x y z, then
+ * basin-1, basin-2, saddle and activation energies and the centre, all
+ * FP32.  Unpacking returns FP64 copies of the FP32 values, energies[4] in
+ * that order, and the event's rate. */
+int akmc_kmc_event_pack(int n, const int *global_no, const double *basin2, const double *saddle, double e1,
+                        double e2, double es, double ea, const double centre[3], int *veci, float *vecf);
+int akmc_kmc_event_unpack(int n, const int *veci, const float *vecf, double prefactor, double temperature,
+                          int *global_no, double *basin2, double *saddle, double energies[4], double centre[3],
+                          double *rate);
+
+/* ---- text formats (host) ----------------------------------------------
+ * species: the species names separated by blanks; an atom's label indexes
+ * it.  Positions are SoA [3][n].  The writers use the reference's iostream
+ * formats, byte for byte. */
+/* Basis::dumpBasis (Common/Basis.cpp:600-649): the title and "0 3 N"
+ * header when header != 0 (the reference writes it on an object's first
+ * dump only), then "timeStep it N 0 3 time energy", the lattice and the
+ * atoms; append != 0 adds to the file (the archive) */
+int akmc_io_dump_basis(const char *path, int append, int header, const char *title, const char *species, int n,
+                       const int *label, const double *pos, const double lat[9], double simulation_time,
+                       double energy, int iteration);
+/* runKMC's restart file (kmc/KineticMonteCarlo.cpp:210-218): "restart
+ * title", " 0 3 N N", then dumpBasis */
+int akmc_io_write_restart(const char *path, int header, const char *title, const char *species, int n,
+                          const int *label, const double *pos, const double lat[9], double simulation_time,
+                          double energy, int iteration);
+/* Basis::inputBasis (Common/Basis.cpp:356-501): akmc_io_basis_size reads the
+ * atom count and levcfg; akmc_io_read_basis the labels, positions, lattice,
+ * the restart iteration and time (0 when absent) and the title */
+int akmc_io_basis_size(const char *path, int *n, int *levcfg);
+int akmc_io_read_basis(const char *path, const char *species, int n, int *label, double *pos, double lat[9],
+                       int *iteration, double *time, char *title, int title_len);
+/* writeEventList (kmc/KineticMonteCarlo.cpp:1561-1616) with
+ * Event::writeCentre / writeSaddle / writeBasin2 (kmc/Event.cpp:216-292):
+ * basin 1, then per event its activation energy, centre[3], saddle energy and
+ * positions (saddles [nev][3][n]) and basin-2 energy and positions.
+ * fp32_events != 0 prints the event quantities as the reference holds them,
+ * rounded to FP32 by its event packet. */
+int akmc_io_write_events(const char *path, const char *species, int n, const int *label, const double *basin1,
+                         const double lat[9], double basin1_energy, int nev, const double *act_energy,
+                         const double *centres, const double *saddle_energy, const double *saddles,
+                         const double *basin2_energy, const double *basin2s, int chosen, double simulation_time,
+                         int fp32_events);
+
 /* ---- learned proposal (no counterpart in the reference) ---------------
  * Per-atom descriptors (32 radial + 16 angular symmetry functions with
  * vacancy-site and candidate-mover channels) and the MLP 48 -> 128 -> 128

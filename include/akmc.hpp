@@ -13,11 +13,13 @@
 //   ART, Dimer optim/ART.h:136-138, optim/Dimer.h:68-70 (findSaddlePoint)
 //   KMCCntrl   the kmc{} keys (docs/docs/source/ACDCUserGuide.rst:150-180;
 //              member names as read in kmc/KineticMonteCarlo.cpp)
-//   Basis::checkBasin  Common/Basis.cpp:731-781
-//   Event      kmc/Event.cpp (rate, setPositions, getDisplacedAtoms)
+//   Basis::checkBasin  Common/Basis.cpp:731-781; inputBasis, dumpBasis
+//              (:356-501, :600-649)
+//   Event      kmc/Event.cpp (rate, the packet constructor, setPositions,
+//              getDisplacedAtoms)
 //   KineticMonteCarlo  kmc/KineticMonteCarlo.cpp: checkForNewBasin,
 //              calculateTime, the event choice of runKMC, clearEventList,
-//              updateExcludedList
+//              updateExcludedList, writeEventList
 // The MPIComms/Species/Basis/JobControl arguments of the reference become the
 // plain arrays they carry (positions SoA, lattice, labels, charges, frozen).
 #pragma once
@@ -230,6 +232,47 @@ struct Basis {
         p.insert(p.end(), posZ.begin(), posZ.end());
         return p;
     }
+    // Basis::inputBasis (Common/Basis.cpp:356-501) from a file; species
+    // gives the label of each symbol
+    std::string title;
+    bool firstWrite = true;
+    void inputBasis(const std::string &path, const std::vector<std::string> &species, int &restartIteration,
+                    double &restartTime)
+    {
+        int n = 0, lev = 0;
+        check(akmc_io_basis_size(path.c_str(), &n, &lev));
+        std::string names = join(species);
+        numberOfAtoms = n;
+        std::vector<double> p(3 * (size_t)n);
+        atmLabel.assign(n, 0);
+        frozen.assign(n, 0);
+        std::vector<char> t(4096);
+        check(akmc_io_read_basis(path.c_str(), names.c_str(), n, atmLabel.data(), p.data(), latVector,
+                                 &restartIteration, &restartTime, t.data(), (int)t.size()));
+        title = t.data();
+        posX.assign(p.begin(), p.begin() + n);
+        posY.assign(p.begin() + n, p.begin() + 2 * n);
+        posZ.assign(p.begin() + 2 * n, p.end());
+    }
+    // Basis::dumpBasis (Common/Basis.cpp:600-649), appending to path; the
+    // header on this object's first dump, as the reference does
+    void dumpBasis(const std::string &path, const std::vector<std::string> &species, double simulationTime,
+                   double energy, int iteration, bool append = true)
+    {
+        std::vector<double> p = soa();
+        std::string names = join(species);
+        check(akmc_io_dump_basis(path.c_str(), append ? 1 : 0, firstWrite ? 1 : 0, title.c_str(), names.c_str(),
+                                 numberOfAtoms, atmLabel.data(), p.data(), latVector, simulationTime, energy,
+                                 iteration));
+        firstWrite = false;
+    }
+    static std::string join(const std::vector<std::string> &v)
+    {
+        std::string s;
+        for (const std::string &x : v) s += x + " ";
+        return s;
+    }
+
     // Basis::checkBasin: some atom moved by >= detectBasinRadius and none
     // by more than maxBasinRadius
     bool checkBasin(const Basis &basin2, double detectBasinRadius, double maxBasinRadius) const
@@ -269,6 +312,27 @@ public:
           actEnergy(eact), rate(akmc_kmc_rate(eact, prefactor, temperature))
     {
         for (int k = 0; k < 3; ++k) centre[k] = c[k];
+    }
+    // Event(int* veci, float* vecf, natoms, prefactor, temperature)
+    // (kmc/Event.cpp:66-105): an event from the FP32 packet of sendEvent
+    std::vector<int> globNumber;
+    Event(const int *veci, const float *vecf, int natoms, double prefactor, double temperature)
+    {
+        std::vector<double> b2(3 * (size_t)natoms), sd(3 * (size_t)natoms);
+        double en[4];
+        globNumber.assign(natoms, 0);
+        check(akmc_kmc_event_unpack(natoms, veci, vecf, prefactor, temperature, globNumber.data(), b2.data(),
+                                    sd.data(), en, centre, &rate));
+        bas2PosX.assign(b2.begin(), b2.begin() + natoms);
+        bas2PosY.assign(b2.begin() + natoms, b2.begin() + 2 * natoms);
+        bas2PosZ.assign(b2.begin() + 2 * natoms, b2.end());
+        sadPosX.assign(sd.begin(), sd.begin() + natoms);
+        sadPosY.assign(sd.begin() + natoms, sd.begin() + 2 * natoms);
+        sadPosZ.assign(sd.begin() + 2 * natoms, sd.end());
+        basin1Energy = en[0];
+        basin2Energy = en[1];
+        saddleEnergy = en[2];
+        actEnergy = en[3];
     }
     double getRate() const { return rate; }
     double getActivationEnergy() const { return actEnergy; }
@@ -434,6 +498,28 @@ public:
         if (found) relStatus.setStatusSuccess();
         else relStatus.setStatusFailed();
         return found;
+    }
+
+    // writeEventList (:1561-1616): events.<iteration> in the working directory
+    void writeEventList(const std::vector<std::string> &species, const Basis &basin1,
+                        const std::vector<Event> &eventList, double simulationTime, double basin1Energy,
+                        int chosenEvent, int iteration)
+    {
+        const int n = basin1.numberOfAtoms, m = (int)eventList.size();
+        std::vector<double> act, cen, es, sad, e2, b2;
+        for (const Event &e : eventList) {
+            act.push_back(e.actEnergy);
+            cen.insert(cen.end(), e.centre, e.centre + 3);
+            es.push_back(e.saddleEnergy);
+            e2.push_back(e.basin2Energy);
+            for (const auto *v : {&e.sadPosX, &e.sadPosY, &e.sadPosZ}) sad.insert(sad.end(), v->begin(), v->end());
+            for (const auto *v : {&e.bas2PosX, &e.bas2PosY, &e.bas2PosZ}) b2.insert(b2.end(), v->begin(), v->end());
+        }
+        std::vector<double> p = basin1.soa();
+        std::string names = Basis::join(species), file = "events." + std::to_string(iteration);
+        check(akmc_io_write_events(file.c_str(), names.c_str(), n, basin1.atmLabel.data(), p.data(),
+                                   basin1.latVector, basin1Energy, m, act.data(), cen.data(), es.data(), sad.data(),
+                                   e2.data(), b2.data(), chosenEvent, simulationTime, 1));
     }
 
     // updateExcludedList (:1291-1313)

@@ -3,6 +3,7 @@
 // Relax::RelaxStructure, Dimer/ART::findSaddlePoint).
 //
 //   host_driver <action> <potentials file> <config file> <out file>
+//   host_driver basis "<species names>" <basis in> <basis out>
 //   action: forces | relax | dimer | art | kmc
 // config: N, 9 lattice numbers, N lines "x y z type frozen", then for dimer
 // and art N lines "mx my mz" (start direction), for kmc N lines of saddle
@@ -13,6 +14,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -20,6 +22,34 @@
 
 int main(int argc, char **argv)
 {
+    if (argc == 5 && std::string(argv[1]) == "basis") {
+        // host_driver basis "<species names>" <basis in> <basis out>:
+        // Basis::inputBasis, then Basis::dumpBasis of what was read
+        std::vector<std::string> species;
+        std::istringstream sp(argv[2]);
+        for (std::string w; sp >> w;) species.push_back(w);
+        akmc::Basis b;
+        int it = 0;
+        double t = 0.0;
+        try {
+            b.inputBasis(argv[3], species, it, t);
+            std::ifstream in(argv[3]);
+            std::string line;
+            std::getline(in, line);
+            std::getline(in, line);
+            std::getline(in, line);  // the energy sits in the timeStep line
+            std::istringstream ls(line);
+            std::string w;
+            double energy = 0.0;
+            for (int k = 0; k < 7 && ls >> w; ++k)
+                if (k == 6) energy = std::stod(w);
+            b.dumpBasis(argv[4], species, t, energy, it, false);
+        } catch (const std::exception &e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            return 1;
+        }
+        return 0;
+    }
     if (argc != 5) {
         std::fprintf(stderr, "usage: host_driver action potentials config out\n");
         return 2;

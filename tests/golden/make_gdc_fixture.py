@@ -1,8 +1,11 @@
 """Writes tests/golden/gdc_example.json from the reference's example input
 data (docs/examples/gdc/{basis,potentials}): cell, species, charges,
 Buckingham parameters, positions and the total energy the reference printed
-in the basis header (docs/examples/gdc/basis:3).  Run in the build
-container, where the reference tree is mounted; the JSON is the fixture.
+in the basis header (docs/examples/gdc/basis:3).  Also keeps the basis file
+itself, byte for byte, as tests/golden/gdc_basis.txt: it is a
+Basis::dumpBasis output (its "timeStep" header line), the golden file of the
+basis reader and writer.  Run in the build container, where the reference
+tree is mounted; the JSON and the copy are the fixtures.
 """
 import json
 import os
@@ -10,6 +13,7 @@ import sys
 
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/docs/examples/gdc"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gdc_example.json")
+BASIS_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gdc_basis.txt")
 
 
 def main():
@@ -47,6 +51,9 @@ def main():
                pos=[xs, ys, zs], energy=energy)
     json.dump(out, open(OUT, "w"))
     print("wrote", OUT, n, "ions, energy", energy)
+    with open(os.path.join(REF, "basis"), "rb") as src, open(BASIS_OUT, "wb") as dst:
+        dst.write(src.read())
+    print("wrote", BASIS_OUT)
 
 
 if __name__ == "__main__":

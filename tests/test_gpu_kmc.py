@@ -102,7 +102,7 @@ def test_basin2_stage_matches_oracle(lib, oracle, c1, bits):
     assert n_new >= 1
 
 
-def test_kmc_steps_rechecked_on_cpu(lib, oracle):
+def test_kmc_steps_rechecked_on_cpu(lib, oracle, tmp_path):
     """Two systems x 4 walkers, 2 KMC steps of 3 events each: every step
     ends in a new basin whose FP64 energy is the oracle's at the chosen
     event's basin-2 positions; rates and the total rate are the oracle's
@@ -112,7 +112,8 @@ def test_kmc_steps_rechecked_on_cpu(lib, oracle):
     text = lib.sc_potentials_text(_pots(), 2 * CU["a"])
     p = M.KMCParams(kmcSteps=2, kmcImages=3)
     run = M.KineticMonteCarlo(text, cells, walkers_per_system=4, params=p,
-                              search_params=lib.dimer_defaults(maxParIter=400), seed=2)
+                              search_params=lib.dimer_defaults(maxParIter=400), seed=2,
+                              output_dir=str(tmp_path), write_events=True)
     st = run.run(max_micro_steps=40000)
     print(f"KMC: {st.kmc_steps} steps, {st.searches} searches, {st.converged} converged, {st.verified} verified, "
           f"{st.new_basin} new basins, {st.valid_events} valid, {st.out_of_window} out of window, "
@@ -135,3 +136,12 @@ def test_kmc_steps_rechecked_on_cpu(lib, oracle):
         e_final = oracle.sc_energy(of, s.basin, c.types, c.lat)[0]
         assert abs(e_final - s.e_basin64) < 1e-9 * abs(e_final)
         assert len(by_sys[si]) == 2
+        # runKMC's files: the restart holds the final basin (10 decimals), the
+        # archive both steps, events.<n> each step's catalogue
+        d = tmp_path / f"system{si}"
+        back = lib.read_basis(d / "restart", c.species)
+        assert back["iteration"] == 2 and np.abs(back["pos"] - s.basin).max() <= 5e-11
+        assert (d / "archive").read_text().count("timeStep") == 2
+        for rec in by_sys[si]:
+            head = (d / f"events.{rec.iteration}").read_text().split("\n")[0].split()
+            assert head[:2] == ["events", str(rec.events)] and int(head[4]) == rec.chosen
