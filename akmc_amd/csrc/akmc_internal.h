@@ -47,10 +47,18 @@ struct EamTabDev {
     const double2 *emb;    // [nrho][2] F(rho), F'(rho)
 };
 
+// short-range forms of the two-body table (TwoBody::shortrangeone,
+// potentials/pair/TwoBody.h:173-276), parameters in the order of the
+// potentials text: buck A rho C; morse D alpha r0 C12; ljones eps sigma;
+// bhm A B sigma C D; severnsix eps sigma
+enum : int { ION_BUCK = 0, ION_MORSE = 1, ION_LJONES = 2, ION_BHM = 3, ION_SEVNSIX = 4 };
 struct IonPairDev {
     float A, inv_rho, C;
     double Ad, inv_rhod, Cd;
     int on;
+    int form;
+    float pf[6];
+    double pd[6];
 };
 
 struct FieldDev {
@@ -66,6 +74,7 @@ struct FieldDev {
                      // are evaluated in FP64 on the FP32 path (0: none)
     double eta;
     double cfact;
+    int energy_path;       // short-range energies of the energy-only forms (Morse + C12/r^12)
     SCPairDev sc[AKMC_MAXPAIR];
     IonPairDev ion[AKMC_MAXPAIR];
     // tabulated EAM (kind 0 with eam = 1): species pair -> table (-1 none)

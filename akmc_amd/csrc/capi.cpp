@@ -64,9 +64,13 @@ struct SCPot {
     std::string A, B;
     double eps, a, n, m, c;
 };
+// a two-body entry: form (IonForm) and its parameters in the order of the
+// potentials text (TwoBody.cpp:98-245)
 struct BuckPot {
     std::string A, B;
     double Ab, rho, C;
+    int form = 0;
+    double par[6] = {0, 0, 0, 0, 0, 0};
 };
 // a tabulated EAM potential as readLEAM leaves it
 // (potentials/metal/MetalPotential.cpp:276-361)
@@ -277,12 +281,26 @@ int parse_potentials(akmc_field_t f, const char *text)
                     std::string kl, pl;
                     if (!next_line(kl) || !next_line(pl)) return fail(AKMC_EINVAL, "twobody: missing lines");
                     std::string key = lower(split_ws(kl).at(0));
-                    if (key != "buck")
+                    // potenKey (TwoBody.cpp:440-470) and the parameter counts of loadPotential
+                    int form = -1, npar = 0;
+                    if (key == "buck") form = ION_BUCK, npar = 3;
+                    else if (key == "morse") form = ION_MORSE, npar = 4;
+                    else if (key == "ljones") form = ION_LJONES, npar = 2;
+                    else if (key == "bhm") form = ION_BHM, npar = 5;
+                    else if (key == "severnsix") form = ION_SEVNSIX, npar = 2;
+                    if (form < 0)
                         return fail(AKMC_EUNSUPPORTED, "two-body form '%s' is not supported on the device",
                                     key.c_str());
-                    auto pw = split_ws(pl);  // A B A rho C (TwoBody.cpp:98-112)
-                    f->buck.push_back({pw.at(0), pw.at(1), std::stod(pw.at(2)), std::stod(pw.at(3)),
-                                       std::stod(pw.at(4))});
+                    auto pw = split_ws(pl);
+                    BuckPot bp{pw.at(0), pw.at(1), 0.0, 1.0, 0.0};
+                    bp.form = form;
+                    for (int q = 0; q < npar; ++q) bp.par[q] = std::stod(pw.at(2 + q));
+                    if (form == ION_BUCK) {  // A B A rho C (TwoBody.cpp:98-112)
+                        bp.Ab = bp.par[0];
+                        bp.rho = bp.par[1];
+                        bp.C = bp.par[2];
+                    }
+                    f->buck.push_back(bp);
                 }
             } else if (k == "close") {
                 closed = true;
@@ -383,12 +401,17 @@ void build_tables(akmc_field_t f)
                 if (k < 0 || k >= (int)f->buck.size()) continue;
                 const BuckPot &b = f->buck[k];
                 p.on = 1;
+                p.form = b.form;
                 p.Ad = b.Ab;
                 p.inv_rhod = 1.0 / b.rho;
                 p.Cd = b.C;
                 p.A = (float)p.Ad;
                 p.inv_rho = (float)p.inv_rhod;
                 p.C = (float)p.Cd;
+                for (int q = 0; q < 6; ++q) {
+                    p.pd[q] = b.par[q];
+                    p.pf[q] = (float)b.par[q];
+                }
             }
         }
 }
@@ -1155,7 +1178,22 @@ int akmc_field_energy(akmc_field_t f, double *x, double *y, double *z, const dou
 {
     if (!f || n <= 0) return fail(AKMC_EINVAL, "bad arguments");
     std::vector<double> fx(n), fy(n), fz(n);
-    TRY(akmc_field_forces(f, x, y, z, fx.data(), fy.data(), fz.data(), nullptr, lat, charge, type, nullptr, n, e));
+    // the energy-only short-range forms (TwoBody.h:81-150) differ from the
+    // force forms for Morse, which adds C12/r^12
+    bool morse = false;
+    for (const BuckPot &bp : f->buck) morse |= bp.form == ION_MORSE;
+    if (morse && f->kind == 1) {
+        TRY(ensure_single(f, n, lat));
+        f->host.energy_path = 1;
+        HIPCHK(hipMemcpy(f->dev, &f->host, sizeof(FieldDev), hipMemcpyHostToDevice));
+    }
+    const int rc = akmc_field_forces(f, x, y, z, fx.data(), fy.data(), fz.data(), nullptr, lat, charge, type,
+                                     nullptr, n, e);
+    if (morse && f->kind == 1) {
+        f->host.energy_path = 0;
+        HIPCHK(hipMemcpy(f->dev, &f->host, sizeof(FieldDev), hipMemcpyHostToDevice));
+    }
+    TRY(rc);
     // the energy path of a tabulated EAM field doubles the pair term
     // (calculateManyBodyPairEnergy, potentials/metal/ManyBody.cpp:444-450)
     if (e && f->kind == 0 && !f->eam.empty()) {

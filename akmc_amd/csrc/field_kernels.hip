@@ -998,7 +998,7 @@ __device__ inline void ion_pair_terms(const FieldDev &F, const IonPairDev &p, do
         ec = cq * erfcv * ir;
         forc = cq * (-erfcv * ir * ir * ir - tsp * eta * ex * ir * ir);
     }
-    if (p.on) {
+    if (p.on && p.form == ION_BUCK) {
         T A = sizeof(T) == 8 ? T(p.Ad) : T(p.A);
         T irho = sizeof(T) == 8 ? T(p.inv_rhod) : T(p.inv_rho);
         T Cc = sizeof(T) == 8 ? T(p.Cd) : T(p.C);
@@ -1008,6 +1008,44 @@ __device__ inline void ion_pair_terms(const FieldDev &F, const IonPairDev &p, do
         T e2 = -Cc * ir6;
         eb = e1 + e2;
         forc -= e1 * irho * ir + T(6) * e2 * ir * ir;
+    } else if (p.on) {
+        // the other forms with the reference's own force expressions
+        // (TwoBody.h:217-276): -(dV/dr)/r for Morse and BHM, +(dV/dr)/r for
+        // Lennard-Jones, and the 7-6 expression as written
+        T m[6];
+        for (int k = 0; k < 6; ++k) m[k] = sizeof(T) == 8 ? T(p.pd[k]) : T(p.pf[k]);
+        const T ir2 = ir * ir;
+        T fs = T(0);
+        if (p.form == ION_MORSE) {
+            const T e1 = exp(-m[1] * (r - m[2]));
+            eb = m[0] * (T(1) - e1) * (T(1) - e1) - m[0];
+            if (F.energy_path) {
+                T ir12 = ir2 * ir2 * ir2;
+                ir12 *= ir12;
+                eb += m[3] * ir12;
+            }
+            fs = -(T(2) * m[0] * m[1] * (T(1) - e1) * e1) * ir;
+        } else if (p.form == ION_LJONES) {
+            T t6 = m[1] * ir;
+            t6 = t6 * t6 * t6;
+            t6 *= t6;
+            const T e1 = T(4) * m[0] * t6 * t6, e2 = T(-4) * m[0] * t6;
+            eb = e1 + e2;
+            fs = -(T(12) * e1 + T(6) * e2) * ir2;
+        } else if (p.form == ION_BHM) {
+            const T ir6 = ir2 * ir2 * ir2, ir8 = ir6 * ir2;
+            const T ex = exp(m[1] * (m[2] - r));
+            eb = m[0] * ex - m[3] * ir6 - m[4] * ir8;
+            fs = (r * m[0] * m[1] * ex - T(6) * m[3] * ir6 - T(8) * m[4] * ir8) * ir2;
+        } else if (p.form == ION_SEVNSIX) {
+            const T s = m[1] * ir;
+            T s6 = s * s * s;
+            s6 *= s6;
+            eb = m[0] * s6 * s - T(2) * m[0] * s6;
+            // f1 = -7 eps sigma^7 / r^8, f2 = -12 eps sigma^6 / r^7, fs = -(f1 + f2) / r^2
+            fs = (T(7) * m[0] * s6 * s + T(12) * m[0] * s6) * ir * ir2;
+        }
+        forc -= fs;
     }
 }
 

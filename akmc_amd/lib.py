@@ -601,8 +601,14 @@ def buck_potentials_text(species, charges, pots, rc, noimage=False, ewald_precis
     if ewald_precis is not None:
         lines.append(f"ewald precis {ewald_precis!r}")
     lines.append(f"twobody {len(pots)} ev ev")
-    for (A, B, a, rho, c) in pots:
-        lines += ["buck", f"{A} {B} {a!r} {rho!r} {c!r}"]
+    for pot in pots:
+        # (A, B, A_buck, rho, C) or (form, A, B, params...) with form one of
+        # buck, morse, ljones, bhm, severnsix (TwoBody.cpp:98-245)
+        if pot[0] in ("buck", "morse", "ljones", "bhm", "severnsix"):
+            form, rest = pot[0], pot[1:]
+        else:
+            form, rest = "buck", pot
+        lines += [form, f"{rest[0]} {rest[1]} " + " ".join(repr(float(v)) for v in rest[2:])]
     lines.append("close")
     return "\n".join(lines) + "\n"
 

@@ -593,8 +593,12 @@ static inline void hfunc1(double xv, double *h0, double *h1)
     *h1 = (-*h0 - fact * exp(-xsq)) / xsq;
 }
 
-/* Buckingham energy and -(dV/dr)/r (TwoBody.h:173-215); table default 0
- * for unlisted pairs (TwoBody.cpp:509) is the caller's table */
+/* The short-range pair forms and their "force" (-(dV/dr)/r where the
+ * reference's expression is that) as TwoBody::shortrangeone computes them
+ * (TwoBody.h:173-276); with energy_path the energy is that of the energy-
+ * only form (TwoBody.h:81-150), which differs for Morse by + C12/r^12.
+ * The table default 0 for unlisted pairs (TwoBody.cpp:509) is the
+ * caller's table. */
 static inline void buck(const ako_ion_field *f, int ta, int tb, double r, double rsq, double *e,
                         double *frc)
 {
@@ -603,12 +607,44 @@ static inline void buck(const ako_ion_field *f, int ta, int tb, double r, double
     int p = f->table[ta * f->nspec + tb];
     if (p < 0) return;
     const double *m = f->par + 6 * p;
-    double e1 = m[0] * exp(-r / m[1]);
-    double f1 = e1 / (r * m[1]);
-    double e2 = -m[2] / (rsq * rsq * rsq);
-    double f2 = 6 * e2 / rsq;
-    *e = e1 + e2;
-    *frc = f1 + f2;
+    switch (f->form[p]) {
+    case AKO_MORSE: {
+        double e1 = exp(-m[1] * (r - m[2]));
+        *e = m[0] * (1.0 - e1) * (1.0 - e1) - m[0];
+        if (f->energy_path) *e += m[3] / pow(rsq, 6);
+        *frc = -(2.0 * m[0] * m[1] * (1.0 - e1) * e1) / r;
+        break;
+    }
+    case AKO_LJONES: {
+        double t = m[1] / r;
+        double e1 = 4.0 * m[0] * pow(t, 12.0), e2 = -4.0 * m[0] * pow(t, 6.0);
+        *e = e1 + e2;
+        *frc = -(12.0 * e1 / rsq + 6.0 * e2 / rsq);
+        break;
+    }
+    case AKO_BHM: {
+        double r6 = rsq * rsq * rsq, r8 = r6 * rsq, ex = exp(m[1] * (m[2] - r));
+        *e = m[0] * ex - m[3] / r6 - m[4] / r8;
+        *frc = (r * m[0] * m[1] * ex - 6.0 * m[3] / r6 - 8.0 * m[4] / r8) / rsq;
+        break;
+    }
+    case AKO_SEVNSIX: {
+        double t = m[1] / r;
+        double r7 = rsq * rsq * rsq * r, r8 = r7 * r;
+        double f1 = -m[0] * 7.0 * pow(m[1], 7.0) / r8, f2 = -m[0] * 12.0 * pow(m[1], 6.0) / r7;
+        *e = m[0] * pow(t, 7.0) - 2.0 * m[0] * pow(t, 6.0);
+        *frc = -(f1 + f2) / rsq;
+        break;
+    }
+    default: {
+        double e1 = m[0] * exp(-r / m[1]);
+        double f1 = e1 / (r * m[1]);
+        double e2 = -m[2] / (rsq * rsq * rsq);
+        double f2 = 6 * e2 / rsq;
+        *e = e1 + e2;
+        *frc = f1 + f2;
+    }
+    }
 }
 
 /* one real-space pair term of realSpaceForceImage (pair Field.cpp:780-818) */

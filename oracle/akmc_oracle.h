@@ -86,9 +86,16 @@ long long ako_sc_count_pairs(const ako_sc_field *f, int n, const double *x, cons
                              const double *z, const double lat[9], const double rcp[9]);
 
 /* ------------------------------------------------------------------ */
-/* Buckingham + Ewald ionic field (potentials/pair/)                   */
-/* par: 6 doubles per potential: A, rho, C (TwoBody.h:203-215)         */
+/* Ewald ionic field with short-range pair forms (potentials/pair/)    */
+/* par: 6 doubles per potential, in the order the potentials text      */
+/* gives them (TwoBody.cpp:98-245): buck A rho C; morse D alpha r0 C12; */
+/* ljones eps sigma; bhm A B sigma C D; severnsix eps sigma            */
 /* ------------------------------------------------------------------ */
+#define AKO_BUCK 0
+#define AKO_MORSE 1
+#define AKO_LJONES 2
+#define AKO_BHM 3
+#define AKO_SEVNSIX 4
 typedef struct {
     int nspec;
     int npot;
@@ -102,6 +109,8 @@ typedef struct {
     int cells[3];         /* explicit image shells for min_image == 0 */
     int nk;               /* number of k vectors */
     double *gx, *gy, *gz, *g0, *g1;
+    int form[AKO_MAXPOT]; /* AKO_BUCK ... per potential */
+    int energy_path;      /* 1: short-range energies of TwoBody's energy-only form */
 } ako_ion_field;
 
 void ako_ewald_eta_from_acc(double acc, double rc, double *eta, double *eps, double *tol);

@@ -37,7 +37,12 @@ class IonField(C.Structure):
                 ("par", C.c_double * (MAXPOT * 6)),
                 ("rc", C.c_double), ("min_image", C.c_int), ("do_ewald", C.c_int),
                 ("eta", C.c_double), ("cfact", C.c_double), ("cells", C.c_int * 3),
-                ("nk", C.c_int), ("gx", _D), ("gy", _D), ("gz", _D), ("g0", _D), ("g1", _D)]
+                ("nk", C.c_int), ("gx", _D), ("gy", _D), ("gz", _D), ("g0", _D), ("g1", _D),
+                ("form", C.c_int * MAXPOT), ("energy_path", C.c_int)]
+
+# short-range pair forms of the potentials text (TwoBody.cpp:440-470) and
+# their parameter counts
+PAIR_FORMS = {"buck": (0, 3), "morse": (1, 4), "ljones": (2, 2), "bhm": (3, 5), "severnsix": (4, 2)}
 
 
 class FieldCtx(C.Structure):
@@ -229,21 +234,28 @@ class Ewald:
 
 def ion_field(species, pots, rc, ewald: Ewald | None, min_image=True, lat=None, cfact=CTOEV,
               table_default=0):
-    """pots: list of (A, B, Abuck, rho, C).  table_default=0 reproduces the
-    reference's pair-table initialisation (TwoBody.cpp:509)."""
+    """pots: list of (A, B, Abuck, rho, C) Buckingham entries, or of
+    (form, A, B, params...) with form a key of PAIR_FORMS.  table_default=0
+    reproduces the reference's pair-table initialisation (TwoBody.cpp:509)."""
     f = IonField()
     f.nspec = len(species)
     f.npot = len(pots)
     for i in range(MAXSPEC * MAXSPEC):
         f.table[i] = table_default
-    for k, (A, B, a, rho, c) in enumerate(pots):
+    for k, pot in enumerate(pots):
+        if isinstance(pot[0], str) and pot[0] in PAIR_FORMS:
+            form, npar = PAIR_FORMS[pot[0]]
+            A, B, par = pot[1], pot[2], pot[3:3 + npar]
+        else:
+            form, (A, B, *par) = 0, pot
         for i, si in enumerate(species):
             for j, sj in enumerate(species):
                 if si == A and sj == B:
                     f.table[j * f.nspec + i] = k
                     f.table[i * f.nspec + j] = k
-        for t, v in enumerate((a, rho, c)):
+        for t, v in enumerate(par):
             f.par[6 * k + t] = v
+        f.form[k] = form
     f.rc = rc
     f.min_image = 1 if min_image else 0
     f.cfact = cfact

@@ -216,3 +216,38 @@ def test_sc_non_integer_exponents(lib, oracle, fp64):
     got = f.calculate_forces(c.pos, c.types, c.lat)
     tol = (1e-9, 1e-11) if fp64 else (F_TOL, E_TOL_ATOM)
     _compare(f"SC n=8.7 m=5.6 fp{64 if fp64 else 32}", ref, got, c.n, *tol)
+
+
+_FORMS = {
+    "morse_bhm_lj": [("morse", "M", "X", 0.6, 1.6, 2.1, 4.0), ("bhm", "X", "X", 1200.0, 3.0, 0.9, 20.0, 8.0),
+                     ("ljones", "M", "M", 0.01, 2.5)],
+    "sevnsix": [("severnsix", "M", "X", 0.3, 2.2), ("buck", "X", "X", 22764.0, 0.149, 27.88),
+                ("severnsix", "M", "M", 0.05, 3.0)],
+}
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("forms", sorted(_FORMS))
+def test_ionic_other_pair_forms(lib, oracle, forms, fp64):
+    """Morse, Born-Huggins-Mayer, Lennard-Jones and 7-6 short-range forms
+    with Ewald, as TwoBody::shortrangeone computes them, against the
+    oracle; the energy-only entry point adds Morse's C12/r^12 term."""
+    c = W.rocksalt_cell(seed=5)
+    pots = _FORMS[forms]
+    f = lib.Field(lib.buck_potentials_text(["M", "X"], [2.0, -2.0], pots, 10.0, noimage=True))
+    f.set_ewald(0.35, 8)
+    if fp64:
+        f.set_precision(64)
+    ew = oracle.Ewald(c.lat, 0.35, gcell=8)
+    of = oracle.ion_field(["M", "X"], pots, rc=10.0, ewald=ew, min_image=False, lat=c.lat)
+    ref = oracle.ion_forces(of, c.pos, c.types, c.charges, c.lat)
+    got = f.calculate_forces(c.pos, c.types, c.lat, c.charges)
+    tol = (1e-8, 1e-10) if fp64 else (F_TOL, E_TOL_ATOM)
+    _compare(f"rock-salt {forms} fp{64 if fp64 else 32}", ref, got, c.n, *tol)
+    assert abs(got["energy"]["vdw"] - ref["e_vdw"]) / c.n < tol[1]
+    of.energy_path = 1
+    ref_e = oracle.ion_forces(of, c.pos, c.types, c.charges, c.lat)
+    e = f.calculate_energy(c.pos, c.types, c.lat, c.charges)
+    print(f"  energy path: vdw {e['vdw']:.6f} (oracle {ref_e['e_vdw']:.6f}; force path {ref['e_vdw']:.6f})",
+          flush=True)
+    assert abs(e["total"] - ref_e["energy"]) / c.n < tol[1]
