@@ -309,3 +309,33 @@ def test_ionic_art_batch_rocksalt(lib, oracle):
         assert abs(b_dev - b64) < 0.01
         assert fmax < 0.05 and b64 > 0
     assert ok >= nw // 2
+
+
+@pytest.mark.parametrize("style,width", [(5, 3.0), (5, None)])
+def test_region_proposer_campaign(lib, oracle, style, width):
+    """Batched dimer searches started from the reference's coordination-
+    defect active region (style 5), with the Gaussian and the plain start:
+    every centre sits on a vacancy's shell, and nearly every converged
+    saddle passes the device FP64 check."""
+    import math
+    from akmc_amd import regions as RG
+    from akmc_amd import search as S
+    cells = W.batched_sc_cells(n_walkers=512, seed=3)
+    cells = [cells[i] for i in np.argsort([c.n for c in cells])[:16]]
+    nn = CU["a"] / math.sqrt(2) + 0.1
+    prop = RG.RegionProposer(style=style, region_radius=6.0, d0=0.1, gauss_width=width, species=["Cu"],
+                             art_types=["Cu"], coord_distance=[nn], ideal_coord=[12], seed=7)
+    text = lib.sc_potentials_text(_pots(), 2 * CU["a"])
+    camp = S.SaddleCampaign(text, cells, params=lib.dimer_defaults(maxParIter=400), seed=5, proposer=prop)
+    camp.relax_basins()
+    camp.begin()
+    st = camp.advance(600, sync_every=25)
+    print(f"style {style}: {st.searches} ended, {st.converged} converged, {st.verified} verified, "
+          f"failures {st.failures()}", flush=True)
+    if style == 5:
+        for w, site in camp.cur_site.items():
+            c = cells[w]
+            near_vac = min(np.sqrt(((W.min_image(site - v, c.lat)) ** 2).sum()) for v in c.vacancies)
+            assert near_vac < nn + 0.2
+    assert st.converged >= 4
+    assert st.verified >= 0.9 * st.converged
