@@ -249,6 +249,15 @@ struct SearchDev {
 };
 
 void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st);
+// one walker start packed for akmc_launch_load_walkers: doubles at dbl
+// (mode [3][n], then positions [3][n] when has_x), ints at ints (frozen [n],
+// then the free-atom list [nact])
+struct LoadItem {
+    int walker, has_x, nact, pad;
+    long long dbl, ints;
+};
+void akmc_launch_load_walkers(const BatchDev &b, const SearchDev &s, const LoadItem *items, int nitems,
+                              const double *D, const int *I, hipStream_t st);
 void akmc_launch_probe_init(const BatchDev &b, const SearchDev &s, hipStream_t st);
 
 // learned proposal (mlp_kernels.hip)

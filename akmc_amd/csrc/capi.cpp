@@ -150,6 +150,11 @@ struct akmc_batch_s {
     int *ov_pin = nullptr;
     double *pos_pin = nullptr;  // pinned staging of walker positions
     size_t pos_pin_n = 0;
+    int inflight = 0;           // a chunk launched by akmc_batch_run_async awaits its wait
+    int inflight_steps = 0;
+    void *stage_pin = nullptr;  // pinned host / device staging of walker starts
+    void *stage_dev = nullptr;
+    size_t stage_n = 0;
 };
 
 namespace {
@@ -746,6 +751,8 @@ void batch_free(akmc_batch_t b)
     if (b->hws_pin) (void)hipHostFree(b->hws_pin);
     if (b->ov_pin) (void)hipHostFree(b->ov_pin);
     if (b->pos_pin) (void)hipHostFree(b->pos_pin);
+    if (b->stage_pin) (void)hipHostFree(b->stage_pin);
+    if (b->stage_dev) (void)hipFree(b->stage_dev);
     delete b;
 }
 
@@ -1001,29 +1008,53 @@ int ensure_lanczos(akmc_batch_t b)
     return AKMC_SUCCESS;
 }
 
+// grow a pinned host / device staging pair to n bytes
+int ensure_stage(akmc_batch_t b, size_t n)
+{
+    if (n <= b->stage_n) return AKMC_SUCCESS;
+    if (b->stage_pin) HIPCHK(hipHostFree(b->stage_pin));
+    if (b->stage_dev) HIPCHK(hipFree(b->stage_dev));
+    b->stage_pin = nullptr;
+    b->stage_dev = nullptr;
+    b->stage_n = 0;
+    HIPCHK(hipHostMalloc((void **)&b->stage_pin, n, hipHostMallocDefault));
+    HIPCHK(hipMalloc((void **)&b->stage_dev, n));
+    b->stage_n = n;
+    return AKMC_SUCCESS;
+}
+
 // per-walker start data of a saddle search: positions (optional), a
-// normalised mode with frozen components removed, zero velocity
+// normalised mode with frozen components removed, zero velocity, the frozen
+// mask and the free-atom list.  The selected walkers' data are packed into
+// one pinned buffer, copied in one transfer and scattered by one kernel.
 int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const double *mode, const int *frozen)
 {
-    size_t A = b->A;
-    // normalised modes of every selected walker, staged together and kept
-    // until the one stream sync at the end
-    size_t total = 0;
-    for (int w = 0; w < b->W; ++w)
-        if (!walkers || walkers[w]) total += 3 * (size_t)b->hw[w].n;
-    std::vector<double> stage(total);
-    size_t at = 0;
+    const size_t A = b->A;
+    std::vector<LoadItem> items;
+    size_t dwords = 0, iwords = 0;
     for (int w = 0; w < b->W; ++w) {
         if (walkers && !walkers[w]) continue;
         const int off = b->hw[w].off, n = b->hw[w].n;
-        if (frozen) {
-            std::copy(frozen + off, frozen + off + n, b->hfrozen.begin() + off);
-            HIPCHK(hipMemcpyAsync((void *)(b->s.frozen + off), frozen + off, n * sizeof(int), hipMemcpyHostToDevice,
-                                  b->stream));
-        }
-        // zero frozen components and normalise the walker's mode
-        double *nw = stage.data() + at;
+        if (frozen) std::copy(frozen + off, frozen + off + n, b->hfrozen.begin() + off);
+        LoadItem it{};
+        it.walker = w;
+        it.dbl = (long long)dwords;
+        it.ints = (long long)iwords;
+        dwords += (x0 ? 6 : 3) * (size_t)n;
+        iwords += 2 * (size_t)n;
+        items.push_back(it);
+    }
+    if (items.empty()) return AKMC_SUCCESS;
+    const size_t ibytes = items.size() * sizeof(LoadItem);
+    const size_t dbytes = dwords * sizeof(double), nbytes = iwords * sizeof(int);
+    TRY(ensure_stage(b, ibytes + dbytes + nbytes + 64));
+    char *base = (char *)b->stage_pin;
+    double *D = (double *)(base + ibytes);
+    int *I = (int *)(base + ibytes + dbytes);
+    for (LoadItem &it : items) {
+        const int w = it.walker, off = b->hw[w].off, n = b->hw[w].n;
         const int *fr = b->hfrozen.data() + off;
+        double *nw = D + it.dbl;
         double nn = 0.0;
         for (int c = 0; c < 3; ++c)
             for (int i = 0; i < n; ++i) {
@@ -1031,24 +1062,33 @@ int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const dou
                 nw[(size_t)c * n + i] = v;
                 nn += v * v;
             }
-        if (nn <= 0.0) {
-            (void)hipStreamSynchronize(b->stream);
-            return fail(AKMC_EINVAL, "walker %d: zero start mode", w);
-        }
+        if (nn <= 0.0) return fail(AKMC_EINVAL, "walker %d: zero start mode", w);
         nn = 1.0 / std::sqrt(nn);
         for (size_t k = 0; k < 3 * (size_t)n; ++k) nw[k] *= nn;
-        for (int c = 0; c < 3; ++c) {
-            size_t o = c * A + off;
-            HIPCHK(hipMemcpyAsync(b->s.N + o, nw + (size_t)c * n, n * sizeof(double), hipMemcpyHostToDevice,
-                                  b->stream));
-            HIPCHK(hipMemsetAsync(b->s.V + o, 0, n * sizeof(double), b->stream));
-            if (x0)
-                HIPCHK(hipMemcpyAsync(b->s.X + o, x0 + o, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+        it.has_x = x0 ? 1 : 0;
+        if (x0)
+            for (int c = 0; c < 3; ++c) std::memcpy(nw + (size_t)(3 + c) * n, x0 + c * A + off, n * sizeof(double));
+        // frozen mask and free-atom list (global ids)
+        int *iw = I + it.ints;
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            iw[i] = fr[i];
+            if (!fr[i]) {
+                iw[n + m++] = off + i;
+                b->hact[off + m - 1] = off + i;
+            }
         }
-        at += 3 * (size_t)n;
+        it.nact = m;
+        b->hnact[w] = m;
     }
+    std::memcpy(base, items.data(), ibytes);
+    HIPCHK(hipMemcpyAsync(b->stage_dev, b->stage_pin, ibytes + dbytes + nbytes, hipMemcpyHostToDevice, b->stream));
+    char *dbase = (char *)b->stage_dev;
+    akmc_launch_load_walkers(b->d, b->s, (const LoadItem *)dbase, (int)items.size(), (const double *)(dbase + ibytes),
+                             (const int *)(dbase + ibytes + dbytes), b->stream);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(b->stream));
-    return upload_active(b, walkers);
+    return AKMC_SUCCESS;
 }
 
 int copy_walkers_out(akmc_batch_t b, const double *src, int nw, const int *walkers, double *out);
@@ -1462,35 +1502,50 @@ int akmc_batch_start_art(akmc_batch_t b, const int *walkers, const double *x0, c
     return start_walkers(b, walkers, WK_ART, p->initTimeStep, p->alphaStart);
 }
 
-int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_done)
+int akmc_batch_run_async(akmc_batch_t b, int steps)
 {
-    if (!b || !b->search_alloc || max_steps < 0 || sync_every < 1) return fail(AKMC_EINVAL, "bad arguments");
-    int done = 0, running = 0;
+    if (!b || !b->search_alloc || steps < 1) return fail(AKMC_EINVAL, "bad arguments");
+    if (b->inflight) return fail(AKMC_EINVAL, "a run chunk is already in flight");
     b->s.default_mode = batch_mode(b);  // the mode the state kernels give walkers they move on
     TRY(fetch_states(b));
+    int running = 0;
     TRY(refresh_active(b, &running));
-    while (running > 0 && done < max_steps) {
-        int k = std::min(sync_every, max_steps - done);
-        // k micro-steps as one graph launch; the state kernels keep the
-        // per-walker active flags current (0 when a walker ends, FP64 while
-        // it verifies), so the host only reads states and the overflow flag
-        if (b->ktimer && k > akmc_batch_s::kTimerSteps)
-            return fail(AKMC_EINVAL, "sync_every is limited to %d while the kernel timer runs", akmc_batch_s::kTimerSteps);
-        // HIP events recorded inside a replayed graph cannot be timed, so the
-        // kernel timer launches the micro-steps one by one
-        if (b->use_graphs && !b->ktimer) {
-            if (!b->gexec || b->graph_steps != k || b->graph_prec != batch_mode(b) ||
-                std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0 ||
-                std::memcmp(&b->graph_d, &b->d, sizeof(BatchDev)) != 0)
-                TRY(capture_graph(b, k));
-            HIPCHK(hipGraphLaunch(b->gexec, b->stream));
-        } else {
-            for (int j = 0; j < k; ++j) enqueue_micro_step(b, j);
-            HIPCHK(hipGetLastError());
-        }
-        done += k;
-        HIPCHK(hipMemcpyAsync(b->hws_pin, b->s.ws, b->W * sizeof(WalkerState), hipMemcpyDeviceToHost, b->stream));
-        HIPCHK(hipMemcpyAsync(b->ov_pin, b->d.overflow, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    b->inflight_steps = 0;
+    if (running == 0) {
+        b->inflight = 1;
+        return AKMC_SUCCESS;
+    }
+    // k micro-steps as one graph launch; the state kernels keep the
+    // per-walker active flags current (0 when a walker ends, FP64 while
+    // it verifies), so the host only reads states and the overflow flag
+    if (b->ktimer && steps > akmc_batch_s::kTimerSteps)
+        return fail(AKMC_EINVAL, "sync_every is limited to %d while the kernel timer runs", akmc_batch_s::kTimerSteps);
+    // HIP events recorded inside a replayed graph cannot be timed, so the
+    // kernel timer launches the micro-steps one by one
+    if (b->use_graphs && !b->ktimer) {
+        if (!b->gexec || b->graph_steps != steps || b->graph_prec != batch_mode(b) ||
+            std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0 ||
+            std::memcmp(&b->graph_d, &b->d, sizeof(BatchDev)) != 0)
+            TRY(capture_graph(b, steps));
+        HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    } else {
+        for (int j = 0; j < steps; ++j) enqueue_micro_step(b, j);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(b->hws_pin, b->s.ws, b->W * sizeof(WalkerState), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipMemcpyAsync(b->ov_pin, b->d.overflow, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    b->inflight = 1;
+    b->inflight_steps = steps;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_run_wait(akmc_batch_t b, int *steps_done, int *running)
+{
+    if (!b || !b->inflight) return fail(AKMC_EINVAL, "no run chunk in flight");
+    b->inflight = 0;
+    const int k = b->inflight_steps;
+    int run = 0;
+    if (k > 0) {
         HIPCHK(hipStreamSynchronize(b->stream));
         if (*b->ov_pin) return overflow_error(b, *b->ov_pin);
         if (b->ktimer && batch_mode(b) == 1 && b->f->host.kind == 0)
@@ -1501,9 +1556,23 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
                 b->klaunches += 1;
             }
         std::memcpy(b->hws.data(), b->hws_pin, b->W * sizeof(WalkerState));
-        running = 0;
-        for (int w = 0; w < b->W; ++w)
-            running += b->hws[w].kind != WK_IDLE && b->hws[w].state != ST_DONE;
+        for (int w = 0; w < b->W; ++w) run += b->hws[w].kind != WK_IDLE && b->hws[w].state != ST_DONE;
+    }
+    if (steps_done) *steps_done = k;
+    if (running) *running = run;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_done)
+{
+    if (!b || !b->search_alloc || max_steps < 0 || sync_every < 1) return fail(AKMC_EINVAL, "bad arguments");
+    int done = 0, running = 1;
+    while (running > 0 && done < max_steps) {
+        int k = std::min(sync_every, max_steps - done), got = 0;
+        TRY(akmc_batch_run_async(b, k));
+        TRY(akmc_batch_run_wait(b, &got, &running));
+        if (got == 0) break;
+        done += got;
     }
     if (steps_done) *steps_done = done;
     return AKMC_SUCCESS;

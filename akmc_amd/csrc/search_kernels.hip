@@ -1082,7 +1082,39 @@ __global__ void __launch_bounds__(kBlock) k_probe_init(BatchDev b, SearchDev s)
     FOR_ATOMS(i) C_.probe(i, V3(s.X, 0, i), V3(s.X, 1, i), V3(s.X, 2, i));
 }
 
+// scatter of packed walker starts (one workgroup per walker): mode N,
+// zero velocity, positions X when given, the frozen mask, the free-atom
+// list and its length
+__global__ void __launch_bounds__(kBlock) k_load_walkers(BatchDev b, SearchDev s, const LoadItem *items,
+                                                          const double *D, const int *I)
+{
+    const LoadItem it = items[blockIdx.x];
+    const WalkerDev W = b.walkers[it.walker];
+    const int n = W.n, off = W.off;
+    const size_t A = b.A;
+    const double *d = D + it.dbl;
+    const int *iw = I + it.ints;
+    int *frozen = const_cast<int *>(s.frozen);
+    int *act = const_cast<int *>(s.act);
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        for (int c = 0; c < 3; ++c) {
+            s.N[c * A + off + i] = d[(size_t)c * n + i];
+            s.V[c * A + off + i] = 0.0;
+            if (it.has_x) s.X[c * A + off + i] = d[(size_t)(3 + c) * n + i];
+        }
+        frozen[off + i] = iw[i];
+    }
+    for (int k = threadIdx.x; k < it.nact; k += kBlock) act[off + k] = iw[n + k];
+    if (threadIdx.x == 0) const_cast<int *>(s.nact)[it.walker] = it.nact;
+}
+
 }  // namespace
+
+void akmc_launch_load_walkers(const BatchDev &b, const SearchDev &s, const LoadItem *items, int nitems,
+                              const double *D, const int *I, hipStream_t st)
+{
+    if (nitems > 0) k_load_walkers<<<nitems, kBlock, 0, st>>>(b, s, items, D, I);
+}
 
 void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st)
 {

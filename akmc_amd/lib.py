@@ -164,6 +164,8 @@ def load() -> C.CDLL:
     L.akmc_batch_start_relax.argtypes = [C.c_void_p, _I, C.POINTER(RelaxParams)]
     L.akmc_batch_start_dimer.argtypes = [C.c_void_p, _I, _D, _D, _I, C.POINTER(DimerParams)]
     L.akmc_batch_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _I]
+    L.akmc_batch_run_async.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_run_wait.argtypes = [C.c_void_p, _I, _I]
     L.akmc_batch_status.argtypes = [C.c_void_p, C.POINTER(WalkerStatus)]
     L.akmc_batch_get_positions.argtypes = [C.c_void_p, _D, _D, _D]
     L.akmc_batch_get_mode.argtypes = [C.c_void_p, _D]
@@ -459,6 +461,16 @@ class Batch:
         n = C.c_int()
         check(load().akmc_batch_run(self._h, int(max_steps), int(sync_every), C.byref(n)))
         return n.value
+
+    def run_async(self, steps):
+        """enqueue one chunk of micro-steps and return at once"""
+        check(load().akmc_batch_run_async(self._h, int(steps)))
+
+    def run_wait(self):
+        """wait for the chunk of run_async: (steps run, walkers still running)"""
+        n, r = C.c_int(), C.c_int()
+        check(load().akmc_batch_run_wait(self._h, C.byref(n), C.byref(r)))
+        return n.value, r.value
 
     def status(self):
         st = (WalkerStatus * self.W)()

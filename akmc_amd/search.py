@@ -66,6 +66,14 @@ class CampaignStats:
     device_time: float = 0.0   # inside akmc_batch_run (device work + its host checks)
     harvest_time: float = 0.0  # reading ended searches
     refill_time: float = 0.0   # proposals and restarts
+    # the parts of a chunk: launch, host work overlapped with the device
+    # (record processing, proposal draws), waiting, reading ended searches,
+    # restarting walkers
+    t_launch: float = 0.0
+    t_overlap: float = 0.0
+    t_wait: float = 0.0
+    t_collect: float = 0.0
+    t_start: float = 0.0
     max_barrier_err: float = 0.0
     records: list = field(default_factory=list)
     # converged searches that failed one verification test (a search can fail several)
@@ -87,6 +95,8 @@ class CampaignStats:
             setattr(self, k, getattr(self, k) + getattr(o, k))
         self.search_time += o.search_time
         self.device_time += o.device_time
+        for k in ("t_launch", "t_overlap", "t_wait", "t_collect", "t_start"):
+            setattr(self, k, getattr(self, k) + getattr(o, k))
         self.harvest_time += o.harvest_time
         self.refill_time += o.refill_time
         self.max_barrier_err = max(self.max_barrier_err, o.max_barrier_err)
@@ -117,6 +127,7 @@ class SaddleCampaign:
         self.proposer = proposer  # optional callable(campaign, w) -> (x0, mode, frozen, site)
         self.found = [[] for _ in range(self.W)]
         self._regions = {}
+        self._next = {}  # walker -> its next proposal, drawn ahead
         self.pending = set()
         self.cur_site = {}
         A = self.batch.A
@@ -134,6 +145,7 @@ class SaddleCampaign:
         self.basin_ok = np.array([s["status"] == 0 for s in st])
         self.basins = self.batch.positions()
         self._regions = {}
+        self._next = {}
         self.batch.save_basin()
         self.basin32 = self.batch.search_energies(32)
         self.basin64 = self.batch.search_energies(64)
@@ -167,6 +179,16 @@ class SaddleCampaign:
                                           int(self.rng.integers(0, 2**31 - 1)))
         return basin + disp, mode, frozen, c.vacancies[v]
 
+    def precompute(self, walkers):
+        """draw the next proposal of each listed walker now (random proposals
+        do not depend on how the current search ends), so a refill between
+        device chunks only uploads arrays"""
+        if self.proposer is not None:
+            return
+        for w in walkers:
+            if w not in self._next and self.basin_ok[w]:
+                self._next[w] = self.proposal(w)
+
     def start(self, walkers):
         walkers = [int(w) for w in walkers if self.basin_ok[w]]
         if not walkers:
@@ -174,7 +196,8 @@ class SaddleCampaign:
         A = self.batch.A
         mask = np.zeros(self.W, dtype=np.int32)
         for w in walkers:
-            x0, mode, frozen, site = (self.proposer or SaddleCampaign.proposal)(self, w)
+            nxt = self._next.pop(w, None) if self.proposer is None else None
+            x0, mode, frozen, site = nxt if nxt is not None else (self.proposer or SaddleCampaign.proposal)(self, w)
             self.cur_site[w] = site
             a, b = self.off[w], self.off[w + 1]
             for c in range(3):
@@ -188,10 +211,20 @@ class SaddleCampaign:
     def harvest(self, stats: CampaignStats):
         """Collect ended searches (their FP64 verification already ran on
         the device); returns the walkers that ended."""
+        done, raw = self.collect(stats)
+        self.process(raw, stats)
+        return done
+
+    def collect(self, stats: CampaignStats):
+        """The device part of a harvest: statuses and saddle positions of
+        the ended searches, with the context their records need (site, free
+        atoms), taken before the walkers are refilled.  Returns (ended
+        walkers, raw records for process())."""
         st = self.batch.status_array()
         done = [w for w in sorted(self.pending) if st["done"][w]]
         good = [w for w in done if st["status"][w] == 0 and st["verified"][w]]
         positions = self.batch.walkers_positions(good) if good else {}
+        raw = []
         for w in done:
             s = st[w]
             self.pending.discard(w)
@@ -203,18 +236,25 @@ class SaddleCampaign:
             if not s["verified"]:
                 stats.fail_unchecked += 1
                 continue
+            act = np.nonzero(self._frozen[self.off[w]:self.off[w + 1]] == 0)[0]
+            raw.append((w, s, positions[w], self.cur_site.get(w), act))
+        return done, raw
+
+    def process(self, raw, stats: CampaignStats):
+        """The host part of a harvest: FP64 tests, records, de-duplication
+        (needs nothing from the device, so it can run while the next chunk
+        does)."""
+        for (w, s, pos, site, act) in raw:
             b32 = s["energy32"] - self.basin32[w]
             b64 = s["energy64"] - self.basin64[w]
             err = abs(b32 - b64)
             stats.max_barrier_err = max(stats.max_barrier_err, err)
-            pos = positions[w]
             d = W.min_image(pos - self.basins[w], self.cells[w].lat)
             dn = np.sqrt((d * d).sum(0))
             atoms = np.nonzero(dn > 0.3)[0]
-            act = np.nonzero(self._frozen[self.off[w]:self.off[w + 1]] == 0)[0]
             rec = SaddleRecord(w, s["energy32"], s["energy64"], float(b32), float(b64), s["curvature32"],
                                (s["lambda1"], s["lambda2"]), s["fmax64"], atoms, d[:, atoms],
-                               site=self.cur_site.get(w), active=act, active_disp=d[:, act])
+                               site=site, active=act, active_disp=d[:, act])
             tests = dict(fmax=s["fmax64"] < self.fmax_verify, barrier=err <= self.barrier_tol,
                          lambda1=s["lambda1"] < 0.0, lambda2=s["lambda2"] > 0.0, b64=b64 > 0.0)
             for k, ok in tests.items():
@@ -228,7 +268,6 @@ class SaddleCampaign:
                     self.found[w].append(rec)
                     stats.distinct += 1
             stats.records.append(rec)
-        return done
 
     @staticmethod
     def _same(a: SaddleRecord, b: SaddleRecord):
@@ -245,22 +284,45 @@ class SaddleCampaign:
     def advance(self, micro_steps, sync_every=25, refill=True):
         """Run micro_steps micro-steps (a multiple of sync_every), refilling
         walkers whose search ended; returns the searches that ended in this
-        window.  Searches still running carry over to the next window."""
+        window.  Searches still running carry over to the next window.
+
+        Each chunk of sync_every micro-steps runs on the device while the
+        host processes the previous chunk's records and draws the next
+        proposals; between chunks the host only reads the ended searches
+        and uploads the drawn proposals."""
         stats = CampaignStats()
         t0 = time.perf_counter()
         steps = 0
+        raw = []
         while steps < micro_steps and self.pending:
             k = min(sync_every, micro_steps - steps)
             t1 = time.perf_counter()
-            steps += self.batch.run(k, sync_every=k)
+            self.batch.run_async(k)
+            ta = time.perf_counter()
+            self.process(raw, stats)
+            tb = time.perf_counter()
+            if refill:
+                self.precompute(sorted(self.pending))
+            tc = time.perf_counter()
+            got, _ = self.batch.run_wait()
+            steps += got
             t2 = time.perf_counter()
-            done = self.harvest(stats)
+            done, raw = self.collect(stats)
             t3 = time.perf_counter()
             if refill and done:
                 self.start(done)
-            stats.device_time += t2 - t1
-            stats.harvest_time += t3 - t2
-            stats.refill_time += time.perf_counter() - t3
+            t4 = time.perf_counter()
+            stats.device_time += (t2 - tc) + (ta - t1)
+            stats.harvest_time += (t3 - t2) + (tb - ta)
+            stats.refill_time += (t4 - t3) + (tc - tb)
+            stats.t_launch += ta - t1
+            stats.t_overlap += tc - ta
+            stats.t_wait += t2 - tc
+            stats.t_collect += t3 - t2
+            stats.t_start += t4 - t3
+            if got == 0:
+                break
+        self.process(raw, stats)
         stats.search_time = time.perf_counter() - t0
         stats.micro_steps = steps
         return stats

@@ -345,7 +345,9 @@ def main():
               f"{st.searches} ended, {st.converged} converged, {st.verified} verified, "
               f"{st.distinct} distinct, {st.micro_steps} micro-steps in {st.search_time:.2f}s, "
               f"max barrier err {st.max_barrier_err:.2e}; failed tests {st.failures()}; device {st.device_time:.2f}s "
-              f"harvest {st.harvest_time:.2f}s refill {st.refill_time:.2f}s", file=sys.stderr, flush=True)
+              f"harvest {st.harvest_time:.2f}s refill {st.refill_time:.2f}s; per chunk part: launch {st.t_launch:.3f} "
+              f"overlapped host {st.t_overlap:.3f} wait {st.t_wait:.3f} collect {st.t_collect:.3f} "
+              f"start {st.t_start:.3f} s", file=sys.stderr, flush=True)
         if r >= args.warmup:
             t_timed += st.search_time
             tot.add(st)

@@ -227,6 +227,14 @@ int akmc_batch_start_art(akmc_batch_t b, const int *walkers, const double *x0, c
  * host looks at the walker states every sync_every steps (1 = the plain,
  * host-driven loop); steps of sync_every > 1 replay a captured hipGraph */
 int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_done);
+/* the two halves of one akmc_batch_run chunk, so the host can work while the
+ * device runs it (no counterpart in the reference): run_async enqueues
+ * steps micro-steps and the status copy and returns; run_wait waits for
+ * them and reports the steps run (0 when no walker was running) and the
+ * walkers still running.  Between the two, no other call may be made on
+ * the batch. */
+int akmc_batch_run_async(akmc_batch_t b, int steps);
+int akmc_batch_run_wait(akmc_batch_t b, int *steps_done, int *running);
 /* enable (default) or disable the hipGraph replay of akmc_batch_run; when
  * disabled the same kernels are launched one by one on the batch's stream
  * (for tools that cannot trace graph launches).  Results are identical. */
