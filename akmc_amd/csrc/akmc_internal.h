@@ -65,7 +65,7 @@ struct FieldDev {
     int kind;        // 0 many-body SC, 1 ionic
     int nspec;
     int min_image;
-    int do_ewald;
+    int do_ewald;    // 0 none, 1 Ewald, 2 damped-shifted Coulomb
     double rc;
     double rc2;      // FP64 cutoff decision
     double rlist2;   // (rc + skin)^2
@@ -75,6 +75,11 @@ struct FieldDev {
     double eta;
     double cfact;
     int energy_path;       // short-range energies of the energy-only forms (Morse + C12/r^12)
+    // do_ewald == 2: damped-shifted Coulomb on an erfc mesh (fieldErfcGen,
+    // potentials/pair/Field.cpp:19-73): points, spacing, shift terms
+    const double *dsf_e, *dsf_f;
+    int dsf_n;
+    double dsf_del, dsf_rfld0, dsf_rfld1;
     SCPairDev sc[AKMC_MAXPAIR];
     IonPairDev ion[AKMC_MAXPAIR];
     // tabulated EAM (kind 0 with eam = 1): species pair -> table (-1 none)

@@ -100,6 +100,8 @@ struct akmc_field_s {
     std::vector<BuckPot> buck;
     std::vector<EamPot> eam;
     double2 *eam_dev = nullptr;  // device copy of every EAM table (one allocation)
+    double dsf_alpha = 0.0;      // ewald damp: the damping of the shifted Coulomb sum
+    double *dsf_dev = nullptr;   // its erfc mesh on the device (energy, force terms)
     FieldDev host{};
     FieldDev *dev = nullptr;
     int table[AKMC_MAXPAIR];
@@ -234,6 +236,10 @@ int parse_potentials(akmc_field_t f, const char *text)
                 if (sub == "precis") {
                     f->do_ewald = 1;
                     f->madelung_acc = std::stod(w.at(2));
+                } else if (sub == "damp") {
+                    // damped-shifted Coulomb with damping alpha (pair Field.cpp:265-270)
+                    f->do_ewald = 2;
+                    f->dsf_alpha = std::stod(w.at(2));
                 } else if (sub == "none") {
                     f->do_ewald = 0;
                 } else {
@@ -554,6 +560,44 @@ int ensure_eam_tables(akmc_field_t f)
     return AKMC_SUCCESS;
 }
 
+// the erfc mesh of the damped-shifted Coulomb sum, Field::fieldErfcGen
+// (potentials/pair/Field.cpp:19-73): erfc(alpha r)/r by Abramowitz and
+// Stegun 7.1.26 and (that + 2 alpha/sqrt(pi) exp(-alpha^2 r^2)) / r^2 at
+// r = i rc / (n - 4); the shift terms are the values at rc.  The
+// reference's point count (MAXMESH) lives in its missing Constants.h; the
+// count here is this library's choice.
+constexpr int kDsfMesh = 5000;
+
+int ensure_dsf_mesh(akmc_field_t f)
+{
+    if (f->kind != 1 || f->do_ewald != 2) return AKMC_SUCCESS;
+    const int n = kDsfMesh;
+    const double rc = f->rc, a = f->dsf_alpha, del = rc / (n - 4);
+    std::vector<double> h(2 * (size_t)n, 0.0);
+    const double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741, a4 = -1.453152027, a5 = 1.061405429,
+                 pp = 0.3275911;
+    for (int i = 1; i < n; ++i) {
+        const double r = (double)i * del, rsq = r * r;
+        const double tt = 1.0 / (1.0 + pp * a * r);
+        const double ex = std::exp(-std::pow(a * r, 2.0));
+        h[i] = tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * ex / r;
+        h[n + i] = (h[i] + 2.0 * (a / 1.7724538509055159) * ex) / rsq;
+    }
+    if (!f->dsf_dev) {
+        HIPCHK(hipMalloc((void **)&f->dsf_dev, 2 * (size_t)n * sizeof(double)));
+        HIPCHK(hipMemcpy(f->dsf_dev, h.data(), 2 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    FieldDev &d = f->host;
+    d.dsf_e = f->dsf_dev;
+    d.dsf_f = f->dsf_dev + n;
+    d.dsf_n = n;
+    d.dsf_del = del;
+    d.dsf_rfld0 = h[n - 4];
+    d.dsf_rfld1 = rc * h[n + n - 4];
+    d.eta = a;
+    return AKMC_SUCCESS;
+}
+
 int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akmc_batch_t b)
 {
     b->f = f;
@@ -584,7 +628,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
         h.cell_off = ncells;
         ncells += h.nc[0] * h.nc[1] * h.nc[2];
         max_density = std::max(max_density, h.n / (h.L[0] * h.L[1] * h.L[2]));
-        if (f->kind == 1 && f->do_ewald) {
+        if (f->kind == 1 && f->do_ewald == 1) {
             double eta;
             int g[3];
             ewald_params(f, h.L, eta, g);
@@ -724,6 +768,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     // batch share the field's eta rule)
     if (!f->dev) HIPCHK(hipMalloc(&f->dev, sizeof(FieldDev)));
     TRY(ensure_eam_tables(f));
+    TRY(ensure_dsf_mesh(f));
     HIPCHK(hipMemcpy(f->dev, &f->host, sizeof(FieldDev), hipMemcpyHostToDevice));
     d.field = f->dev;
     b->hout.resize(W);
@@ -1150,6 +1195,7 @@ void akmc_field_destroy(akmc_field_t f)
     if (f->single) batch_free(f->single);
     if (f->dev) (void)hipFree(f->dev);
     if (f->eam_dev) (void)hipFree(f->eam_dev);
+    if (f->dsf_dev) (void)hipFree(f->dsf_dev);
     delete f;
 }
 

@@ -989,7 +989,25 @@ __device__ inline void ion_pair_terms(const FieldDev &F, const IonPairDev &p, do
     forc = T(0);
     ec = T(0);
     eb = T(0);
-    if (F.do_ewald) {
+    if (F.do_ewald == 2) {
+        // interpolate_shifted_damped_potential (pair Field.cpp:2049-2094):
+        // three-point interpolation of the erfc mesh plus the shift terms
+        const double rstep = 1.0 / F.dsf_del;
+        const double x = double(r) * rstep;
+        int point = int(x);
+        if (point > F.dsf_n - 3) point = F.dsf_n - 3;
+        const T dlt = T(x - double(point));
+        const double *te = F.dsf_e + point, *tf = F.dsf_f + point;
+        T d1 = T(te[0]) + (T(te[1]) - T(te[0])) * dlt;
+        T d2 = T(te[1]) + (T(te[2]) - T(te[1])) * (dlt - T(1));
+        const T omega = d1 + (d2 - d1) * dlt * T(0.5);
+        d1 = T(tf[0]) + (T(tf[1]) - T(tf[0])) * dlt;
+        d2 = T(tf[1]) + (T(tf[2]) - T(tf[1])) * (dlt - T(1));
+        const T gamma = d1 + (d2 - d1) * dlt * T(0.5);
+        const T cq = T(qiq);
+        ec = cq * (omega - T(F.dsf_rfld0) + T(F.dsf_rfld1) * (r - T(F.rc)));
+        forc = -cq * (gamma - T(F.dsf_rfld1) * ir);
+    } else if (F.do_ewald) {
         T eta = T(F.eta);
         T cq = T(qiq);
         T xr = eta * r;
@@ -1242,7 +1260,7 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
     for (int i = W.off + threadIdx.x; i < W.off + W.n; i += blockDim.x) {
         v[0] += b.e_a[i];
         v[1] += b.e_b[i];
-        if (F.kind == 1 && F.do_ewald) {
+        if (F.kind == 1 && F.do_ewald == 1) {
             v[2] += b.e_c[i];
             v[3] += b.q[i] * b.q[i];
         }
@@ -1250,7 +1268,7 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
             for (int c = 0; c < 6; ++c) v[4 + c] += b.st_atom[(size_t)c * b.A + i];
     }
     double phtot = 0.0, rs[6] = {0, 0, 0, 0, 0, 0};
-    if (stress && F.kind == 1 && F.do_ewald) {
+    if (stress && F.kind == 1 && F.do_ewald == 1) {
         // reciprocal stress (Ewald.cpp:613-657) with the phase total from 0
         for (int k = threadIdx.x; k < W.nk; k += blockDim.x) {
             int kk = W.k_off + k;
@@ -1271,7 +1289,7 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
     }
     block_sum<10>(v, sh);
     double r7[7] = {phtot, rs[0], rs[1], rs[2], rs[3], rs[4], rs[5]};
-    if (stress && F.kind == 1 && F.do_ewald) block_sum<7>(r7, sh);
+    if (stress && F.kind == 1 && F.do_ewald == 1) block_sum<7>(r7, sh);
     if (threadIdx.x == 0) {
         WalkerOut o;
         if (F.kind == 0) {
@@ -1281,7 +1299,7 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
             o.total = o.many + o.vdw;
         } else {
             // self term -eta/sqrt(pi) CTOEV sum q^2 (pair Field.cpp:704-720)
-            o.self = F.do_ewald ? -1.1283791670955126 * 0.5 * F.cfact * F.eta * v[3] : 0.0;
+            o.self = F.do_ewald == 1 ? -1.1283791670955126 * 0.5 * F.cfact * F.eta * v[3] : 0.0;
             o.real = v[0] + o.self;
             o.vdw = v[1];
             o.rcp = v[2];
@@ -1289,7 +1307,7 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
             o.total = o.rcp + o.real + o.vdw;
         }
         for (int c = 0; c < 6; ++c) o.stress[c] = stress ? v[4 + c] : 0.0;
-        if (stress && F.kind == 1 && F.do_ewald) {
+        if (stress && F.kind == 1 && F.do_ewald == 1) {
             for (int c = 0; c < 6; ++c) o.stress[c] += r7[1 + c];
             o.stress[0] -= r7[0];
             o.stress[1] -= r7[0];
@@ -1454,7 +1472,7 @@ static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress,
 #define AKMC_FORCE(L_, U_) launch_sc_force<T, L_, U_>(b, hf, want_stress, s)
         AKMC_SC_VARIANT(AKMC_FORCE);
 #undef AKMC_FORCE
-    } else if (hf.do_ewald) {
+    } else if (hf.do_ewald == 1) {
         k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
         dim3 g((b.maxnk + kBlock - 1) / kBlock, b.W);
         k_ewald_sk<<<g, kBlock, 0, s>>>(b);

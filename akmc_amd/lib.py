@@ -603,7 +603,7 @@ def eam_potentials_text(entries, rc, species=None, noimage=False):
     return "\n".join(lines) + "\n"
 
 
-def buck_potentials_text(species, charges, pots, rc, noimage=False, ewald_precis=None):
+def buck_potentials_text(species, charges, pots, rc, noimage=False, ewald_precis=None, ewald_damp=None):
     """Reference 'potentials' text for Buckingham + Ewald
     (potentials/pair/Field.cpp:215-317, TwoBody.cpp:56-112)."""
     lines = [f"cutoff {rc!r}", f"species {len(species)}"]
@@ -612,6 +612,8 @@ def buck_potentials_text(species, charges, pots, rc, noimage=False, ewald_precis
         lines.append("noimage")
     if ewald_precis is not None:
         lines.append(f"ewald precis {ewald_precis!r}")
+    if ewald_damp is not None:
+        lines.append(f"ewald damp {ewald_damp!r}")   # damped-shifted Coulomb (pair Field.cpp:265-270)
     lines.append(f"twobody {len(pots)} ev ev")
     for pot in pots:
         # (A, B, A_buck, rho, C) or (form, A, B, params...) with form one of

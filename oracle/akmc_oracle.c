@@ -647,6 +647,44 @@ static inline void buck(const ako_ion_field *f, int ta, int tb, double r, double
     }
 }
 
+double ako_damped_mesh(double alpha, double rc, int nmesh, double *e, double *f, double *rfld0, double *rfld1)
+{
+    const double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741, a4 = -1.453152027, a5 = 1.061405429,
+                 pp = 0.3275911;  /* Abramowitz & Stegun 7.1.26 */
+    const double del = rc / (nmesh - 4);
+    e[0] = f[0] = 0.0;
+    for (int i = 1; i < nmesh; ++i) {
+        double r = (double)i * del, rsq = r * r;
+        double tt = 1.0 / (1.0 + pp * alpha * r);
+        double ex = exp(-pow(alpha * r, 2.0));
+        e[i] = tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * ex / r;
+        f[i] = (e[i] + 2.0 * (alpha / 1.7724538509055159) * ex) / rsq;
+    }
+    *rfld0 = e[nmesh - 4];
+    *rfld1 = rc * f[nmesh - 4];
+    return del;
+}
+
+/* interpolate_shifted_damped_potential (pair Field.cpp:2049-2094): the
+ * three-point interpolation of the mesh; energy and -(dV/dr)/r with the
+ * shift terms */
+static inline void damped(const ako_ion_field *f, double r, double chg, double *e0, double *f1)
+{
+    double rstep = 1.0 / f->dsf_del;
+    int point = (int)(r * rstep);
+    double dlt = r * rstep - (double)point;
+    const double *t = f->dsf_e;
+    double d1 = t[point] + (t[point + 1] - t[point]) * dlt;
+    double d2 = t[point + 1] + (t[point + 2] - t[point + 1]) * (dlt - 1.0);
+    double omega = d1 + (d2 - d1) * dlt * 0.5;
+    *e0 = chg * (omega - f->dsf_rfld0 + f->dsf_rfld1 * (r - f->rc));
+    t = f->dsf_f;
+    d1 = t[point] + (t[point + 1] - t[point]) * dlt;
+    d2 = t[point + 1] + (t[point + 2] - t[point + 1]) * (dlt - 1.0);
+    double gamma = d1 + (d2 - d1) * dlt * 0.5;
+    *f1 = -chg * (gamma - f->dsf_rfld1 / r);
+}
+
 /* one real-space pair term of realSpaceForceImage (pair Field.cpp:780-818) */
 static inline void ion_pair(const ako_ion_field *f, int ta, int tb, double qa_c, double qb,
                             double rx, double ry, double rz, double rsq, double *fa, double *fb,
@@ -659,6 +697,8 @@ static inline void ion_pair(const ako_ion_field *f, int ta, int tb, double qa_c,
         hfunc1(hf0 * r, &h0, &h1);
         eng = h0 * cp * hf0;
         forc = h1 * cp * hf1;
+    } else if (f->do_ewald == 2) {
+        damped(f, r, cp, &eng, &forc);
     }
     buck(f, ta, tb, r, rsq, &es, &fs);
     *ereal += eng;

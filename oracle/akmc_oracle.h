@@ -111,7 +111,16 @@ typedef struct {
     double *gx, *gy, *gz, *g0, *g1;
     int form[AKO_MAXPOT]; /* AKO_BUCK ... per potential */
     int energy_path;      /* 1: short-range energies of TwoBody's energy-only form */
+    /* do_ewald == 2: damped-shifted Coulomb on the erfc mesh of
+     * ako_damped_mesh (Field::fieldErfcGen, pair Field.cpp:19-73) */
+    const double *dsf_e, *dsf_f;
+    double dsf_del, dsf_rfld0, dsf_rfld1;
 } ako_ion_field;
+
+/* Field::fieldErfcGen (potentials/pair/Field.cpp:19-73): nmesh points
+ * (the reference's MAXMESH) of erfc(alpha r)/r by Abramowitz-Stegun 7.1.26
+ * and of its force term, spacing rc / (nmesh - 4); returns the spacing */
+double ako_damped_mesh(double alpha, double rc, int nmesh, double *e, double *f, double *rfld0, double *rfld1);
 
 void ako_ewald_eta_from_acc(double acc, double rc, double *eta, double *eps, double *tol);
 void ako_ewald_gcell(const double lat[9], const double rcp[9], double eta, double rc,

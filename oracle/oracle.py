@@ -38,7 +38,28 @@ class IonField(C.Structure):
                 ("rc", C.c_double), ("min_image", C.c_int), ("do_ewald", C.c_int),
                 ("eta", C.c_double), ("cfact", C.c_double), ("cells", C.c_int * 3),
                 ("nk", C.c_int), ("gx", _D), ("gy", _D), ("gz", _D), ("g0", _D), ("g1", _D),
-                ("form", C.c_int * MAXPOT), ("energy_path", C.c_int)]
+                ("form", C.c_int * MAXPOT), ("energy_path", C.c_int),
+                ("dsf_e", _D), ("dsf_f", _D), ("dsf_del", C.c_double), ("dsf_rfld0", C.c_double),
+                ("dsf_rfld1", C.c_double)]
+
+# points of the damped-shifted Coulomb mesh: the reference's MAXMESH, a
+# constant of its missing Constants.h; this value is the build's choice
+DSF_MESH = 5000
+
+
+def set_damped(f, alpha, rc, nmesh=DSF_MESH):
+    """switch an ion field to damped-shifted Coulomb (ewald damp alpha,
+    pair Field.cpp:119-126): builds the erfc mesh of fieldErfcGen"""
+    e, fm = np.zeros(nmesh), np.zeros(nmesh)
+    r0, r1 = C.c_double(), C.c_double()
+    lib().ako_damped_mesh.restype = C.c_double
+    dl = lib().ako_damped_mesh(C.c_double(alpha), C.c_double(rc), nmesh, _p(e), _p(fm), C.byref(r0), C.byref(r1))
+    f.do_ewald = 2
+    f.eta = alpha
+    f.dsf_e, f.dsf_f = _p(e), _p(fm)
+    f.dsf_del, f.dsf_rfld0, f.dsf_rfld1 = dl, r0.value, r1.value
+    f._dsf = (e, fm)  # keep alive
+    return e, fm
 
 # short-range pair forms of the potentials text (TwoBody.cpp:440-470) and
 # their parameter counts

@@ -251,3 +251,24 @@ def test_ionic_other_pair_forms(lib, oracle, forms, fp64):
     print(f"  energy path: vdw {e['vdw']:.6f} (oracle {ref_e['e_vdw']:.6f}; force path {ref['e_vdw']:.6f})",
           flush=True)
     assert abs(e["total"] - ref_e["energy"]) / c.n < tol[1]
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_ionic_damped_shifted_coulomb(lib, oracle, fp64):
+    """ewald damp: the damped-shifted Coulomb sum on the erfc mesh of
+    fieldErfcGen (5000 points, this build's choice of the reference's
+    MAXMESH) with Buckingham, explicit images, against the oracle"""
+    c = W.rocksalt_cell(seed=5)
+    pots = [("M", "X") + W.ROCKSALT_BUCK[("M", "X")], ("X", "X") + W.ROCKSALT_BUCK[("X", "X")],
+            ("M", "M") + W.ROCKSALT_BUCK[("M", "M")]]
+    f = lib.Field(lib.buck_potentials_text(["M", "X"], [2.0, -2.0], pots, 10.0, noimage=True, ewald_damp=0.3))
+    if fp64:
+        f.set_precision(64)
+    of = oracle.ion_field(["M", "X"], pots, rc=10.0, ewald=None, min_image=False, lat=c.lat)
+    oracle.set_damped(of, 0.3, 10.0)
+    ref = oracle.ion_forces(of, c.pos, c.types, c.charges, c.lat)
+    got = f.calculate_forces(c.pos, c.types, c.lat, c.charges)
+    tol = (1e-8, 1e-10) if fp64 else (F_TOL, E_TOL_ATOM)
+    _compare(f"rock-salt damped-shifted Coulomb fp{64 if fp64 else 32}", ref, got, c.n, *tol)
+    assert got["energy"]["rcp"] == 0.0
+    assert abs(got["energy"]["real"] - ref["e_real"]) / c.n < tol[1]

@@ -64,3 +64,41 @@ def test_seven_six_energy(oracle):
     assert e == pytest.approx(-eps, rel=1e-14)
     e, _ = _dimer(oracle, pot, 3.5)
     assert e == pytest.approx(eps * (sig / 3.5) ** 7 - 2 * eps * (sig / 3.5) ** 6, rel=1e-13)
+
+
+def _dsf_dimer(oracle, r, alpha=0.3, rc=9.0, q=(1.5, -2.0)):
+    f = oracle.ion_field(["A", "B"], [("A", "B", 0.0, 1.0, 0.0)], rc=rc, ewald=None, lat=LAT)
+    oracle.set_damped(f, alpha, rc)
+    pos = np.array([[5.0, 5.0 + r], [5.0, 5.0], [5.0, 5.0]])
+    out = oracle.ion_forces(f, pos, np.array([0, 1], np.int32), np.array(q), LAT)
+    return out["e_real"], out["force"][0, 1]
+
+
+def test_damped_mesh_is_abramowitz_stegun_erfc(oracle):
+    """the mesh holds erfc(alpha r)/r to the A&S 7.1.26 bound (1.5e-7 on erfc)"""
+    from scipy.special import erfc
+    e, f = oracle.set_damped(oracle.ion_field(["A"], [("A", "A", 0.0, 1.0, 0.0)], rc=9.0, ewald=None, lat=LAT),
+                             0.3, 9.0)
+    r = np.arange(1, e.size) * 9.0 / (e.size - 4)
+    assert np.all(np.abs(e[1:] * r - erfc(0.3 * r)) <= 1.5e-7)
+    ex = 2 * 0.3 / math.sqrt(math.pi) * np.exp(-(0.3 * r) ** 2)
+    assert np.allclose(f[1:], (e[1:] + ex) / r ** 2, rtol=1e-14)
+
+
+def test_damped_shifted_coulomb_closed_form(oracle):
+    """E = k qi qj [erfc(a r)/r - erfc(a rc)/rc + (r - rc)(erfc(a rc)/rc^2 + 2a/sqrt(pi) e^{-a^2 rc^2}/rc)]
+    (the damped-shifted-force Coulomb), its force -dE/dr, and both vanish at rc"""
+    from scipy.special import erfc
+    a, rc, k = 0.3, 9.0, 14.3997584 * 1.5 * -2.0
+
+    def exact(r):
+        s = erfc(a * rc) / rc ** 2 + 2 * a / math.sqrt(math.pi) * math.exp(-(a * rc) ** 2) / rc
+        return k * (erfc(a * r) / r - erfc(a * rc) / rc + (r - rc) * s)
+
+    for r in (2.1, 4.37, 7.9):
+        e, f = _dsf_dimer(oracle, r)
+        assert e == pytest.approx(exact(r), abs=2e-6 * abs(k))
+        h = 1e-5
+        assert f == pytest.approx(-(exact(r + h) - exact(r - h)) / (2 * h), abs=1e-5 * abs(k))
+    e, f = _dsf_dimer(oracle, rc - 1e-6)
+    assert abs(e) < 1e-6 and abs(f) < 1e-5
