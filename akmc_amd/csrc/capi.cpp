@@ -94,7 +94,7 @@ struct akmc_field_s {
     bool explicit_ewald = false;
     double eta = 0.0;
     int gcell[3] = {0, 0, 0};
-    double skin = 0.6;
+    double skin = 0.7;  // best of a 0.45-1.6 A sweep on the batched dimer workload
     std::vector<SpeciesRec> species;
     std::vector<SCPot> sc;
     std::vector<BuckPot> buck;

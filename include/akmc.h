@@ -58,7 +58,7 @@ int akmc_field_kind(akmc_field_t f);
  * accuracy-derived ones of potentials/pair/Field.cpp:129-132 and
  * Ewald.cpp:88-120; call before akmc_field_setup. */
 int akmc_field_set_ewald(akmc_field_t f, double eta, int gx, int gy, int gz);
-/* Verlet skin of the device neighbour lists (default 0.6 A). */
+/* Verlet skin of the device neighbour lists (default 0.7 A). */
 int akmc_field_set_skin(akmc_field_t f, double skin);
 /* Pair-physics precision of later force calls: 32 (FP32 pair terms, the
  * default) or 64 (FP64 throughout; the verification path).  Cutoff tests,
