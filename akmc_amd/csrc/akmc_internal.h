@@ -10,6 +10,8 @@
 #define AKMC_NB_JMASK 0xFFFFFu
 #define AKMC_NB_ZERO_SHIFT (0x777u << 20)
 #define AKMC_MAX_WALKER_ATOMS (1 << 20)
+// workgroup size of the LDS-staged Sutton-Chen kernels (1024 / lpa atoms each)
+#define AKMC_STAGE_BLOCK 1024
 // cells per axis are capped so a walker's grid fits one workgroup's LDS
 #define AKMC_MAX_CELLS_AXIS 16
 #define AKMC_MAX_WALKER_CELLS (AKMC_MAX_CELLS_AXIS * AKMC_MAX_CELLS_AXIS * AKMC_MAX_CELLS_AXIS)
@@ -113,7 +115,12 @@ struct BatchDev {
     int lpa;              // lanes per atom of the Sutton-Chen pair kernels (1, 2, 4)
     int nb16;             // list entries are 16-bit local indices (minimum-image
                           // many-body lists of walkers below 65536 atoms)
-    int unroll;           // neighbour gathers in flight per lane (1, 4)
+    int unroll;           // neighbour gathers in flight per lane (1, 2, 4, 8)
+    int staged;           // Sutton-Chen pair kernels read the gather records of
+                          // a walker from LDS (one workgroup per chunk of atoms)
+    int nchunk;           // workgroups of a staged launch
+    const int2 *chunk;    // staged chunk -> (walker, first local atom), for lpa
+    int stage_bytes;      // dynamic LDS of a staged launch
     const FieldDev *field;
     const WalkerDev *walkers;
     const int *wk;        // atom -> walker

@@ -123,6 +123,11 @@ struct akmc_batch_s {
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     std::vector<WalkerOut> hout;
+    // chunk tables of the LDS-staged Sutton-Chen launches, per lanes per atom
+    // (1, 2, 4, 8); stage_ok when every walker's gather records fit the LDS
+    const int2 *chunk_tab[4] = {nullptr, nullptr, nullptr, nullptr};
+    int chunk_n[4] = {0, 0, 0, 0};
+    bool stage_ok = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long rebuild_base = 0;
     // search state
@@ -743,6 +748,30 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     HIPCHK(hipMemcpy(wd, b->hw.data(), W * sizeof(WalkerDev), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(wk, hwk.data(), A * sizeof(int), hipMemcpyHostToDevice));
     {
+        // LDS staging: a walker's packed gather records (32 bytes per atom)
+        // next to the static pair table of the multi-species kernels
+        int nmax = 0, dev = 0, maxlds = 0;
+        for (int w = 0; w < W; ++w) nmax = std::max(nmax, b->hw[w].n);
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t bytes = (size_t)nmax * sizeof(double4);
+        b->stage_ok = f->kind == 0 && !f->host.eam && bytes + sizeof(SCPairDev) * AKMC_MAXPAIR <= (size_t)maxlds;
+        d.stage_bytes = (int)bytes;
+        for (int li = 0; li < 4; ++li) {
+            const int apc = AKMC_STAGE_BLOCK >> li;  // atoms per chunk at 1, 2, 4, 8 lanes per atom
+            std::vector<int2> ch;
+            for (int w = 0; w < W; ++w)
+                for (int a0 = 0; a0 < b->hw[w].n; a0 += apc) ch.push_back(make_int2(w, a0));
+            int2 *dc;
+            TRY(dalloc(b, &dc, ch.size()));
+            HIPCHK(hipMemcpy(dc, ch.data(), ch.size() * sizeof(int2), hipMemcpyHostToDevice));
+            b->chunk_tab[li] = dc;
+            b->chunk_n[li] = (int)ch.size();
+        }
+        d.chunk = b->chunk_tab[0];
+        d.nchunk = b->chunk_n[0];
+    }
+    {
         std::vector<int> hcw(ncells);
         for (int w = 0; w < W; ++w) {
             const WalkerDev &h = b->hw[w];
@@ -1360,28 +1389,58 @@ int akmc_batch_set_variant(akmc_batch_t b, int lanes, int unroll)
     if (b->d.lpa != lanes || b->d.unroll != unroll) drop_graph(b);  // the captured graph holds the old kernels
     b->d.lpa = lanes;
     b->d.unroll = unroll;
+    const int li = lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3;
+    b->d.chunk = b->chunk_tab[li];
+    b->d.nchunk = b->chunk_n[li];
     return AKMC_SUCCESS;
 }
 
-int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[2], double ms[AKMC_NVARIANTS])
+int akmc_batch_set_staging(akmc_batch_t b, int on)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    if (on && !b->stage_ok)
+        return fail(AKMC_EUNSUPPORTED,
+                    "LDS staging needs a Sutton-Chen field whose largest walker's gather records "
+                    "(%d bytes) fit one workgroup's LDS", b->d.stage_bytes);
+    if (b->d.staged != (on ? 1 : 0)) drop_graph(b);
+    b->d.staged = on ? 1 : 0;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_staging(akmc_batch_t b, int *on, int *possible)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    if (on) *on = b->d.staged;
+    if (possible) *possible = b->stage_ok ? 1 : 0;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[3], double ms[AKMC_NVARIANTS])
 {
     if (!b || nrep <= 0) return fail(AKMC_EINVAL, "bad arguments");
     static const int lanes[4] = {1, 2, 4, 8}, unroll[4] = {1, 2, 4, 8};
     double t[AKMC_NVARIANTS];
     TRY(run_forces(b, 0));  // lists current before timing
+    const int nv = b->stage_ok ? AKMC_NVARIANTS : AKMC_NVARIANTS / 2;
     for (int v = 0; v < AKMC_NVARIANTS; ++v) {
-        TRY(akmc_batch_set_variant(b, lanes[v / 4], unroll[v % 4]));
+        t[v] = -1.0;
+        if (v >= nv) continue;
+        TRY(akmc_batch_set_staging(b, v / 16));
+        TRY(akmc_batch_set_variant(b, lanes[v / 4 % 4], unroll[v % 4]));
         TRY(run_forces(b, 0));  // first launch of this instantiation
         TRY(akmc_batch_forces_repeat(b, nrep, &t[v]));
-        if (ms) ms[v] = t[v];
     }
+    if (ms)
+        for (int v = 0; v < AKMC_NVARIANTS; ++v) ms[v] = t[v];
     int k = 0;
-    for (int v = 1; v < AKMC_NVARIANTS; ++v)
+    for (int v = 1; v < nv; ++v)
         if (t[v] < t[k]) k = v;
-    TRY(akmc_batch_set_variant(b, lanes[k / 4], unroll[k % 4]));
+    TRY(akmc_batch_set_staging(b, k / 16));
+    TRY(akmc_batch_set_variant(b, lanes[k / 4 % 4], unroll[k % 4]));
     if (best) {
-        best[0] = lanes[k / 4];
+        best[0] = lanes[k / 4 % 4];
         best[1] = unroll[k % 4];
+        best[2] = k / 16;
     }
     return AKMC_SUCCESS;
 }

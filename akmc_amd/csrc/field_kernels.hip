@@ -85,6 +85,12 @@ __device__ inline T pow_int(T u2, int k)
     return r;
 }
 
+// square root for the integer powers: the hardware instruction (1 ulp) on
+// the FP32 path, whose arguments (a/r)^2 are normal numbers far from the
+// denormal range the correctly rounded expansion guards; libm on FP64
+__device__ inline double root(double x) { return sqrt(x); }
+__device__ inline float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // pow_int for U values at once: the uniform loop control runs once per
 // batch of pairs instead of once per pair (same operations per value)
 template <typename T, int U>
@@ -106,7 +112,7 @@ __device__ inline void pow_int_batch(const T (&u2)[U], int k, T (&r)[U])
     }
     if (k & 1) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) r[u] *= sqrt(u2[u]);
+        for (int u = 0; u < U; ++u) r[u] *= root(u2[u]);
     }
 }
 
@@ -565,14 +571,98 @@ __device__ inline uint32_t self_entry(int i, const WalkerDev &W)
 // agree bitwise; FP64 ones to the last bit of FMA contraction); LPA variants
 // differ by summation order.
 
-// list entries k, k + LPA, ..., k + (U-1) LPA of atom i ("none" past the end)
-template <int LPA, int U>
+// list entries k, k + LPA, ..., k + (U-1) LPA of atom i ("none" past the end).
+// NB16: the batch is known to store 16-bit minimum-image entries, read
+// without the run-time width test (their image bits are never used).
+template <int LPA, int U, bool NB16>
 __device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, uint32_t none, uint32_t (&e)[U])
 {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int kk = k + u * LPA;
-        e[u] = kk < cnt ? nbr_entry(b, kk, i) : none;
+        if constexpr (NB16)
+            e[u] = kk < cnt ? uint32_t(reinterpret_cast<const uint16_t *>(b.nbr)[(size_t)kk * b.A + i]) : none;
+        else
+            e[u] = kk < cnt ? nbr_entry(b, kk, i) : none;
+    }
+}
+
+// The 16-bit entries of one lane walked by pointer: entry k0 + u LPA sits
+// u * (LPA A) elements past the lane's cursor, which advances by U LPA A
+// per batch (two 64-bit adds instead of a 64-bit multiply-add per entry)
+template <int LPA, int U>
+struct Entries16 {
+    const uint16_t *at;
+    size_t step;  // LPA * A elements
+    int k, cnt;
+    uint32_t none;
+    __device__ Entries16(const BatchDev &b, int i, int sub, int cnt_, uint32_t none_)
+        : at(reinterpret_cast<const uint16_t *>(b.nbr) + (size_t)sub * b.A + i), step((size_t)LPA * b.A), k(sub),
+          cnt(cnt_), none(none_) {}
+    __device__ void next(uint32_t (&e)[U])
+    {
+#pragma unroll
+        for (int u = 0; u < U; ++u) e[u] = k + u * LPA < cnt ? uint32_t(at[u * step]) : none;
+        at += U * step;
+        k += U * LPA;
+    }
+};
+
+// A neighbour's packed gather record, whole (W4) or its position only
+// (24 bytes: one 16-byte and one 8-byte read, w = 0)
+template <bool W4>
+__device__ inline double4 gather_rec(const double4 *base, int j)
+{
+    if constexpr (W4) {
+        return base[j];
+    } else {
+        const double2 xy = reinterpret_cast<const double2 *>(base)[2 * j];
+        const double z = reinterpret_cast<const double *>(base)[4 * j + 2];
+        return make_double4(xy.x, xy.y, z, 0.0);
+    }
+}
+
+// 1/sqrt(rho) of an FP32 walker in the fourth word of its packed gather
+// record (single pair type): the float's bits in the low half, so the pass-2
+// gather reads it without a conversion
+__device__ inline double isr32_word(float v) { return __longlong_as_double((long long)__float_as_uint(v)); }
+__device__ inline float isr32_of(double w) { return __uint_as_float(uint32_t(__double_as_longlong(w))); }
+
+// The atom a thread serves.  Unstaged launches: LPA consecutive threads per
+// atom over the whole batch (XCD-ordered).  Staged launches: one workgroup of
+// AKMC_STAGE_BLOCK threads per chunk of AKMC_STAGE_BLOCK / LPA atoms of one
+// walker; the workgroup first copies the walker's packed gather records
+// (p4) into LDS, so every neighbour gather of the pair loop is an LDS read
+// instead of a texture-unit gather from L2.  A workgroup whose walker runs
+// in the other precision, or is inactive, leaves before staging.
+struct AtomSlot {
+    int i, sub;
+    bool ok;
+};
+
+template <typename T, int LPA, bool STAGED>
+__device__ inline AtomSlot sc_slot(const BatchDev &b, WalkerDev &W, double4 *stage)
+{
+    if constexpr (STAGED) {
+        const int per = (b.nchunk + kXCD - 1) / kXCD;
+        const int c = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+        if (c >= b.nchunk) return {0, 0, false};
+        const int2 ch = b.chunk[c];
+        if (other_precision<T>(b, ch.x)) return {0, 0, false};
+        W = b.walkers[ch.x];
+        for (int k = threadIdx.x; k < W.n; k += blockDim.x) stage[k] = b.p4[W.off + k];
+        __syncthreads();
+        const int la = ch.y + int(threadIdx.x) / LPA;
+        return {W.off + la, int(threadIdx.x) % LPA, la < W.n};
+    } else {
+        const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
+        if (t < 0) return {0, 0, false};
+        const int i = t / LPA;
+        if (i >= b.A) return {0, 0, false};
+        const int w = walker_of(b, i);
+        if (other_precision<T>(b, w)) return {0, 0, false};
+        W = b.walkers[w];
+        return {i, t % LPA, true};
     }
 }
 
@@ -581,20 +671,20 @@ __device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, u
 // LPA lanes share one atom's neighbour list (strided) and combine by shuffles.
 // ONE: the field has a single species pair with integer exponents, whose
 // coefficients stay in registers and whose powers take uniform control flow.
-template <typename T, bool MINIMG, int LPA, int U, bool ONE>
-__global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
+template <typename T, bool MINIMG, int LPA, int U, bool ONE, bool STAGED>
+__global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_density(BatchDev b)
 {
     __shared__ SCPairDev tab[ONE ? 1 : AKMC_MAXPAIR];
+    extern __shared__ double4 stage[];
+    constexpr bool NB16 = STAGED && MINIMG;  // staged batches store 16-bit minimum-image entries
     const FieldDev &F = *b.field;
     if (!ONE) stage_sc_table(F, tab);
     const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
     const int m1 = F.sc[0].m_int;
-    const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
-    if (t < 0) return;
-    const int i = t / LPA, sub = t % LPA;
-    if (i >= b.A) return;
-    if (other_precision<T>(b, walker_of(b, i))) return;
-    const WalkerDev W = b.walkers[walker_of(b, i)];
+    WalkerDev W;
+    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage);
+    if (!sl.ok) return;
+    const int i = sl.i, sub = sl.sub;
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
     const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
@@ -602,15 +692,22 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
     const uint32_t none = self_entry(i, W);
     double rho = 0.0;
     uint32_t en[U];
-    fetch_entries<LPA, U>(b, i, sub, cnt, none, en);
+    Entries16<LPA, U> cur(b, i, sub, cnt, none);
+    if constexpr (NB16) cur.next(en);
+    else fetch_entries<LPA, U, false>(b, i, sub, cnt, none, en);
     for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
         uint32_t e[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) e[u] = en[u];
-        fetch_entries<LPA, U>(b, i, k0 + LPA * U, cnt, none, en);  // next batch in flight
+        // next batch in flight
+        if constexpr (NB16) cur.next(en);
+        else fetch_entries<LPA, U, false>(b, i, k0 + LPA * U, cnt, none, en);
         double4 pj[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) pj[u] = b.p4[W.off + int(e[u] & AKMC_NB_JMASK)];
+        for (int u = 0; u < U; ++u) {
+            const int jl = int(e[u] & AKMC_NB_JMASK);
+            pj[u] = STAGED ? gather_rec<!ONE>(stage, jl) : gather_rec<!ONE>(b.p4, W.off + jl);
+        }
         double r2[U];
         bool in[U];
 #pragma unroll
@@ -628,8 +725,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
             for (int u = 0; u < U; ++u) u2[u] = c1.a * c1.a * recip<T>(T(r2[u]));
             pow_int_batch<T, U>(u2, m1, um);
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (in[u]) rho += double(c1.rho_coef * um[u]);
+            for (int u = 0; u < U; ++u) rho += double(in[u] ? c1.rho_coef * um[u] : T(0));  // + exact zeros
         } else {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -653,27 +749,27 @@ __global__ void __launch_bounds__(kBlock) k_sc_density(BatchDev b)
     if (sizeof(T) == 8) b.isr64[i] = isr64;
     // single pair type: the packed gather record carries 1/sqrt(rho) in place
     // of the species for pass 2 (k_pack restores the species every step)
-    if (ONE) reinterpret_cast<double *>(&b.p4[i])[3] = sizeof(T) == 8 ? isr64 : double(isr32);
+    if (ONE) reinterpret_cast<double *>(&b.p4[i])[3] = sizeof(T) == 8 ? isr64 : isr32_word(isr32);
 }
 
 // pass 2: F_i = sum_j r_ij (fs - fm)/r^2, fs = n eps (a/r)^n,
 // fm = 1/2 m (eps c)^2 (1/sqrt(rho_i) + 1/sqrt(rho_j)) (a/r)^m
 // (potentials/metal/Field.cpp:587-640, ManyBody.cpp:621-757); each atom
 // also carries half of each pair's V = eps (a/r)^n
-template <typename T, bool MINIMG, bool STRESS, int LPA, int U, bool ONE>
-__global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
+template <typename T, bool MINIMG, bool STRESS, int LPA, int U, bool ONE, bool STAGED>
+__global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force(BatchDev b)
 {
     __shared__ SCPairDev tab[ONE ? 1 : AKMC_MAXPAIR];
+    extern __shared__ double4 stage[];
+    constexpr bool NB16 = STAGED && MINIMG;
     const FieldDev &F = *b.field;
     if (!ONE) stage_sc_table(F, tab);
     const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
     const int n1 = F.sc[0].n_int, m1 = F.sc[0].m_int;
-    const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
-    if (t < 0) return;
-    const int i = t / LPA, sub = t % LPA;
-    if (i >= b.A) return;
-    if (other_precision<T>(b, walker_of(b, i))) return;
-    const WalkerDev W = b.walkers[walker_of(b, i)];
+    WalkerDev W;
+    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage);
+    if (!sl.ok) return;
+    const int i = sl.i, sub = sl.sub;
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
     const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
@@ -684,22 +780,27 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
     double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
     uint32_t en[U];
-    fetch_entries<LPA, U>(b, i, sub, cnt, none, en);
+    Entries16<LPA, U> cur(b, i, sub, cnt, none);
+    if constexpr (NB16) cur.next(en);
+    else fetch_entries<LPA, U, false>(b, i, sub, cnt, none, en);
     for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
         uint32_t e[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) e[u] = en[u];
-        fetch_entries<LPA, U>(b, i, k0 + LPA * U, cnt, none, en);  // next batch in flight
+        // next batch in flight
+        if constexpr (NB16) cur.next(en);
+        else fetch_entries<LPA, U, false>(b, i, k0 + LPA * U, cnt, none, en);
         double4 pj[U];
         float isj[U];
         double isj64[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int j = W.off + int(e[u] & AKMC_NB_JMASK);
-            pj[u] = b.p4[j];
+            const int jl = int(e[u] & AKMC_NB_JMASK);
+            const int j = W.off + jl;
+            pj[u] = STAGED ? stage[jl] : b.p4[j];
             if (ONE) {  // 1/sqrt(rho_j) rides in the gather record (pass 1)
                 if (sizeof(T) == 8) isj64[u] = pj[u].w;
-                else isj[u] = float(pj[u].w);
+                else isj[u] = isr32_of(pj[u].w);
             } else {
                 if (sizeof(T) == 8) isj64[u] = b.isr64[j];
                 else isj[u] = b.inv_sqrt_rho[j];
@@ -728,6 +829,34 @@ __global__ void __launch_bounds__(kBlock) k_sc_force(BatchDev b)
             }
             pow_int_batch<T, U>(u2, n1, un);
             pow_int_batch<T, U>(u2, m1, um);
+        }
+        if constexpr (ONE && !STRESS) {
+            // branch-free: a pair outside the cutoff (or a padding slot)
+            // adds exact zeros, which leaves the FP64 sums bitwise as if
+            // it were skipped
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const T fs = c1.fs_coef * un[u];
+                T fm;
+                if constexpr (sizeof(T) == 8)
+                    fm = (isi64 >= 0.0 && isj64[u] >= 0.0) ? c1.fm_coef * T(isi64 + isj64[u]) * um[u] : T(0);
+                else
+                    fm = (isi >= 0.0f && isj[u] >= 0.0f) ? c1.fm_coef * (T(isi) + T(isj[u])) * um[u] : T(0);
+                T ft = in[u] ? (fs - fm) * ir2[u] : T(0);
+                T eh = in[u] ? T(0.5) * c1.eps * un[u] : T(0);
+                if constexpr (sizeof(T) == 4) {
+                    // keep the selects on the FP32 values (one instruction
+                    // each) instead of letting them sink past the widening
+                    asm("" : "+v"(ft));
+                    asm("" : "+v"(eh));
+                }
+                const double dft = double(ft);
+                fx += dft * dx[u];
+                fy += dft * dy[u];
+                fz += dft * dz[u];
+                ep += double(eh);
+            }
+            continue;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1355,38 +1484,62 @@ static bool sc_one(const FieldDev &hf)
     return hf.nspec == 1 && hf.sc[0].on && hf.sc[0].n_int >= 0 && hf.sc[0].m_int >= 0;
 }
 
+// grid, workgroup size and dynamic LDS of a Sutton-Chen pair launch
+struct ScLaunch {
+    int grid, block, lds;
+};
+
+template <int LPA>
+static ScLaunch sc_launch(const BatchDev &b)
+{
+    if (b.staged) return {xcd_grid(b.nchunk), AKMC_STAGE_BLOCK, b.stage_bytes};
+    return {xcd_grid((b.A * LPA + kBlock - 1) / kBlock), kBlock, 0};
+}
+
+template <typename T, int LPA, int U, bool ONE, bool STAGED>
+static void launch_sc_density_one(const BatchDev &b, const FieldDev &hf, hipStream_t s)
+{
+    const ScLaunch L = sc_launch<LPA>(b);
+    if (hf.min_image) k_sc_density<T, true, LPA, U, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+    else k_sc_density<T, false, LPA, U, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+}
+
 template <typename T, int LPA, int U>
 static void launch_sc_density(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
-    int g = xcd_grid((b.A * LPA + kBlock - 1) / kBlock);
     if (sc_one(hf)) {
-        if (hf.min_image) k_sc_density<T, true, LPA, U, true><<<g, kBlock, 0, s>>>(b);
-        else k_sc_density<T, false, LPA, U, true><<<g, kBlock, 0, s>>>(b);
+        if (b.staged) launch_sc_density_one<T, LPA, U, true, true>(b, hf, s);
+        else launch_sc_density_one<T, LPA, U, true, false>(b, hf, s);
     } else {
-        if (hf.min_image) k_sc_density<T, true, LPA, U, false><<<g, kBlock, 0, s>>>(b);
-        else k_sc_density<T, false, LPA, U, false><<<g, kBlock, 0, s>>>(b);
+        if (b.staged) launch_sc_density_one<T, LPA, U, false, true>(b, hf, s);
+        else launch_sc_density_one<T, LPA, U, false, false>(b, hf, s);
     }
 }
 
-template <typename T, int LPA, int U, bool ONE>
+template <typename T, int LPA, int U, bool ONE, bool STAGED>
 static void launch_sc_force_one(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
-    int g = xcd_grid((b.A * LPA + kBlock - 1) / kBlock);
+    const ScLaunch L = sc_launch<LPA>(b);
     // the stress variant exists for U = 1 only: U does not change results
     if (hf.min_image) {
-        if (want_stress) k_sc_force<T, true, true, LPA, 1, ONE><<<g, kBlock, 0, s>>>(b);
-        else k_sc_force<T, true, false, LPA, U, ONE><<<g, kBlock, 0, s>>>(b);
+        if (want_stress) k_sc_force<T, true, true, LPA, 1, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+        else k_sc_force<T, true, false, LPA, U, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
     } else {
-        if (want_stress) k_sc_force<T, false, true, LPA, 1, ONE><<<g, kBlock, 0, s>>>(b);
-        else k_sc_force<T, false, false, LPA, U, ONE><<<g, kBlock, 0, s>>>(b);
+        if (want_stress) k_sc_force<T, false, true, LPA, 1, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+        else k_sc_force<T, false, false, LPA, U, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
     }
 }
 
 template <typename T, int LPA, int U>
 static void launch_sc_force(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
-    if (sc_one(hf)) launch_sc_force_one<T, LPA, U, true>(b, hf, want_stress, s);
-    else launch_sc_force_one<T, LPA, U, false>(b, hf, want_stress, s);
+    if (sc_one(hf)) {
+        if (b.staged) launch_sc_force_one<T, LPA, U, true, true>(b, hf, want_stress, s);
+        else launch_sc_force_one<T, LPA, U, true, false>(b, hf, want_stress, s);
+    } else {
+        if (b.staged) launch_sc_force_one<T, LPA, U, false, true>(b, hf, want_stress, s);
+        else launch_sc_force_one<T, LPA, U, false, false>(b, hf, want_stress, s);
+    }
 }
 
 // (lanes per atom, gathers in flight per lane) of the batch

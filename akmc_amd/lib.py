@@ -177,6 +177,8 @@ def load() -> C.CDLL:
     L.akmc_batch_kernel_time.argtypes = [C.c_void_p, _D, C.POINTER(C.c_longlong)]
     L.akmc_batch_eval_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.akmc_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
+    L.akmc_batch_set_staging.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_staging.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.akmc_batch_save_basin.argtypes = [C.c_void_p]
     L.akmc_art_defaults.argtypes = [C.POINTER(ARTParams)]
     L.akmc_batch_start_art.argtypes = [C.c_void_p, _I, _D, _D, _I, C.POINTER(ARTParams)]
@@ -517,16 +519,28 @@ class Batch:
     def set_variant(self, lanes, unroll=1):
         check(load().akmc_batch_set_variant(self._h, int(lanes), int(unroll)))
 
+    def set_staging(self, on=True):
+        """LDS staging of the Sutton-Chen pair kernels (raises when the
+        largest walker does not fit one workgroup's LDS)."""
+        check(load().akmc_batch_set_staging(self._h, 1 if on else 0))
+
+    def staging(self):
+        """(staged now, staging possible)"""
+        on, ok = C.c_int(0), C.c_int(0)
+        check(load().akmc_batch_staging(self._h, C.byref(on), C.byref(ok)))
+        return bool(on.value), bool(ok.value)
+
     def autotune(self, nrep=10):
         """Time the Sutton-Chen kernel variants (lanes per atom 1/2/4/8 x
-        gathers in flight 1/2/4/8) on the resident positions and keep the
-        fastest; returns ((lanes, unroll), {(lanes, unroll): ms per force
-        evaluation})."""
-        ms = np.zeros(16)
-        best = (C.c_int * 2)()
+        gathers in flight 1/2/4/8 x LDS staging off/on where possible) on
+        the resident positions and keep the fastest; returns ((lanes, unroll,
+        staged), {(lanes, unroll, staged): ms per force evaluation})."""
+        ms = np.zeros(32)
+        best = (C.c_int * 3)()
         check(load().akmc_batch_autotune(self._h, int(nrep), best, _dp(ms)))
-        table = {(l, u): ms[4 * li + ui] / nrep for li, l in enumerate((1, 2, 4, 8)) for ui, u in enumerate((1, 2, 4, 8))}
-        return (best[0], best[1]), table
+        table = {(l, u, st): ms[16 * st + 4 * li + ui] / nrep for st in (0, 1) for li, l in enumerate((1, 2, 4, 8))
+                 for ui, u in enumerate((1, 2, 4, 8)) if ms[16 * st + 4 * li + ui] >= 0}
+        return (best[0], best[1], best[2]), table
 
     def save_basin(self):
         check(load().akmc_batch_save_basin(self._h))

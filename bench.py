@@ -311,7 +311,7 @@ def main():
     # kernel variant chosen on the seeded cells, before the relaxation that uses it
     camp.batch.forces()
     variant, variant_ms = camp.batch.autotune(10)
-    print(f"[bench] autotune (lanes, unroll): {variant} (ms per evaluation "
+    print(f"[bench] autotune (lanes, unroll, staged): {variant} (ms per evaluation "
           f"{ {k: round(v, 3) for k, v in variant_ms.items()} })", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     ok = camp.relax_basins()
@@ -424,8 +424,9 @@ def main():
         "force_evals_per_s_batched_cells": round(batched_force_evals, 1),
         "force_eval_ms_all_walkers": round(eval_ms, 4),
         "force_evals_per_s_cells_device": round(len(cells) / (eval_ms * 1e-3), 1),
-        "autotune": {"lanes_per_atom": variant[0], "gathers_in_flight": variant[1],
-                     "ms_per_eval": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in variant_ms.items()}},
+        "autotune": {"lanes_per_atom": variant[0], "gathers_in_flight": variant[1], "lds_staged": bool(variant[2]),
+                     "ms_per_eval": {f"{k[0]}x{k[1]}{' lds' if k[2] else ''}": round(v, 4)
+                                     for k, v in variant_ms.items()}},
         "roofline": {"bound": "mfma", "bound_note": "compute-bound on the FP32 vector ALU (pair physics has no "
                      "matrix form); peak is gfx950's FP32 vector figure, equal to its FP32 matrix one",
                      "kernel": "k_sc_force (Sutton-Chen pass 2, autotuned variant)",

@@ -105,23 +105,33 @@ int akmc_batch_forces_repeat(akmc_batch_t b, int nrep, double *ms);
  * totals over the nrep evaluations */
 int akmc_batch_time_phases(akmc_batch_t b, int nrep, double ms[4]);
 /* Autotuner of the Sutton-Chen pair kernels.  A kernel variant is
- * (lanes, unroll): each atom's neighbour list is split over `lanes`
+ * (lanes, unroll, staged): each atom's neighbour list is split over `lanes`
  * consecutive lanes of a wavefront (1, 2, 4 or 8), combined by cross-lane
  * sums, and every lane keeps `unroll` neighbour gathers in flight (1, 2, 4
  * or 8).  Small walkers fill the chip better with more lanes per atom; more
- * gathers in flight hide memory latency at the cost of registers.
- * akmc_batch_set_variant fixes the choice (default 1, 1; the graph of a
- * running search is recaptured).  akmc_batch_autotune times nrep force
- * evaluations at the resident positions for each of the AKMC_NVARIANTS
- * variants, ms[4*l + u] for lanes (1, 2, 4, 8)[l] and unroll (1, 2, 4, 8)[u],
- * keeps the fastest and returns it in best[0] (lanes), best[1] (unroll).
- * Variants with the same lanes agree bitwise in FP32 (to FMA-contraction
- * round-off in FP64); different lanes agree to summation-order round-off.
+ * gathers in flight hide memory latency at the cost of registers.  Staged
+ * variants run one 1024-thread workgroup per chunk of 1024 / lanes atoms of
+ * one walker, which first copies the walker's packed positions into LDS, so
+ * the neighbour gathers read LDS instead of L2 (possible when the largest
+ * walker's 32 bytes per atom fit one workgroup's LDS; MI355X: 160 KiB, about
+ * 4900 atoms).  akmc_batch_set_variant and akmc_batch_set_staging fix the
+ * choice (default 1, 1, unstaged; the graph of a running search is
+ * recaptured); akmc_batch_staging reports the current mode and whether
+ * staging is possible (AKMC_EUNSUPPORTED from set_staging otherwise).
+ * akmc_batch_autotune times nrep force evaluations at the resident positions
+ * for each variant, ms[16*s + 4*l + u] for staged s (0, 1), lanes (1, 2, 4,
+ * 8)[l] and unroll (1, 2, 4, 8)[u] (-1 where staging is not possible), keeps
+ * the fastest and returns it in best[0] (lanes), best[1] (unroll), best[2]
+ * (staged).  Variants with the same lanes agree bitwise in FP32 (to
+ * FMA-contraction round-off in FP64), staged or not; different lanes agree
+ * to summation-order round-off.
  * No reference counterpart: the reference evaluates one configuration per
  * call on the host (potentials/metal/Field.cpp:293-490). */
-#define AKMC_NVARIANTS 16
+#define AKMC_NVARIANTS 32
 int akmc_batch_set_variant(akmc_batch_t b, int lanes, int unroll);
-int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[2], double ms[AKMC_NVARIANTS]);
+int akmc_batch_set_staging(akmc_batch_t b, int on);
+int akmc_batch_staging(akmc_batch_t b, int *on, int *possible);
+int akmc_batch_autotune(akmc_batch_t b, int nrep, int best[3], double ms[AKMC_NVARIANTS]);
 int akmc_batch_total_atoms(akmc_batch_t b);
 /* neighbour-list rebuild count since creation (diagnostics) */
 long long akmc_batch_rebuilds(akmc_batch_t b);

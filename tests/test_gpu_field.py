@@ -190,13 +190,32 @@ def test_kernel_variants_and_autotune(lib, oracle, fp64):
                     assert np.abs(got[(lanes, 1)][w] - got[(lanes, unroll)][w]).max() < 1e-12
                 else:
                     assert np.array_equal(got[(lanes, 1)][w], got[(lanes, unroll)][w])
+    # LDS-staged variants: the same per-lane summation order, so bitwise equal
+    # to the unstaged ones in FP32 (to FMA contraction in FP64)
+    assert b.staging() == (False, True)
+    b.set_staging(True)
+    for lanes in (1, 2, 4, 8):
+        for unroll in (1, 4):
+            b.set_variant(lanes, unroll)
+            b.forces()
+            _, frc = b.download()
+            for w in range(len(cells)):
+                if fp64:
+                    assert np.abs(frc[w] - got[(lanes, unroll)][w]).max() < 1e-12, (lanes, unroll, w)
+                else:
+                    assert np.array_equal(frc[w], got[(lanes, unroll)][w]), (lanes, unroll, w)
+    b.set_staging(False)
     best, ms = b.autotune(nrep=5)
     print(f"autotune: best {best}, ms per evaluation {ms}", flush=True)
-    assert best in got
+    assert best[:2] in got and len(ms) == 32
+    assert b.staging()[0] == bool(best[2])
     b.forces()
     _, frc = b.download()
     for w in range(len(cells)):
-        assert np.array_equal(frc[w], got[best][w])
+        if fp64 and best[2]:
+            assert np.abs(frc[w] - got[best[:2]][w]).max() < 1e-12
+        else:
+            assert np.array_equal(frc[w], got[best[:2]][w])
     with pytest.raises(Exception):
         b.set_variant(3, 1)
     with pytest.raises(Exception):
@@ -216,6 +235,57 @@ def test_sc_non_integer_exponents(lib, oracle, fp64):
     got = f.calculate_forces(c.pos, c.types, c.lat)
     tol = (1e-9, 1e-11) if fp64 else (F_TOL, E_TOL_ATOM)
     _compare(f"SC n=8.7 m=5.6 fp{64 if fp64 else 32}", ref, got, c.n, *tol)
+
+
+_TWO_SPECIES = [("A", "A", 0.0124, 3.61, 9.0, 6.0, 39.4), ("B", "B", 0.0158, 3.89, 10.0, 8.0, 34.4),
+                ("A", "B", 0.0140, 3.75, 9.5, 7.0, 36.8)]
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("case", ["two_species", "noimage", "non_integer"])
+def test_lds_staged_kernels(lib, oracle, case, fp64):
+    """The LDS-staged Sutton-Chen kernels (a walker's gather records copied
+    to LDS by its workgroups) on the multi-species, explicit-image and
+    non-integer-exponent paths: the oracle's forces and energies, and the
+    unstaged kernels' results bitwise in FP32 (to FMA contraction in FP64)."""
+    if case == "two_species":
+        cells = [W.sc_cell(9, 9, 9, n_vac=3, jitter=0.05, seed=7 + k, two_species=True, a=3.7) for k in range(2)]
+        text = lib.sc_potentials_text(_TWO_SPECIES, 7.3, species=["A", "B"])
+        of = oracle.sc_field(["A", "B"], _TWO_SPECIES, 7.3)
+    elif case == "noimage":
+        cells = [W.sc_cell(3, 3, 3, n_vac=1, jitter=0.02, seed=3 + k) for k in range(3)]
+        text = lib.sc_potentials_text(_sc_pots(), 7.0, noimage=True)
+        of = oracle.sc_field(["Cu"], _sc_pots(), 7.0, min_image=False)
+    else:
+        pots = [("Cu", "Cu", CU["eps"], CU["a"], 8.7, 5.6, CU["c"])]
+        cells = W.batched_sc_cells(n_walkers=3, seed=21)
+        text = lib.sc_potentials_text(pots, 2 * CU["a"])
+        of = oracle.sc_field(["Cu"], pots, 2 * CU["a"])
+    f = lib.Field(text)
+    if fp64:
+        f.set_precision(64)
+    b = lib.Batch(f, cells)
+    assert b.staging() == (False, True)
+    ftol, etol = (1e-9, 1e-11) if fp64 else (F_TOL, E_TOL_ATOM)
+    for lanes, unroll in ((1, 1), (4, 4), (8, 2)):
+        b.set_variant(lanes, unroll)
+        b.set_staging(False)
+        e0 = b.forces()
+        _, f0 = b.download()
+        b.set_staging(True)
+        e1 = b.forces()
+        _, f1 = b.download()
+        for w, c in enumerate(cells):
+            ref = oracle.sc_forces(of, c.pos, c.types, c.lat)
+            df = np.abs(f1[w] - ref["force"]).max()
+            de = abs(e1[w]["total"] - ref["energy"]) / c.n
+            print(f"staged {case} {lanes}x{unroll} fp{64 if fp64 else 32} walker {w}: max|dF| {df:.2e}, "
+                  f"|dE|/N {de:.2e}", flush=True)
+            assert df < ftol and de < etol
+            if fp64:
+                assert np.abs(f1[w] - f0[w]).max() < 1e-12
+            else:
+                assert np.array_equal(f1[w], f0[w]) and e1[w]["total"] == e0[w]["total"]
 
 
 _FORMS = {
