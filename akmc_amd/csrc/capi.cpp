@@ -768,7 +768,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
             b->chunk_tab[li] = dc;
             b->chunk_n[li] = (int)ch.size();
         }
-        d.chunk = b->chunk_tab[0];
+        d.chunk = b->chunk_tab[0];  // lanes 1
         d.nchunk = b->chunk_n[0];
     }
     {

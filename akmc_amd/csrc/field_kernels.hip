@@ -1633,6 +1633,18 @@ static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress,
     }
 }
 
+// The FP64 pair kernels of a mixed launch (prec 3) serve the few walkers
+// under FP64 verification: they run unstaged, whose 256-thread workgroups of
+// 256 / lpa atoms spread those walkers over more CUs than staged chunks
+// would.  An FP64-only launch serves every walker and keeps the batch's
+// staging.
+static BatchDev fp64_view(const BatchDev &b, int prec)
+{
+    BatchDev v = b;
+    if (prec & 1) v.staged = 0;
+    return v;
+}
+
 // one phase of a force evaluation: 0 lists, 1 pass 1 (SC density / ionic
 // real space), 2 pass 2 (SC force / Ewald reciprocal), 3 walker reduction.
 // prec: bit 0 launches the FP32 kernels, bit 1 the FP64 ones; each serves
@@ -1646,7 +1658,7 @@ void akmc_launch_phase(const BatchDev &b, const FieldDev &hf, int prec, int want
         break;
     case 1:
         if (prec & 1) launch_pass1<float>(b, hf, want_stress, s);
-        if (prec & 2) launch_pass1<double>(b, hf, want_stress, s);
+        if (prec & 2) launch_pass1<double>(fp64_view(b, prec), hf, want_stress, s);
         break;
     case 2:
         if (hf.kind == 1) {
@@ -1654,7 +1666,7 @@ void akmc_launch_phase(const BatchDev &b, const FieldDev &hf, int prec, int want
             if (prec) launch_pass2<double>(b, hf, want_stress, s);
         } else {
             if (prec & 1) launch_pass2<float>(b, hf, want_stress, s);
-            if (prec & 2) launch_pass2<double>(b, hf, want_stress, s);
+            if (prec & 2) launch_pass2<double>(fp64_view(b, prec), hf, want_stress, s);
         }
         break;
     default:

@@ -138,7 +138,7 @@ class KineticMonteCarlo:
     def __init__(self, potentials_text, systems, walkers_per_system=8, params: KMCParams | None = None,
                  method="dimer", search_params=None, relax=None, region_radius=6.0, d0=0.1, seed=0,
                  fmax_verify=0.05, barrier_tol=0.01, verify_nl=16, verify_delta=0.01, skin=None, sync_every=25,
-                 output_dir=None, write_events=False):
+                 output_dir=None, write_events=False, autotune=True):
         if method not in ("dimer", "art"):
             raise ValueError("the kmc method is not recognised")  # KineticMonteCarlo.cpp:1034-1046
         self.p = params or KMCParams()
@@ -149,6 +149,11 @@ class KineticMonteCarlo:
             self.field.set_skin(skin)
         cells = [c for c in systems for _ in range(self.G)]
         self.batch = L.Batch(self.field, cells)
+        if autotune:
+            # kernel variant of the Sutton-Chen pair kernels timed on the seeded cells
+            if self.field.kind == 0:
+                self.batch.forces()
+                self.batch.autotune(5)
         self.W = len(cells)
         self.A = self.batch.A
         self.off = self.batch.offsets

@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on `bench.py --forces-only 5`,
 # calibrated on a 1 GiB device copy (FETCH_SIZE reads half the copied bytes,
 # WRITE_SIZE all of them); the summary is under profiles/
-TRAFFIC_BYTES_PER_PAIR = 3.288
+TRAFFIC_BYTES_PER_PAIR = 3.281
 
 
 def ordered_pairs(cells, positions, rc):
@@ -266,7 +266,7 @@ def main():
     ap.add_argument("--walkers", type=int, default=512)
     ap.add_argument("--roofline-window", type=int, default=100,
                     help="micro-steps of the eagerly launched window that times the dominant kernel")
-    ap.add_argument("--plain", type=int, default=12, help="plain GPU dimer searches")
+    ap.add_argument("--plain", type=int, default=32, help="plain GPU dimer searches")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skin", type=float, default=None, help="Verlet skin of the device lists (A)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal section")
@@ -420,6 +420,10 @@ def main():
         "distinct": tot.distinct, "max_barrier_error_eV": tot.max_barrier_err,
         "verification_failures": tot.failures(),
         "plain_gpu_dimer_saddles_per_s": round(plain_rate, 3),
+        "plain_gpu_dimer": {"searches": args.plain, "verified": n_ok, "seconds": round(t_plain, 3),
+                            "force_calls_per_search": round(plain_calls / max(1, args.plain), 1),
+                            "ms_per_force_call": round(1e3 * t_plain / max(1, plain_calls), 4)},
+        "batched_force_calls_per_verified": round(tot.force_calls / max(1, tot.verified), 1),
         "batched_over_plain": round(verified_rate / plain_rate, 2) if plain_rate > 0 else None,
         "force_evals_per_s_batched_cells": round(batched_force_evals, 1),
         "force_eval_ms_all_walkers": round(eval_ms, 4),

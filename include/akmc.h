@@ -124,7 +124,9 @@ int akmc_batch_time_phases(akmc_batch_t b, int nrep, double ms[4]);
  * the fastest and returns it in best[0] (lanes), best[1] (unroll), best[2]
  * (staged).  Variants with the same lanes agree bitwise in FP32 (to
  * FMA-contraction round-off in FP64), staged or not; different lanes agree
- * to summation-order round-off.
+ * to summation-order round-off.  In a micro-step that mixes FP32 walkers
+ * with walkers under FP64 verification, the FP64 kernels run unstaged (they
+ * serve few walkers, which finer workgroups spread over more CUs).
  * No reference counterpart: the reference evaluates one configuration per
  * call on the host (potentials/metal/Field.cpp:293-490). */
 #define AKMC_NVARIANTS 32
