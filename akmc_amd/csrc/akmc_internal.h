@@ -9,6 +9,9 @@
 // image shift (+7) along a, b, c.  Minimum-image lists store shift 0.
 #define AKMC_NB_JMASK 0xFFFFFu
 #define AKMC_NB_ZERO_SHIFT (0x777u << 20)
+// spare 32-bit rows after a batch's neighbour lists (over-read by the
+// unconditional entry prefetch of the 16-bit lists)
+#define AKMC_NB_PAD_ROWS 64
 #define AKMC_MAX_WALKER_ATOMS (1 << 20)
 // workgroup size of the LDS-staged Sutton-Chen kernels (1024 / lpa atoms each)
 #define AKMC_STAGE_BLOCK 1024

@@ -698,7 +698,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     TRY(dalloc(b, &d.fx, A));
     TRY(dalloc(b, &d.fy, A));
     TRY(dalloc(b, &d.fz, A));
-    TRY(dalloc(b, &d.nbr, (size_t)maxnb * A));
+    TRY(dalloc(b, &d.nbr, (size_t)(maxnb + AKMC_NB_PAD_ROWS) * A));
     TRY(dalloc(b, &d.nnb, A));
     TRY(dalloc(b, &d.nnear, A));
     TRY(dalloc(b, &d.rebuild, W));

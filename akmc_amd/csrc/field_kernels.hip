@@ -301,6 +301,19 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
     }
 }
 
+// The entry after atom i's list (row n <= maxnb; the buffer holds
+// AKMC_NB_PAD_ROWS spare rows) is i's own "no neighbour" entry: local index
+// i, zero image shift.  The pair kernels' entry prefetch clamps its row to
+// the list length and reads this entry for every slot past the end, whose
+// separation 0 the r^2 floor rejects.
+__device__ inline void put_sentinel(const BatchDev &b, int i, int n, int off)
+{
+    const size_t o = (size_t)n * b.A + i;
+    const uint32_t e = uint32_t(i - off) | AKMC_NB_ZERO_SHIFT;
+    if (b.nb16) reinterpret_cast<uint16_t *>(b.nbr)[o] = uint16_t(e & 0xFFFFu);
+    else b.nbr[o] = e;
+}
+
 __device__ inline void push_nbr(const BatchDev &b, int i, int &cnt, uint32_t e)
 {
     if (cnt < b.maxnb) {
@@ -436,8 +449,10 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
         if (pass == 0) nnear = cnt;
     }
     if (have && lane == 0) {
-        b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
+        const int n = cnt < b.maxnb ? cnt : b.maxnb;
+        b.nnb[i] = n;
         b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
+        put_sentinel(b, i, n, W.off);
     }
 }
 
@@ -509,8 +524,10 @@ __global__ void k_nbr_build_images(BatchDev b)
         }
         if (pass == 0) nnear = cnt;
     }
-    b.nnb[i] = cnt < b.maxnb ? cnt : b.maxnb;
+    const int n = cnt < b.maxnb ? cnt : b.maxnb;
+    b.nnb[i] = n;
     b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
+    put_sentinel(b, i, n, W.off);
 }
 
 __global__ void k_rebuild_done(BatchDev b, unsigned long long *count)
@@ -554,13 +571,6 @@ __device__ inline void stage_sc_table(const FieldDev &F, SCPairDev *tab)
     __syncthreads();
 }
 
-// List entry that stands for "no neighbour": atom i itself with no image
-// shift.  Its separation is exactly 0, which the r^2 > floor test rejects,
-// so padding slots of a U-wide batch of gathers need no other branch.
-__device__ inline uint32_t self_entry(int i, const WalkerDev &W)
-{
-    return uint32_t(i - W.off) | (0x777u << 20);
-}
 
 // The neighbour loops below hold U list entries, fetch the next U entries
 // while gathering the current U neighbours' positions (and densities), then
@@ -571,42 +581,24 @@ __device__ inline uint32_t self_entry(int i, const WalkerDev &W)
 // agree bitwise; FP64 ones to the last bit of FMA contraction); LPA variants
 // differ by summation order.
 
-// list entries k, k + LPA, ..., k + (U-1) LPA of atom i ("none" past the end).
-// NB16: the batch is known to store 16-bit minimum-image entries, read
-// without the run-time width test (their image bits are never used).
+// list entries k, k + LPA, ..., k + (U-1) LPA of atom i.  A slot past the
+// end reads row cnt, the list's sentinel (i's own entry, rejected by the
+// r^2 floor), so the loads are unconditional: no branch separates them from
+// the batch they prefetch for, and the wait before a batch's first use
+// counts only the loads issued before it.  NB16: the batch is known to store
+// 16-bit minimum-image entries, read without the run-time width test (their
+// image bits are never used).
 template <int LPA, int U, bool NB16>
-__device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, uint32_t none, uint32_t (&e)[U])
+__device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, uint32_t (&e)[U])
 {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int kk = k + u * LPA;
-        if constexpr (NB16)
-            e[u] = kk < cnt ? uint32_t(reinterpret_cast<const uint16_t *>(b.nbr)[(size_t)kk * b.A + i]) : none;
-        else
-            e[u] = kk < cnt ? nbr_entry(b, kk, i) : none;
+        const int row = kk < cnt ? kk : cnt;
+        if constexpr (NB16) e[u] = reinterpret_cast<const uint16_t *>(b.nbr)[(size_t)row * b.A + i];
+        else e[u] = nbr_entry(b, row, i);
     }
 }
-
-// The 16-bit entries of one lane walked by pointer: entry k0 + u LPA sits
-// u * (LPA A) elements past the lane's cursor, which advances by U LPA A
-// per batch (two 64-bit adds instead of a 64-bit multiply-add per entry)
-template <int LPA, int U>
-struct Entries16 {
-    const uint16_t *at;
-    size_t step;  // LPA * A elements
-    int k, cnt;
-    uint32_t none;
-    __device__ Entries16(const BatchDev &b, int i, int sub, int cnt_, uint32_t none_)
-        : at(reinterpret_cast<const uint16_t *>(b.nbr) + (size_t)sub * b.A + i), step((size_t)LPA * b.A), k(sub),
-          cnt(cnt_), none(none_) {}
-    __device__ void next(uint32_t (&e)[U])
-    {
-#pragma unroll
-        for (int u = 0; u < U; ++u) e[u] = k + u * LPA < cnt ? uint32_t(at[u * step]) : none;
-        at += U * step;
-        k += U * LPA;
-    }
-};
 
 // A neighbour's packed gather record, whole (W4) or its position only
 // (24 bytes: one 16-byte and one 8-byte read, w = 0)
@@ -689,23 +681,19 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
     const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
-    const uint32_t none = self_entry(i, W);
     double rho = 0.0;
     uint32_t en[U];
-    Entries16<LPA, U> cur(b, i, sub, cnt, none);
-    if constexpr (NB16) cur.next(en);
-    else fetch_entries<LPA, U, false>(b, i, sub, cnt, none, en);
+    fetch_entries<LPA, U, NB16>(b, i, sub, cnt, en);
     for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
         uint32_t e[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) e[u] = en[u];
         // next batch in flight
-        if constexpr (NB16) cur.next(en);
-        else fetch_entries<LPA, U, false>(b, i, k0 + LPA * U, cnt, none, en);
+        fetch_entries<LPA, U, NB16>(b, i, k0 + LPA * U, cnt, en);
         double4 pj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int jl = int(e[u] & AKMC_NB_JMASK);
+            const int jl = NB16 ? int(e[u]) : int(e[u] & AKMC_NB_JMASK);
             pj[u] = STAGED ? gather_rec<!ONE>(stage, jl) : gather_rec<!ONE>(b.p4, W.off + jl);
         }
         double r2[U];
@@ -776,26 +764,22 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
     const int cnt = b.nnb[i];
     const float isi = sizeof(T) == 8 ? 0.0f : b.inv_sqrt_rho[i];
     const double isi64 = sizeof(T) == 8 ? b.isr64[i] : 0.0;
-    const uint32_t none = self_entry(i, W);
     double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
     uint32_t en[U];
-    Entries16<LPA, U> cur(b, i, sub, cnt, none);
-    if constexpr (NB16) cur.next(en);
-    else fetch_entries<LPA, U, false>(b, i, sub, cnt, none, en);
+    fetch_entries<LPA, U, NB16>(b, i, sub, cnt, en);
     for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
         uint32_t e[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) e[u] = en[u];
         // next batch in flight
-        if constexpr (NB16) cur.next(en);
-        else fetch_entries<LPA, U, false>(b, i, k0 + LPA * U, cnt, none, en);
+        fetch_entries<LPA, U, NB16>(b, i, k0 + LPA * U, cnt, en);
         double4 pj[U];
         float isj[U];
         double isj64[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int jl = int(e[u] & AKMC_NB_JMASK);
+            const int jl = NB16 ? int(e[u]) : int(e[u] & AKMC_NB_JMASK);
             const int j = W.off + jl;
             pj[u] = STAGED ? stage[jl] : b.p4[j];
             if (ONE) {  // 1/sqrt(rho_j) rides in the gather record (pass 1)
