@@ -91,10 +91,48 @@ __device__ inline T pow_int(T u2, int k)
 __device__ inline double root(double x) { return sqrt(x); }
 __device__ inline float root(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// pow_int for U values at once: the uniform loop control runs once per
-// batch of pairs instead of once per pair (same operations per value)
+// 1/sqrt(x) in FP64 for normal x > 0: the hardware estimate refined by two
+// Newton steps y += y (1 - x y^2) / 2 (quadratic convergence), a few ulp
+// from the correctly rounded value at a third of the cost of a division and
+// a square root
+__device__ inline double rsqrt_fp64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double r = fma(-(x * y), y, 1.0);
+    y = fma(0.5 * y, r, y);
+    r = fma(-(x * y), y, 1.0);
+    return fma(0.5 * y, r, y);
+}
+
+// The single-species power inputs of U pairs: ir2 = 1/r^2, u2 = (a/r)^2 and,
+// in FP64, rt = a/r.  FP32: hardware reciprocal (the power takes sqrt(u2)
+// from the hardware square root for odd exponents).  FP64: one refined 1/r,
+// so that neither a division nor a square root is needed.
 template <typename T, int U>
-__device__ inline void pow_int_batch(const T (&u2)[U], int k, T (&r)[U])
+__device__ inline void sc_powers_in(const double (&r2)[U], T a, T (&ir2)[U], T (&u2)[U], T (&rt)[U])
+{
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double ir = rsqrt_fp64(r2[u]);
+            ir2[u] = ir * ir;
+            u2[u] = a * a * ir2[u];
+            rt[u] = a * ir;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ir2[u] = recip<T>(T(r2[u]));
+            u2[u] = a * a * ir2[u];
+            rt[u] = T(0);  // the FP32 power takes its root from u2 itself
+        }
+    }
+}
+
+// u2^(k/2) for U values at once, sqrt(u2) given for odd k: the uniform loop
+// control runs once per batch of pairs instead of once per pair
+template <typename T, int U>
+__device__ inline void pow_int_batch_rt(const T (&u2)[U], const T (&rt)[U], int k, T (&r)[U])
 {
     T base[U];
 #pragma unroll
@@ -112,7 +150,7 @@ __device__ inline void pow_int_batch(const T (&u2)[U], int k, T (&r)[U])
     }
     if (k & 1) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) r[u] *= root(u2[u]);
+        for (int u = 0; u < U; ++u) r[u] *= sizeof(T) == 8 ? rt[u] : root(u2[u]);
     }
 }
 
@@ -708,10 +746,9 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
         if constexpr (ONE) {
             // every slot is evaluated (padding gives r2 = 0 and infinities),
             // only the pairs inside the cutoff are summed
-            T u2[U], um[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) u2[u] = c1.a * c1.a * recip<T>(T(r2[u]));
-            pow_int_batch<T, U>(u2, m1, um);
+            T ir2[U], u2[U], rt[U], um[U];
+            sc_powers_in<T, U>(r2, c1.a, ir2, u2, rt);
+            pow_int_batch_rt<T, U>(u2, rt, m1, um);
 #pragma unroll
             for (int u = 0; u < U; ++u) rho += double(in[u] ? c1.rho_coef * um[u] : T(0));  // + exact zeros
         } else {
@@ -805,14 +842,10 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
         if constexpr (ONE) {
             // every slot is evaluated (padding gives r2 = 0 and infinities),
             // only the pairs inside the cutoff are summed
-            T u2[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                ir2[u] = recip<T>(T(r2[u]));
-                u2[u] = c1.a * c1.a * ir2[u];
-            }
-            pow_int_batch<T, U>(u2, n1, un);
-            pow_int_batch<T, U>(u2, m1, um);
+            T u2[U], rt[U];
+            sc_powers_in<T, U>(r2, c1.a, ir2, u2, rt);
+            pow_int_batch_rt<T, U>(u2, rt, n1, un);
+            pow_int_batch_rt<T, U>(u2, rt, m1, um);
         }
         if constexpr (ONE && !STRESS) {
             // branch-free: a pair outside the cutoff (or a padding slot)
