@@ -1651,14 +1651,19 @@ static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress,
 }
 
 // The FP64 pair kernels of a mixed launch (prec 3) serve the few walkers
-// under FP64 verification: they run unstaged, whose 256-thread workgroups of
-// 256 / lpa atoms spread those walkers over more CUs than staged chunks
-// would.  An FP64-only launch serves every walker and keeps the batch's
-// staging.
+// under FP64 verification: they run unstaged at 4 lanes per atom and 4
+// gathers in flight, whose 256-thread workgroups of 64 atoms spread those
+// walkers over more CUs than staged chunks would (the batch's own variant is
+// tuned for the FP32 pass over every walker).  An FP64-only launch serves
+// every walker and keeps the batch's variant and staging.
 static BatchDev fp64_view(const BatchDev &b, int prec)
 {
     BatchDev v = b;
-    if (prec & 1) v.staged = 0;
+    if (prec & 1) {
+        v.staged = 0;
+        v.lpa = 4;  // measured best for this pass on the batched dimer workload
+        v.unroll = 4;
+    }
     return v;
 }
 
