@@ -1166,6 +1166,7 @@ int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const dou
 }
 
 int copy_walkers_out(akmc_batch_t b, const double *src, int nw, const int *walkers, double *out);
+int copy_walkers_out3(akmc_batch_t b, const double *const src[3], int nw, const int *walkers, double *out);
 
 int capture_graph(akmc_batch_t b, int steps)
 {
@@ -1757,18 +1758,28 @@ int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out)
 
 int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z)
 {
-    if (!b || !b->search_alloc) return fail(AKMC_EINVAL, "no search state");
+    if (!b) return fail(AKMC_EINVAL, "null batch");
     size_t A = b->A;
-    if (x) HIPCHK(hipMemcpy(x, b->s.X, A * sizeof(double), hipMemcpyDeviceToHost));
-    if (y) HIPCHK(hipMemcpy(y, b->s.X + A, A * sizeof(double), hipMemcpyDeviceToHost));
-    if (z) HIPCHK(hipMemcpy(z, b->s.X + 2 * A, A * sizeof(double), hipMemcpyDeviceToHost));
+    // before the first search or relax the batch's own positions are current
+    const double *src[3] = {b->d.x, b->d.y, b->d.z};
+    if (b->search_alloc) {
+        src[0] = b->s.X;
+        src[1] = b->s.X + A;
+        src[2] = b->s.X + 2 * A;
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (x) HIPCHK(hipMemcpy(x, src[0], A * sizeof(double), hipMemcpyDeviceToHost));
+    if (y) HIPCHK(hipMemcpy(y, src[1], A * sizeof(double), hipMemcpyDeviceToHost));
+    if (z) HIPCHK(hipMemcpy(z, src[2], A * sizeof(double), hipMemcpyDeviceToHost));
     return AKMC_SUCCESS;
 }
 
 int akmc_batch_get_walkers_positions(akmc_batch_t b, int nw, const int *walkers, double *out)
 {
-    if (!b || !b->search_alloc) return fail(AKMC_EINVAL, "no search state");
-    return copy_walkers_out(b, b->s.X, nw, walkers, out);
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    if (b->search_alloc) return copy_walkers_out(b, b->s.X, nw, walkers, out);
+    const double *const src[3] = {b->d.x, b->d.y, b->d.z};
+    return copy_walkers_out3(b, src, nw, walkers, out);
 }
 
 int akmc_batch_get_walkers_saddles(akmc_batch_t b, int nw, const int *walkers, double *out)
@@ -1785,6 +1796,14 @@ namespace {
 // transfer
 int copy_walkers_out(akmc_batch_t b, const double *src, int nw, const int *walkers, double *out)
 {
+    const double *const c3[3] = {src, src + b->A, src + 2 * (size_t)b->A};
+    return copy_walkers_out3(b, c3, nw, walkers, out);
+}
+
+// the flagged walkers' (3, n_w) slices of three [A] component arrays, packed
+// walker after walker, through the pinned staging buffer
+int copy_walkers_out3(akmc_batch_t b, const double *const src[3], int nw, const int *walkers, double *out)
+{
     if (nw < 0 || (nw && (!walkers || !out))) return fail(AKMC_EINVAL, "bad arguments");
     size_t total = 0;
     for (int k = 0; k < nw; ++k) {
@@ -1799,11 +1818,10 @@ int copy_walkers_out(akmc_batch_t b, const double *src, int nw, const int *walke
         b->pos_pin_n = total;
     }
     size_t o = 0;
-    const size_t A = b->A;
     for (int k = 0; k < nw; ++k) {
         const WalkerDev &h = b->hw[walkers[k]];
         for (int c = 0; c < 3; ++c)
-            HIPCHK(hipMemcpyAsync(b->pos_pin + o + (size_t)c * h.n, src + c * A + h.off, h.n * sizeof(double),
+            HIPCHK(hipMemcpyAsync(b->pos_pin + o + (size_t)c * h.n, src[c] + h.off, h.n * sizeof(double),
                                   hipMemcpyDeviceToHost, b->stream));
         o += 3 * (size_t)h.n;
     }
@@ -1818,12 +1836,14 @@ extern "C" {
 
 int akmc_batch_get_walker_positions(akmc_batch_t b, int walker, double *x, double *y, double *z)
 {
-    if (!b || !b->search_alloc || walker < 0 || walker >= b->W) return fail(AKMC_EINVAL, "bad arguments");
+    if (!b || walker < 0 || walker >= b->W) return fail(AKMC_EINVAL, "bad arguments");
     size_t A = b->A, off = b->hw[walker].off, n = b->hw[walker].n;
     double *dst[3] = {x, y, z};
+    const double *src[3] = {b->d.x, b->d.y, b->d.z};  // before the first search or relax
+    if (b->search_alloc)
+        for (int c = 0; c < 3; ++c) src[c] = b->s.X + c * A;
     for (int c = 0; c < 3; ++c)
-        if (dst[c]) HIPCHK(hipMemcpyAsync(dst[c], b->s.X + c * A + off, n * sizeof(double), hipMemcpyDeviceToHost,
-                                          b->stream));
+        if (dst[c]) HIPCHK(hipMemcpyAsync(dst[c], src[c] + off, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return AKMC_SUCCESS;
 }

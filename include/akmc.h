@@ -264,7 +264,9 @@ int akmc_batch_kernel_time(akmc_batch_t b, double *ms, long long *launches);
  * consumed since the search state was allocated (counts[W]) */
 int akmc_batch_eval_counts(akmc_batch_t b, long long *counts);
 int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out);
-/* current search positions (the dimer centre / relaxed positions) */
+/* current search positions (the dimer centre / relaxed positions); before
+ * the batch's first search or relax, the positions it was created or last
+ * loaded with (akmc_batch_get_walker(s)_positions alike) */
 int akmc_batch_get_positions(akmc_batch_t b, double *x, double *y, double *z);
 int akmc_batch_get_mode(akmc_batch_t b, double *mode);
 /* search positions of one walker (x, y, z of its n atoms) */

@@ -86,6 +86,27 @@ def test_dimer_matches_oracle(lib, oracle, c1):
         assert st["curvature"] < 0
 
 
+def test_positions_before_first_search(lib, c1):
+    """A fresh batch reports the positions it was created with (every
+    position getter), and a dimer started from a displacement on it (no
+    frozen mask set first) starts at those positions plus the displacement."""
+    c, of, h, r = c1
+    basin = r["pos"]
+    f = lib.Field(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
+    cells = [W.Cell(basin, c.types, c.lat), W.Cell(basin + 0.01, c.types, c.lat)]
+    b = lib.Batch(f, cells)
+    got = b.positions()
+    assert all(np.array_equal(got[w], cells[w].pos) for w in range(2))
+    assert np.array_equal(b.walkers_positions([1])[1], cells[1].pos)
+    assert np.array_equal(b.walker_positions(0), cells[0].pos)
+    frozen, disp, mode = _dimer_start(c, basin, 0)
+    b.start_dimer([mode, mode], [disp, disp], lib.dimer_defaults(maxParIter=5))
+    got = b.positions()
+    for w in range(2):
+        assert np.abs(got[w] - (cells[w].pos + disp)).max() < 1e-12
+    assert b.run(50, sync_every=10) > 0
+
+
 def test_batched_walkers_equal_plain(lib, oracle, c1):
     """The same walker in a batch of 3 (with two different cells beside it)
     and alone: identical trajectories bit for bit."""
