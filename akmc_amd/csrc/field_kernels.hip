@@ -324,18 +324,48 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
         __syncthreads();
     }
     // queue this walker's list-build items: home chunks of kNbSlots atoms per
-    // cell (chunk indices above 255 are served by the item of chunk % 256)
+    // cell (chunk indices above 255 are served by the item of chunk % 256),
+    // in cell order.  After the placement fill[c] is the end of cell c, so a
+    // cell's count is fill[c] minus the end of cell c - 1; the items per
+    // cell are scanned across the workgroup and the walker takes its queue
+    // range with one atomic.
+    __shared__ int qbase;
+    auto items_of = [&](int c) {
+        const int n = fill[c] - (c == 0 ? W.off : fill[c - 1]);
+        const int k = (n + kNbSlotsQ - 1) / kNbSlotsQ;
+        return k < 256 ? k : 256;
+    };
+    int mine = 0;
+    for (int c = tid; c < nc; c += kBlock) mine += items_of(c);
+    chunk[tid] = mine;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (tid < o) chunk[tid] += chunk[tid + o];
+        __syncthreads();
+    }
     if (tid == 0) {
-        int items = 0;
-        for (int c = 0; c < nc; ++c) {
-            const int k = (b.cell_count[W.cell_off + c] + kNbSlotsQ - 1) / kNbSlotsQ;
-            items += k < 256 ? k : 256;
+        qbase = atomicAdd(b.cell_work_n, chunk[0]);
+        carry = 0;
+    }
+    __syncthreads();
+    for (int base = 0; base < nc; base += kBlock) {
+        const int c = base + tid;
+        const int v = c < nc ? items_of(c) : 0;
+        chunk[tid] = v;
+        __syncthreads();
+        for (int o = 1; o < kBlock; o <<= 1) {
+            const int t = tid >= o ? chunk[tid - o] : 0;
+            __syncthreads();
+            chunk[tid] += t;
+            __syncthreads();
         }
-        int at = atomicAdd(b.cell_work_n, items);
-        for (int c = 0; c < nc; ++c) {
-            const int k = (b.cell_count[W.cell_off + c] + kNbSlotsQ - 1) / kNbSlotsQ;
-            for (int ch = 0; ch < (k < 256 ? k : 256); ++ch) b.cell_work[at++] = ((W.cell_off + c) << 8) | ch;
+        if (c < nc) {
+            const int at = qbase + carry + chunk[tid] - v;
+            for (int ch = 0; ch < v; ++ch) b.cell_work[at + ch] = ((W.cell_off + c) << 8) | ch;
         }
+        __syncthreads();
+        if (tid == kBlock - 1) carry += chunk[tid];
+        __syncthreads();
     }
 }
 
