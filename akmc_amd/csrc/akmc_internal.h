@@ -135,9 +135,7 @@ struct BatchDev {
     double *fx, *fy, *fz;
     uint32_t *nbr;        // [maxnb][A] column-major
     int *nnb;
-    int *nnear;           // leading list entries inside the near shell (ionic
-                          // fields) / whose minimum image is the raw
-                          // displacement (many-body minimum-image lists)
+    int *nnear;           // leading list entries inside the near shell
     int *rebuild;         // per walker flag
     int *overflow;        // sticky capacity flag
     int *cell_count, *cell_start, *cell_atoms;

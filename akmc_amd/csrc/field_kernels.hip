@@ -459,14 +459,6 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
     const int ncand = c_pref[n_cells];
     const double rl2 = b.field->rlist2;
     const double rn2 = b.field->rnear2;
-    // many-body fields: pass 0 lists the pairs whose raw displacement stays
-    // within L/2 - skin on every axis.  Positions are never wrapped on the
-    // device and move less than skin/2 each before a rebuild, so these
-    // pairs' minimum image is the raw displacement for the list's lifetime
-    // and the pair kernels can skip the image arithmetic for them.
-    const bool part_img = rn2 <= 0.0 && b.field->kind == 0;
-    const double skin = 2.0 * sqrt(b.field->half_skin2);
-    const double hx = 0.5 * W.L[0] - skin, hy = 0.5 * W.L[1] - skin, hz = 0.5 * W.L[2] - skin;
     const int s0 = b.cell_start[cg], n0 = b.cell_count[cg];
     const int lane = threadIdx.x % kNbLanes, slot = threadIdx.x / kNbLanes;
     const int wlane = threadIdx.x & 63;
@@ -476,13 +468,11 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
     const int i = have ? b.cell_atoms[s0 + q] : 0;
     const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
     int cnt = 0, nnear = 0;
-    // pass 0 lists the near shell (r^2 <= rnear2) of ionic fields or the
-    // image-free pairs of many-body ones, pass 1 the rest
-    const bool two_pass = rn2 > 0.0 || part_img;
-    for (int pass = two_pass ? 0 : 1; pass < 2; ++pass) {
+    // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
+    for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
         for (int seg = 0; seg < ncand; seg += kNbCap) {
             const int m = min(kNbCap, ncand - seg);
-            if (!(pass == 1 && m == ncand && two_pass)) {  // a single segment stays staged
+            if (!(pass == 1 && m == ncand && rn2 > 0.0)) {  // a single segment stays staged
                 __syncthreads();
                 for (int t = threadIdx.x; t < m; t += kBlock) {
                     const int qq = seg + t;
@@ -506,11 +496,10 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
                     if (have && t < m) {
                         jj[u] = sj[t];
                         double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
-                        const bool noimg = fabs(dx) <= hx && fabs(dy) <= hy && fabs(dz) <= hz;
                         min_image(dx, dy, dz, W);
                         const double r2 = dx * dx + dy * dy + dz * dz;
-                        const bool first = part_img ? noimg : r2 <= rn2;
-                        hit[u] = (jj[u] != i && r2 <= rl2 && (pass == 0 ? first : !first)) ? 1 : 0;
+                        const bool near = r2 <= rn2;
+                        hit[u] = (jj[u] != i && r2 <= rl2 && (pass == 0 ? near : !near)) ? 1 : 0;
                     }
                 }
 #pragma unroll
@@ -679,19 +668,6 @@ __device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, u
     }
 }
 
-// Whether a batch whose last entry is k_last needs the minimum image: the
-// list's leading nn entries are image-free (their minimum image is the raw
-// displacement), and the wavefront skips the image arithmetic when every
-// lane's batch lies among them.  Applying it to an image-free pair would
-// change nothing (the image shift is exactly 0), so the test only saves
-// work.  Explicit-image lists always apply their entries' shifts.
-template <bool MINIMG>
-__device__ inline bool needs_image(int k_last, int nn)
-{
-    if constexpr (!MINIMG) return true;
-    return __any(k_last >= nn);
-}
-
 // A neighbour's packed gather record, whole (W4) or its position only
 // (24 bytes: one 16-byte and one 8-byte read, w = 0)
 template <bool W4>
@@ -773,7 +749,6 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
     const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
-    const int nn = MINIMG ? b.nnear[i] : 0;  // leading image-free entries
     double rho = 0.0;
     uint32_t en[U];
     fetch_entries<LPA, U, NB16>(b, i, sub, cnt, en);
@@ -791,11 +766,10 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
         }
         double r2[U];
         bool in[U];
-        const bool img = needs_image<MINIMG>(k0 + (U - 1) * LPA, nn);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             double dx = xi - pj[u].x, dy = yi - pj[u].y, dz = zi - pj[u].z;
-            if (img) entry_disp(e[u], dx, dy, dz, W, MINIMG);
+            entry_disp(e[u], dx, dy, dz, W, MINIMG);
             r2[u] = dx * dx + dy * dy + dz * dz;
             in[u] = r2[u] <= rc2 && r2[u] > (MINIMG ? 1.0e-16 : 1.0e-3);
         }
@@ -855,7 +829,6 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
     const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
-    const int nn = MINIMG ? b.nnear[i] : 0;  // leading image-free entries
     const float isi = sizeof(T) == 8 ? 0.0f : b.inv_sqrt_rho[i];
     const double isi64 = sizeof(T) == 8 ? b.isr64[i] : 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
@@ -886,13 +859,12 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
         }
         double dx[U], dy[U], dz[U], r2[U];
         bool in[U];
-        const bool img = needs_image<MINIMG>(k0 + (U - 1) * LPA, nn);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             dx[u] = xi - pj[u].x;
             dy[u] = yi - pj[u].y;
             dz[u] = zi - pj[u].z;
-            if (img) entry_disp(e[u], dx[u], dy[u], dz[u], W, MINIMG);
+            entry_disp(e[u], dx[u], dy[u], dz[u], W, MINIMG);
             r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
             in[u] = r2[u] <= rc2 && r2[u] > (MINIMG ? 1.0e-16 : 1.0e-3);
         }
