@@ -121,9 +121,9 @@ struct BatchDev {
     int unroll;           // neighbour gathers in flight per lane (1, 2, 4, 8)
     int staged;           // Sutton-Chen pair kernels read the gather records of
                           // a walker from LDS (one workgroup per chunk of atoms)
-    int nchunk;           // workgroups of a staged launch
-    const int2 *chunk;    // staged chunk -> (walker, first local atom), for lpa
-    int stage_bytes;      // dynamic LDS of a staged launch
+    const int2 *chunk_tab[4];  // staged chunk -> (walker, first local atom) at
+    int nchunk_tab[4];         // 1, 2, 4, 8 lanes per atom (1024 / lanes atoms each)
+    int stage_bytes;           // dynamic LDS of a staged launch
     const FieldDev *field;
     const WalkerDev *walkers;
     const int *wk;        // atom -> walker

@@ -123,10 +123,8 @@ struct akmc_batch_s {
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     std::vector<WalkerOut> hout;
-    // chunk tables of the LDS-staged Sutton-Chen launches, per lanes per atom
-    // (1, 2, 4, 8); stage_ok when every walker's gather records fit the LDS
-    const int2 *chunk_tab[4] = {nullptr, nullptr, nullptr, nullptr};
-    int chunk_n[4] = {0, 0, 0, 0};
+    // LDS staging of the many-body pair kernels is possible when every
+    // walker's gather records fit one workgroup's LDS
     bool stage_ok = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long rebuild_base = 0;
@@ -755,7 +753,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         const size_t bytes = (size_t)nmax * sizeof(double4);
-        b->stage_ok = f->kind == 0 && !f->host.eam && bytes + sizeof(SCPairDev) * AKMC_MAXPAIR <= (size_t)maxlds;
+        b->stage_ok = f->kind == 0 && bytes + sizeof(SCPairDev) * AKMC_MAXPAIR <= (size_t)maxlds;
         d.stage_bytes = (int)bytes;
         for (int li = 0; li < 4; ++li) {
             const int apc = AKMC_STAGE_BLOCK >> li;  // atoms per chunk at 1, 2, 4, 8 lanes per atom
@@ -765,11 +763,9 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
             int2 *dc;
             TRY(dalloc(b, &dc, ch.size()));
             HIPCHK(hipMemcpy(dc, ch.data(), ch.size() * sizeof(int2), hipMemcpyHostToDevice));
-            b->chunk_tab[li] = dc;
-            b->chunk_n[li] = (int)ch.size();
+            d.chunk_tab[li] = dc;
+            d.nchunk_tab[li] = (int)ch.size();
         }
-        d.chunk = b->chunk_tab[0];  // lanes 1
-        d.nchunk = b->chunk_n[0];
     }
     {
         std::vector<int> hcw(ncells);
@@ -1390,9 +1386,6 @@ int akmc_batch_set_variant(akmc_batch_t b, int lanes, int unroll)
     if (b->d.lpa != lanes || b->d.unroll != unroll) drop_graph(b);  // the captured graph holds the old kernels
     b->d.lpa = lanes;
     b->d.unroll = unroll;
-    const int li = lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3;
-    b->d.chunk = b->chunk_tab[li];
-    b->d.nchunk = b->chunk_n[li];
     return AKMC_SUCCESS;
 }
 
@@ -1401,7 +1394,7 @@ int akmc_batch_set_staging(akmc_batch_t b, int on)
     if (!b) return fail(AKMC_EINVAL, "null batch");
     if (on && !b->stage_ok)
         return fail(AKMC_EUNSUPPORTED,
-                    "LDS staging needs a Sutton-Chen field whose largest walker's gather records "
+                    "LDS staging needs a many-body field whose largest walker's gather records "
                     "(%d bytes) fit one workgroup's LDS", b->d.stage_bytes);
     if (b->d.staged != (on ? 1 : 0)) drop_graph(b);
     b->d.staged = on ? 1 : 0;

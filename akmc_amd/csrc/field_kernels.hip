@@ -704,10 +704,12 @@ template <typename T, int LPA, bool STAGED>
 __device__ inline AtomSlot sc_slot(const BatchDev &b, WalkerDev &W, double4 *stage)
 {
     if constexpr (STAGED) {
-        const int per = (b.nchunk + kXCD - 1) / kXCD;
+        constexpr int li = LPA == 1 ? 0 : LPA == 2 ? 1 : LPA == 4 ? 2 : 3;
+        const int nch = b.nchunk_tab[li];
+        const int per = (nch + kXCD - 1) / kXCD;
         const int c = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
-        if (c >= b.nchunk) return {0, 0, false};
-        const int2 ch = b.chunk[c];
+        if (c >= nch) return {0, 0, false};
+        const int2 ch = b.chunk_tab[li][c];
         if (other_precision<T>(b, ch.x)) return {0, 0, false};
         W = b.walkers[ch.x];
         for (int k = threadIdx.x; k < W.n; k += blockDim.x) stage[k] = b.p4[W.off + k];
@@ -994,16 +996,15 @@ constexpr double kEamNoDeriv = -1.0e300;  // F'(rho) of an atom whose rho < tiny
 // gather, not arithmetic, so the FP32 path gains nothing from FP32 here.
 // ONE: one table for every pair; F'(rho_i) then rides in the packed gather
 // record for pass 2 (k_pack restores the species every step).
-template <typename T, bool MINIMG, int LPA, bool ONE>
-__global__ void __launch_bounds__(kBlock) k_eam_density(BatchDev b)
+template <typename T, bool MINIMG, int LPA, bool ONE, bool STAGED>
+__global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_eam_density(BatchDev b)
 {
+    extern __shared__ double4 stage[];
     const FieldDev &F = *b.field;
-    const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
-    if (t < 0) return;
-    const int i = t / LPA, sub = t % LPA;
-    if (i >= b.A) return;
-    if (other_precision<T>(b, walker_of(b, i))) return;
-    const WalkerDev W = b.walkers[walker_of(b, i)];
+    WalkerDev W;
+    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage);
+    if (!sl.ok) return;
+    const int i = sl.i, sub = sl.sub;
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
@@ -1011,7 +1012,8 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(BatchDev b)
     double rho = 0.0;
     for (int k = sub; k < cnt; k += LPA) {
         const uint32_t e = nbr_entry(b, k, i);
-        const double4 pj = b.p4[W.off + int(e & AKMC_NB_JMASK)];
+        const int jl = int(e & AKMC_NB_JMASK);
+        const double4 pj = STAGED ? stage[jl] : b.p4[W.off + jl];
         double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
         entry_disp(e, dx, dy, dz, W, MINIMG);
         const double r2 = dx * dx + dy * dy + dz * dz;
@@ -1049,16 +1051,15 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(BatchDev b)
 // (fs - fm) / r^2 over both orderings of each pair, and half of phi(r) to
 // the pair energy.  phi, phi' and rho' are those of the (t_i, t_j) table,
 // F' too, evaluated at rho_i and rho_j, as the reference does.
-template <typename T, bool MINIMG, bool STRESS, int LPA, bool ONE>
-__global__ void __launch_bounds__(kBlock) k_eam_force(BatchDev b)
+template <typename T, bool MINIMG, bool STRESS, int LPA, bool ONE, bool STAGED>
+__global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_eam_force(BatchDev b)
 {
+    extern __shared__ double4 stage[];
     const FieldDev &F = *b.field;
-    const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
-    if (t < 0) return;
-    const int i = t / LPA, sub = t % LPA;
-    if (i >= b.A) return;
-    if (other_precision<T>(b, walker_of(b, i))) return;
-    const WalkerDev W = b.walkers[walker_of(b, i)];
+    WalkerDev W;
+    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage);
+    if (!sl.ok) return;
+    const int i = sl.i, sub = sl.sub;
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
@@ -1070,8 +1071,9 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(BatchDev b)
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
     for (int k = sub; k < cnt; k += LPA) {
         const uint32_t e = nbr_entry(b, k, i);
-        const int j = W.off + int(e & AKMC_NB_JMASK);
-        const double4 pj = b.p4[j];
+        const int jl = int(e & AKMC_NB_JMASK);
+        const int j = W.off + jl;
+        const double4 pj = STAGED ? stage[jl] : b.p4[j];
         double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
         entry_disp(e, dx, dy, dz, W, MINIMG);
         const double r2 = dx * dx + dy * dy + dz * dz;
@@ -1539,7 +1541,8 @@ struct ScLaunch {
 template <int LPA>
 static ScLaunch sc_launch(const BatchDev &b)
 {
-    if (b.staged) return {xcd_grid(b.nchunk), AKMC_STAGE_BLOCK, b.stage_bytes};
+    constexpr int li = LPA == 1 ? 0 : LPA == 2 ? 1 : LPA == 4 ? 2 : 3;
+    if (b.staged) return {xcd_grid(b.nchunk_tab[li]), AKMC_STAGE_BLOCK, b.stage_bytes};
     return {xcd_grid((b.A * LPA + kBlock - 1) / kBlock), kBlock, 0};
 }
 
@@ -1608,37 +1611,51 @@ static void launch_sc_force(const BatchDev &b, const FieldDev &hf, int want_stre
 // the EAM kernels: 4 lanes per atom
 constexpr int kEamLanes = 4;
 
+// (the staged variants read the gather records from LDS like the
+// Sutton-Chen ones; the tables stay in global memory)
+template <typename T, bool ONE, bool STAGED>
+static void launch_eam_density_one(const BatchDev &b, const FieldDev &hf, hipStream_t s)
+{
+    const ScLaunch L = sc_launch<kEamLanes>(b);
+    if (hf.min_image) k_eam_density<T, true, kEamLanes, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+    else k_eam_density<T, false, kEamLanes, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+}
+
 template <typename T>
 static void launch_eam_density(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
-    int g = xcd_grid((b.A * kEamLanes + kBlock - 1) / kBlock);
     if (hf.eam_one) {
-        if (hf.min_image) k_eam_density<T, true, kEamLanes, true><<<g, kBlock, 0, s>>>(b);
-        else k_eam_density<T, false, kEamLanes, true><<<g, kBlock, 0, s>>>(b);
+        if (b.staged) launch_eam_density_one<T, true, true>(b, hf, s);
+        else launch_eam_density_one<T, true, false>(b, hf, s);
     } else {
-        if (hf.min_image) k_eam_density<T, true, kEamLanes, false><<<g, kBlock, 0, s>>>(b);
-        else k_eam_density<T, false, kEamLanes, false><<<g, kBlock, 0, s>>>(b);
+        if (b.staged) launch_eam_density_one<T, false, true>(b, hf, s);
+        else launch_eam_density_one<T, false, false>(b, hf, s);
     }
 }
 
-template <typename T, bool ONE>
+template <typename T, bool ONE, bool STAGED>
 static void launch_eam_force_one(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
-    int g = xcd_grid((b.A * kEamLanes + kBlock - 1) / kBlock);
+    const ScLaunch L = sc_launch<kEamLanes>(b);
     if (hf.min_image) {
-        if (want_stress) k_eam_force<T, true, true, kEamLanes, ONE><<<g, kBlock, 0, s>>>(b);
-        else k_eam_force<T, true, false, kEamLanes, ONE><<<g, kBlock, 0, s>>>(b);
+        if (want_stress) k_eam_force<T, true, true, kEamLanes, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+        else k_eam_force<T, true, false, kEamLanes, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
     } else {
-        if (want_stress) k_eam_force<T, false, true, kEamLanes, ONE><<<g, kBlock, 0, s>>>(b);
-        else k_eam_force<T, false, false, kEamLanes, ONE><<<g, kBlock, 0, s>>>(b);
+        if (want_stress) k_eam_force<T, false, true, kEamLanes, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+        else k_eam_force<T, false, false, kEamLanes, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
     }
 }
 
 template <typename T>
 static void launch_eam_force(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
-    if (hf.eam_one) launch_eam_force_one<T, true>(b, hf, want_stress, s);
-    else launch_eam_force_one<T, false>(b, hf, want_stress, s);
+    if (hf.eam_one) {
+        if (b.staged) launch_eam_force_one<T, true, true>(b, hf, want_stress, s);
+        else launch_eam_force_one<T, true, false>(b, hf, want_stress, s);
+    } else {
+        if (b.staged) launch_eam_force_one<T, false, true>(b, hf, want_stress, s);
+        else launch_eam_force_one<T, false, false>(b, hf, want_stress, s);
+    }
 }
 
 template <typename T>

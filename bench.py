@@ -203,7 +203,8 @@ def eam_section(L, W, cells, basins, P, args):
     """The tabulated-EAM field (LAMMPS setfl tables sampling the Cu
     Sutton-Chen functions on a 1e-3 mesh) on the same 512 walkers at their
     basins: one force evaluation of every walker, per-pass device times
-    (HIP events) and ordered pairs per second."""
+    (HIP events) and ordered pairs per second of the faster of the unstaged
+    and LDS-staged kernels (both totals reported)."""
     import tempfile
     cu = W.SC_CU
     rc = 2 * cu["a"]
@@ -212,9 +213,16 @@ def eam_section(L, W, cells, basins, P, args):
     f = L.Field(L.eam_potentials_text([("Cu", "Cu", path)], rc))
     b = L.Batch(f, [W.Cell(p, c.types, c.lat) for p, c in zip(basins, cells)])
     b.forces()
-    ph = b.time_phases(20)
+    ph_global = b.time_phases(20)
+    b.set_staging(True)  # the walkers' gather records in LDS (the tables stay in global memory)
+    b.forces()
+    ph_staged = b.time_phases(20)
+    staged = bool(ph_staged.sum() < ph_global.sum())
+    ph = ph_staged if staged else ph_global
     Ptot = int(P.sum())
     out = {"table": "LAMMPS setfl, Cu Sutton-Chen functions sampled at dr = drho = 1e-3",
+           "lds_staged": staged, "force_eval_ms_all_walkers_unstaged": round(float(ph_global.sum()), 4),
+           "force_eval_ms_all_walkers_staged": round(float(ph_staged.sum()), 4),
            "ms_lists": round(float(ph[0]), 4), "ms_density": round(float(ph[1]), 4),
            "ms_force": round(float(ph[2]), 4), "ms_reduce": round(float(ph[3]), 4),
            "force_eval_ms_all_walkers": round(float(ph.sum()), 4),

@@ -78,6 +78,49 @@ def test_eam_explicit_images(lib, tmp_path):
     _compare(f"EAM no-image {c.n} atoms", ref, got, c.n)
 
 
+@pytest.mark.parametrize("case", ["one_table", "two_species", "noimage"])
+def test_eam_lds_staged(lib, tmp_path, case):
+    """The LDS-staged EAM kernels (a walker's gather records in LDS, the
+    tables in global memory) give the unstaged kernels' forces and energies
+    bitwise (same FP64 operations in the same order) and the oracle's to
+    round-off, on the one-table, per-pair-table and explicit-image paths."""
+    if case == "two_species":
+        cells = [W.sc_cell(5, 5, 5, n_vac=2, jitter=0.05, seed=7 + k, two_species=True, a=3.7) for k in range(2)]
+        rc = 7.3
+        pots = {("A", "A"): (0.0124, 3.61, 9.0, 6.0, 39.4), ("B", "B"): (0.0158, 3.89, 10.0, 8.0, 34.4),
+                ("A", "B"): (0.0140, 3.75, 9.5, 7.0, 36.8)}
+        entries = []
+        for (A, B), (eps, a, n, m, cc) in pots.items():
+            p = FX.sc_setfl(tmp_path / f"{A}{B}.eam", eps, a, n, m, cc, rc, dr=2e-3, drho=1e-2, rho_max=300.0)
+            entries.append((A, B, str(p)))
+        text = lib.eam_potentials_text(entries, rc, species=["A", "B"])
+        of = E.Field(["A", "B"], [(A, B, E.read_lammps(p)) for A, B, p in entries], rc)
+    else:
+        rc = 2 * CU["a"]
+        p = FX.sc_setfl(tmp_path / "cu.eam", CU["eps"], CU["a"], CU["n"], CU["m"], CU["c"], rc, dr=2e-3, drho=2e-3)
+        if case == "noimage":
+            cells = [W.sc_cell(3, 3, 3, n_vac=1, jitter=0.02, seed=3 + k) for k in range(3)]
+            text = lib.eam_potentials_text([("Cu", "Cu", str(p))], rc, noimage=True)
+        else:
+            cells = [W.plain_sc_workload(seed=1), W.sc_cell(5, 5, 5, n_vac=2, jitter=0.03, seed=9)]
+            text = lib.eam_potentials_text([("Cu", "Cu", str(p))], rc)
+        of = E.Field(["Cu"], [("Cu", "Cu", E.read_lammps(p))], rc, min_image=case != "noimage")
+    f = lib.Field(text)
+    b = lib.Batch(f, cells)
+    assert b.staging() == (False, True)
+    e0 = b.forces()
+    _, f0 = b.download()
+    b.set_staging(True)
+    e1 = b.forces()
+    _, f1 = b.download()
+    for w, c in enumerate(cells):
+        ref = E.forces(of, c.pos, c.types, c.lat)
+        df = np.abs(f1[w] - ref["force"]).max()
+        print(f"staged EAM {case} walker {w} ({c.n} atoms): max|dF| {df:.2e} eV/A", flush=True)
+        assert df < 1e-9 and abs(e1[w]["total"] - ref["energy"]) < 1e-11 * abs(ref["energy"])
+        assert np.array_equal(f1[w], f0[w]) and e1[w]["total"] == e0[w]["total"]
+
+
 def test_eam_relax_and_dimer_on_device(lib, tmp_path):
     """FIRE relax and a dimer search on the tabulated field: the oracle's
     forces at the device minimum are below the relax tolerance, and the
