@@ -271,6 +271,9 @@ struct LoadItem {
     int walker, has_x, nact, pad;
     long long dbl, ints;
 };
+// the listed walkers' slices of three [A] arrays packed into out (device)
+void akmc_launch_gather_walkers(const WalkerDev *walkers, const double *const src[3], const int *ws,
+                                const long long *dst, int nw, double *out, hipStream_t st);
 void akmc_launch_load_walkers(const BatchDev &b, const SearchDev &s, const LoadItem *items, int nitems,
                               const double *D, const int *I, hipStream_t st);
 void akmc_launch_probe_init(const BatchDev &b, const SearchDev &s, hipStream_t st);

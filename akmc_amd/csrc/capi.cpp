@@ -153,8 +153,9 @@ struct akmc_batch_s {
     // pinned status staging: one stream sync per host check of a run
     WalkerState *hws_pin = nullptr;
     int *ov_pin = nullptr;
-    double *pos_pin = nullptr;  // pinned staging of walker positions
-    size_t pos_pin_n = 0;
+    char *gat_pin = nullptr;    // pinned / device index lists and packed output
+    char *gat_dev = nullptr;    // of akmc_batch_get_walkers_* (gather kernel)
+    size_t gat_n = 0;
     int inflight = 0;           // a chunk launched by akmc_batch_run_async awaits its wait
     int inflight_steps = 0;
     void *stage_pin = nullptr;  // pinned host / device staging of walker starts
@@ -820,7 +821,8 @@ void batch_free(akmc_batch_t b)
             if (e) (void)hipEventDestroy(e);
     if (b->hws_pin) (void)hipHostFree(b->hws_pin);
     if (b->ov_pin) (void)hipHostFree(b->ov_pin);
-    if (b->pos_pin) (void)hipHostFree(b->pos_pin);
+    if (b->gat_pin) (void)hipHostFree(b->gat_pin);
+    if (b->gat_dev) (void)hipFree(b->gat_dev);
     if (b->stage_pin) (void)hipHostFree(b->stage_pin);
     if (b->stage_dev) (void)hipFree(b->stage_dev);
     delete b;
@@ -1803,23 +1805,37 @@ int copy_walkers_out3(akmc_batch_t b, const double *const src[3], int nw, const 
         if (walkers[k] < 0 || walkers[k] >= b->W) return fail(AKMC_EINVAL, "walker %d out of range", walkers[k]);
         total += 3 * (size_t)b->hw[walkers[k]].n;
     }
-    if (total > b->pos_pin_n) {
-        if (b->pos_pin) HIPCHK(hipHostFree(b->pos_pin));
-        b->pos_pin = nullptr;
-        b->pos_pin_n = 0;
-        HIPCHK(hipHostMalloc((void **)&b->pos_pin, total * sizeof(double), hipHostMallocDefault));
-        b->pos_pin_n = total;
+    if (nw == 0) return AKMC_SUCCESS;
+    // one buffer: walker ids, their output offsets, then the packed doubles
+    const size_t ibytes = ((size_t)nw * sizeof(int) + 7) / 8 * 8, obytes = (size_t)nw * sizeof(long long);
+    const size_t need = ibytes + obytes + total * sizeof(double);
+    if (need > b->gat_n) {
+        if (b->gat_pin) HIPCHK(hipHostFree(b->gat_pin));
+        if (b->gat_dev) HIPCHK(hipFree(b->gat_dev));
+        b->gat_pin = b->gat_dev = nullptr;
+        b->gat_n = 0;
+        const size_t cap = need + need / 2;
+        HIPCHK(hipHostMalloc((void **)&b->gat_pin, cap, hipHostMallocDefault));
+        HIPCHK(hipMalloc((void **)&b->gat_dev, cap));
+        b->gat_n = cap;
     }
-    size_t o = 0;
+    int *ws = reinterpret_cast<int *>(b->gat_pin);
+    long long *dst = reinterpret_cast<long long *>(b->gat_pin + ibytes);
+    long long o = 0;
     for (int k = 0; k < nw; ++k) {
-        const WalkerDev &h = b->hw[walkers[k]];
-        for (int c = 0; c < 3; ++c)
-            HIPCHK(hipMemcpyAsync(b->pos_pin + o + (size_t)c * h.n, src[c] + h.off, h.n * sizeof(double),
-                                  hipMemcpyDeviceToHost, b->stream));
-        o += 3 * (size_t)h.n;
+        ws[k] = walkers[k];
+        dst[k] = o;
+        o += 3 * (long long)b->hw[walkers[k]].n;
     }
+    HIPCHK(hipMemcpyAsync(b->gat_dev, b->gat_pin, ibytes + obytes, hipMemcpyHostToDevice, b->stream));
+    double *dout = reinterpret_cast<double *>(b->gat_dev + ibytes + obytes);
+    akmc_launch_gather_walkers(b->d.walkers, src, reinterpret_cast<const int *>(b->gat_dev),
+                               reinterpret_cast<const long long *>(b->gat_dev + ibytes), nw, dout, b->stream);
+    HIPCHK(hipGetLastError());
+    double *hout = reinterpret_cast<double *>(b->gat_pin + ibytes + obytes);
+    HIPCHK(hipMemcpyAsync(hout, dout, total * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    if (total) std::memcpy(out, b->pos_pin, total * sizeof(double));
+    std::memcpy(out, hout, total * sizeof(double));
     return AKMC_SUCCESS;
 }
 

@@ -1116,6 +1116,27 @@ void akmc_launch_load_walkers(const BatchDev &b, const SearchDev &s, const LoadI
     if (nitems > 0) k_load_walkers<<<nitems, kBlock, 0, st>>>(b, s, items, D, I);
 }
 
+// the listed walkers' x, y, z slices packed walker after walker (out +
+// dst[k]: x[n], y[n], z[n] of walker ws[k]); one workgroup per walker
+__global__ void __launch_bounds__(kBlock) k_gather_walkers(const WalkerDev *walkers, const double *sx,
+                                                           const double *sy, const double *sz, const int *ws,
+                                                           const long long *dst, double *out)
+{
+    const WalkerDev W = walkers[ws[blockIdx.x]];
+    double *o = out + dst[blockIdx.x];
+    for (int a = threadIdx.x; a < W.n; a += kBlock) {
+        o[a] = sx[W.off + a];
+        o[W.n + a] = sy[W.off + a];
+        o[2 * W.n + a] = sz[W.off + a];
+    }
+}
+
+void akmc_launch_gather_walkers(const WalkerDev *walkers, const double *const src[3], const int *ws,
+                                const long long *dst, int nw, double *out, hipStream_t st)
+{
+    if (nw > 0) k_gather_walkers<<<nw, kBlock, 0, st>>>(walkers, src[0], src[1], src[2], ws, dst, out);
+}
+
 void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st)
 {
     k_search_logic<<<b.W, kBlock, 0, st>>>(b, s);
