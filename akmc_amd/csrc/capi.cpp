@@ -1013,7 +1013,7 @@ void enqueue_micro_step(akmc_batch_t b, int j)
         (void)hipEventRecord(b->kev[0][j], b->stream);
         akmc_launch_phase(b->d, b->f->host, 1, 0, 2, b->stream);
         (void)hipEventRecord(b->kev[1][j], b->stream);
-        if (prec & 2) akmc_launch_phase(b->d, b->f->host, 2, 0, 2, b->stream);
+        if (prec & 2) akmc_launch_phase(b->d, b->f->host, 2 | 4, 0, 2, b->stream);  // FP64 part of a mixed step
         akmc_launch_phase(b->d, b->f->host, prec, 0, 3, b->stream);
     } else {
         akmc_launch_forces_impl(b->d, b->f->host, prec, 0, b->stream);

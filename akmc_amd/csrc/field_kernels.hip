@@ -1706,7 +1706,7 @@ static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress,
 static BatchDev fp64_view(const BatchDev &b, int prec)
 {
     BatchDev v = b;
-    if (prec & 1) {
+    if (prec & 5) {
         v.staged = 0;
         v.lpa = 4;  // measured best for this pass on the batched dimer workload
         v.unroll = 4;
@@ -1717,7 +1717,9 @@ static BatchDev fp64_view(const BatchDev &b, int prec)
 // one phase of a force evaluation: 0 lists, 1 pass 1 (SC density / ionic
 // real space), 2 pass 2 (SC force / Ewald reciprocal), 3 walker reduction.
 // prec: bit 0 launches the FP32 kernels, bit 1 the FP64 ones; each serves
-// the walkers whose active mode matches.
+// the walkers whose active mode matches.  Bit 2 marks an FP64-only launch
+// that belongs to a mixed micro-step (the FP32 kernels launched apart), so
+// the FP64 kernels take their mixed-step variant.
 void akmc_launch_phase(const BatchDev &b, const FieldDev &hf, int prec, int want_stress, int phase,
                        hipStream_t s)
 {
