@@ -271,6 +271,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--window", type=int, default=500, help="micro-steps per step")
+    ap.add_argument("--sync-every", type=int, default=25,
+                    help="micro-steps per device chunk between host checks (refills)")
     ap.add_argument("--walkers", type=int, default=512)
     ap.add_argument("--roofline-window", type=int, default=100,
                     help="micro-steps of the eagerly launched window that times the dominant kernel")
@@ -348,7 +350,7 @@ def main():
     t_timed = 0.0
     for r in range(args.warmup + args.steps):
         rb0 = camp.batch.rebuilds
-        st = camp.advance(args.window, sync_every=25)
+        st = camp.advance(args.window, sync_every=args.sync_every)
         print(f"[bench] window {r}: {camp.batch.rebuilds - rb0} walker list rebuilds; "
               f"{st.searches} ended, {st.converged} converged, {st.verified} verified, "
               f"{st.distinct} distinct, {st.micro_steps} micro-steps in {st.search_time:.2f}s, "
