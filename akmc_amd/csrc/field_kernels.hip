@@ -194,24 +194,21 @@ __device__ inline void entry_disp(uint32_t e, double &dx, double &dy, double &dz
 
 // ---------------------------------------------------------------- lists
 
-__global__ void k_nbr_check(BatchDev b)
+// Per atom of an active walker, in one pass over the positions: the gather
+// copy of the positions and species (one 32-byte record per neighbour
+// read), and the skin test that flags the walker's list for a rebuild when
+// the atom moved more than skin/2 since the list was built.
+__global__ void k_pack_check(BatchDev b)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *b.cell_work_n = 0;  // k_cell_bin queues this step's cells
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.A) return;
     int w = walker_of(b, i);
     if (skipped(b, w)) return;
-    double dx = b.x[i] - b.xr[i], dy = b.y[i] - b.yr[i], dz = b.z[i] - b.zr[i];
+    const double x = b.x[i], y = b.y[i], z = b.z[i];
+    b.p4[i] = make_double4(x, y, z, (double)b.type[i]);
+    double dx = x - b.xr[i], dy = y - b.yr[i], dz = z - b.zr[i];
     if (dx * dx + dy * dy + dz * dz > b.field->half_skin2) b.rebuild[w] = 1;
-}
-
-// gather copy of the positions and species: one 32-byte load per neighbour
-__global__ void k_pack(BatchDev b)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.A) return;
-    if (skipped(b, walker_of(b, i))) return;
-    b.p4[i] = make_double4(b.x[i], b.y[i], b.z[i], (double)b.type[i]);
 }
 
 // record the positions the lists are built from (positions are never
@@ -805,7 +802,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
     b.inv_sqrt_rho[i] = isr32;
     if (sizeof(T) == 8) b.isr64[i] = isr64;
     // single pair type: the packed gather record carries 1/sqrt(rho) in place
-    // of the species for pass 2 (k_pack restores the species every step)
+    // of the species for pass 2 (k_pack_check restores the species every step)
     if (ONE) reinterpret_cast<double *>(&b.p4[i])[3] = sizeof(T) == 8 ? isr64 : isr32_word(isr32);
 }
 
@@ -995,7 +992,7 @@ constexpr double kEamNoDeriv = -1.0e300;  // F'(rho) of an atom whose rho < tiny
 // interpolation runs in FP64 on both precision paths: a table lookup is a
 // gather, not arithmetic, so the FP32 path gains nothing from FP32 here.
 // ONE: one table for every pair; F'(rho_i) then rides in the packed gather
-// record for pass 2 (k_pack restores the species every step).
+// record for pass 2 (k_pack_check restores the species every step).
 template <typename T, bool MINIMG, int LPA, bool ONE, bool STAGED>
 __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_eam_density(BatchDev b)
 {
@@ -1515,8 +1512,7 @@ extern "C" unsigned long long *akmc_rebuild_counter_ptr()
 static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     int gA = blocks(b.A);
-    k_pack<<<gA, kBlock, 0, s>>>(b);
-    k_nbr_check<<<gA, kBlock, 0, s>>>(b);
+    k_pack_check<<<gA, kBlock, 0, s>>>(b);
     k_ref_positions<<<gA, kBlock, 0, s>>>(b);
     if (hf.min_image) {
         k_cell_bin<<<b.W, kBlock, 0, s>>>(b);
