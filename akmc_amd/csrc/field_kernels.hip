@@ -1408,19 +1408,29 @@ __global__ void __launch_bounds__(kBlock) k_ewald_force(BatchDev b)
 
 // ----------------------------------------------------- walker reductions
 
+// Workgroup sums of NV values in a fixed order: a butterfly over the 64
+// lanes of each wavefront, then the kBlock / 64 wavefront partials in
+// order, so every thread gets the same bits whatever the batch holds.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double *sh)
 {
+    constexpr int kW = kBlock / 64;
+    static_assert(NV * kW <= kBlock, "partials fit the scratch");
+#pragma unroll
+    for (int c = 0; c < NV; ++c)
+        for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) sh[c * kW + wave] = v[c];
+    __syncthreads();
+#pragma unroll
     for (int c = 0; c < NV; ++c) {
-        sh[threadIdx.x] = v[c];
-        __syncthreads();
-        for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
-            if (threadIdx.x < (unsigned)o) sh[threadIdx.x] += sh[threadIdx.x + o];
-            __syncthreads();
-        }
-        v[c] = sh[0];
-        __syncthreads();
+        double t = sh[c * kW];
+        for (int k = 1; k < kW; ++k) t += sh[c * kW + k];
+        v[c] = t;
     }
+    __syncthreads();
 }
 
 // one workgroup per walker; fixed summation order (deterministic)
