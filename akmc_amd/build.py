@@ -24,9 +24,9 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-x", "hip",
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
-    for src in SOURCES:
+
+    def compile_one(src):
         obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
@@ -35,7 +35,13 @@ def build(verbose: bool = False) -> str:
         err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
         if r.returncode != 0:
             raise RuntimeError(f"hipcc failed on {src}:\n{err}")
-        objs.append(obj)
+        return obj
+
+    # the sources compile independently; a few hipcc processes at once
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "6"))))
+    with ThreadPoolExecutor(jobs) as pool:
+        objs = list(pool.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -79,7 +79,6 @@ struct FieldDev {
                      // are evaluated in FP64 on the FP32 path (0: none)
     double eta;
     double cfact;
-    int energy_path;       // short-range energies of the energy-only forms (Morse + C12/r^12)
     // do_ewald == 2: damped-shifted Coulomb on an erfc mesh (fieldErfcGen,
     // potentials/pair/Field.cpp:19-73): points, spacing, shift terms
     const double *dsf_e, *dsf_f;
@@ -156,6 +155,9 @@ struct BatchDev {
     double *Ck, *Sk;      // structure factors
     WalkerOut *out;
     int *active;          // per walker: 0 skip, 1 FP32 pair terms, 2 FP64
+    int energy_only;      // this launch evaluates energies only (Field::calculateEnergy):
+                          // no force pass, the energy-only short-range forms
+    unsigned long long *rebuild_count;  // list rebuilds, counted by the list kernels
 };
 
 // entry k of atom i's neighbour list in the 32-bit form (local index and
@@ -173,6 +175,15 @@ void akmc_launch_forces_impl(const BatchDev &b, const FieldDev &hf, int fp64, in
 void akmc_launch_phase(const BatchDev &b, const FieldDev &hf, int fp64, int want_stress, int phase,
                        hipStream_t s);
 extern "C" unsigned long long *akmc_rebuild_counter_ptr();
+
+// what the allocation-free force entry needs of a field handle (capi.cpp)
+struct FieldView {
+    const FieldDev *dev;   // device copy of the field tables (set up by akmc_field_setup)
+    const FieldDev *host;  // host copy (launch decisions)
+    double rc, skin;
+    int kind, min_image, eam, fp64;
+};
+int akmc_field_view(struct akmc_field_s *f, FieldView *v);
 // records the message akmc_last_error() returns and passes code through
 extern "C" int akmc_fail(int code, const char *fmt, ...);
 
@@ -200,8 +211,11 @@ enum : int {
 // FIRE relax parameters (RelaxCntrl, Common/RelaxCntrl.h:15-48)
 struct RelaxParamsDev {
     double normTol, forceTol, initTimeStep, maxTimeStep, alphaStart, alphaDecrease, stepIncrease,
-        stepDecrease, timeStepUnit;
+        stepDecrease, timeStepUnit, minTimeStep;
     int maxIter, N_min;
+    int method;   // 0 FIRE, 1 FIRE2, 2 force and energy only (AKMC_RELAX_*)
+    int verlet;   // velocity-Verlet instead of semi-implicit Euler
+    int N_delaySteps, N_maxUphill, initialDelay;
 };
 
 // ART parameters (ARTCntrl names, Common/ArtCntrl.h:14-74)
@@ -229,7 +243,8 @@ struct WalkerState {
     // ART: norms of the modified forces before the move, F.v, half step,
     // last and previous Lanczos eigenvalue, Lanczos beta
     double fnpar, fnperp, fmpar, fmperp, dotfv, half, lam, lamold, lbeta;
-    int lconv, pad3;
+    int lconv;
+    int nuphill;  // FIRE2: uphill steps in a row
     // basin 2 of a converged saddle: energy (search precision, FP64),
     // largest squared displacement from the basin, new-basin flag, FIRE
     // status, force calls and iterations of its relax

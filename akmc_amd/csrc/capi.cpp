@@ -800,6 +800,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     b->hout.resize(W);
     unsigned long long *cnt = akmc_rebuild_counter_ptr();
     if (!cnt) return fail(AKMC_EHIP, "rebuild counter allocation failed");
+    d.rebuild_count = cnt;
     HIPCHK(hipMemcpy(&b->rebuild_base, cnt, sizeof b->rebuild_base, hipMemcpyDeviceToHost));
     return AKMC_SUCCESS;
 }
@@ -1264,6 +1265,24 @@ int akmc_field_setup(akmc_field_t f, const double lat[9], int max_atoms)
 
 int akmc_field_num_kvectors(akmc_field_t f) { return f ? f->nk_last : 0; }
 
+}  // extern "C"
+
+int akmc_field_view(akmc_field_s *f, FieldView *v)
+{
+    if (!f || !v) return fail(AKMC_EINVAL, "bad arguments");
+    v->dev = f->dev;
+    v->host = &f->host;
+    v->rc = f->rc;
+    v->skin = f->skin;
+    v->kind = f->kind;
+    v->min_image = f->min_image ? 1 : 0;
+    v->eam = f->eam.empty() ? 0 : 1;
+    v->fp64 = f->fp64;
+    return AKMC_SUCCESS;
+}
+
+extern "C" {
+
 int akmc_field_forces(akmc_field_t f, double *x, double *y, double *z, double *fx, double *fy, double *fz,
                       double stress[6], const double lat[9], const double *charge, const int *type,
                       const int *frozen, int n, akmc_energy *e)
@@ -1290,24 +1309,23 @@ int akmc_field_forces(akmc_field_t f, double *x, double *y, double *z, double *f
 int akmc_field_energy(akmc_field_t f, double *x, double *y, double *z, const double lat[9], const double *charge,
                       const int *type, int n, akmc_energy *e)
 {
-    if (!f || n <= 0) return fail(AKMC_EINVAL, "bad arguments");
-    std::vector<double> fx(n), fy(n), fz(n);
-    // the energy-only short-range forms (TwoBody.h:81-150) differ from the
-    // force forms for Morse, which adds C12/r^12
-    bool morse = false;
-    for (const BuckPot &bp : f->buck) morse |= bp.form == ION_MORSE;
-    if (morse && f->kind == 1) {
-        TRY(ensure_single(f, n, lat));
-        f->host.energy_path = 1;
-        HIPCHK(hipMemcpy(f->dev, &f->host, sizeof(FieldDev), hipMemcpyHostToDevice));
-    }
-    const int rc = akmc_field_forces(f, x, y, z, fx.data(), fy.data(), fz.data(), nullptr, lat, charge, type,
-                                     nullptr, n, e);
-    if (morse && f->kind == 1) {
-        f->host.energy_path = 0;
-        HIPCHK(hipMemcpy(f->dev, &f->host, sizeof(FieldDev), hipMemcpyHostToDevice));
-    }
-    TRY(rc);
+    if (!f || !x || !y || !z || !type || n <= 0) return fail(AKMC_EINVAL, "bad arguments");
+    if (f->kind == 1 && f->do_ewald && !charge) return fail(AKMC_EINVAL, "ionic field needs charges");
+    TRY(ensure_single(f, n, lat));
+    akmc_batch_t b = f->single;
+    TRY(upload(b, x, y, z, type, charge));
+    // this launch's own copy of the batch descriptor carries the energy-only
+    // flag: no force pass, and the energy-only short-range forms of the
+    // two-body table (TwoBody.h:81-150: Morse adds C12/r^12).  Tabulated EAM
+    // keeps its force pass (its pair energy comes from it).
+    BatchDev d = b->d;
+    d.energy_only = 1;
+    akmc_launch_forces_impl(d, f->host, batch_mode(b), 0, b->stream);
+    HIPCHK(hipGetLastError());
+    TRY(check_overflow(b));
+    TRY(fetch_out(b));
+    if (e) to_energy(b->hout[0], e);
+    host_wrap(f->min_image, lat, n, x, y, z);
     // the energy path of a tabulated EAM field doubles the pair term
     // (calculateManyBodyPairEnergy, potentials/metal/ManyBody.cpp:444-450)
     if (e && f->kind == 0 && !f->eam.empty()) {
@@ -1466,6 +1484,12 @@ void akmc_relax_defaults(akmc_relax_params *p)
     p->alphaStart = 0.1;
     p->maxTimeStep = 0.00075;
     p->timeStepUnit = 9648.530821;
+    p->method = AKMC_RELAX_FIRE;
+    p->verletIntegration = 0;
+    p->N_delaySteps = 20;
+    p->N_maxUphill = 50;
+    p->initialDelay = 1;
+    p->minTimeStep = 0.0001;
 }
 
 void akmc_dimer_defaults(akmc_dimer_params *p)
@@ -1524,6 +1548,8 @@ int akmc_batch_start_relax(akmc_batch_t b, const int *walkers, const akmc_relax_
 {
     if (!b || !p) return fail(AKMC_EINVAL, "bad arguments");
     TRY(search_alloc(b));
+    if (p->method != AKMC_RELAX_FIRE && p->method != AKMC_RELAX_FIRE2 && p->method != AKMC_RELAX_NONE)
+        return fail(AKMC_EINVAL, "relax method %d is not one of AKMC_RELAX_FIRE, _FIRE2, _NONE", p->method);
     RelaxParamsDev &r = b->s.rp;
     r.normTol = p->normTol;
     r.forceTol = p->forceTol;
@@ -1534,8 +1560,14 @@ int akmc_batch_start_relax(akmc_batch_t b, const int *walkers, const akmc_relax_
     r.stepIncrease = p->stepIncrease;
     r.stepDecrease = p->stepDecrease;
     r.timeStepUnit = p->timeStepUnit;
+    r.minTimeStep = p->minTimeStep;
     r.maxIter = p->maxIter;
     r.N_min = p->N_min;
+    r.method = p->method;
+    r.verlet = p->verletIntegration ? 1 : 0;
+    r.N_delaySteps = p->N_delaySteps;
+    r.N_maxUphill = p->N_maxUphill;
+    r.initialDelay = p->initialDelay ? 1 : 0;
     // velocities start at zero
     if (!walkers) {
         HIPCHK(hipMemset(b->s.V, 0, 3 * (size_t)b->A * sizeof(double)));

@@ -967,6 +967,41 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
     }
 }
 
+// energy-only pass 2 of Sutton-Chen (Field::calculateEnergy,
+// potentials/metal/Field.cpp:1088-1297): each atom's half of its pairs'
+// V = eps (a/r)^n, no forces.  One lane per atom; species from the type
+// array (the ONE kernels' gather records carry 1/sqrt(rho) in their fourth
+// word after pass 1).
+template <typename T, bool MINIMG>
+__global__ void __launch_bounds__(kBlock) k_sc_pair_energy(BatchDev b)
+{
+    const int i = xcd_atom((b.A + kBlock - 1) / kBlock);
+    if (i < 0 || i >= b.A) return;
+    const int w = walker_of(b, i);
+    if (other_precision<T>(b, w)) return;
+    const FieldDev &F = *b.field;
+    const WalkerDev W = b.walkers[w];
+    const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    const int row = b.type[i] * F.nspec;
+    const int cnt = b.nnb[i];
+    double ep = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+        const uint32_t e = nbr_entry(b, k, i);
+        const int j = W.off + int(e & AKMC_NB_JMASK);
+        const double4 pj = b.p4[j];
+        double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+        entry_disp(e, dx, dy, dz, W, MINIMG);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 <= F.rc2) || !(r2 > (MINIMG ? 1.0e-16 : 1.0e-3))) continue;
+        const SCPairDev &p = F.sc[row + b.type[j]];
+        if (!p.on) continue;
+        const SCCoef<T> &c = sc_coef<T>(p);
+        const T u2 = c.a * c.a * recip<T>(T(r2));
+        ep += double(T(0.5) * c.eps * pow_half<T>(u2, p.n_int, c.n));
+    }
+    b.e_b[i] = ep;
+}
+
 // ------------------------------------------------------- tabulated EAM
 
 // eamMeshEnergy (potentials/metal/ManyBody.cpp:455-460) on a table stored as
@@ -1155,7 +1190,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_eam_forc
 // factor forc = (dE/dr)/r and the pair energies
 template <typename T>
 __device__ inline void ion_pair_terms(const FieldDev &F, const IonPairDev &p, double r2, double qiq, T &forc, T &ec,
-                                      T &eb)
+                                      T &eb, int energy_path)
 {
     const T tsp = T(1.1283791670955126);  // 2/sqrt(pi)
     T r2t = T(r2);
@@ -1212,7 +1247,7 @@ __device__ inline void ion_pair_terms(const FieldDev &F, const IonPairDev &p, do
         if (p.form == ION_MORSE) {
             const T e1 = exp(-m[1] * (r - m[2]));
             eb = m[0] * (T(1) - e1) * (T(1) - e1) - m[0];
-            if (F.energy_path) {
+            if (energy_path) {
                 T ir12 = ir2 * ir2 * ir2;
                 ir12 *= ir12;
                 eb += m[3] * ir12;
@@ -1278,13 +1313,13 @@ __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
         double df, dec, deb;
         if (k < nnear) {
             double forc, ec, eb;
-            ion_pair_terms<double>(F, p, r2, qi * b.q[j], forc, ec, eb);
+            ion_pair_terms<double>(F, p, r2, qi * b.q[j], forc, ec, eb, b.energy_only);
             df = forc;
             dec = ec;
             deb = eb;
         } else {
             T forc, ec, eb;
-            ion_pair_terms<T>(F, p, r2, qi * b.q[j], forc, ec, eb);
+            ion_pair_terms<T>(F, p, r2, qi * b.q[j], forc, ec, eb, b.energy_only);
             df = double(forc);
             dec = double(ec);
             deb = double(eb);
@@ -1453,7 +1488,8 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
             for (int c = 0; c < 6; ++c) v[4 + c] += b.st_atom[(size_t)c * b.A + i];
     }
     double phtot = 0.0, rs[6] = {0, 0, 0, 0, 0, 0};
-    if (stress && F.kind == 1 && F.do_ewald == 1) {
+    const bool rcp_sk = b.energy_only && F.kind == 1 && F.do_ewald == 1;
+    if ((stress || rcp_sk) && F.kind == 1 && F.do_ewald == 1) {
         // reciprocal stress (Ewald.cpp:613-657) with the phase total from 0
         for (int k = threadIdx.x; k < W.nk; k += blockDim.x) {
             int kk = W.k_off + k;
@@ -1474,7 +1510,7 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
     }
     block_sum<10>(v, sh);
     double r7[7] = {phtot, rs[0], rs[1], rs[2], rs[3], rs[4], rs[5]};
-    if (stress && F.kind == 1 && F.do_ewald == 1) block_sum<7>(r7, sh);
+    if ((stress || rcp_sk) && F.kind == 1 && F.do_ewald == 1) block_sum<7>(r7, sh);
     if (threadIdx.x == 0) {
         WalkerOut o;
         if (F.kind == 0) {
@@ -1487,7 +1523,9 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
             o.self = F.do_ewald == 1 ? -1.1283791670955126 * 0.5 * F.cfact * F.eta * v[3] : 0.0;
             o.real = v[0] + o.self;
             o.vdw = v[1];
-            o.rcp = v[2];
+            // energy only: 1/2 sum_k g0 (C^2 + S^2) (Ewald::recipSpaceEnergy,
+            // Ewald.cpp:368-451) instead of the per-atom sums of the force pass
+            o.rcp = rcp_sk ? r7[0] : v[2];
             o.many = 0.0;
             o.total = o.rcp + o.real + o.vdw;
         }
@@ -1530,7 +1568,7 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
     } else {
         k_nbr_build_images<<<gA, kBlock, 0, s>>>(b);
     }
-    k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, akmc_rebuild_counter_ptr());
+    k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, b.rebuild_count);
 }
 
 // a single species pair with integer exponents (the ONE kernels)
@@ -1689,6 +1727,20 @@ template <typename T>
 static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     int gA = blocks(b.A);
+    if (b.energy_only && hf.kind == 0 && !hf.eam) {
+        if (hf.min_image) k_sc_pair_energy<T, true><<<xcd_grid(gA), kBlock, 0, s>>>(b);
+        else k_sc_pair_energy<T, false><<<xcd_grid(gA), kBlock, 0, s>>>(b);
+        return;
+    }
+    if (b.energy_only && hf.kind == 1) {
+        // the reciprocal energy needs the structure factors only
+        if (hf.do_ewald == 1) {
+            k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
+            dim3 g((b.maxnk + kBlock - 1) / kBlock, b.W);
+            k_ewald_sk<<<g, kBlock, 0, s>>>(b);
+        }
+        return;
+    }
     if (hf.kind == 0 && hf.eam) {
         launch_eam_force<T>(b, hf, want_stress, s);
     } else if (hf.kind == 0) {

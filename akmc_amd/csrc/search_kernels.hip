@@ -142,6 +142,7 @@ __device__ void enter_basin2(const Ctx &C_, WalkerState &w, double *sh)
     w.state = ST_R_INIT;
     w.it = 0;
     w.nstep = 1;  // Relax.cpp:49
+    w.nuphill = 0;
     w.dt = C_.s.rp.initTimeStep;
     w.alpha = C_.s.rp.alphaStart;
     w.b2_calls = 0;
@@ -176,11 +177,24 @@ __device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double cur
 
 // ------------------------------------------------------------------ relax
 
-// the part of a FIRE iteration before its force call (Relax.cpp:125-193,
-// semi-implicit Euler)
+__device__ void relax_end(const Ctx &C_, WalkerState &w, double *sh, int status);
+
+// the part of a FIRE / FIRE2 iteration before its force call: the power
+// test and the dt / alpha schedule, the velocity mixing and the first half
+// of the integrator.  FIRE: optim/Relax.cpp:125-225; FIRE2: :429-560, which
+// mixes inside the integration loop with the alpha of before the schedule
+// (the same values), counts uphill steps (failure past N_maxUphill),
+// changes dt and alpha on an uphill step only from iteration N_delaySteps
+// on when initialDelay is set (then clamped at minTimeStep), and puts the
+// free atoms back by half a step of the new dt before zeroing the
+// velocities.  Semi-implicit Euler by default; velocity Verlet
+// (verletIntegration, :194-225 / :519-560) kicks the free atoms' velocities
+// by half a step of the old forces and zeroes the frozen ones before the
+// drift.
 __device__ void relax_pre(const Ctx &C_, WalkerState &w, double *sh)
 {
     const RelaxParamsDev &p = C_.s.rp;
+    const bool fire2 = p.method == 1;
     double acc[3] = {0, 0, 0};  // P, |F|^2, |V|^2
     FOR_ATOMS(i)
     {
@@ -202,12 +216,28 @@ __device__ void relax_pre(const Ctx &C_, WalkerState &w, double *sh)
             w.alpha *= p.alphaDecrease;
         }
         w.nstep += 1;
+        w.nuphill = 0;
+    } else if (fire2) {
+        w.nstep = 0;
+        w.nuphill += 1;
+        if (w.nuphill > p.N_maxUphill) {  // Relax.cpp:446-459: positions as they are
+            relax_end(C_, w, sh, 1);
+            return;
+        }
+        if (!p.initialDelay) {
+            w.dt *= p.stepDecrease;
+            w.alpha = p.alphaStart;
+        } else if (w.it >= p.N_delaySteps) {
+            w.dt *= p.stepDecrease;
+            if (w.dt < p.minTimeStep) w.dt = p.minTimeStep;
+            w.alpha = p.alphaStart;
+        }
     } else {
         w.nstep = 0;
         w.dt *= p.stepDecrease;
         w.alpha = p.alphaStart;
     }
-    double dt = w.dt, fact = dt * p.timeStepUnit;
+    const double dt = w.dt, fact = dt * p.timeStepUnit, half = 0.5 * dt * p.timeStepUnit;
     FOR_ATOMS(i)
     {
         bool fr = C_.s.frozen[i] != 0;
@@ -216,9 +246,17 @@ __device__ void relax_pre(const Ctx &C_, WalkerState &w, double *sh)
             if (down) {
                 if (!fr) v = oma * v + af * F;
             } else {
+                if (fire2 && !fr) V3(C_.s.X, c, i) -= 0.5 * dt * v;
                 v = 0.0;
             }
-            if (!fr) {
+            if (p.verlet) {
+                if (!fr) {
+                    v += half * F;
+                    V3(C_.s.X, c, i) += dt * v;
+                } else {
+                    v = 0.0;
+                }
+            } else if (!fr) {
                 V3(C_.s.X, c, i) += dt * v;
                 v += fact * F;
             }
@@ -290,10 +328,28 @@ __device__ void relax_logic(const Ctx &C_, WalkerState &w, double *sh)
     if (w.kind != WK_BASIN2) w.E0 = w.E_last;  // a basin-2 relax keeps the saddle energy in E0
     __syncthreads();
     if (w.state == ST_R_INIT) {
+        if (p.method == 2) {  // calculateConfig only (Relax.cpp:33-39)
+            relax_end(C_, w, sh, 0);
+            return;
+        }
         w.it = 1;
         w.state = ST_R_STEP;
         relax_pre(C_, w, sh);
         return;
+    }
+    if (p.verlet) {
+        // second half kick with the new forces; frozen velocities stay zero
+        // (Relax.cpp:233-257, :569-585)
+        const double half = 0.5 * w.dt * p.timeStepUnit;
+        FOR_ATOMS(i)
+        {
+            const bool fr = C_.s.frozen[i] != 0;
+            for (int c = 0; c < 3; ++c) {
+                if (!fr) V3(C_.s.V, c, i) += half * V3(C_.s.F0, c, i);
+                else V3(C_.s.V, c, i) = 0.0;
+            }
+        }
+        __syncthreads();
     }
     // centre-of-mass velocity removal over all atoms (Relax.cpp:964-996)
     double m[3] = {0, 0, 0};

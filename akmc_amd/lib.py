@@ -38,7 +38,18 @@ class RelaxParams(C.Structure):
     _fields_ = [("maxIter", C.c_int), ("normTol", C.c_double), ("forceTol", C.c_double), ("N_min", C.c_int),
                 ("initTimeStep", C.c_double), ("alphaDecrease", C.c_double), ("stepIncrease", C.c_double),
                 ("stepDecrease", C.c_double), ("alphaStart", C.c_double), ("maxTimeStep", C.c_double),
-                ("timeStepUnit", C.c_double)]
+                ("timeStepUnit", C.c_double), ("method", C.c_int), ("verletIntegration", C.c_int),
+                ("N_delaySteps", C.c_int), ("N_maxUphill", C.c_int), ("initialDelay", C.c_int),
+                ("minTimeStep", C.c_double)]
+
+
+# Relax::RelaxStructure's method strings (optim/Relax.cpp:25-40); any other
+# string is a single force and energy evaluation
+RELAX_FIRE, RELAX_FIRE2, RELAX_NONE = 0, 1, 2
+
+
+def relax_method(name: str) -> int:
+    return {"fire": RELAX_FIRE, "fire2": RELAX_FIRE2}.get(name, RELAX_NONE)
 
 
 class DimerParams(C.Structure):
@@ -97,9 +108,12 @@ def basin2_defaults(**kw) -> Basin2Params:
 
 
 def relax_defaults(**kw) -> RelaxParams:
+    """RelaxCntrl defaults; method may be given as the reference's string"""
     p = RelaxParams()
     load().akmc_relax_defaults(C.byref(p))
     for k, v in kw.items():
+        if k == "method" and isinstance(v, str):
+            v = relax_method(v)
         setattr(p, k, v)
     return p
 

@@ -144,15 +144,28 @@ long long akmc_batch_rebuilds(akmc_batch_t b);
  * machine.  All walkers of a batch move together; finished walkers are
  * skipped by the force kernels. */
 
-/* RelaxCntrl (Common/RelaxCntrl.h:15-48); FIRE with semi-implicit Euler
- * as Relax::FIRERelax (optim/Relax.cpp:44-315).  time_step_unit is the
- * eV -> 10 J/mol factor EVTODL (potentials/metal/Energy.h:40). */
+/* RelaxCntrl (Common/RelaxCntrl.h:15-48).  Relax::RelaxStructure
+ * (optim/Relax.cpp:19-42) dispatches on method: AKMC_RELAX_FIRE is
+ * Relax::FIRERelax (:44-315), AKMC_RELAX_FIRE2 Relax::FIRE2Relax
+ * (:317-648), AKMC_RELAX_NONE one force and energy evaluation reported as
+ * success (calculateConfig, :33-39).  verletIntegration != 0 selects the
+ * velocity-Verlet branches of both (:194-262, :519-585) instead of the
+ * semi-implicit Euler default.  N_delaySteps, N_maxUphill, initialDelay and
+ * minTimeStep are read by FIRE2 only.  timeStepUnit is the eV -> 10 J/mol
+ * factor EVTODL (potentials/metal/Energy.h:40). */
+#define AKMC_RELAX_FIRE 0
+#define AKMC_RELAX_FIRE2 1
+#define AKMC_RELAX_NONE 2
 typedef struct {
     int maxIter;
     double normTol, forceTol;
     int N_min;
     double initTimeStep, alphaDecrease, stepIncrease, stepDecrease, alphaStart, maxTimeStep;
     double timeStepUnit;
+    int method;
+    int verletIntegration;
+    int N_delaySteps, N_maxUphill, initialDelay;
+    double minTimeStep;
 } akmc_relax_params;
 
 /* DimerCntrl (Common/DimerCntrl.h:18-88) plus the translation time steps

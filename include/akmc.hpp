@@ -91,6 +91,14 @@ struct RelaxCntrl {
         p.stepDecrease = stepDecrease;
         p.alphaStart = alphaStart;
         p.maxTimeStep = maxTimeStep;
+        p.minTimeStep = minTimeStep;
+        p.N_delaySteps = N_delaySteps;
+        p.N_maxUphill = N_maxUphill;
+        p.initialDelay = initialDelay ? 1 : 0;
+        p.verletIntegration = verletIntegration ? 1 : 0;
+        // Relax::RelaxStructure (optim/Relax.cpp:25-40): "fire", "fire2", or
+        // anything else for a single force and energy evaluation
+        p.method = method == "fire" ? AKMC_RELAX_FIRE : (method == "fire2" ? AKMC_RELAX_FIRE2 : AKMC_RELAX_NONE);
         return p;
     }
 };
@@ -383,7 +391,9 @@ struct OneWalker {
 };
 }  // namespace detail
 
-// Relax::RelaxStructure(mpi, bas, fld, natoms, const RelaxCntrl&, Energy&, Status&, out)
+// Relax::RelaxStructure(mpi, bas, fld, natoms, const RelaxCntrl&, Energy&, Status&, out):
+// dispatches on RelaxCntrl::method ("fire", "fire2", anything else a single
+// force and energy evaluation) and verletIntegration (optim/Relax.cpp:19-42)
 class Relax {
 public:
     void RelaxStructure(Basis &bas, Field *fld, const RelaxCntrl &c, Energy &eng, Status &status)

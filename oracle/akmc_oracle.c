@@ -853,6 +853,10 @@ void ako_relax_defaults(ako_relax_cntrl *c)
     c->maxTimeStep = 0.00075;
     c->euler = 1;
     c->timeStepUnit = AKO_EVTODL;
+    c->N_delaySteps = 20;
+    c->N_maxUphill = 50;
+    c->initialDelay = 1;
+    c->minTimeStep = 0.0001;
 }
 
 /* Relax::vectorNormMax over atoms (optim/Relax.cpp:926-946) */
@@ -953,6 +957,145 @@ int ako_fire_relax(ako_force_fn fn, void *ctx, int n, double *x, double *y, doub
             }
         }
         /* Relax::centreOfMass over all atoms (optim/Relax.cpp:964-996) */
+        double mx = 0.0, my = 0.0, mz = 0.0;
+        for (int i = 0; i < n; ++i) {
+            mx += vx[i];
+            my += vy[i];
+            mz += vz[i];
+        }
+        mx /= (double)n;
+        my /= (double)n;
+        mz /= (double)n;
+        for (int i = 0; i < n; ++i) {
+            vx[i] -= mx;
+            vy[i] -= my;
+            vz[i] -= mz;
+        }
+        norm_max3(n, fx, fy, fz, &fnorm, &fmax);
+        if (fnorm < c->normTol || fmax < c->forceTol) {
+            status = AKO_SUCCESS;
+            break;
+        }
+    }
+    if (iters) *iters = (it > c->maxIter) ? c->maxIter : it;
+    if (energy) *energy = e;
+    free(vx); free(vy); free(vz); free(fx); free(fy); free(fz);
+    return status;
+}
+
+/* Relax::FIRE2Relax (optim/Relax.cpp:317-648).  Differences from FIRE:
+ * the velocity mixing moves into the integration loop (:496-540), an
+ * uphill step counts towards N_maxUphill (fail past it, :446-459), only
+ * decreases dt and resets alpha from iteration N_delaySteps on when
+ * initialDelay is set (:461-478), clamps dt at minTimeStep, and puts the
+ * free atoms back by half a step of the new dt before zeroing the
+ * velocities (:481-491). */
+int ako_fire2_relax(ako_force_fn fn, void *ctx, int n, double *x, double *y, double *z,
+                    const int *frozen, const ako_relax_cntrl *c, double *energy, int *iters)
+{
+    double *vx = (double *)calloc(n, sizeof(double)), *vy = (double *)calloc(n, sizeof(double)),
+           *vz = (double *)calloc(n, sizeof(double)), *fx = (double *)calloc(n, sizeof(double)),
+           *fy = (double *)calloc(n, sizeof(double)), *fz = (double *)calloc(n, sizeof(double));
+    double dt = c->initTimeStep, dtmax = c->maxTimeStep, dtmin = c->minTimeStep, alpha = c->alphaStart;
+    double fnorm, fmax, vnorm, vmax, e;
+    int nstep = 1, nuphill = 0, status = AKO_FAILED, it;
+    e = fn(ctx, n, x, y, z, fx, fy, fz);
+    for (int i = 0; i < n; ++i)
+        if (frozen && frozen[i] == 1) fx[i] = fy[i] = fz[i] = 0.0;
+    double half = 0.5 * dt * c->timeStepUnit;
+    for (it = 1; it <= c->maxIter; ++it) {
+        double P = 0.0;
+        for (int i = 0; i < n; ++i) P += fx[i] * vx[i] + fy[i] * vy[i] + fz[i] * vz[i];
+        double oma = 1.0 - alpha;
+        norm_max3(n, fx, fy, fz, &fnorm, &fmax);
+        norm_max3(n, vx, vy, vz, &vnorm, &vmax);
+        double af = alpha * vnorm / fnorm;
+        if (P > 0) {
+            if (nstep > c->N_min) {
+                dt = fmin(dt * c->stepIncrease, dtmax);
+                alpha *= c->alphaDecrease;
+            }
+            nstep += 1;
+            nuphill = 0;
+        } else {
+            nstep = 0;
+            nuphill += 1;
+            if (nuphill > c->N_maxUphill) {
+                status = AKO_FAILED;
+                break;
+            }
+            if (c->initialDelay) {
+                if (it >= c->N_delaySteps) {
+                    dt *= c->stepDecrease;
+                    if (dt < dtmin) dt = dtmin;
+                    alpha = c->alphaStart;
+                }
+            } else {
+                dt *= c->stepDecrease;
+                alpha = c->alphaStart;
+            }
+            for (int i = 0; i < n; ++i)
+                if (!frozen || frozen[i] == 0) {
+                    x[i] -= 0.5 * dt * vx[i];
+                    y[i] -= 0.5 * dt * vy[i];
+                    z[i] -= 0.5 * dt * vz[i];
+                }
+            for (int i = 0; i < n; ++i) vx[i] = vy[i] = vz[i] = 0.0;
+        }
+        if (c->euler) {
+            double fact = dt * c->timeStepUnit;
+            for (int i = 0; i < n; ++i)
+                if (!frozen || frozen[i] == 0) {
+                    if (P > 0.0) {
+                        vx[i] = oma * vx[i] + af * fx[i];
+                        vy[i] = oma * vy[i] + af * fy[i];
+                        vz[i] = oma * vz[i] + af * fz[i];
+                    }
+                    x[i] += dt * vx[i];
+                    y[i] += dt * vy[i];
+                    z[i] += dt * vz[i];
+                    vx[i] += fact * fx[i];
+                    vy[i] += fact * fy[i];
+                    vz[i] += fact * fz[i];
+                }
+        } else {
+            half = 0.5 * dt * c->timeStepUnit;
+            for (int i = 0; i < n; ++i) {
+                if (!frozen || frozen[i] == 0) {
+                    if (P > 0.0) {
+                        vx[i] = oma * vx[i] + af * fx[i];
+                        vy[i] = oma * vy[i] + af * fy[i];
+                        vz[i] = oma * vz[i] + af * fz[i];
+                    }
+                    vx[i] += half * fx[i];
+                    vy[i] += half * fy[i];
+                    vz[i] += half * fz[i];
+                } else {
+                    vx[i] = vy[i] = vz[i] = 0.0;
+                }
+            }
+            for (int i = 0; i < n; ++i)
+                if (!frozen || frozen[i] == 0) {
+                    x[i] += dt * vx[i];
+                    y[i] += dt * vy[i];
+                    z[i] += dt * vz[i];
+                }
+        }
+        e = fn(ctx, n, x, y, z, fx, fy, fz);
+        if (frozen)
+            for (int i = 0; i < n; ++i)
+                if (frozen[i] == 1) fx[i] = fy[i] = fz[i] = 0.0;
+        if (!c->euler) {
+            for (int i = 0; i < n; ++i) {
+                if (!frozen || frozen[i] == 0) {
+                    vx[i] += half * fx[i];
+                    vy[i] += half * fy[i];
+                    vz[i] += half * fz[i];
+                } else {
+                    vx[i] = vy[i] = vz[i] = 0.0;
+                }
+            }
+        }
         double mx = 0.0, my = 0.0, mz = 0.0;
         for (int i = 0; i < n; ++i) {
             mx += vx[i];

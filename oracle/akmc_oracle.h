@@ -174,10 +174,16 @@ typedef struct {
            maxTimeStep;
     int euler;            /* 1 = semi-implicit Euler (reference default) */
     double timeStepUnit;  /* EVTODL */
+    /* FIRE2 (Relax::FIRE2Relax) */
+    int N_delaySteps, N_maxUphill, initialDelay;
+    double minTimeStep;
 } ako_relax_cntrl;
 void ako_relax_defaults(ako_relax_cntrl *c);
 int ako_fire_relax(ako_force_fn fn, void *ctx, int n, double *x, double *y, double *z,
                    const int *frozen, const ako_relax_cntrl *c, double *energy, int *iters);
+/* Relax::FIRE2Relax (optim/Relax.cpp:317-648) */
+int ako_fire2_relax(ako_force_fn fn, void *ctx, int n, double *x, double *y, double *z,
+                    const int *frozen, const ako_relax_cntrl *c, double *energy, int *iters);
 
 /* ------------------------------------------------------------------ */
 /* saddle-search helpers (optim/SaddleSearch.cpp)                      */

@@ -76,7 +76,8 @@ class RelaxCntrl(C.Structure):
                 ("N_min", C.c_int), ("initTimeStep", C.c_double), ("alphaDecrease", C.c_double),
                 ("stepIncrease", C.c_double), ("stepDecrease", C.c_double),
                 ("alphaStart", C.c_double), ("maxTimeStep", C.c_double), ("euler", C.c_int),
-                ("timeStepUnit", C.c_double)]
+                ("timeStepUnit", C.c_double), ("N_delaySteps", C.c_int), ("N_maxUphill", C.c_int),
+                ("initialDelay", C.c_int), ("minTimeStep", C.c_double)]
 
 
 class ARTCntrl(C.Structure):
@@ -374,15 +375,17 @@ def dimer_defaults(fast=False):
     return c
 
 
-def fire_relax(h: FieldHandle, pos, frozen=None, cntrl=None):
+def fire_relax(h: FieldHandle, pos, frozen=None, cntrl=None, method="fire"):
+    """Relax::FIRERelax (method "fire") or Relax::FIRE2Relax ("fire2");
+    cntrl.euler = 0 is the reference's verletIntegration."""
     x, y, z = _xyz(pos)
     n = x.size
     fr = np.ascontiguousarray(frozen if frozen is not None else np.zeros(n), dtype=np.int32)
     c = cntrl or relax_defaults()
     e = C.c_double()
     it = C.c_int()
-    st = lib().ako_fire_relax(h.fn, C.byref(h.ctx), n, _p(x), _p(y), _p(z), _p(fr, C.c_int), C.byref(c),
-                              C.byref(e), C.byref(it))
+    fn = {"fire": lib().ako_fire_relax, "fire2": lib().ako_fire2_relax}[method]
+    st = fn(h.fn, C.byref(h.ctx), n, _p(x), _p(y), _p(z), _p(fr, C.c_int), C.byref(c), C.byref(e), C.byref(it))
     return dict(status=st, pos=np.stack([x, y, z]), energy=e.value, iters=it.value)
 
 

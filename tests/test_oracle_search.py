@@ -75,6 +75,57 @@ def test_fire_reaches_lbfgs_minimum(oracle, relaxed_c1):
     assert abs(res.fun - r["energy"]) / c.n < 1e-6
 
 
+@pytest.mark.parametrize("method,euler", [("fire", 0), ("fire2", 1), ("fire2", 0)])
+def test_fire_variants_reach_lbfgs_minimum(oracle, relaxed_c1, method, euler):
+    """FIRE with velocity Verlet and FIRE2 with either integrator stop at the
+    minimum an independent optimiser (scipy L-BFGS) finds from the same
+    start, with the reference's stop rule met."""
+    f, c, h, r = relaxed_c1
+    cn = oracle.relax_defaults()
+    cn.euler = euler
+    o = oracle.fire_relax(h, c.pos, cntrl=cn, method=method)
+    assert o["status"] == 0
+    fr = oracle.sc_forces(f, o["pos"], c.types, c.lat)
+    fmax = np.sqrt((fr["force"] ** 2).sum(0).max())
+    assert fmax < 1e-4 or np.linalg.norm(fr["force"]) < 1e-5
+    assert abs(o["energy"] - r["energy"]) / c.n < 1e-6
+
+
+def test_fire2_uphill_limit_and_delay(oracle):
+    """FIRE2's first iteration is uphill (zero velocity): with N_maxUphill = 0
+    it fails there before moving any atom (optim/Relax.cpp:446-459).  With
+    the initial delay on, uphill steps before N_delaySteps leave dt alone,
+    so on a quadratic well the first FIRE2 step moves an atom by a full
+    dt^2 unit F (semi-implicit Euler from rest), while FIRE's first step
+    halves dt first (4x less).  Without the delay FIRE2 halves dt too."""
+    k = 0.7
+
+    def ef(pos):
+        return 0.5 * k * float((pos * pos).sum()), -k * pos
+
+    start = np.array([[0.3, -0.2], [0.1, 0.05], [-0.4, 0.2]])
+    h = oracle.PyFieldHandle(ef)
+    cn = oracle.relax_defaults()
+    cn.N_maxUphill = 0
+    o = oracle.fire_relax(h, start.copy(), cntrl=cn, method="fire2")
+    assert o["status"] == 1 and o["iters"] == 1
+    assert np.array_equal(o["pos"], start)
+    # two free atoms, one iteration each: x1 = x0 + dt^2 u F(x0) after the
+    # uphill first step of zero velocity (the velocity is mixed and kicked
+    # after the drift), so the moves are proportional to dt^2
+    cn = oracle.relax_defaults()
+    cn.maxIter = 2
+    moves = {}
+    for method, delay in (("fire", 1), ("fire2", 1), ("fire2", 0)):
+        cn.initialDelay = delay
+        o = oracle.fire_relax(h, start.copy(), cntrl=cn, method=method)
+        moves[(method, delay)] = o["pos"] - start
+    base = moves[("fire2", 1)]
+    assert np.abs(base).max() > 0
+    np.testing.assert_allclose(moves[("fire", 1)][:, 0] / base[:, 0], 0.25, rtol=0.2)
+    np.testing.assert_allclose(moves[("fire2", 0)], moves[("fire", 1)], rtol=1e-12, atol=1e-15)
+
+
 def test_lanczos_quadratic_form(oracle):
     rng = np.random.default_rng(2)
     m = 12
