@@ -80,6 +80,29 @@ int akmc_field_energy(akmc_field_t f, double *x, double *y, double *z, const dou
 /* number of k vectors of the last setup (0 for many-body fields) */
 int akmc_field_num_kvectors(akmc_field_t f);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  One Sutton-Chen force evaluation of the configuration
+ * in the caller's device arrays x, y, z [n] (FP64, not wrapped) and type
+ * [n], with the kernels and default variant of akmc_field_forces, so the
+ * same results bit for bit: forces into the device arrays fx, fy, fz [n],
+ * the energy record into the device akmc_energy *energy (may be NULL) and
+ * the overflow bits (0: the list capacity sufficed) into the device int
+ * *flags (may be NULL).  Enqueued on stream (a hipStream_t; NULL: the null
+ * stream); returns without waiting.  Allocates nothing: every other buffer
+ * lives in one fixed device array of the library, akmc_noalloc_scratch_bytes()
+ * per GPU, held until the process ends; calls that overlap on one GPU race
+ * on it.  The field must have been set up (akmc_field_setup) and be a
+ * Sutton-Chen field (AKMC_EUNSUPPORTED otherwise); n above
+ * AKMC_NOALLOC_MAX_ATOMS is AKMC_ECAPACITY; neighbour lists hold up to
+ * AKMC_NOALLOC_MAX_NEIGHBOURS entries per atom. */
+#define AKMC_NOALLOC_MAX_ATOMS 8192
+#define AKMC_NOALLOC_MAX_NEIGHBOURS 320
+int akmc_field_forces_noalloc(akmc_field_t f, const double *x, const double *y, const double *z,
+                              const int *type, int n, const double lat[9], double *fx, double *fy,
+                              double *fz, akmc_energy *energy, int *flags, void *stream);
+unsigned long akmc_noalloc_scratch_bytes(void);
+
 /* ---- device-resident walker batches ----------------------------------
  * W independent cells of uneven size kept in HBM; the batched counterpart
  * of the reference's MPI task farm (kmc/KineticMonteCarlo.cpp:352-598),
