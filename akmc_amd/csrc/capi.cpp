@@ -95,6 +95,7 @@ struct akmc_field_s {
     double eta = 0.0;
     int gcell[3] = {0, 0, 0};
     double skin = 0.7;  // best of a 0.45-1.6 A sweep on the batched dimer workload
+    double list_density = 0.0;  // atoms per A^3 the neighbour lists are sized for (0: each walker's N / V)
     std::vector<SpeciesRec> species;
     std::vector<SCPot> sc;
     std::vector<BuckPot> buck;
@@ -632,6 +633,7 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
         h.cell_off = ncells;
         ncells += h.nc[0] * h.nc[1] * h.nc[2];
         max_density = std::max(max_density, h.n / (h.L[0] * h.L[1] * h.L[2]));
+        max_density = std::max(max_density, f->list_density);
         if (f->kind == 1 && f->do_ewald == 1) {
             double eta;
             int g[3];
@@ -1246,6 +1248,13 @@ int akmc_field_set_skin(akmc_field_t f, double skin)
     if (!f || skin < 0) return fail(AKMC_EINVAL, "bad skin");
     f->skin = skin;
     build_tables(f);
+    return AKMC_SUCCESS;
+}
+
+int akmc_field_set_list_density(akmc_field_t f, double atoms_per_A3)
+{
+    if (!f || !(atoms_per_A3 >= 0.0) || atoms_per_A3 > 10.0) return fail(AKMC_EINVAL, "bad list density");
+    f->list_density = atoms_per_A3;
     return AKMC_SUCCESS;
 }
 
@@ -1923,7 +1932,8 @@ int akmc_batch_save_basin(akmc_batch_t b) { return akmc_batch_set_basins(b, null
 
 int akmc_batch_set_basins(akmc_batch_t b, const int *walkers, const double *x)
 {
-    if (!b || !b->search_alloc) return fail(AKMC_EINVAL, "no search state");
+    if (!b) return fail(AKMC_EINVAL, "bad arguments");
+    TRY(search_alloc(b));  // search positions start at the resident ones
     if (!b->basin) TRY(dalloc(b, &b->basin, 3 * (size_t)b->A));
     b->s.basin = b->basin;
     const size_t A = b->A;

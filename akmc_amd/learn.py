@@ -164,42 +164,57 @@ def hop_frame(h):
     return np.stack([h, e2, np.cross(h, e2)])
 
 
-def _basin_state(camp):
-    """Put the campaign's batch at its basins with fresh lists."""
-    camp.batch.upload([W.Cell(p, c.types, c.lat) for p, c in zip(camp.basins, camp.cells)])
+def _basin_state(camp, pool=None):
+    """Put the campaign's batch at its basins (pool entry `pool` of every
+    walker when the campaign turns basins over) with fresh lists."""
+    if pool is None or camp.pool is None:
+        cells, basins = camp.cells, camp.basins
+    else:
+        ent = [camp.pool[w][pool % len(camp.pool[w])] for w in range(camp.W)]
+        cells, basins = [e[0] for e in ent], [e[1] for e in ent]
+    camp.batch.upload([W.Cell(p, c.types, c.lat) for p, c in zip(basins, cells)])
     camp.batch.forces()
+    return cells, basins
 
 
 def dataset_from_records(camp, records):
     """(X, T) from verified saddle records of a campaign: one row per active
     atom of each saddle; T is that atom's saddle displacement in the hop
-    frame of the saddle's mover (the atom that moved most).  Call it when no
-    search is running: it puts the batch back on its basins."""
-    _basin_state(camp)
-    qa, qv, qm, T = [], [], [], []
-    for r in records:
-        if not r.verified or r.atoms.size == 0:
-            continue
-        w = r.walker
-        c, basin = camp.cells[w], camp.basins[w]
-        mover = int(r.atoms[np.argmax(np.sqrt((r.disp ** 2).sum(0)))])
-        site = r.site
-        if site is None:
-            continue
-        F = hop_frame(W.min_image(site - basin[:, mover], c.lat))
-        act = r.active
-        disp = np.zeros((3, c.n))
-        disp[:, act] = r.active_disp
-        off = camp.batch.offsets[w]
-        for i in act:
-            qa.append(off + i)
-            qv.append(site)
-            qm.append(basin[:, mover])
-            T.append(F @ disp[:, i])
-    if not qa:
+    frame of the saddle's mover (the atom that moved most).  Records of
+    different pooled basins are described at their own basins.  Call it
+    when no search is running: it leaves the batch on the campaign's
+    current basins."""
+    good = [r for r in records if r.verified and r.atoms.size and r.site is not None]
+    groups = sorted({r.pool for r in good}) if camp.pool is not None else [None]
+    Xs, Ts = [], []
+    for g in groups:
+        cells, basins = _basin_state(camp, g)
+        qa, qv, qm, T = [], [], [], []
+        for r in good:
+            if g is not None and r.pool != g:
+                continue
+            w = r.walker
+            c, basin = cells[w], basins[w]
+            mover = int(r.atoms[np.argmax(np.sqrt((r.disp ** 2).sum(0)))])
+            site = r.site
+            F = hop_frame(W.min_image(site - basin[:, mover], c.lat))
+            act = r.active
+            disp = np.zeros((3, c.n))
+            disp[:, act] = r.active_disp
+            off = camp.batch.offsets[w]
+            for i in act:
+                qa.append(off + i)
+                qv.append(site)
+                qm.append(basin[:, mover])
+                T.append(F @ disp[:, i])
+        if qa:
+            Xs.append(batch_descriptors(camp.batch, qa, np.array(qv), np.array(qm)))
+            Ts.append(_f32(np.array(T)))
+    if groups != [None]:
+        _basin_state(camp)
+    if not Xs:
         return np.zeros((0, 48), np.float32), np.zeros((0, 3), np.float32)
-    X = batch_descriptors(camp.batch, qa, np.array(qv), np.array(qm))
-    return X, _f32(np.array(T))
+    return np.concatenate(Xs), np.concatenate(Ts)
 
 
 def _active_region(cell, pos, site, radius):

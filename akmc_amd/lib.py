@@ -157,6 +157,7 @@ def load() -> C.CDLL:
     L.akmc_field_kind.argtypes = [C.c_void_p]
     L.akmc_field_set_ewald.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]
     L.akmc_field_set_skin.argtypes = [C.c_void_p, C.c_double]
+    L.akmc_field_set_list_density.argtypes = [C.c_void_p, C.c_double]
     L.akmc_field_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.akmc_field_setup.argtypes = [C.c_void_p, _D, C.c_int]
     L.akmc_field_forces.argtypes = [C.c_void_p, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I, _I, C.c_int,
@@ -288,6 +289,10 @@ class Field:
 
     def set_skin(self, skin):
         check(load().akmc_field_set_skin(self._h, float(skin)))
+
+    def set_list_density(self, atoms_per_A3):
+        """size later batches' neighbour lists for this number density"""
+        check(load().akmc_field_set_list_density(self._h, float(atoms_per_A3)))
 
     def set_precision(self, bits):
         check(load().akmc_field_set_precision(self._h, int(bits)))

@@ -17,6 +17,14 @@ the search's own barrier, the two lowest Lanczos curvatures bracket zero
 (exactly one negative direction) and the barrier is positive.  Verified
 saddles are de-duplicated per basin (barrier within 2 meV and the same
 moved atoms within 0.15 A).
+
+Basin turnover (``searches_per_basin`` = k): each walker owns a pool of
+pre-relaxed basins -- the same cell shape and vacancy count, vacancies and
+jitter re-drawn from seeds -- and moves to the next one after k searches,
+as the reference's KMC loop moves on once a step's catalogue holds
+kmcevents events (docs/examples/gdc/control: 5).  Each basin keeps its own
+catalogue, so the distinct-saddle rate does not decay as a campaign runs
+on (until the pool wraps around).
 """
 from __future__ import annotations
 
@@ -52,6 +60,7 @@ class SaddleRecord:
     site: np.ndarray | None = None  # vacancy site of the proposal
     active: np.ndarray | None = None      # free atoms of the search
     active_disp: np.ndarray | None = None  # their saddle displacement (3, n_active)
+    pool: int = 0                         # index of the walker's basin in its pool
 
 
 @dataclass
@@ -106,7 +115,7 @@ class CampaignStats:
 class SaddleCampaign:
     def __init__(self, potentials_text, cells, method="dimer", region_radius=6.0, d0=0.1, params=None,
                  relax=None, seed=0, fmax_verify=0.05, barrier_tol=0.01, verify_nl=16, verify_delta=0.01,
-                 proposer=None, skin=None):
+                 proposer=None, skin=None, searches_per_basin=None):
         self.text = potentials_text
         self.cells = cells
         self.W = len(cells)
@@ -126,6 +135,16 @@ class SaddleCampaign:
         self.verify_delta = verify_delta
         self.proposer = proposer  # optional callable(campaign, w) -> (x0, mode, frozen, site)
         self.found = [[] for _ in range(self.W)]
+        # basin turnover: pool[w] = [(cell, basin, basin32, basin64, ok)], the
+        # pool index of the walker's current basin, searches started on it,
+        # and the catalogues of the basins it left (by pool index)
+        self.searches_per_basin = searches_per_basin
+        self.pool = None
+        self.pool_idx = np.zeros(self.W, dtype=np.int64)
+        self._starts = np.zeros(self.W, dtype=np.int64)
+        self._catalogues = {}
+        self._ctx = {}  # walker -> (pool index, basin32, basin64, basin) of its running search
+        self.basin_moves = 0
         self._regions = {}
         self._next = {}  # walker -> its next proposal, drawn ahead
         self.pending = set()
@@ -152,11 +171,71 @@ class SaddleCampaign:
         self.batch.set_verify(self.verify_nl, self.verify_delta)
         return self.basin_ok
 
+    def relax_pool(self, size, seed, sync_every=50, max_steps=5000):
+        """Relax `size` basins per walker (pool entry 0 is the campaign's own
+        cells, as relax_basins leaves them; entries 1.. re-draw the vacancy
+        sites and jitter of the same cell shape from seeds) for basin
+        turnover.  Leaves the batch on pool entry 0."""
+        first = (list(self.cells), [b.copy() for b in self.basins], self.basin32.copy(), self.basin64.copy(),
+                 self.basin_ok.copy())
+        rng = np.random.default_rng(seed)
+        rounds = [first]
+        for _ in range(1, size):
+            cells = []
+            for c in first[0]:
+                m = c.meta
+                cells.append(W.sc_cell(m["nx"], m["ny"], m["nz"], n_vac=len(c.vacancies), jitter=m.get("jitter", 0.01),
+                                       seed=int(rng.integers(0, 2**31 - 1))))
+            self.cells = cells
+            self.batch.upload(cells)
+            self.relax_basins(sync_every=sync_every, max_steps=max_steps)
+            rounds.append((cells, [b.copy() for b in self.basins], self.basin32.copy(), self.basin64.copy(),
+                           self.basin_ok.copy()))
+        self.pool = [[(r[0][w], r[1][w], float(r[2][w]), float(r[3][w]), bool(r[4][w])) for r in rounds]
+                     for w in range(self.W)]
+        self.pool_idx[:] = 0
+        self._starts[:] = 0
+        self._use_pool_entry(range(self.W), upload=True)
+
+    def _use_pool_entry(self, walkers, upload=False):
+        """make each listed walker's current pool entry its basin (host
+        state); with upload, put every walker on its basin on the device"""
+        for w in walkers:
+            c, basin, b32, b64, ok = self.pool[w][int(self.pool_idx[w]) % len(self.pool[w])]
+            self.cells[w] = c
+            self.basins[w] = basin
+            self.basin32[w] = b32
+            self.basin64[w] = b64
+            self.basin_ok[w] = ok
+        if upload:
+            self.batch.upload([W.Cell(p, c.types, c.lat) for p, c in zip(self.basins, self.cells)])
+            self.batch.save_basin()
+
+    def _advance_basin(self, w):
+        """after searches_per_basin searches the walker moves to its next
+        pooled basin, with that basin's own catalogue"""
+        k = self.searches_per_basin
+        if self.pool is None or not k or self._starts[w] < k:
+            return
+        self._catalogues[(w, int(self.pool_idx[w]) % len(self.pool[w]))] = self.found[w]
+        self.pool_idx[w] += 1
+        self._starts[w] = 0
+        self._use_pool_entry([w])
+        self.found[w] = self._catalogues.get((w, int(self.pool_idx[w]) % len(self.pool[w])), [])
+        self._regions = {key: v for key, v in self._regions.items() if key[0] != w}
+        self.basin_moves += 1
+
+    def _catalogue(self, w, pool):
+        """the de-duplication list of pool entry `pool` of walker w"""
+        if pool == int(self.pool_idx[w]) % (len(self.pool[w]) if self.pool else 1):
+            return self.found[w]
+        return self._catalogues.setdefault((w, pool), [])
+
     # ---------------------------------------------------------- searches
     def _site_region(self, w, v):
         """(frozen mask, the 12 neighbours) of vacancy v of walker w at its
         basin; fixed per basin, so computed once."""
-        key = (w, v)
+        key = (w, int(self.pool_idx[w]), v)
         hit = self._regions.get(key)
         if hit is None:
             c = self.cells[w]
@@ -179,6 +258,17 @@ class SaddleCampaign:
                                           int(self.rng.integers(0, 2**31 - 1)))
         return basin + disp, mode, frozen, c.vacancies[v]
 
+    def _draw(self, w):
+        """the walker's next proposal, on the basin its next search runs on
+        (moving to the next pooled basin first when the current one has had
+        its searches); returns (x0, mode, frozen, site, context)"""
+        self._advance_basin(w)
+        pool = int(self.pool_idx[w]) % (len(self.pool[w]) if self.pool else 1)
+        ctx = (pool, self.basin32[w], self.basin64[w], self.basins[w], self.cells[w].lat)
+        self._starts[w] += 1
+        x0, mode, frozen, site = (self.proposer or SaddleCampaign.proposal)(self, w)
+        return x0, mode, frozen, site, ctx
+
     def precompute(self, walkers):
         """draw the next proposal of each listed walker now (random proposals
         do not depend on how the current search ends), so a refill between
@@ -187,7 +277,7 @@ class SaddleCampaign:
             return
         for w in walkers:
             if w not in self._next and self.basin_ok[w]:
-                self._next[w] = self.proposal(w)
+                self._next[w] = self._draw(w)
 
     def start(self, walkers):
         walkers = [int(w) for w in walkers if self.basin_ok[w]]
@@ -197,8 +287,9 @@ class SaddleCampaign:
         mask = np.zeros(self.W, dtype=np.int32)
         for w in walkers:
             nxt = self._next.pop(w, None) if self.proposer is None else None
-            x0, mode, frozen, site = nxt if nxt is not None else (self.proposer or SaddleCampaign.proposal)(self, w)
+            x0, mode, frozen, site, ctx = nxt if nxt is not None else self._draw(w)
             self.cur_site[w] = site
+            self._ctx[w] = ctx
             a, b = self.off[w], self.off[w + 1]
             for c in range(3):
                 self._x0[c * A + a:c * A + b] = x0[c]
@@ -237,24 +328,25 @@ class SaddleCampaign:
                 stats.fail_unchecked += 1
                 continue
             act = np.nonzero(self._frozen[self.off[w]:self.off[w + 1]] == 0)[0]
-            raw.append((w, s, positions[w], self.cur_site.get(w), act))
+            raw.append((w, s, positions[w], self.cur_site.get(w), act, self._ctx.get(w)))
         return done, raw
 
     def process(self, raw, stats: CampaignStats):
         """The host part of a harvest: FP64 tests, records, de-duplication
         (needs nothing from the device, so it can run while the next chunk
         does)."""
-        for (w, s, pos, site, act) in raw:
-            b32 = s["energy32"] - self.basin32[w]
-            b64 = s["energy64"] - self.basin64[w]
+        for (w, s, pos, site, act, ctx) in raw:
+            pool, e32, e64, basin, lat = ctx
+            b32 = s["energy32"] - e32
+            b64 = s["energy64"] - e64
             err = abs(b32 - b64)
             stats.max_barrier_err = max(stats.max_barrier_err, err)
-            d = W.min_image(pos - self.basins[w], self.cells[w].lat)
+            d = W.min_image(pos - basin, lat)
             dn = np.sqrt((d * d).sum(0))
             atoms = np.nonzero(dn > 0.3)[0]
             rec = SaddleRecord(w, s["energy32"], s["energy64"], float(b32), float(b64), s["curvature32"],
                                (s["lambda1"], s["lambda2"]), s["fmax64"], atoms, d[:, atoms],
-                               site=site, active=act, active_disp=d[:, act])
+                               site=site, active=act, active_disp=d[:, act], pool=pool)
             tests = dict(fmax=s["fmax64"] < self.fmax_verify, barrier=err <= self.barrier_tol,
                          lambda1=s["lambda1"] < 0.0, lambda2=s["lambda2"] > 0.0, b64=b64 > 0.0)
             for k, ok in tests.items():
@@ -263,9 +355,10 @@ class SaddleCampaign:
             rec.verified = all(tests.values())
             if rec.verified:
                 stats.verified += 1
-                rec.distinct = all(not self._same(rec, o) for o in self.found[w])
+                cat = self._catalogue(w, pool)
+                rec.distinct = all(not self._same(rec, o) for o in cat)
                 if rec.distinct:
-                    self.found[w].append(rec)
+                    cat.append(rec)
                     stats.distinct += 1
             stats.records.append(rec)
 

@@ -15,6 +15,19 @@ import numpy as np
 # Sutton-Chen Cu of the plain-method workload: eps, a, n, m, c
 SC_CU = dict(eps=0.012382, a=3.61, n=9.0, m=6.0, c=39.432)
 
+# The two-species alloy of the full-loop workload: Sutton-Chen Cu and Ni
+# (Sutton & Chen 1990: eps eV, a A, n, m, c) and an explicit Cu-Ni line
+# (geometric-mean eps, mean a and c), FCC at the mean lattice constant
+SC_NI = dict(eps=0.015707, a=3.52, n=9.0, m=6.0, c=39.432)
+ALLOY_POTS = [
+    ("Cu", "Cu", SC_CU["eps"], SC_CU["a"], SC_CU["n"], SC_CU["m"], SC_CU["c"]),
+    ("Ni", "Ni", SC_NI["eps"], SC_NI["a"], SC_NI["n"], SC_NI["m"], SC_NI["c"]),
+    ("Cu", "Ni", (SC_CU["eps"] * SC_NI["eps"]) ** 0.5, 0.5 * (SC_CU["a"] + SC_NI["a"]), 9.0, 6.0,
+     0.5 * (SC_CU["c"] + SC_NI["c"])),
+]
+ALLOY_A = 0.5 * (SC_CU["a"] + SC_NI["a"])
+ALLOY_RC = 2.0 * SC_CU["a"]
+
 # Buckingham parameters of the rock-salt workload (A eV, rho A, C eV A^6)
 ROCKSALT_BUCK = {
     ("M", "X"): (1284.38, 0.29969, 0.0),
@@ -75,7 +88,7 @@ def sc_cell(nx, ny, nz, n_vac=1, jitter=0.01, seed=0, a=SC_CU["a"], two_species=
         species = ["Cu"]
     lat = np.diag([a * nx, a * ny, a * nz]).reshape(9).astype(np.float64)
     return Cell(pos=np.ascontiguousarray(pos), types=types, lat=lat, species=species,
-                vacancies=[sites[:, v].copy() for v in vac], meta=dict(seed=seed, nx=nx, ny=ny, nz=nz))
+                vacancies=[sites[:, v].copy() for v in vac], meta=dict(seed=seed, nx=nx, ny=ny, nz=nz, jitter=jitter))
 
 
 def neighbours_of_site(cell: Cell, site: np.ndarray, count: int = 12) -> np.ndarray:
@@ -128,6 +141,28 @@ def rocksalt_cell(nx=4, ny=4, nz=4, a=4.212, jitter=0.02, seed=0, charge=2.0, sc
     charges = np.where(types == 0, charge, -charge).astype(np.float64)
     return Cell(pos=np.ascontiguousarray(pos), types=types.astype(np.int32), lat=lat, species=["M", "X"],
                 charges=charges, vacancies=vac, meta=dict(seed=seed, a=a))
+
+
+def alloy_cell(n=20, n_vac=20, jitter=0.01, seed=7, frac_a=0.5):
+    """The full-loop workload's cell: FCC n x n x n at ALLOY_A, species 0
+    (Cu) and 1 (Ni) drawn 50/50 per site from the seed, n_vac vacancies at
+    seeded sites, N(0, jitter) displacements (20 x 20 x 20, 20 vacancies:
+    31,980 atoms)."""
+    rng = np.random.default_rng(seed)
+    sites = fcc_sites(ALLOY_A, n, n, n)
+    ns = sites.shape[1]
+    kind = (rng.random(ns) >= frac_a).astype(np.int32)
+    vac = np.sort(rng.choice(ns, size=n_vac, replace=False))
+    keep = np.setdiff1d(np.arange(ns), vac)
+    pos = sites[:, keep] + rng.normal(0.0, jitter, size=(3, keep.size))
+    lat = np.diag([ALLOY_A * n] * 3).reshape(9).astype(np.float64)
+    return Cell(pos=np.ascontiguousarray(pos), types=kind[keep], lat=lat, species=["Cu", "Ni"],
+                vacancies=[sites[:, v].copy() for v in vac], meta=dict(seed=seed, nx=n, ny=n, nz=n, jitter=jitter))
+
+
+def alloy_potentials_text():
+    from .lib import sc_potentials_text
+    return sc_potentials_text(ALLOY_POTS, ALLOY_RC, species=["Cu", "Ni"])
 
 
 def batched_sc_cells(n_walkers=512, n_min=500, n_max=4000, seed=3, jitter=0.01):

@@ -13,7 +13,12 @@ window of 500 batched micro-steps (a micro-step is one force evaluation of
 every running walker plus its state update).
 
 value = FP64-verified distinct saddles ended in the timed windows / their
-wall time.
+wall time.  Basins turn over: every walker owns a pool of pre-relaxed
+basins (same cell shape and vacancy count, re-drawn vacancy sites and
+jitter) and moves to the next after --searches-per-basin searches (5, the
+kmcevents of the reference's example control file), each basin with its
+own de-duplication catalogue, so the rate is a steady state rather than a
+catalogue that saturates as the run goes on.
 Also reported: the plain unbatched GPU dimer (one walker, host-driven
 micro-steps) on the same cells, force evaluations per second, the roofline
 of the dominant kernel (Sutton-Chen pass-2 force) timed with HIP events on
@@ -91,12 +96,55 @@ def plain_dimer_rate(L, W, text, cells, basins, proposal_fn, params, n_search, b
     return ok, t_total, calls
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _omp_threads(n):
+    """set the OpenMP thread count of the oracle's fast build (libgomp)"""
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def cpu_force_timing(cells, basins, cores, repeats=10, warmup=3):
+    """median wall time of one FP64 Sutton-Chen force evaluation of the
+    restatement (steady_clock-like perf_counter, `warmup` untimed calls, then
+    the median of `repeats`) on the median-size walker cell, at `cores`
+    OpenMP threads and at 1 thread"""
+    from oracle import oracle as O
+    from akmc_amd import workloads as W
+    cu = W.SC_CU
+    f = O.sc_field(["Cu"], [("Cu", "Cu", cu["eps"], cu["a"], cu["n"], cu["m"], cu["c"])], 2 * cu["a"])
+    w = int(np.argsort([c.n for c in cells])[len(cells) // 2])
+    c = cells[w]
+    out = {"cell_atoms": c.n}
+    for label, nt in (("all_cores", cores), ("one_core", 1)):
+        _omp_threads(nt)
+        ts = []
+        for k in range(warmup + repeats):
+            t0 = time.perf_counter()
+            O.sc_forces(f, basins[w], c.types, c.lat, omp=True, fast=True)
+            if k >= warmup:
+                ts.append(time.perf_counter() - t0)
+        out[label] = {"threads": nt, "median_ms": round(1e3 * float(np.median(ts)), 3),
+                      "force_evals_per_s": round(1.0 / float(np.median(ts)), 2)}
+    _omp_threads(cores)
+    return out
+
+
 def cpu_baseline(cells, basins, proposal_fn, budget_s=20.0, max_searches=3):
     """FP64 C restatement of the reference (OpenMP Sutton-Chen force, as the
     reference's metal build) running the same dimer searches."""
     from oracle import oracle as O
     from akmc_amd import workloads as W
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    _omp_threads(cores)
     cu = W.SC_CU
     f = O.sc_field(["Cu"], [("Cu", "Cu", cu["eps"], cu["a"], cu["n"], cu["m"], cu["c"])], 2 * cu["a"])
     order = np.argsort([c.n for c in cells])
@@ -117,7 +165,8 @@ def cpu_baseline(cells, basins, proposal_fn, budget_s=20.0, max_searches=3):
             break
     sizes = [cells[w].n for w in pick[:done]]
     return dict(saddles_per_s=ok / t_total if t_total > 0 else 0.0, force_evals_per_s=calls / t_total,
-                cores=cores, searches=done, converged=ok, seconds=t_total, sizes=sizes)
+                cores=cores, searches=done, converged=ok, seconds=t_total, sizes=sizes,
+                force_timing=cpu_force_timing(cells, basins, cores), model=cpu_model())
 
 
 def copy_calibration(mib):
@@ -183,6 +232,8 @@ def learned_ab(camp, records, window, epochs):
                         "forward_tflops": round(fwd_flop / (fwd_ms * 1e-3) / 1e12, 2),
                         "forward_backward_tflops": round(fb_flop / (fb_ms * 1e-3) / 1e12, 2),
                         "peak_bf16_dense_tflops": 2500.0}}
+    k_turnover = camp.searches_per_basin
+    camp.searches_per_basin = None  # both arms search the same basins
     for name, prop in (("random", None), ("learned", LR.LearnedProposer(camp, net, seed=4))):
         drain(camp, S.CampaignStats())
         camp.proposer = prop
@@ -196,6 +247,7 @@ def learned_ab(camp, records, window, epochs):
                      "force_calls_per_verified": round(st.force_calls / max(st.verified, 1), 1)}
         print(f"[bench] {name} proposals: {out[name]}", file=sys.stderr, flush=True)
     camp.proposer = None
+    camp.searches_per_basin = k_turnover
     return out
 
 
@@ -265,6 +317,51 @@ def kmc_section(L, W, text, args):
     return out
 
 
+def kmc_c5_section(L, W, args):
+    """The full AKMC loop on one GPU: the 31,980-atom Cu/Ni Sutton-Chen alloy
+    (20 x 20 x 20 FCC, 50/50 seeded occupancy, 20 vacancies), T = 600 K,
+    prefactor 1e13 1/s, each KMC step's catalogue rebuilt by --c5-walkers
+    batched searches on local search cells (the 3600 atoms nearest each
+    vacancy), learned proposals for half of the walkers once the network has
+    been trained (after the first step) with a random-displacement fallback,
+    every saddle FP64-verified and followed by its device basin-2 stage;
+    --c5-steps KMC steps of the 1000 the workload names."""
+    from akmc_amd import kmc_cluster as KC
+    from akmc_amd.kmc import KMCParams
+    cell = W.alloy_cell(seed=7)
+    p = KMCParams(kmcTemperature=600.0, kmcPreFactor=1.0e13)
+    t0 = time.perf_counter()
+    run = KC.ClusterKMC(W.alloy_potentials_text(), cell, walkers=args.c5_walkers, cluster_atoms=3600, params=p,
+                        seed=args.seed, learned_fraction=0.5, skin=args.skin)
+    t_init = time.perf_counter() - t0
+    st = run.run(args.c5_steps)
+    steps = st.kmc_steps
+    per_step = [r.seconds for r in st.steps]
+    out = {"atoms": cell.n, "vacancies": len(cell.vacancies), "species": "Cu/Ni 50/50", "temperature_K": 600.0,
+           "prefactor_per_s": 1.0e13, "walkers": args.c5_walkers, "search_cell_atoms": run.K,
+           "search_cell_r_in_A": round(float(min(run.r_in)), 3), "kmc_steps": steps,
+           "kmc_steps_of_workload": 1000, "seconds": round(st.seconds, 2), "init_seconds": round(t_init, 2),
+           "kmc_steps_per_s": round(steps / st.seconds, 4),
+           "seconds_per_kmc_step": round(st.seconds / max(steps, 1), 2),
+           "projected_seconds_1000_steps": round(1000 * st.seconds / max(steps, 1), 0),
+           "valid_events_per_s": round(st.valid_events / st.seconds, 2),
+           "searches": st.searches, "learned_searches": st.learned_searches,
+           "fallback_searches": st.fallback_searches, "converged": st.converged, "verified": st.verified,
+           "new_basin": st.new_basin, "valid_events": st.valid_events, "learned_events": st.learned_events,
+           "learned_share_of_valid_events": round(st.learned_events / max(st.valid_events, 1), 3),
+           "out_of_window": st.out_of_window, "outside_search_cell": st.outside_cell, "cancelled": st.cancelled,
+           "force_calls_per_valid_event": round(st.force_calls / max(st.valid_events, 1), 1),
+           "micro_steps": st.micro_steps, "training_rows": st.training_rows,
+           "time_split_s": {"basin1_relax": round(st.t_relax, 2), "search_cells": round(st.t_cells, 2),
+                            "searches": round(st.t_search, 2), "training": round(st.t_train, 2)},
+           "per_step": [{"events": r.events, "learned_events": r.learned_events, "Ea_eV": round(r.activation_energy, 4),
+                         "dt_s": r.dt, "seconds": r.seconds} for r in st.steps],
+           "simulated_time_s": run.time, "basin1_energy_eV": round(run.e_full, 6)}
+    run.batch.close()
+    run.full.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,9 +380,15 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--no-kmc", action="store_true", help="skip the KMC-step section")
     ap.add_argument("--no-eam", action="store_true", help="skip the tabulated-EAM section")
+    ap.add_argument("--no-c5", action="store_true", help="skip the full-loop (31,980-atom alloy) section")
+    ap.add_argument("--c5-steps", type=int, default=3, help="KMC steps of the full-loop section")
+    ap.add_argument("--c5-walkers", type=int, default=1024)
     ap.add_argument("--kmc-systems", type=int, default=64)
     ap.add_argument("--kmc-steps", type=int, default=3)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--searches-per-basin", type=int, default=5,
+                    help="searches on a basin before the walker moves to its next pooled basin (0: never)")
+    ap.add_argument("--basin-pool", type=int, default=8, help="pre-relaxed basins per walker")
     ap.add_argument("--eager", action="store_true",
                     help="launch the micro-step kernels one by one instead of replaying hipGraphs "
                          "(for kernel tracers that cannot follow graph launches)")
@@ -328,6 +431,12 @@ def main():
     t_relax = time.perf_counter() - t0
     print(f"[bench] {len(cells)} walkers, {natoms.sum()} atoms; relaxed {ok.sum()} basins in {t_relax:.1f} s",
           file=sys.stderr, flush=True)
+    if args.searches_per_basin > 0 and args.basin_pool > 1:
+        t0 = time.perf_counter()
+        camp.relax_pool(args.basin_pool, seed=args.seed + 17)
+        camp.searches_per_basin = args.searches_per_basin
+        print(f"[bench] basin pool: {args.basin_pool} basins per walker relaxed in {time.perf_counter() - t0:.1f} s; "
+              f"a walker moves on after {args.searches_per_basin} searches", file=sys.stderr, flush=True)
 
     # ---- roofline of the force kernels at the basins (lists settled)
     camp.batch.upload([W.Cell(p, c.types, c.lat) for p, c in zip(camp.basins, cells)])
@@ -378,12 +487,13 @@ def main():
           f"{timed_tf:.3f} TFLOP/s ({int(evals.sum())} walker evaluations)", file=sys.stderr, flush=True)
     value = tot.distinct / t_timed
     verified_rate = tot.verified / t_timed
+    moves_timed = camp.basin_moves
     batched_force_evals = tot.force_calls / t_timed
 
     # ---- plain GPU dimer on the same cells
     n_ok, t_plain, plain_calls = plain_dimer_rate(L, W, text, cells, camp.basins, camp.proposal, dimer,
                                                   args.plain, camp.basin32, camp.basin64, eager=args.eager)
-    plain_rate = n_ok / t_plain if t_plain > 0 else 0.0
+    plain_rate = n_ok / t_plain if t_plain > 0 else 0.0  # 32 searches on 32 cells: every verified one is distinct
     print(f"[bench] plain GPU dimer: {n_ok}/{args.plain} converged in {t_plain:.2f}s "
           f"({plain_calls} force calls)", file=sys.stderr, flush=True)
 
@@ -402,6 +512,12 @@ def main():
     if not args.no_kmc:
         kmc = kmc_section(L, W, text, args)
         print(f"[bench] KMC steps: {kmc}", file=sys.stderr, flush=True)
+
+    c5 = None
+    if not args.no_c5:
+        c5 = kmc_c5_section(L, W, args)
+        print(f"[bench] full loop (alloy): { {k: v for k, v in c5.items() if k != 'per_step'} }", file=sys.stderr,
+              flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -434,18 +550,26 @@ def main():
                             "force_calls_per_search": round(plain_calls / max(1, args.plain), 1),
                             "ms_per_force_call": round(1e3 * t_plain / max(1, plain_calls), 4)},
         "batched_force_calls_per_verified": round(tot.force_calls / max(1, tot.verified), 1),
-        "batched_over_plain": round(verified_rate / plain_rate, 2) if plain_rate > 0 else None,
+        "basin_turnover": {"searches_per_basin": args.searches_per_basin, "pool_basins_per_walker": args.basin_pool,
+                           "basin_moves": int(moves_timed)},
+        "batched_over_plain": round(value / plain_rate, 2) if plain_rate > 0 else None,
+        "batched_over_plain_basis": "FP64-verified distinct saddles/s, batched (value) over the plain GPU dimer",
+        "batched_over_plain_verified": round(verified_rate / plain_rate, 2) if plain_rate > 0 else None,
         "force_evals_per_s_batched_cells": round(batched_force_evals, 1),
         "force_eval_ms_all_walkers": round(eval_ms, 4),
         "force_evals_per_s_cells_device": round(len(cells) / (eval_ms * 1e-3), 1),
         "autotune": {"lanes_per_atom": variant[0], "gathers_in_flight": variant[1], "lds_staged": bool(variant[2]),
                      "ms_per_eval": {f"{k[0]}x{k[1]}{' lds' if k[2] else ''}": round(v, 4)
                                      for k, v in variant_ms.items()}},
-        "roofline": {"bound": "mfma", "bound_note": "compute-bound on the FP32 vector ALU (pair physics has no "
-                     "matrix form); peak is gfx950's FP32 vector figure, equal to its FP32 matrix one",
+        "roofline": {"bound": "valu", "bound_note": "compute-bound on the FP32 vector ALU (the pair physics has "
+                     "no matrix form and uses no MFMA); peak is MI355X's published FP32 vector figure (packed "
+                     "FP32). This build compiles without packed FP32 instructions, so its reachable FP32 "
+                     "vector ceiling is half that: frac_vs_unpacked_ceiling",
                      "kernel": "k_sc_force (Sutton-Chen pass 2, autotuned variant)",
                      "achieved": round(timed_tf, 3), "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(timed_tf / FP32_VECTOR_PEAK_TFLOPS, 4),
+                     "unpacked_ceiling": FP32_VECTOR_PEAK_TFLOPS / 2,
+                     "frac_vs_unpacked_ceiling": round(timed_tf / (FP32_VECTOR_PEAK_TFLOPS / 2), 4),
                      "traffic": round(TRAFFIC_BYTES_PER_PAIR * Ptot) if TRAFFIC_BYTES_PER_PAIR else None,
                      "traffic_note": "HBM bytes of one all-walker launch at the basins (FETCH_SIZE x 2.0 + "
                                      "WRITE_SIZE x 1.0, scales from a 1 GiB copy), from the hardware-counter run",
@@ -464,12 +588,15 @@ def main():
         out["kmc"] = kmc
     if eam:
         out["tabulated_eam"] = eam
+    if c5:
+        out["kmc_c5"] = c5
     if cpu:
         out["cpu_baseline"] = {"value": round(cpu["saddles_per_s"], 5), "unit": "saddles/s",
-                               "cores": cpu["cores"], "kind": "port",
+                               "cores": cpu["cores"], "kind": "port", "cpu_model": cpu["model"],
                                "sample": f"{cpu['searches']} dimer searches on cells of {cpu['sizes']} atoms "
                                          f"({cpu['seconds']:.1f} s, O(N^2) FP64 Sutton-Chen, OpenMP)",
-                               "force_evals_per_s": round(cpu["force_evals_per_s"], 2)}
+                               "force_evals_per_s": round(cpu["force_evals_per_s"], 2),
+                               "force_eval_timing": cpu["force_timing"]}
         out["gpu_over_cpu_saddles"] = round(verified_rate / cpu["saddles_per_s"], 1) if cpu["saddles_per_s"] else None
         out["plain_gpu_over_cpu_saddles"] = round(plain_rate / cpu["saddles_per_s"], 1) if cpu["saddles_per_s"] else None
     print(json.dumps(out), flush=True)

@@ -60,6 +60,12 @@ int akmc_field_kind(akmc_field_t f);
 int akmc_field_set_ewald(akmc_field_t f, double eta, int gx, int gy, int gz);
 /* Verlet skin of the device neighbour lists (default 0.7 A). */
 int akmc_field_set_skin(akmc_field_t f, double skin);
+/* Number density (atoms per A^3) the neighbour lists of later batches are
+ * sized for, when it exceeds a walker's own N / V: walkers that hold a
+ * local cluster of a larger crystal in that crystal's periodic box (0, the
+ * default: each walker's N / V).  No reference counterpart (the reference
+ * keeps no neighbour lists on this path). */
+int akmc_field_set_list_density(akmc_field_t f, double atoms_per_A3);
 /* Pair-physics precision of later force calls: 32 (FP32 pair terms, the
  * default) or 64 (FP64 throughout; the verification path).  Cutoff tests,
  * minimum images, reciprocal sums and all accumulations are FP64 in both. */

@@ -441,3 +441,40 @@ def test_region_proposer_campaign(lib, oracle, style, width):
             assert near_vac < nn + 0.2
     assert st.converged >= 4
     assert st.verified >= 0.9 * st.converged
+
+
+def test_basin_turnover_campaign(lib, oracle):
+    """Basin turnover: with one search per basin and a pool of three
+    pre-relaxed basins per walker, walkers move on, and every verified
+    record's FP64 barrier is the oracle's energy difference between its
+    saddle and the pool basin it was searched on (so each record is judged
+    against its own basin's energies, catalogue and positions)."""
+    from akmc_amd import search as S
+    cells = W.batched_sc_cells(n_walkers=512, seed=3)
+    cells = [cells[i] for i in np.argsort([c.n for c in cells])[:8]]
+    text = lib.sc_potentials_text(_pots(), 2 * CU["a"])
+    camp = S.SaddleCampaign(text, cells, params=lib.dimer_defaults(maxParIter=400), seed=9)
+    camp.relax_basins()
+    camp.relax_pool(3, seed=21)
+    camp.searches_per_basin = 1
+    camp.begin()
+    st = camp.advance(1500, sync_every=25)
+    pools = sorted({r.pool for r in st.records})
+    print(f"turnover: {st.searches} ended, {st.verified} verified, {st.distinct} distinct, "
+          f"{camp.basin_moves} basin moves, pool entries seen {pools}", flush=True)
+    assert camp.basin_moves >= len(cells)
+    assert len(pools) >= 2
+    of = oracle.sc_field(["Cu"], _pots(), 2 * CU["a"])
+    checked = 0
+    for r in st.records:
+        if not r.verified or checked >= 6:
+            continue
+        c, basin = camp.pool[r.walker][r.pool][0], camp.pool[r.walker][r.pool][1]
+        sad = basin.copy()
+        sad[:, r.active] += r.active_disp
+        e_s = oracle.sc_energy(of, sad, c.types, c.lat)[0]
+        e_b = oracle.sc_energy(of, basin, c.types, c.lat)[0]
+        print(f"walker {r.walker} pool {r.pool}: barrier {r.barrier64:.6f} oracle {e_s - e_b:.6f}", flush=True)
+        assert abs((e_s - e_b) - r.barrier64) < 1e-6
+        checked += 1
+    assert checked >= 3
