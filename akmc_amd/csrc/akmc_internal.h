@@ -164,7 +164,8 @@ struct BatchDev {
 // image shift), whichever width the batch stores
 __device__ inline uint32_t nbr_entry(const BatchDev &b, int k, int i)
 {
-    const size_t o = (size_t)k * b.A + i;
+    // batches keep A below 2^24 and (maxnb + pad) A below 2^32
+    const uint32_t o = __umul24(uint32_t(k), uint32_t(b.A)) + uint32_t(i);
     return b.nb16 ? (uint32_t(reinterpret_cast<const uint16_t *>(b.nbr)[o]) | AKMC_NB_ZERO_SHIFT) : b.nbr[o];
 }
 

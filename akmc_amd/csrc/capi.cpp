@@ -659,6 +659,9 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     // neighbour capacity: 1.6x the expected count in the list sphere + 64
     double expect = 4.0 / 3.0 * kPI * rl * rl * rl * max_density;
     int maxnb = ((int)(1.6 * expect) + 64 + 31) / 32 * 32;
+    // list offsets are 32-bit row A + i products (24-bit operands)
+    if (A >= (1 << 24) || (size_t)(maxnb + AKMC_NB_PAD_ROWS) * (size_t)A >= ((size_t)1 << 32))
+        return fail(AKMC_ECAPACITY, "batch of %d atoms with %d list entries per atom is too large", A, maxnb);
     BatchDev &d = b->d;
     std::memset(&d, 0, sizeof d);
     d.A = A;

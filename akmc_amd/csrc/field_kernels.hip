@@ -129,29 +129,26 @@ __device__ inline void sc_powers_in(const double (&r2)[U], T a, T (&ir2)[U], T (
     }
 }
 
-// u2^(k/2) for U values at once, sqrt(u2) given for odd k: the uniform loop
-// control runs once per batch of pairs instead of once per pair
+// (a/r)^9 and (a/r)^6 of the single-pair kernels from u2 = (a/r)^2, with
+// the compile-time exponents of the Sutton-Chen pairs they serve: the
+// products of the repeated squaring, u8 sqrt(u2) and u2 u4, without its
+// loop
 template <typename T, int U>
-__device__ inline void pow_int_batch_rt(const T (&u2)[U], const T (&rt)[U], int k, T (&r)[U])
+__device__ inline void pow_9_6(const T (&u2)[U], const T (&rt)[U], T (&u9)[U], T (&u6)[U])
 {
-    T base[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        r[u] = T(1);
-        base[u] = u2[u];
+        const T u4 = u2[u] * u2[u];
+        const T u8 = u4 * u4;
+        u9[u] = u8 * (sizeof(T) == 8 ? rt[u] : root(u2[u]));
+        u6[u] = u2[u] * u4;
     }
-    for (int p = k >> 1; p; p >>= 1) {
-        if (p & 1) {
+}
+template <typename T, int U>
+__device__ inline void pow_6(const T (&u2)[U], T (&u6)[U])
+{
 #pragma unroll
-            for (int u = 0; u < U; ++u) r[u] *= base[u];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) base[u] *= base[u];
-    }
-    if (k & 1) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) r[u] *= sizeof(T) == 8 ? rt[u] : root(u2[u]);
-    }
+    for (int u = 0; u < U; ++u) u6[u] = u2[u] * (u2[u] * u2[u]);
 }
 
 template <typename T>
@@ -546,49 +543,76 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
     }
 }
 
-// explicit-image list (fields without the image convention): every (j, S)
-// with |r_i - r_j - S L| <= rc + skin except j == i, S == 0; the shell loop
-// is centred on the nearest image so unwrapped positions are handled
-__global__ void k_nbr_build_images(BatchDev b)
+// Explicit-image lists (fields without the image convention) hold every
+// (j, S) with |r_i - r_j - S L| <= rc + skin except j == i, S == 0; the shell
+// loop is centred on the nearest image so unwrapped positions are handled.
+// image_hits: one candidate j of atom i over those shells; calls put(entry)
+// for each image inside the list radius and on the pass's side of the near
+// shell, in (sa, sb, sc) ascending order
+template <typename Put>
+__device__ inline void image_hits(const BatchDev &b, const WalkerDev &W, int i, int jl, double xi, double yi,
+                                  double zi, double rl2, double rn2, int pass, Put put)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.A) return;
-    int w = walker_of(b, i);
-    if (!b.rebuild[w]) return;
-    const WalkerDev W = b.walkers[w];
-    const double rl2 = b.field->rlist2;
-    double xi = b.x[i], yi = b.y[i], zi = b.z[i];
-    int cnt = 0, nnear = 0;
-    const double rn2 = b.field->rnear2;
-    for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
-        for (int jl = 0; jl < W.n; ++jl) {
-            int j = W.off + jl;
-            double bx = xi - b.x[j], by = yi - b.y[j], bz = zi - b.z[j];
-            int ma = (int)rint(bx * W.invL[0]), mb = (int)rint(by * W.invL[1]), mc = (int)rint(bz * W.invL[2]);
-            for (int sa = ma - W.shells[0]; sa <= ma + W.shells[0]; ++sa) {
-                double dx = bx - sa * W.L[0];
-                for (int sb = mb - W.shells[1]; sb <= mb + W.shells[1]; ++sb) {
-                    double dy = by - sb * W.L[1];
-                    for (int sc = mc - W.shells[2]; sc <= mc + W.shells[2]; ++sc) {
-                        if (j == i && sa == 0 && sb == 0 && sc == 0) continue;
-                        double dz = bz - sc * W.L[2];
-                        double r2 = dx * dx + dy * dy + dz * dz;
-                        bool near = r2 <= rn2;
-                        if (r2 <= rl2 && (pass == 0 ? near : !near)) {
-                            if (sa < -7 || sa > 7 || sb < -7 || sb > 7 || sc < -7 || sc > 7) {
-                                *b.overflow = 1;
-                                continue;
-                            }
-                            push_nbr(b, i, cnt,
-                                     uint32_t(jl) | (uint32_t(sa + 7) << 20) | (uint32_t(sb + 7) << 24) |
-                                         (uint32_t(sc + 7) << 28));
-                        }
+    const int j = W.off + jl;
+    const double bx = xi - b.x[j], by = yi - b.y[j], bz = zi - b.z[j];
+    const int ma = (int)rint(bx * W.invL[0]), mb = (int)rint(by * W.invL[1]), mc = (int)rint(bz * W.invL[2]);
+    for (int sa = ma - W.shells[0]; sa <= ma + W.shells[0]; ++sa) {
+        const double dx = bx - sa * W.L[0];
+        for (int sb = mb - W.shells[1]; sb <= mb + W.shells[1]; ++sb) {
+            const double dy = by - sb * W.L[1];
+            for (int sc = mc - W.shells[2]; sc <= mc + W.shells[2]; ++sc) {
+                if (j == i && sa == 0 && sb == 0 && sc == 0) continue;
+                const double dz = bz - sc * W.L[2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const bool near = r2 <= rn2;
+                if (r2 <= rl2 && (pass == 0 ? near : !near)) {
+                    if (sa < -7 || sa > 7 || sb < -7 || sb > 7 || sc < -7 || sc > 7) {
+                        *b.overflow = 1;
+                        continue;
                     }
+                    put(uint32_t(jl) | (uint32_t(sa + 7) << 20) | (uint32_t(sb + 7) << 24) | (uint32_t(sc + 7) << 28));
                 }
             }
         }
+    }
+}
+
+// Explicit-image list of a rebuilding walker (no-image fields, rc > L/2):
+// one wavefront per atom.  Candidates are taken 64 at a time, one per
+// lane; each lane counts its hits, a wavefront prefix sum gives every lane
+// its first slot, and the lanes write their entries there.  The list holds
+// the near-shell entries first, then the rest, each in (j, image)
+// ascending order -- the order of a serial scan, so the list does not
+// depend on scheduling.
+__global__ void __launch_bounds__(kBlock) k_nbr_build_images(BatchDev b)
+{
+    constexpr int kAtomsPerBlock = kBlock / 64;
+    const int nblk = (b.A + kAtomsPerBlock - 1) / kAtomsPerBlock;
+    const int i = blockIdx.x * kAtomsPerBlock + int(threadIdx.x) / 64, lane = int(threadIdx.x) % 64;
+    if (blockIdx.x >= nblk || i >= b.A) return;  // whole wavefronts leave together
+    const int w = walker_of(b, i);
+    if (!b.rebuild[w]) return;
+    const WalkerDev W = b.walkers[w];
+    const double rl2 = b.field->rlist2, rn2 = b.field->rnear2;
+    const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
+    int cnt = 0, nnear = 0;
+    for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
+        for (int jb = 0; jb < W.n; jb += 64) {
+            const int jl = jb + lane;
+            int c = 0;
+            if (jl < W.n) image_hits(b, W, i, jl, xi, yi, zi, rl2, rn2, pass, [&](uint32_t) { ++c; });
+            int incl = c;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o);
+                if (lane >= o) incl += t;
+            }
+            int slot = cnt + incl - c;
+            if (c > 0) image_hits(b, W, i, jl, xi, yi, zi, rl2, rn2, pass, [&](uint32_t e) { put_nbr(b, i, slot++, e); });
+            cnt += __shfl(incl, 63);
+        }
         if (pass == 0) nnear = cnt;
     }
+    if (lane != 0) return;
     const int n = cnt < b.maxnb ? cnt : b.maxnb;
     b.nnb[i] = n;
     b.nnear[i] = nnear < b.maxnb ? nnear : b.maxnb;
@@ -660,7 +684,10 @@ __device__ inline void fetch_entries(const BatchDev &b, int i, int k, int cnt, u
     for (int u = 0; u < U; ++u) {
         const int kk = k + u * LPA;
         const int row = kk < cnt ? kk : cnt;
-        if constexpr (NB16) e[u] = reinterpret_cast<const uint16_t *>(b.nbr)[(size_t)row * b.A + i];
+        // element offset row A + i in 32 bits, one 24-bit multiply-add (the
+        // batch keeps A below 2^24 and the list's (maxnb + pad) A below 2^32)
+        const uint32_t o = __umul24(uint32_t(row), uint32_t(b.A)) + uint32_t(i);
+        if constexpr (NB16) e[u] = reinterpret_cast<const uint16_t *>(b.nbr)[o];
         else e[u] = nbr_entry(b, row, i);
     }
 }
@@ -728,8 +755,8 @@ __device__ inline AtomSlot sc_slot(const BatchDev &b, WalkerDev &W, double4 *sta
 // pass 1: rho_i = sum_j (eps c)^2 (a/r)^m (potentials/metal/Field.cpp:1782-1853),
 // then F(rho) = -sqrt(rho) (ManyBody.cpp:554-575) and 1/sqrt(rho) for pass 2.
 // LPA lanes share one atom's neighbour list (strided) and combine by shuffles.
-// ONE: the field has a single species pair with integer exponents, whose
-// coefficients stay in registers and whose powers take uniform control flow.
+// ONE: the field has a single species pair with n = 9, m = 6, whose
+// coefficients stay in registers and whose powers are straight products.
 template <typename T, bool MINIMG, int LPA, int U, bool ONE, bool STAGED>
 __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_density(BatchDev b)
 {
@@ -739,7 +766,6 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
     const FieldDev &F = *b.field;
     if (!ONE) stage_sc_table(F, tab);
     const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
-    const int m1 = F.sc[0].m_int;
     WalkerDev W;
     const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage);
     if (!sl.ok) return;
@@ -749,14 +775,24 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
     double rho = 0.0;
-    uint32_t en[U];
+    // entries two batches ahead: the next batch and the one after it are in
+    // flight while a batch's gathers and pair terms run (more bytes of the
+    // streamed list outstanding per wavefront)
+    uint32_t en[U], en2[U];
     fetch_entries<LPA, U, NB16>(b, i, sub, cnt, en);
+    fetch_entries<LPA, U, NB16>(b, i, sub + LPA * U, cnt, en2);
     for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
         uint32_t e[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) e[u] = en[u];
-        // next batch in flight
-        fetch_entries<LPA, U, NB16>(b, i, k0 + LPA * U, cnt, en);
+        for (int u = 0; u < U; ++u) {
+            e[u] = en[u];
+            en[u] = en2[u];
+        }
+        // the batch after next in flight; in the staged kernels (gathers from
+        // LDS) a scheduling barrier keeps these loads here, ahead of this
+        // batch's work, instead of next to their use two batches on
+        fetch_entries<LPA, U, NB16>(b, i, k0 + 2 * LPA * U, cnt, en2);
+        if constexpr (STAGED) __builtin_amdgcn_sched_barrier(0);
         double4 pj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -777,7 +813,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
             // only the pairs inside the cutoff are summed
             T ir2[U], u2[U], rt[U], um[U];
             sc_powers_in<T, U>(r2, c1.a, ir2, u2, rt);
-            pow_int_batch_rt<T, U>(u2, rt, m1, um);
+            pow_6<T, U>(u2, um);
 #pragma unroll
             for (int u = 0; u < U; ++u) rho += double(in[u] ? c1.rho_coef * um[u] : T(0));  // + exact zeros
         } else {
@@ -819,7 +855,6 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
     const FieldDev &F = *b.field;
     if (!ONE) stage_sc_table(F, tab);
     const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
-    const int n1 = F.sc[0].n_int, m1 = F.sc[0].m_int;
     WalkerDev W;
     const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage);
     if (!sl.ok) return;
@@ -832,14 +867,24 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
     const double isi64 = sizeof(T) == 8 ? b.isr64[i] : 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-    uint32_t en[U];
+    // entries two batches ahead: the next batch and the one after it are in
+    // flight while a batch's gathers and pair terms run (more bytes of the
+    // streamed list outstanding per wavefront)
+    uint32_t en[U], en2[U];
     fetch_entries<LPA, U, NB16>(b, i, sub, cnt, en);
+    fetch_entries<LPA, U, NB16>(b, i, sub + LPA * U, cnt, en2);
     for (int k0 = sub; k0 < cnt; k0 += LPA * U) {
         uint32_t e[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) e[u] = en[u];
-        // next batch in flight
-        fetch_entries<LPA, U, NB16>(b, i, k0 + LPA * U, cnt, en);
+        for (int u = 0; u < U; ++u) {
+            e[u] = en[u];
+            en[u] = en2[u];
+        }
+        // the batch after next in flight; in the staged kernels (gathers from
+        // LDS) a scheduling barrier keeps these loads here, ahead of this
+        // batch's work, instead of next to their use two batches on
+        fetch_entries<LPA, U, NB16>(b, i, k0 + 2 * LPA * U, cnt, en2);
+        if constexpr (STAGED) __builtin_amdgcn_sched_barrier(0);
         double4 pj[U];
         float isj[U];
         double isj64[U];
@@ -873,8 +918,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
             // only the pairs inside the cutoff are summed
             T u2[U], rt[U];
             sc_powers_in<T, U>(r2, c1.a, ir2, u2, rt);
-            pow_int_batch_rt<T, U>(u2, rt, n1, un);
-            pow_int_batch_rt<T, U>(u2, rt, m1, um);
+            pow_9_6<T, U>(u2, rt, un, um);
         }
         if constexpr (ONE && !STRESS) {
             // branch-free: a pair outside the cutoff (or a padding slot)
@@ -1284,14 +1328,26 @@ __device__ inline void ion_pair_terms(const FieldDev &F, const IonPairDev &p, do
 // On the FP32 path the leading near-shell entries of each list are
 // evaluated in FP64: they carry the large, cancelling Coulomb and
 // Buckingham terms.
+// The ionic kernels give each atom one wavefront: its list (about 550
+// entries at rc = 10 A) or the walker's k vectors are split over the 64
+// lanes (entry or k = lane, lane + 64, ...), whose sums meet by shuffles in
+// a fixed order.  256-thread workgroups of 4 atoms, dealt XCD-major, spread
+// even a single 510-ion cell over 128 CUs.  The lane count is fixed, so an
+// atom's sums do not depend on the batch.
+constexpr int kIonLanes = 64;
+constexpr int kIonAtomsPerBlock = kBlock / kIonLanes;
+
 template <typename T, bool MINIMG, bool STRESS>
 __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
 {
-    int i = xcd_atom((b.A + kBlock - 1) / kBlock);
-    if (i < 0 || i >= b.A) return;
+    const int nblk = (b.A + kIonAtomsPerBlock - 1) / kIonAtomsPerBlock;
+    const int per = (nblk + kXCD - 1) / kXCD;
+    const int blk = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+    const int i = blk * kIonAtomsPerBlock + int(threadIdx.x) / kIonLanes, sub = int(threadIdx.x) % kIonLanes;
+    if (blk >= nblk || i >= b.A) return;  // whole wavefronts leave together
     if (other_precision<T>(b, walker_of(b, i))) return;
-    const FieldDev &F = *b.field;
     const WalkerDev W = b.walkers[walker_of(b, i)];
+    const FieldDev &F = *b.field;
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
     const int ti = b.type[i];
     const double rc2 = F.rc2;
@@ -1300,11 +1356,12 @@ __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
     const double qi = F.cfact * b.q[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, er = 0.0, ev = 0.0;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-    for (int k = 0; k < cnt; ++k) {
+    for (int k = sub; k < cnt; k += kIonLanes) {
         uint32_t e = nbr_entry(b, k, i);
         int jl = int(e & AKMC_NB_JMASK);
         int j = W.off + jl;
         const double4 pj = b.p4[j];
+        const double qj = b.q[j];
         double dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
         entry_disp(e, dx, dy, dz, W, MINIMG);
         double r2 = dx * dx + dy * dy + dz * dz;
@@ -1313,13 +1370,13 @@ __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
         double df, dec, deb;
         if (k < nnear) {
             double forc, ec, eb;
-            ion_pair_terms<double>(F, p, r2, qi * b.q[j], forc, ec, eb, b.energy_only);
+            ion_pair_terms<double>(F, p, r2, qi * qj, forc, ec, eb, b.energy_only);
             df = forc;
             dec = ec;
             deb = eb;
         } else {
             T forc, ec, eb;
-            ion_pair_terms<T>(F, p, r2, qi * b.q[j], forc, ec, eb, b.energy_only);
+            ion_pair_terms<T>(F, p, r2, qi * qj, forc, ec, eb, b.energy_only);
             df = double(forc);
             dec = double(ec);
             deb = double(eb);
@@ -1340,6 +1397,20 @@ __global__ void __launch_bounds__(kBlock) k_ion_real(BatchDev b)
             s5 += h * dx * dy;
         }
     }
+    fx = lane_group_sum<kIonLanes>(fx);
+    fy = lane_group_sum<kIonLanes>(fy);
+    fz = lane_group_sum<kIonLanes>(fz);
+    er = lane_group_sum<kIonLanes>(er);
+    ev = lane_group_sum<kIonLanes>(ev);
+    if (STRESS) {
+        s0 = lane_group_sum<kIonLanes>(s0);
+        s1 = lane_group_sum<kIonLanes>(s1);
+        s2 = lane_group_sum<kIonLanes>(s2);
+        s3 = lane_group_sum<kIonLanes>(s3);
+        s4 = lane_group_sum<kIonLanes>(s4);
+        s5 = lane_group_sum<kIonLanes>(s5);
+    }
+    if (sub != 0) return;
     b.fx[i] = fx;
     b.fy[i] = fy;
     b.fz[i] = fz;
@@ -1392,49 +1463,86 @@ __device__ inline double2 phase_of(const double2 *t, int G1, int nx, int ny, int
     return make_double2(ab.x * c.x - ab.y * c.y, ab.x * c.y + ab.y * c.x);
 }
 
-// C_k = sum_i q_i cos(k.r_i), S_k = sum_i q_i sin(k.r_i) (Ewald.cpp:596-611);
-// grid (k blocks, walkers), FP64 sums in atom order
-__global__ void __launch_bounds__(kBlock) k_ewald_sk(BatchDev b)
+// C_k = sum_i q_i cos(k.r_i), S_k = sum_i q_i sin(k.r_i) (Ewald.cpp:596-611).
+// One 1024-thread workgroup per (tile of 64 k vectors, walker): lane l of
+// every wavefront owns k = 64 tile + l, wavefront v sums the atoms
+// v, v + 16, ... of the walker, and the 16 wavefront partials are added in
+// wavefront order through LDS (FP64 throughout; a fixed order, so the sums
+// do not depend on the batch).  The per-k products g0 C and g0 S of the
+// force pass are written with them.
+constexpr int kSkWaves = AKMC_STAGE_BLOCK / 64;
+
+__global__ void __launch_bounds__(AKMC_STAGE_BLOCK) k_ewald_sk(BatchDev b)
 {
-    int w = blockIdx.y;
+    __shared__ double part[2][kSkWaves][64];
+    const int w = blockIdx.y;
     if (skipped(b, w)) return;
     const WalkerDev W = b.walkers[w];
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= W.nk) return;
-    int kk = W.k_off + k;
-    int nx = b.kn[kk], ny = b.kn[b.nk_total + kk], nz = b.kn[2 * b.nk_total + kk];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    if (blockIdx.x * 64 >= W.nk) return;
+    const bool has = k < W.nk;
+    const int kk = W.k_off + (has ? k : 0);
+    const int nx = b.kn[kk], ny = b.kn[b.nk_total + kk], nz = b.kn[2 * b.nk_total + kk];
     double C = 0.0, S = 0.0;
-    for (int i = W.off; i < W.off + W.n; ++i) {
-        double2 p = phase_of(b.ptab + (size_t)i * 3 * b.G1, b.G1, nx, ny, nz);
-        double q = b.q[i];
-        C += q * p.x;
-        S += q * p.y;
+    if (has) {
+#pragma unroll 4
+        for (int i = W.off + wave; i < W.off + W.n; i += kSkWaves) {
+            const double2 p = phase_of(b.ptab + (size_t)i * 3 * b.G1, b.G1, nx, ny, nz);
+            const double q = b.q[i];
+            C += q * p.x;
+            S += q * p.y;
+        }
     }
-    b.Ck[kk] = C;
-    b.Sk[kk] = S;
+    part[0][wave][lane] = C;
+    part[1][wave][lane] = S;
+    __syncthreads();
+    if (wave == 0 && has) {
+        double Ct = part[0][0][lane], St = part[1][0][lane];
+        for (int v = 1; v < kSkWaves; ++v) {
+            Ct += part[0][v][lane];
+            St += part[1][v][lane];
+        }
+        b.Ck[kk] = Ct;
+        b.Sk[kk] = St;
+    }
 }
 
 // F_i += -g0 (-q sin C + q cos S) k;  E_i = 1/2 sum_k g0 q (cos C + sin S)
-// (Ewald.cpp:636-646)
+// (Ewald.cpp:636-646).  One wavefront per atom, its lanes over the walker's
+// k vectors (k = lane, lane + 64, ...: every load of k data is one coalesced
+// 64-wide read), lane sums by shuffles.
 __global__ void __launch_bounds__(kBlock) k_ewald_force(BatchDev b)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.A) return;
+    const int nblk = (b.A + kIonAtomsPerBlock - 1) / kIonAtomsPerBlock;
+    const int per = (nblk + kXCD - 1) / kXCD;
+    const int blk = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+    const int i = blk * kIonAtomsPerBlock + int(threadIdx.x) / kIonLanes, sub = int(threadIdx.x) % kIonLanes;
+    if (blk >= nblk || i >= b.A) return;
     if (skipped(b, walker_of(b, i))) return;
     const WalkerDev W = b.walkers[walker_of(b, i)];
     const double2 *t = b.ptab + (size_t)i * 3 * b.G1;
     const double q = b.q[i];
+    const int nk = b.nk_total;
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-    for (int k = 0; k < W.nk; ++k) {
-        int kk = W.k_off + k;
-        double2 p = phase_of(t, b.G1, b.kn[kk], b.kn[b.nk_total + kk], b.kn[2 * b.nk_total + kk]);
-        double cs = q * p.x, sn = q * p.y, C = b.Ck[kk], S = b.Sk[kk], g0 = b.g0[kk];
-        double pf = -g0 * (-sn * C + cs * S);
+#pragma unroll 2
+    for (int k = sub; k < W.nk; k += kIonLanes) {
+        const int kk = W.k_off + k;
+        const double2 p = phase_of(t, b.G1, b.kn[kk], b.kn[nk + kk], b.kn[2 * nk + kk]);
+        const double g0 = b.g0[kk];
+        const double gC = g0 * b.Ck[kk], gS = g0 * b.Sk[kk];
+        const double cs = q * p.x, sn = q * p.y;
+        const double pf = -(-sn * gC + cs * gS);
         fx += pf * b.kv[kk];
-        fy += pf * b.kv[b.nk_total + kk];
-        fz += pf * b.kv[2 * b.nk_total + kk];
-        e += g0 * (cs * C + sn * S);
+        fy += pf * b.kv[nk + kk];
+        fz += pf * b.kv[2 * nk + kk];
+        e += cs * gC + sn * gS;
     }
+    fx = lane_group_sum<kIonLanes>(fx);
+    fy = lane_group_sum<kIonLanes>(fy);
+    fz = lane_group_sum<kIonLanes>(fz);
+    e = lane_group_sum<kIonLanes>(e);
+    if (sub != 0) return;
     b.fx[i] += fx;
     b.fy[i] += fy;
     b.fz[i] += fz;
@@ -1566,15 +1674,16 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
         k_cell_bin<<<b.W, kBlock, 0, s>>>(b);
         k_nbr_build_cells<<<kNbGrid, kBlock, 0, s>>>(b);
     } else {
-        k_nbr_build_images<<<gA, kBlock, 0, s>>>(b);
+        k_nbr_build_images<<<(b.A + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(b);
     }
     k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, b.rebuild_count);
 }
 
-// a single species pair with integer exponents (the ONE kernels)
+// a single species pair with the exponents n = 9, m = 6 (the ONE kernels;
+// other pairs take the table kernels)
 static bool sc_one(const FieldDev &hf)
 {
-    return hf.nspec == 1 && hf.sc[0].on && hf.sc[0].n_int >= 0 && hf.sc[0].m_int >= 0;
+    return hf.nspec == 1 && hf.sc[0].on && hf.sc[0].n_int == 9 && hf.sc[0].m_int == 6;
 }
 
 // grid, workgroup size and dynamic LDS of a Sutton-Chen pair launch
@@ -1713,12 +1822,13 @@ static void launch_pass1(const BatchDev &b, const FieldDev &hf, int want_stress,
         AKMC_SC_VARIANT(AKMC_DENS);
 #undef AKMC_DENS
     } else {
+        const int gI = xcd_grid((b.A + kIonAtomsPerBlock - 1) / kIonAtomsPerBlock);
         if (hf.min_image) {
-            if (want_stress) k_ion_real<T, true, true><<<gA, kBlock, 0, s>>>(b);
-            else k_ion_real<T, true, false><<<gA, kBlock, 0, s>>>(b);
+            if (want_stress) k_ion_real<T, true, true><<<gI, kBlock, 0, s>>>(b);
+            else k_ion_real<T, true, false><<<gI, kBlock, 0, s>>>(b);
         } else {
-            if (want_stress) k_ion_real<T, false, true><<<gA, kBlock, 0, s>>>(b);
-            else k_ion_real<T, false, false><<<gA, kBlock, 0, s>>>(b);
+            if (want_stress) k_ion_real<T, false, true><<<gI, kBlock, 0, s>>>(b);
+            else k_ion_real<T, false, false><<<gI, kBlock, 0, s>>>(b);
         }
     }
 }
@@ -1736,8 +1846,8 @@ static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress,
         // the reciprocal energy needs the structure factors only
         if (hf.do_ewald == 1) {
             k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
-            dim3 g((b.maxnk + kBlock - 1) / kBlock, b.W);
-            k_ewald_sk<<<g, kBlock, 0, s>>>(b);
+            dim3 g((b.maxnk + 63) / 64, b.W);
+            k_ewald_sk<<<g, AKMC_STAGE_BLOCK, 0, s>>>(b);
         }
         return;
     }
@@ -1749,9 +1859,9 @@ static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress,
 #undef AKMC_FORCE
     } else if (hf.do_ewald == 1) {
         k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
-        dim3 g((b.maxnk + kBlock - 1) / kBlock, b.W);
-        k_ewald_sk<<<g, kBlock, 0, s>>>(b);
-        k_ewald_force<<<gA, kBlock, 0, s>>>(b);
+        dim3 g((b.maxnk + 63) / 64, b.W);
+        k_ewald_sk<<<g, AKMC_STAGE_BLOCK, 0, s>>>(b);
+        k_ewald_force<<<xcd_grid((b.A + kIonAtomsPerBlock - 1) / kIonAtomsPerBlock), kBlock, 0, s>>>(b);
     }
 }
 

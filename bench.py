@@ -45,6 +45,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64-verified saddles/sec at ≤0.01 eV barrier error; force evals/sec vs CPU"
 FP32_VECTOR_PEAK_TFLOPS = 157.3
+FP64_VECTOR_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per ordered pair of one all-walker k_sc_force launch, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on `bench.py --forces-only 5`,
@@ -317,6 +318,131 @@ def kmc_section(L, W, text, args):
     return out
 
 
+def ion_pairs_with_images(c, rc):
+    """unordered in-cutoff pairs of a cell including periodic images (the
+    reference's no-image loop over a <= b and image shifts): half the
+    ordered (i, j-image) pairs plus the self-image pairs, counted over the
+    27 nearest image cells (rc below the shortest cell length)"""
+    from scipy.spatial import cKDTree
+    Lb = np.array([c.lat[0], c.lat[4], c.lat[8]])
+    assert rc < Lb.min()
+    shifts = np.array([(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)], dtype=np.float64)
+    pts = (c.pos.T[None, :, :] + (shifts * Lb)[:, None, :]).reshape(-1, 3)
+    owner = np.tile(np.arange(c.n), 27)
+    tree = cKDTree(pts)
+    ordered, selfimg = 0, 0
+    for i, nb in enumerate(tree.query_ball_point(c.pos.T, rc)):
+        nb = np.asarray(nb)
+        ordered += nb.size - 1                               # minus (i, no shift)
+        selfimg += int((owner[nb] == i).sum()) - 1
+    return (ordered - selfimg) // 2 + selfimg
+
+
+def c2_section(L, W, args):
+    """Config 2, the ionic path: the rock-salt cell (510 ions, charges +-2,
+    Buckingham, Ewald eta = 0.35 1/A, rc = 10 A over explicit images, |n| < 8,
+    1147 k vectors).  Per-pass device times of one cell and of the 16-walker
+    batch (HIP events, time_phases), force evaluations per second, the
+    real-space roofline (45 FLOP per unordered in-cutoff pair, images
+    included, FP32 pair terms) and the reciprocal one (20 N K FP64 FLOP),
+    then 16 batched ART searches from seeded starts with device FP64
+    verification (ART saddles/s)."""
+    c = W.rocksalt_cell(seed=5)
+    pots = [("M", "X") + W.ROCKSALT_BUCK[("M", "X")], ("X", "X") + W.ROCKSALT_BUCK[("X", "X")],
+            ("M", "M") + W.ROCKSALT_BUCK[("M", "M")]]
+    f = L.Field(L.buck_potentials_text(["M", "X"], [2.0, -2.0], pots, 10.0, noimage=True))
+    f.set_ewald(0.35, 8)
+    nrep = 200
+    b1 = L.Batch(f, [c])
+    b1.forces()
+    K = f.num_kvectors
+    ph1 = b1.time_phases(nrep)  # ms per evaluation
+    Pc = ion_pairs_with_images(c, 10.0)
+    real_flop, recip_flop = 45.0 * Pc, 20.0 * c.n * K
+    out = {"cell": {"ions": c.n, "k_vectors": K, "unordered_pairs_with_images": int(Pc),
+                    "ms_lists": round(float(ph1[0]), 5), "ms_real_space": round(float(ph1[1]), 5),
+                    "ms_reciprocal": round(float(ph1[2]), 5), "ms_reduce": round(float(ph1[3]), 5),
+                    "force_evals_per_s_device": round(1e3 / float(ph1.sum()), 1),
+                    "real_space_tflops": round(real_flop / (ph1[1] * 1e-3) / 1e12, 4),
+                    "reciprocal_fp64_tflops": round(recip_flop / (ph1[2] * 1e-3) / 1e12, 4)}}
+    b1.close()
+    nw = 16
+    b = L.Batch(f, [c] * nw)
+    b.forces()
+    ph = b.time_phases(50)
+    out["batch16"] = {"ms_real_space": round(float(ph[1]), 5), "ms_reciprocal": round(float(ph[2]), 5),
+                      "ms_total": round(float(ph.sum()), 5),
+                      "force_evals_per_s_cells_device": round(nw * 1e3 / float(ph.sum()), 1)}
+    out["roofline_real_space"] = {"bound": "valu", "kernel": "k_ion_real (erfc real space + Buckingham)",
+                                  "achieved": round(nw * real_flop / (ph[1] * 1e-3) / 1e12, 4), "unit": "TFLOP/s",
+                                  "peak": FP32_VECTOR_PEAK_TFLOPS,
+                                  "frac": round(nw * real_flop / (ph[1] * 1e-3) / 1e12 / FP32_VECTOR_PEAK_TFLOPS, 5),
+                                  "flops_per_launch": nw * real_flop, "ms_per_launch": round(float(ph[1]), 5)}
+    out["roofline_reciprocal"] = {"bound": "valu", "kernel": "k_ewald_tables + k_ewald_sk + k_ewald_force (FP64)",
+                                  "achieved": round(nw * recip_flop / (ph[2] * 1e-3) / 1e12, 4), "unit": "TFLOP/s",
+                                  "peak": FP64_VECTOR_PEAK_TFLOPS,
+                                  "frac": round(nw * recip_flop / (ph[2] * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS, 5),
+                                  "flops_per_launch": nw * recip_flop, "ms_per_pass": round(float(ph[2]), 5)}
+    if args.c2_no_art:
+        b.close()
+        f.close()
+        return out
+    # 16 batched ART searches with device FP64 verification
+    b.set_frozen([np.zeros(c.n, np.int32)] * nw)
+    b.start_relax(walkers=[0])
+    b.run(20000, sync_every=50)
+    basin = b.positions()[0]
+    cb = W.Cell(basin, c.types, c.lat, charges=c.charges)
+    b.upload([cb] * nw)
+    b.save_basin()
+    e32, e64 = b.search_energies(32), b.search_energies(64)
+    site = c.vacancies[0]
+    d = W.min_image(basin - site[:, None], c.lat)
+    r = np.sqrt((d * d).sum(0))
+    frozen = (r >= 6.0).astype(np.int32)
+    cations = np.nonzero(c.types == 0)[0]
+    movers = cations[np.argsort(r[cations])[:12]]
+    x0s, modes = [], []
+    for w in range(nw):
+        disp, mode = W.local_displacement(cb, movers, 0.1, 100 + w)
+        x0s.append(basin + disp)
+        modes.append(mode)
+    b.set_verify(16, 0.01)
+    t0 = time.perf_counter()
+    b.start_art(modes, params=L.art_defaults(initTimeStep=0.0005, maxTimeStep=0.002, forceTol=1e-2, parDamp=0.25,
+                                             numLanczosVectors=25), x0_per_walker=x0s, frozen_per_walker=[frozen] * nw)
+    steps = b.run(200000, sync_every=50)
+    t_art = time.perf_counter() - t0
+    st = b.status()
+    conv = [s for s in st if s["status"] == 0]
+    ver = [s for w, s in enumerate(st) if s["status"] == 0 and s["verified"] and s["fmax64"] < 0.05
+           and abs((s["energy32"] - e32[w]) - (s["energy64"] - e64[w])) <= 0.01 and s["lambda1"] < 0 < s["lambda2"]]
+    out["art16"] = {"searches": nw, "converged": len(conv), "fp64_verified": len(ver), "seconds": round(t_art, 3),
+                    "micro_steps": steps, "force_calls": int(sum(s["force_calls"] for s in st)),
+                    "verified_saddles_per_s": round(len(ver) / t_art, 2),
+                    "barriers_eV": sorted(round(s["energy64"] - e64[w], 4) for w, s in enumerate(st)
+                                          if s["status"] == 0 and s["verified"])}
+    b.close()
+    f.close()
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ew = O.Ewald(c.lat, 0.35, gcell=8)
+        of = O.ion_field(["M", "X"], pots, rc=10.0, ewald=ew, min_image=False, lat=c.lat)
+        ts = []
+        for k in range(13):
+            t1 = time.perf_counter()
+            O.ion_forces(of, c.pos, c.types, c.charges, c.lat)
+            if k >= 3:
+                ts.append(time.perf_counter() - t1)
+        med = float(np.median(ts))
+        out["cpu_baseline"] = {"kind": "port", "cores": 1, "cpu_model": cpu_model(),
+                               "sample": "serial FP64 Buckingham + Ewald force evaluations of the same cell "
+                                         "(3 untimed, median of 10)",
+                               "median_ms": round(1e3 * med, 3), "force_evals_per_s": round(1.0 / med, 2),
+                               "gpu_cell_over_cpu": round((1e3 / float(ph1.sum())) / (1.0 / med), 1)}
+    return out
+
+
 def kmc_c5_section(L, W, args):
     """The full AKMC loop on one GPU: the 31,980-atom Cu/Ni Sutton-Chen alloy
     (20 x 20 x 20 FCC, 50/50 seeded occupancy, 20 vacancies), T = 600 K,
@@ -381,6 +507,9 @@ def main():
     ap.add_argument("--no-kmc", action="store_true", help="skip the KMC-step section")
     ap.add_argument("--no-eam", action="store_true", help="skip the tabulated-EAM section")
     ap.add_argument("--no-c5", action="store_true", help="skip the full-loop (31,980-atom alloy) section")
+    ap.add_argument("--no-c2", action="store_true", help="skip the ionic (rock-salt Ewald) section")
+    ap.add_argument("--c2-only", action="store_true", help="only the ionic section (for kernel traces)")
+    ap.add_argument("--c2-no-art", action="store_true", help="ionic section without the ART searches")
     ap.add_argument("--c5-steps", type=int, default=3, help="KMC steps of the full-loop section")
     ap.add_argument("--c5-walkers", type=int, default=1024)
     ap.add_argument("--kmc-systems", type=int, default=64)
@@ -411,6 +540,9 @@ def main():
     L.load()
     if args.forces_only:
         forces_only(L, W, args)
+        return
+    if args.c2_only:
+        print(json.dumps({"c2": c2_section(L, W, args)}), flush=True)
         return
     cu = W.SC_CU
     rc = 2 * cu["a"]
@@ -513,6 +645,11 @@ def main():
         kmc = kmc_section(L, W, text, args)
         print(f"[bench] KMC steps: {kmc}", file=sys.stderr, flush=True)
 
+    c2 = None
+    if not args.no_c2:
+        c2 = c2_section(L, W, args)
+        print(f"[bench] ionic section: {c2}", file=sys.stderr, flush=True)
+
     c5 = None
     if not args.no_c5:
         c5 = kmc_c5_section(L, W, args)
@@ -590,6 +727,8 @@ def main():
         out["tabulated_eam"] = eam
     if c5:
         out["kmc_c5"] = c5
+    if c2:
+        out["ionic_c2"] = c2
     if cpu:
         out["cpu_baseline"] = {"value": round(cpu["saddles_per_s"], 5), "unit": "saddles/s",
                                "cores": cpu["cores"], "kind": "port", "cpu_model": cpu["model"],
