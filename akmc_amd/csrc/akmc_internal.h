@@ -305,5 +305,6 @@ void akmc_mlp_launch_backward(const float *X, const float *T, int M, const uint1
                               float *grad, hipStream_t s);
 void akmc_mlp_launch_adam(float *p, const float *g, float *m1, float *m2, int n, float lr, int t, hipStream_t s);
 void akmc_mlp_launch_gather(const float *src, int width, const int *idx, int M, float *dst, hipStream_t s);
+void akmc_mlp_launch_loss_accum(const double *part, int nb, double scale, double *acc, hipStream_t s);
 void akmc_mlp_launch_descriptors(const BatchDev &b, int nq, const int *qatom, const double *qvac,
                                  const double *qmov, float *X, hipStream_t s);

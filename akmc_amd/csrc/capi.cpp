@@ -2274,13 +2274,17 @@ int akmc_mlp_train(akmc_mlp_t m, const float *X, const float *T, int M, int epoc
         return fail(AKMC_EINVAL, "bad training arguments (batch must be <= max_rows)");
     float *dX = nullptr, *dT = nullptr;
     int *didx = nullptr;
+    double *dacc = nullptr;  // per-epoch loss accumulator (no host sync per minibatch)
     HIPCHK(hipMalloc(&dX, (size_t)M * 48 * sizeof(float)));
     HIPCHK(hipMalloc(&dT, (size_t)M * 3 * sizeof(float)));
     HIPCHK(hipMalloc(&didx, (size_t)M * sizeof(int)));
+    HIPCHK(hipMalloc(&dacc, (size_t)epochs * sizeof(double)));
+    HIPCHK(hipMemset(dacc, 0, (size_t)epochs * sizeof(double)));
     auto cleanup = [&]() {
         (void)hipFree(dX);
         (void)hipFree(dT);
         (void)hipFree(didx);
+        (void)hipFree(dacc);
     };
     if (hipMemcpy(dX, X, (size_t)M * 48 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dT, T, (size_t)M * 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
@@ -2296,7 +2300,13 @@ int akmc_mlp_train(akmc_mlp_t m, const float *X, const float *T, int M, int epoc
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
         return z ^ (z >> 31);
     };
+    int nbatch_ep = 1;
     for (int ep = 0; ep < epochs; ++ep) {
+        // the previous epoch's copy of perm must be done before perm changes
+        if (ep > 0 && hipStreamSynchronize(m->stream) != hipSuccess) {
+            cleanup();
+            return fail(AKMC_EHIP, "training step failed");
+        }
         for (int i = 0; i < M; ++i) perm[i] = i;
         for (int i = M - 1; i > 0; --i) std::swap(perm[i], perm[next() % (unsigned long long)(i + 1)]);
         if (hipMemcpyAsync(didx, perm.data(), (size_t)M * sizeof(int), hipMemcpyHostToDevice, m->stream) !=
@@ -2304,33 +2314,35 @@ int akmc_mlp_train(akmc_mlp_t m, const float *X, const float *T, int M, int epoc
             cleanup();
             return fail(AKMC_EHIP, "permutation upload failed");
         }
-        double lsum = 0.0;
         int nb = 0;
         for (int b0 = 0; b0 + batch <= M || (b0 == 0 && M < batch); b0 += batch) {
             int n = std::min(batch, M - b0);
             akmc_mlp_launch_gather(dX, 48, didx + b0, n, m->X, m->stream);
             akmc_mlp_launch_gather(dT, 3, didx + b0, n, m->T, m->stream);
-            double l = 0.0;
-            int rc = mlp_loss_grad_dev(m, n, &l);
+            int rc = mlp_loss_grad_dev(m, n, nullptr);
             if (rc != AKMC_SUCCESS) {
                 cleanup();
                 return rc;
             }
+            akmc_mlp_launch_loss_accum(m->lpart, (n + 255) / 256, 1.0 / n, dacc + ep, m->stream);
             akmc_mlp_launch_adam(m->p, m->g, m->m1, m->m2, AKMC_MLP_NPARAM, lr, ++m->adam_t, m->stream);
             rc = mlp_refresh_weights(m);
             if (rc != AKMC_SUCCESS) {
                 cleanup();
                 return rc;
             }
-            lsum += l;
             ++nb;
             if (M < batch) break;
         }
-        if (loss_history) loss_history[ep] = lsum / std::max(nb, 1);
+        nbatch_ep = std::max(nb, 1);
     }
-    hipError_t e = hipStreamSynchronize(m->stream);
+    std::vector<double> acc(epochs);
+    hipError_t e = hipMemcpyAsync(acc.data(), dacc, (size_t)epochs * sizeof(double), hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     cleanup();
     if (e != hipSuccess) return fail(AKMC_EHIP, "training failed: %s", hipGetErrorString(e));
+    if (loss_history)
+        for (int ep = 0; ep < epochs; ++ep) loss_history[ep] = acc[ep] / nbatch_ep;
     return AKMC_SUCCESS;
 }
 

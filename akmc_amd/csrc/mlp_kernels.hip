@@ -208,7 +208,10 @@ __device__ inline float fcut(float r, float rc) { return r < rc ? 0.5f * (__cosf
 
 // one query per thread: atom i of walker w with vacancy site v and candidate
 // mover position u; neighbours from the walker's list at its resident
-// positions (minimum image); see oracle/mlp_oracle.py for the definitions
+// positions (minimum image).  48 descriptors: 32 radial G2 (16 over the
+// atoms, 8 over the vacancy site and 8 over the mover as pseudo-neighbours
+// of their own kinds) and 16 angular G4; the definitions are restated in
+// FP64 by the test oracle
 __global__ void __launch_bounds__(kB) k_descriptors(BatchDev b, int nq, const int *__restrict__ qatom,
                                                     const double *__restrict__ qvac, const double *__restrict__ qmov,
                                                     float *__restrict__ X)
@@ -232,8 +235,8 @@ __global__ void __launch_bounds__(kB) k_descriptors(BatchDev b, int nq, const in
         float r = (float)sqrt(dx * dx + dy * dy + dz * dz);
         if (r < 1.0e-8f || r >= RC) continue;
         float f = fcut(r, RC);
-        for (int t = 0; t < 24; ++t) {
-            float d = r - (2.0f + 5.0f * t / 23.0f);
+        for (int t = 0; t < 16; ++t) {
+            float d = r - (2.0f + 5.0f * t / 15.0f);
             g[t] += __expf(-ETA_R * d * d) * f;
         }
         if (r >= RCA) continue;
@@ -285,11 +288,14 @@ __global__ void __launch_bounds__(kB) k_descriptors(BatchDev b, int nq, const in
         d2 -= W.L[a] * rint(d2 * W.invL[a]);
         du2 += d2 * d2;
     }
+    // the vacancy site and the candidate mover as pseudo-neighbours of their
+    // own kinds: 8 radial G2 terms each, with the same cutoff function
     float dv = (float)sqrt(dv2), du = (float)sqrt(du2);
-    const float S[4] = {2.55f, 3.61f, 4.42f, 5.11f}, Mc[4] = {0.0f, 2.55f, 3.61f, 4.42f};
-    for (int t = 0; t < 4; ++t) {
-        g[24 + t] = __expf(-ETA_S * (dv - S[t]) * (dv - S[t]));
-        g[28 + t] = __expf(-ETA_S * (du - Mc[t]) * (du - Mc[t]));
+    const float fv = fcut(dv, RC), fu = fcut(du, RC);
+    for (int t = 0; t < 8; ++t) {
+        const float s = dv - (1.5f + 5.0f * t / 7.0f), m = du - 6.0f * t / 7.0f;
+        g[16 + t] = __expf(-ETA_S * s * s) * fv;
+        g[24 + t] = __expf(-ETA_S * m * m) * fu;
     }
     for (int t = 0; t < 48; ++t) X[(size_t)q * 48 + t] = g[t];
 }
@@ -349,6 +355,21 @@ void akmc_mlp_launch_adam(float *p, const float *g, float *m1, float *m2, int n,
     const float b1 = 0.9f, b2 = 0.999f;
     float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
     k_adam<<<(n + kB - 1) / kB, kB, 0, s>>>(p, g, m1, m2, n, lr, b1, b2, 1.0e-8f, c1, c2);
+}
+
+// the training loss of one minibatch, added to a device accumulator: the
+// batch's per-block partials summed in block order and scaled by 1/rows
+__global__ void k_loss_accum(const double *part, int nb, double scale, double *acc)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double t = 0.0;
+    for (int i = 0; i < nb; ++i) t += part[i];
+    *acc += t * scale;
+}
+
+void akmc_mlp_launch_loss_accum(const double *part, int nb, double scale, double *acc, hipStream_t s)
+{
+    k_loss_accum<<<1, 64, 0, s>>>(part, nb, scale, acc);
 }
 
 void akmc_mlp_launch_gather(const float *src, int width, const int *idx, int M, float *dst, hipStream_t s)

@@ -114,7 +114,7 @@ __device__ inline double wrap(double d, double L, double iL) { return d - L * ri
 // norm = basin2Delta * |Delta|; the saddle is kept in Xs and the walker
 // FIRE-relaxes from there with the search's frozen mask
 // (KineticMonteCarlo.cpp:1686)
-__device__ void enter_basin2(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void enter_basin2(const Ctx &C_, WalkerState &w, double *sh)
 {
     const WalkerDev &W = C_.b.walkers[blockIdx.x];
     double acc[1] = {0.0};
@@ -153,14 +153,14 @@ __device__ void enter_basin2(const Ctx &C_, WalkerState &w, double *sh)
 
 // a converged search that is not verified (or whose verification ended)
 // goes on to its basin 2 when that stage is on
-__device__ void end_converged(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void end_converged(const Ctx &C_, WalkerState &w, double *sh)
 {
     if (C_.s.basin2_on && C_.s.basin) enter_basin2(C_, w, sh);
     else finish(C_, w);
 }
 
 // a converged saddle is re-verified in FP64 when verification is enabled
-__device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature, double *sh)
+__device__ __attribute__((always_inline)) void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature, double *sh)
 {
     w.e32 = w.E0;
     w.c32 = curvature;
@@ -177,7 +177,7 @@ __device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double cur
 
 // ------------------------------------------------------------------ relax
 
-__device__ void relax_end(const Ctx &C_, WalkerState &w, double *sh, int status);
+__device__ __attribute__((always_inline)) void relax_end(const Ctx &C_, WalkerState &w, double *sh, int status);
 
 // the part of a FIRE / FIRE2 iteration before its force call: the power
 // test and the dt / alpha schedule, the velocity mixing and the first half
@@ -191,7 +191,7 @@ __device__ void relax_end(const Ctx &C_, WalkerState &w, double *sh, int status)
 // (verletIntegration, :194-225 / :519-560) kicks the free atoms' velocities
 // by half a step of the old forces and zeroes the frozen ones before the
 // drift.
-__device__ void relax_pre(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void relax_pre(const Ctx &C_, WalkerState &w, double *sh)
 {
     const RelaxParamsDev &p = C_.s.rp;
     const bool fire2 = p.method == 1;
@@ -276,7 +276,7 @@ __device__ void relax_pre(const Ctx &C_, WalkerState &w, double *sh)
 // the decision.  With final64 on, one FP64 energy evaluation at the relaxed
 // positions follows (their forces were just evaluated, so the probes
 // already sit at X).
-__device__ void relax_end(const Ctx &C_, WalkerState &w, double *sh, int status)
+__device__ __attribute__((always_inline)) void relax_end(const Ctx &C_, WalkerState &w, double *sh, int status)
 {
     if (w.kind == WK_BASIN2) {
         const WalkerDev &W = C_.b.walkers[blockIdx.x];
@@ -307,7 +307,7 @@ __device__ void relax_end(const Ctx &C_, WalkerState &w, double *sh, int status)
     finish(C_, w);
 }
 
-__device__ void relax_logic(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void relax_logic(const Ctx &C_, WalkerState &w, double *sh)
 {
     const RelaxParamsDev &p = C_.s.rp;
     if (w.state == ST_R_E64) {
@@ -387,11 +387,11 @@ __device__ void relax_logic(const Ctx &C_, WalkerState &w, double *sh)
 
 // ------------------------------------------------------------------ dimer
 
-__device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh);
+__device__ __attribute__((always_inline)) void dimer_translate(const Ctx &C_, WalkerState &w, double *sh);
 
 // rotation bookkeeping: decide between another trial rotation and the
 // translation (the rotation loop of the CPU oracle's dimer)
-__device__ void dimer_rot_eval(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void dimer_rot_eval(const Ctx &C_, WalkerState &w, double *sh)
 {
     const DimerParamsDev &p = C_.s.dp;
     const double D = p.dimerDisplacement;
@@ -440,7 +440,7 @@ __device__ void dimer_rot_eval(const Ctx &C_, WalkerState &w, double *sh)
 }
 
 // translation by FIRE on the effective force; convergence test
-__device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
 {
     const DimerParamsDev &p = C_.s.dp;
     const double D = p.dimerDisplacement;
@@ -531,7 +531,7 @@ __device__ void dimer_translate(const Ctx &C_, WalkerState &w, double *sh)
     write_probe(C_, nullptr, 0.0);
 }
 
-__device__ void dimer_logic(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void dimer_logic(const Ctx &C_, WalkerState &w, double *sh)
 {
     const DimerParamsDev &p = C_.s.dp;
     const double D = p.dimerDisplacement;
@@ -624,7 +624,7 @@ __device__ inline int sturm_count(int n, const double *d, const double *e, doubl
 // evaluate Sturm counts at 64 interior points of the bracket, which shrinks
 // 65-fold per round, from the Gershgorin interval down to the last bits.
 // Every lane of the wavefront must call it; all return the same value.
-__device__ double tridiag_kth(int n, const double *d, const double *e, int k)
+__device__ __attribute__((always_inline)) double tridiag_kth(int n, const double *d, const double *e, int k)
 {
     double lo = d[0], hi = d[0];
     for (int i = 0; i < n; ++i) {
@@ -654,7 +654,7 @@ __device__ double tridiag_kth(int n, const double *d, const double *e, int k)
 // the one-sided product of the reference's ART Lanczos carries an
 // O(delta E''') bias that, for vectors concentrated on stiff contacts, can
 // put a Ritz value below the true lowest curvature.
-__device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
 {
     const double delta = C_.s.verify_delta;
     const int nl = C_.s.verify_nl < AKMC_LZ_MAX ? C_.s.verify_nl : AKMC_LZ_MAX;
@@ -795,7 +795,7 @@ __device__ void verify_logic(const Ctx &C_, WalkerState &w, double *sh)
 
 // lowest eigenvector of the n x n symmetric tridiagonal (d, e) for the
 // eigenvalue lam by inverse iteration (three sweeps); single thread
-__device__ void tridiag_lowest_vector(int n, const double *d, const double *e, double lam, double *y)
+__device__ __attribute__((always_inline)) void tridiag_lowest_vector(int n, const double *d, const double *e, double lam, double *y)
 {
     double scale = 0.0;
     for (int i = 0; i < n; ++i) scale = fmax(scale, fabs(d[i]) + (i < n - 1 ? fabs(e[i]) : 0.0));
@@ -822,12 +822,12 @@ __device__ void tridiag_lowest_vector(int n, const double *d, const double *e, d
     }
 }
 
-__device__ void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh);
-__device__ void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature, double *sh);
+__device__ __attribute__((always_inline)) void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh);
+__device__ __attribute__((always_inline)) void enter_verify_or_finish(const Ctx &C_, WalkerState &w, double curvature, double *sh);
 
 // start of ART::calculateLanczosForce (optim/ART.cpp:159-190): force at x,
 // q_0 = r / |r| with r the current lowest-mode estimate
-__device__ void art_lanczos_begin(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void art_lanczos_begin(const Ctx &C_, WalkerState &w, double *sh)
 {
     const ARTParamsDev &p = C_.s.ap;
     double nn[1] = {0.0};
@@ -861,7 +861,7 @@ __device__ void art_lanczos_begin(const Ctx &C_, WalkerState &w, double *sh)
 
 // ART after a Lanczos evaluation: velocity half kick and convergence test
 // of the previous iteration (optim/ART.cpp:660-720), or entry into the loop
-__device__ void art_after_lanczos(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void art_after_lanczos(const Ctx &C_, WalkerState &w, double *sh)
 {
     const ARTParamsDev &p = C_.s.ap;
     if (!w.inloop) {
@@ -897,7 +897,7 @@ __device__ void art_after_lanczos(const Ctx &C_, WalkerState &w, double *sh)
 
 // one Lanczos step (optim/ART.cpp:191-283); at the end the Ritz vector,
 // F_par = F - 2(F.v)v and F_perp = -(F.v)v (ART.cpp:318-349)
-__device__ void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
 {
     const ARTParamsDev &p = C_.s.ap;
     const size_t A3 = 3 * (size_t)C_.A;
@@ -1016,7 +1016,7 @@ __device__ void art_lanczos_step(const Ctx &C_, WalkerState &w, double *sh)
 
 // the FIRE part of one ART iteration before the next Lanczos evaluation
 // (optim/ART.cpp:549-656)
-__device__ void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh)
 {
     const ARTParamsDev &p = C_.s.ap;
     if (w.it > p.maxParIter) {
@@ -1092,7 +1092,7 @@ __device__ void art_fire_pre(const Ctx &C_, WalkerState &w, double *sh)
     write_probe(C_, nullptr, 0.0);
 }
 
-__device__ void art_logic(const Ctx &C_, WalkerState &w, double *sh)
+__device__ __attribute__((always_inline)) void art_logic(const Ctx &C_, WalkerState &w, double *sh)
 {
     w.fcalls += 1;
     if (w.state == ST_A_L0) art_lanczos_begin(C_, w, sh);

@@ -51,7 +51,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on `bench.py --forces-only 5`,
 # calibrated on a 1 GiB device copy (FETCH_SIZE reads half the copied bytes,
 # WRITE_SIZE all of them); the summary is under profiles/
-TRAFFIC_BYTES_PER_PAIR = 3.281
+TRAFFIC_BYTES_PER_PAIR = 3.35
 
 
 def ordered_pairs(cells, positions, rc):
@@ -208,15 +208,25 @@ def drain(camp, stats):
             camp.batch.run(25, sync_every=25)
 
 
-def learned_ab(camp, records, window, epochs):
-    """Config 4 on the bench's own walkers: train the MLP on the timed
-    windows' FP64-verified saddles, time its MFMA kernels, then run the same
-    cells from random and from learned proposals (fresh de-duplication, one
-    warm-up and one timed window each)."""
+def learned_ab(camp, records, window, epochs, train_saddles):
+    """Config 4 on the bench's own walkers: train the MLP on FP64-verified
+    saddles of the batched campaign (the timed windows' and, until
+    train_saddles are in, further windows of the same campaign), time its
+    MFMA kernels, then run the same cells from random and from learned
+    proposals (fresh de-duplication, one warm-up and one timed window
+    each)."""
     from akmc_amd import learn as LR
     from akmc_amd import search as S
+    recs = [r for r in records if r.verified]
+    t0 = time.perf_counter()
+    extra = 0
+    while len(recs) < train_saddles:
+        st = camp.advance(window, sync_every=25)
+        recs.extend(r for r in st.records if r.verified)
+        extra += 1
+    t_gen = time.perf_counter() - t0
     drain(camp, S.CampaignStats())
-    X, T = LR.dataset_from_records(camp, [r for r in records if r.verified])
+    X, T = LR.dataset_from_records(camp, recs)
     net = LR.MLP(max_rows=8192, seed=1)
     net.fit_scaling(X)
     t0 = time.perf_counter()
@@ -226,7 +236,9 @@ def learned_ab(camp, records, window, epochs):
     fwd_ms, fb_ms = net.time(rows, 20)
     fwd_flop = 2.0 * 22912 * rows               # 48x128 + 128x128 + 128x3 MACs per row
     fb_flop = fwd_flop + 2.0 * 39680 * rows     # + dZ2, dZ1 and the three weight gradients
-    out = {"training_rows": int(X.shape[0]), "training_saddles": int(sum(r.verified for r in records)),
+    out = {"training_rows": int(X.shape[0]), "training_saddles": len(recs),
+           "training_data": f"FP64-verified saddles of the batched campaign ({extra} windows beyond the timed ones, "
+                            f"{t_gen:.1f} s)",
            "epochs": epochs, "train_seconds": round(t_train, 2),
            "loss_first_last": [round(float(hist[0]), 5), round(float(hist[-1]), 5)],
            "mlp_mfma": {"rows": rows, "forward_ms": round(fwd_ms, 4), "forward_backward_ms": round(fb_ms, 4),
@@ -504,6 +516,8 @@ def main():
     ap.add_argument("--skin", type=float, default=None, help="Verlet skin of the device lists (A)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal section")
     ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--train-saddles", type=int, default=50000,
+                    help="FP64-verified saddles of the campaign the MLP is trained on")
     ap.add_argument("--no-kmc", action="store_true", help="skip the KMC-step section")
     ap.add_argument("--no-eam", action="store_true", help="skip the tabulated-EAM section")
     ap.add_argument("--no-c5", action="store_true", help="skip the full-loop (31,980-atom alloy) section")
@@ -631,7 +645,7 @@ def main():
 
     learned = None
     if not args.no_learned:
-        learned = learned_ab(camp, tot.records, args.window, args.train_epochs)
+        learned = learned_ab(camp, tot.records, args.window, args.train_epochs, args.train_saddles)
         print(f"[bench] learned section: { {k: v for k, v in learned.items() if k not in ('random', 'learned')} }",
               file=sys.stderr, flush=True)
 

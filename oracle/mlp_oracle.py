@@ -8,11 +8,18 @@ learned component, so this oracle is pinned by its own checks in
 tests/test_mlp_oracle.py (finite differences of the loss, descriptor
 invariances and closed forms).
 
-Descriptor vector of atom i (48 values), cosine cutoff fc(r) = (cos(pi r/rc)+1)/2:
-  0..23  G2 over neighbours:  sum_j exp(-ETA_R (r_ij - R_k)^2) fc(r_ij),
-         R_k = 2.0 + 5.0 k / 23
-  24..27 vacancy channel:     exp(-ETA_S (|r_i - v| - S_k)^2), S = (2.55, 3.61, 4.42, 5.11)
-  28..31 mover channel:       exp(-ETA_S (|r_i - r_m| - M_k)^2), M = (0, 2.55, 3.61, 4.42)
+Descriptor vector of atom i (48 values): 32 radial G2 and 16 angular G4
+symmetry functions.  The radial ones are resolved by neighbour kind, the
+atoms of the cell and two pseudo-neighbours that carry the proposal's
+context: the vacancy site v and the candidate mover m (the way
+element-resolved G2 sets treat a second species).  Cosine cutoff
+fc(r) = (cos(pi r/rc)+1)/2 at RC = 7.22 A:
+  0..15  G2 over atoms:     sum_j exp(-ETA_R (r_ij - R_k)^2) fc(r_ij),
+         R_k = 2.0 + 5.0 k / 15
+  16..23 G2 over the site:  exp(-ETA_S (|r_i - v| - S_k)^2) fc(|r_i - v|),
+         S_k = 1.5 + 5.0 k / 7
+  24..31 G2 over the mover: exp(-ETA_S (|r_i - r_m| - M_k)^2) fc(|r_i - r_m|),
+         M_k = 6.0 k / 7
   32..47 G4 over neighbour pairs j < k inside RC_ANG:
          2^(1-zeta) sum (1 + lam cos t_ijk)^zeta exp(-eta (r_ij^2 + r_ik^2 + r_jk^2))
          fa(r_ij) fa(r_ik) fa(r_jk), fa the cosine cutoff at RC_ANG = 5.0,
@@ -26,10 +33,10 @@ import numpy as np
 RC = 7.22
 RC_ANG = 5.0
 ETA_R = 8.0
-R_CENTRES = 2.0 + 5.0 * np.arange(24) / 23.0
+R_CENTRES = 2.0 + 5.0 * np.arange(16) / 15.0
 ETA_S = 2.0
-S_CENTRES = np.array([2.55, 3.61, 4.42, 5.11])
-M_CENTRES = np.array([0.0, 2.55, 3.61, 4.42])
+S_CENTRES = 1.5 + 5.0 * np.arange(8) / 7.0
+M_CENTRES = 6.0 * np.arange(8) / 7.0
 G4_PARAMS = [(z, l, e) for z in (1.0, 2.0, 4.0, 16.0) for l in (1.0, -1.0) for e in (0.005, 0.02)]
 NDESC = 48
 HIDDEN = 128
@@ -55,11 +62,11 @@ def descriptors(pos, lat, atoms, vac, mover, rc=RC):
         nb = np.nonzero((r > 1e-8) & (r < rc))[0]
         rn, dn = r[nb], d[nb]
         f = fc(rn, rc)
-        out[a, :24] = (np.exp(-ETA_R * (rn[:, None] - R_CENTRES[None, :]) ** 2) * f[:, None]).sum(0)
+        out[a, :16] = (np.exp(-ETA_R * (rn[:, None] - R_CENTRES[None, :]) ** 2) * f[:, None]).sum(0)
         dv = np.linalg.norm(min_image(P[i] - vac, L))
-        out[a, 24:28] = np.exp(-ETA_S * (dv - S_CENTRES) ** 2)
+        out[a, 16:24] = np.exp(-ETA_S * (dv - S_CENTRES) ** 2) * fc(dv, rc)
         dm = np.linalg.norm(min_image(P[i] - mover, L))
-        out[a, 28:32] = np.exp(-ETA_S * (dm - M_CENTRES) ** 2)
+        out[a, 24:32] = np.exp(-ETA_S * (dm - M_CENTRES) ** 2) * fc(dm, rc)
         an = np.nonzero(rn < RC_ANG)[0]
         if len(an) >= 2:
             ra, da = rn[an], dn[an]

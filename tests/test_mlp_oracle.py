@@ -41,8 +41,14 @@ def test_descriptor_closed_forms():
     vac = np.array([40.0, 40.0, 40.0])
     d = M.descriptors(pos, L, [0], vac, pos[:, 1])[0]
     f = 0.5 * (np.cos(np.pi * r / M.RC) + 1)
-    assert np.allclose(d[:24], np.exp(-M.ETA_R * (r - M.R_CENTRES) ** 2) * f, rtol=1e-13, atol=0)
-    assert np.allclose(d[28:32], np.exp(-M.ETA_S * (r - M.M_CENTRES) ** 2), rtol=1e-13)
+    assert np.allclose(d[:16], np.exp(-M.ETA_R * (r - M.R_CENTRES) ** 2) * f, rtol=1e-13, atol=0)
+    # the mover 3.1 A away is a pseudo-neighbour; the vacancy 52 A away is
+    # past the cutoff
+    assert np.allclose(d[24:32], np.exp(-M.ETA_S * (r - M.M_CENTRES) ** 2) * f, rtol=1e-13)
+    assert np.all(d[16:24] == 0.0)
+    dv = M.descriptors(pos, L, [0], pos[:, 0] + np.array([0.0, 0.0, 2.2]), pos[:, 1])[0]
+    fv = 0.5 * (np.cos(np.pi * 2.2 / M.RC) + 1)
+    assert np.allclose(dv[16:24], np.exp(-M.ETA_S * (2.2 - M.S_CENTRES) ** 2) * fv, rtol=1e-13)
     assert np.all(d[32:] == 0.0)  # a single neighbour has no angle
     # equilateral triangle: cos = 1/2 at the apex
     a = 2.7
