@@ -15,6 +15,13 @@
 #define AKMC_MAX_WALKER_ATOMS (1 << 20)
 // workgroup size of the LDS-staged Sutton-Chen kernels (1024 / lpa atoms each)
 #define AKMC_STAGE_BLOCK 1024
+// active-zone bits of a search walker (BatchDev.zflag): zone on, a full
+// FP32 / FP64 evaluation pending (the walker's first in that precision since
+// its start), membership reset pending (set at the start)
+#define AKMC_ZONE_ON 1
+#define AKMC_ZONE_FULL32 2
+#define AKMC_ZONE_FULL64 4
+#define AKMC_ZONE_RESET 8
 // cells per axis are capped so a walker's grid fits one workgroup's LDS
 #define AKMC_MAX_CELLS_AXIS 16
 #define AKMC_MAX_WALKER_CELLS (AKMC_MAX_CELLS_AXIS * AKMC_MAX_CELLS_AXIS * AKMC_MAX_CELLS_AXIS)
@@ -158,6 +165,27 @@ struct BatchDev {
     int energy_only;      // this launch evaluates energies only (Field::calculateEnergy):
                           // no force pass, the energy-only short-range forms
     unsigned long long *rebuild_count;  // list rebuilds, counted by the list kernels
+    // per-atom energies of the FP64 Sutton-Chen kernels (null: e_a, e_b), so
+    // each precision keeps its own values of the atoms it did not visit
+    double *e_a64, *e_b64;
+    // Active zone of the saddle-search walkers of a Sutton-Chen field (null
+    // outside search micro-steps).  A search moves only its free atoms, so
+    // a force evaluation needs the forces on the free atoms, which need the
+    // densities of the atoms within the cutoff of a free atom, and an energy
+    // change involves only those atoms' energies.  A zoned walker's pass 2
+    // visits its free atoms, its pass 1 the zone: the free atoms and every
+    // atom on a free atom's Verlet list (rc + skin, so the zone holds every
+    // atom within rc of a free atom until the next rebuild), kept as a union
+    // over the rebuilds since the walker started.  Every other atom has had
+    // no free atom within rc since the walker's first full evaluation in
+    // each precision, so its energy, kept from that evaluation, is the one a
+    // full evaluation would give, bit for bit.
+    int *zflag;           // [W] AKMC_ZONE_* bits
+    int *zmark;           // [A] zone membership
+    int *zlist;           // [A] walker w's zone (local ids, ascending) at zlist[off_w ..)
+    int *nz;              // [W] zone size
+    const int *zact;      // [A] walker w's free atoms (global ids) at zact[off_w ..)
+    const int *znact;     // [W] free atoms per walker
 };
 
 // entry k of atom i's neighbour list in the 32-bit form (local index and

@@ -151,6 +151,11 @@ struct akmc_batch_s {
     hipEvent_t evp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // host copy of the frozen mask and the per-walker free-atom lists
     std::vector<int> hfrozen, hact, hnact;
+    // active zones of the search micro-steps (BatchDev.zflag ...): on by
+    // default for minimum-image Sutton-Chen fields
+    bool zone_on = true;
+    int *zflag = nullptr, *zmark = nullptr, *zlist = nullptr, *nz = nullptr;
+    std::vector<int> hzflag;
     // pinned status staging: one stream sync per host check of a run
     WalkerState *hws_pin = nullptr;
     int *ov_pin = nullptr;
@@ -715,6 +720,8 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
     TRY(dalloc(b, &d.isr64, A));
     TRY(dalloc(b, &d.e_a, A));
     TRY(dalloc(b, &d.e_b, A));
+    TRY(dalloc(b, &d.e_a64, A));
+    TRY(dalloc(b, &d.e_b64, A));
     TRY(dalloc(b, &d.e_c, A));
     TRY(dalloc(b, &d.st_atom, (size_t)6 * A));
     TRY(dalloc(b, &d.out, W));
@@ -930,6 +937,8 @@ int upload(akmc_batch_t b, const double *x, const double *y, const double *z, co
     std::vector<int> ones(b->W, 1), mode(b->W, batch_mode(b));
     HIPCHK(hipMemcpyAsync(b->d.rebuild, ones.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d.active, mode.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    // a running search's zone does not survive new positions
+    if (b->zflag) HIPCHK(hipMemsetAsync(b->zflag, 0, b->W * sizeof(int), b->stream));
     if (b->search_alloc) {
         HIPCHK(hipMemcpyAsync(b->s.X, x, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
         HIPCHK(hipMemcpyAsync(b->s.X + A, y, A * sizeof(double), hipMemcpyHostToDevice, b->stream));
@@ -986,6 +995,13 @@ int search_alloc(akmc_batch_t b)
     b->hnact.assign(b->W, 0);
     TRY(dalloc(b, &b->s.ws, b->W));
     HIPCHK(hipMemset(b->s.ws, 0, b->W * sizeof(WalkerState)));
+    TRY(dalloc(b, &b->zflag, b->W));
+    TRY(dalloc(b, &b->zmark, b->A));
+    TRY(dalloc(b, &b->zlist, b->A));
+    TRY(dalloc(b, &b->nz, b->W));
+    HIPCHK(hipMemset(b->zflag, 0, b->W * sizeof(int)));
+    HIPCHK(hipMemset(b->nz, 0, b->W * sizeof(int)));
+    b->hzflag.assign(b->W, 0);
     // X starts at the resident positions
     size_t A = b->A;
     HIPCHK(hipMemcpy(b->s.X, b->d.x, A * sizeof(double), hipMemcpyDeviceToDevice));
@@ -1008,23 +1024,46 @@ void drop_graph(akmc_batch_t b)
     b->graph_steps = 0;
 }
 
+// active zones apply to minimum-image Sutton-Chen fields
+bool zone_usable(akmc_batch_t b)
+{
+    const FieldDev &h = b->f->host;
+    return b->zone_on && b->zflag && h.kind == 0 && !h.eam && h.min_image;
+}
+
+// the batch view of the search micro-steps: the plain one plus the zones
+BatchDev step_dev(akmc_batch_t b)
+{
+    BatchDev d = b->d;
+    if (zone_usable(b)) {
+        d.zflag = b->zflag;
+        d.zmark = b->zmark;
+        d.zlist = b->zlist;
+        d.nz = b->nz;
+        d.zact = b->s.act;
+        d.znact = b->s.nact;
+    }
+    return d;
+}
+
 // one micro-step; step j of a run chunk records the kernel-timer events
 void enqueue_micro_step(akmc_batch_t b, int j)
 {
     // FP64 pair terms run when a walker may ask for them: verification or
     // the closing FP64 energy of a relax
     int prec = batch_mode(b) | ((b->s.verify_nl > 0 || b->s.final64) ? 2 : 0);
+    const BatchDev d = step_dev(b);
     if (b->ktimer && (prec & 1) && b->f->host.kind == 0) {
-        for (int ph = 0; ph < 2; ++ph) akmc_launch_phase(b->d, b->f->host, prec, 0, ph, b->stream);
+        for (int ph = 0; ph < 2; ++ph) akmc_launch_phase(d, b->f->host, prec, 0, ph, b->stream);
         (void)hipEventRecord(b->kev[0][j], b->stream);
-        akmc_launch_phase(b->d, b->f->host, 1, 0, 2, b->stream);
+        akmc_launch_phase(d, b->f->host, 1, 0, 2, b->stream);
         (void)hipEventRecord(b->kev[1][j], b->stream);
-        if (prec & 2) akmc_launch_phase(b->d, b->f->host, 2 | 4, 0, 2, b->stream);  // FP64 part of a mixed step
-        akmc_launch_phase(b->d, b->f->host, prec, 0, 3, b->stream);
+        if (prec & 2) akmc_launch_phase(d, b->f->host, 2 | 4, 0, 2, b->stream);  // FP64 part of a mixed step
+        akmc_launch_phase(d, b->f->host, prec, 0, 3, b->stream);
     } else {
-        akmc_launch_forces_impl(b->d, b->f->host, prec, 0, b->stream);
+        akmc_launch_forces_impl(d, b->f->host, prec, 0, b->stream);
     }
-    akmc_launch_search_logic(b->d, b->s, b->stream);
+    akmc_launch_search_logic(d, b->s, b->stream);
 }
 
 int fetch_states(akmc_batch_t b)
@@ -1070,6 +1109,18 @@ int start_walkers(akmc_batch_t b, const int *walkers, int kind, double dt0, doub
         b->hws[w] = ws;
     }
     HIPCHK(hipMemcpyAsync(b->s.ws, b->hws.data(), b->W * sizeof(WalkerState), hipMemcpyHostToDevice, b->stream));
+    // a started walker with frozen atoms gets a fresh zone: membership reset,
+    // and a full evaluation first in each precision
+    HIPCHK(hipMemcpyAsync(b->hzflag.data(), b->zflag, b->W * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int w = 0; w < b->W; ++w) {
+        if (walkers && !walkers[w]) continue;
+        const int m = b->hnact[w];
+        b->hzflag[w] = (m > 0 && m < b->hw[w].n)
+                           ? (AKMC_ZONE_ON | AKMC_ZONE_FULL32 | AKMC_ZONE_FULL64 | AKMC_ZONE_RESET)
+                           : 0;
+    }
+    HIPCHK(hipMemcpyAsync(b->zflag, b->hzflag.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
     int run;
     TRY(refresh_active(b, &run));
     akmc_launch_probe_init(b->d, b->s, b->stream);
@@ -1184,7 +1235,7 @@ int capture_graph(akmc_batch_t b, int steps)
     HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     b->graph_steps = steps;
     b->graph_args = b->s;
-    b->graph_d = b->d;
+    b->graph_d = step_dev(b);
     b->graph_prec = batch_mode(b);
     return AKMC_SUCCESS;
 }
@@ -1668,9 +1719,10 @@ int akmc_batch_run_async(akmc_batch_t b, int steps)
     // HIP events recorded inside a replayed graph cannot be timed, so the
     // kernel timer launches the micro-steps one by one
     if (b->use_graphs && !b->ktimer) {
+        const BatchDev bd = step_dev(b);
         if (!b->gexec || b->graph_steps != steps || b->graph_prec != batch_mode(b) ||
             std::memcmp(&b->graph_args, &b->s, sizeof(SearchDev)) != 0 ||
-            std::memcmp(&b->graph_d, &b->d, sizeof(BatchDev)) != 0)
+            std::memcmp(&b->graph_d, &bd, sizeof(BatchDev)) != 0)
             TRY(capture_graph(b, steps));
         HIPCHK(hipGraphLaunch(b->gexec, b->stream));
     } else {
@@ -1760,6 +1812,14 @@ int akmc_batch_set_graphs(akmc_batch_t b, int enable)
     if (!b) return fail(AKMC_EINVAL, "null batch");
     b->use_graphs = enable != 0;
     if (!b->use_graphs) drop_graph(b);
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_set_search_zone(akmc_batch_t b, int enable)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    b->zone_on = enable != 0;
+    drop_graph(b);
     return AKMC_SUCCESS;
 }
 
@@ -1929,6 +1989,42 @@ int akmc_batch_time_phases(akmc_batch_t b, int nrep, double ms[4])
         }
     }
     return check_overflow(b);
+}
+
+int akmc_batch_time_search_phases(akmc_batch_t b, int nrep, double ms[4])
+{
+    if (!b || nrep <= 0 || !ms) return fail(AKMC_EINVAL, "bad arguments");
+    if (!b->search_alloc) return fail(AKMC_EINVAL, "no search has been started");
+    for (auto &e : b->evp)
+        if (!e) HIPCHK(hipEventCreate(&e));
+    const BatchDev d = step_dev(b);
+    for (int ph = 0; ph < 4; ++ph) ms[ph] = 0.0;
+    for (int r = 0; r < nrep; ++r) {
+        HIPCHK(hipEventRecord(b->evp[0], b->stream));
+        for (int ph = 0; ph < 4; ++ph) {
+            akmc_launch_phase(d, b->f->host, batch_mode(b), 0, ph, b->stream);
+            HIPCHK(hipEventRecord(b->evp[ph + 1], b->stream));
+        }
+        HIPCHK(hipEventSynchronize(b->evp[4]));
+        for (int ph = 0; ph < 4; ++ph) {
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, b->evp[ph], b->evp[ph + 1]));
+            ms[ph] += t;
+        }
+    }
+    return check_overflow(b);
+}
+
+int akmc_batch_get_zone(akmc_batch_t b, int *member, int *size, int *flags, int *active)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    if (!b->search_alloc) return fail(AKMC_EINVAL, "no search has been started");
+    if (member) HIPCHK(hipMemcpyAsync(member, b->zmark, b->A * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    if (size) HIPCHK(hipMemcpyAsync(size, b->nz, b->W * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    if (flags) HIPCHK(hipMemcpyAsync(flags, b->zflag, b->W * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    if (active) HIPCHK(hipMemcpyAsync(active, b->d.active, b->W * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return AKMC_SUCCESS;
 }
 
 int akmc_batch_save_basin(akmc_batch_t b) { return akmc_batch_set_basins(b, nullptr, nullptr); }

@@ -38,6 +38,29 @@ __device__ inline bool other_precision(const BatchDev &b, int w)
     return b.active[w] != (sizeof(T) == 8 ? 2 : 1);
 }
 
+// the per-atom energy arrays the many-body kernels of type T write
+template <typename T>
+__device__ inline double *energy_a(const BatchDev &b)
+{
+    return (sizeof(T) == 8 && b.e_a64) ? b.e_a64 : b.e_a;
+}
+template <typename T>
+__device__ inline double *energy_b(const BatchDev &b)
+{
+    return (sizeof(T) == 8 && b.e_b64) ? b.e_b64 : b.e_b;
+}
+
+// walker w evaluates only its active zone in a pass of type T: a search
+// micro-step, the zone on, and the walker's full evaluation in this
+// precision done
+template <typename T>
+__device__ inline bool zoned(const BatchDev &b, int w)
+{
+    if (!b.zflag) return false;
+    const int f = b.zflag[w];
+    return (f & AKMC_ZONE_ON) && !(f & (sizeof(T) == 8 ? AKMC_ZONE_FULL64 : AKMC_ZONE_FULL32));
+}
+
 template <typename T>
 __device__ inline const SCCoef<T> &sc_coef(const SCPairDev &p);
 template <>
@@ -619,6 +642,53 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_images(BatchDev b)
     put_sentinel(b, i, n, W.off);
 }
 
+// The active zone of a search walker (one workgroup per walker), after its
+// lists were rebuilt or at its start (membership reset first): its free
+// atoms and every atom on a free atom's list join the zone, then the
+// members are listed in ascending index order.
+__global__ void __launch_bounds__(kBlock) k_zone_build(BatchDev b)
+{
+    __shared__ int wcount[kBlock / 64];
+    __shared__ int total;
+    const int w = blockIdx.x;
+    const int f = b.zflag[w];
+    if (!(f & AKMC_ZONE_ON) || !((f & AKMC_ZONE_RESET) || b.rebuild[w])) return;
+    const WalkerDev W = b.walkers[w];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (f & AKMC_ZONE_RESET) {
+        for (int l = threadIdx.x; l < W.n; l += kBlock) b.zmark[W.off + l] = 0;
+        __syncthreads();
+    }
+    const int nf = b.znact[w];
+    for (int k = threadIdx.x; k < nf; k += kBlock) b.zmark[b.zact[W.off + k]] = 1;
+    // one wavefront per free atom, its lanes over the atom's list
+    for (int k = wave; k < nf; k += kBlock / 64) {
+        const int i = b.zact[W.off + k];
+        const int cnt = b.nnb[i] < b.maxnb ? b.nnb[i] : b.maxnb;
+        for (int e = lane; e < cnt; e += 64) b.zmark[W.off + int(nbr_entry(b, e, i) & AKMC_NB_JMASK)] = 1;
+    }
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    for (int l0 = 0; l0 < W.n; l0 += kBlock) {
+        const int l = l0 + (int)threadIdx.x;
+        const bool m = l < W.n && b.zmark[W.off + l] != 0;
+        const unsigned long long bal = __ballot(m);
+        if (lane == 0) wcount[wave] = __popcll(bal);
+        __syncthreads();
+        int at = total;
+        for (int v = 0; v < wave; ++v) at += wcount[v];
+        if (m) b.zlist[W.off + at + __popcll(bal & ((1ull << lane) - 1ull))] = l;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int v = 0; v < kBlock / 64; ++v) total += wcount[v];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        b.nz[w] = total;
+        b.zflag[w] = f & ~AKMC_ZONE_RESET;
+    }
+}
+
 __global__ void k_rebuild_done(BatchDev b, unsigned long long *count)
 {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -719,13 +789,23 @@ __device__ inline float isr32_of(double w) { return __uint_as_float(uint32_t(__d
 // (p4) into LDS, so every neighbour gather of the pair loop is an LDS read
 // instead of a texture-unit gather from L2.  A workgroup whose walker runs
 // in the other precision, or is inactive, leaves before staging.
+// With zp (1: pass 1, 2: pass 2) a zoned walker's slots run over its zone
+// (pass 1) or its free atoms (pass 2) instead of all its atoms: slot k of
+// the walker serves the k-th listed atom, and workgroups whose slots all
+// lie past the list leave before staging.
 struct AtomSlot {
     int i, sub;
     bool ok;
 };
 
+__device__ inline int zone_count(const BatchDev &b, int w, int zp) { return zp == 1 ? b.nz[w] : b.znact[w]; }
+__device__ inline int zone_atom(const BatchDev &b, const WalkerDev &W, int k, int zp)
+{
+    return zp == 1 ? W.off + b.zlist[W.off + k] : b.zact[W.off + k];
+}
+
 template <typename T, int LPA, bool STAGED>
-__device__ inline AtomSlot sc_slot(const BatchDev &b, WalkerDev &W, double4 *stage)
+__device__ inline AtomSlot sc_slot(const BatchDev &b, WalkerDev &W, double4 *stage, int zp = 0)
 {
     if constexpr (STAGED) {
         constexpr int li = LPA == 1 ? 0 : LPA == 2 ? 1 : LPA == 4 ? 2 : 3;
@@ -735,20 +815,29 @@ __device__ inline AtomSlot sc_slot(const BatchDev &b, WalkerDev &W, double4 *sta
         if (c >= nch) return {0, 0, false};
         const int2 ch = b.chunk_tab[li][c];
         if (other_precision<T>(b, ch.x)) return {0, 0, false};
+        const bool z = zp && zoned<T>(b, ch.x);
+        const int cnt = z ? zone_count(b, ch.x, zp) : 0;
+        if (z && ch.y >= cnt) return {0, 0, false};
         W = b.walkers[ch.x];
         for (int k = threadIdx.x; k < W.n; k += blockDim.x) stage[k] = b.p4[W.off + k];
         __syncthreads();
         const int la = ch.y + int(threadIdx.x) / LPA;
+        if (z) return {la < cnt ? zone_atom(b, W, la, zp) : 0, int(threadIdx.x) % LPA, la < cnt};
         return {W.off + la, int(threadIdx.x) % LPA, la < W.n};
     } else {
         const int t = xcd_thread((b.A * LPA + kBlock - 1) / kBlock);
         if (t < 0) return {0, 0, false};
-        const int i = t / LPA;
-        if (i >= b.A) return {0, 0, false};
-        const int w = walker_of(b, i);
+        const int p = t / LPA;
+        if (p >= b.A) return {0, 0, false};
+        const int w = walker_of(b, p);
         if (other_precision<T>(b, w)) return {0, 0, false};
         W = b.walkers[w];
-        return {i, t % LPA, true};
+        if (zp && zoned<T>(b, w)) {
+            const int k = p - W.off;
+            if (k >= zone_count(b, w, zp)) return {0, 0, false};
+            return {zone_atom(b, W, k, zp), t % LPA, true};
+        }
+        return {p, t % LPA, true};
     }
 }
 
@@ -757,7 +846,11 @@ __device__ inline AtomSlot sc_slot(const BatchDev &b, WalkerDev &W, double4 *sta
 // LPA lanes share one atom's neighbour list (strided) and combine by shuffles.
 // ONE: the field has a single species pair with n = 9, m = 6, whose
 // coefficients stay in registers and whose powers are straight products.
-template <typename T, bool MINIMG, int LPA, int U, bool ONE, bool STAGED>
+// ZONE (search micro-steps with active zones): zoned walkers visit their
+// zone, and every visited atom also gets its half of its pairs' V, the
+// value pass 2 writes (the same terms in the same order), since pass 2
+// visits only the free atoms.
+template <typename T, bool MINIMG, int LPA, int U, bool ONE, bool STAGED, bool ZONE>
 __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_density(BatchDev b)
 {
     __shared__ SCPairDev tab[ONE ? 1 : AKMC_MAXPAIR];
@@ -767,14 +860,14 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
     if (!ONE) stage_sc_table(F, tab);
     const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
     WalkerDev W;
-    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage);
+    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage, ZONE ? 1 : 0);
     if (!sl.ok) return;
     const int i = sl.i, sub = sl.sub;
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
     const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
     const int cnt = b.nnb[i];
-    double rho = 0.0;
+    double rho = 0.0, ep = 0.0;
     // entries two batches ahead: the next batch and the one after it are in
     // flight while a batch's gathers and pair terms run (more bytes of the
     // streamed list outstanding per wavefront)
@@ -813,7 +906,18 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
             // only the pairs inside the cutoff are summed
             T ir2[U], u2[U], rt[U], um[U];
             sc_powers_in<T, U>(r2, c1.a, ir2, u2, rt);
-            pow_6<T, U>(u2, um);
+            if constexpr (ZONE) {
+                T un[U];
+                pow_9_6<T, U>(u2, rt, un, um);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    T eh = in[u] ? T(0.5) * c1.eps * un[u] : T(0);
+                    if constexpr (sizeof(T) == 4) asm("" : "+v"(eh));
+                    ep += double(eh);
+                }
+            } else {
+                pow_6<T, U>(u2, um);
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) rho += double(in[u] ? c1.rho_coef * um[u] : T(0));  // + exact zeros
         } else {
@@ -825,14 +929,17 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
                     const SCCoef<T> &c = sc_coef<T>(p);
                     T u2 = c.a * c.a * recip<T>(T(r2[u]));
                     rho += double(c.rho_coef * pow_half<T>(u2, p.m_int, c.m));
+                    if constexpr (ZONE) ep += double(T(0.5) * c.eps * pow_half<T>(u2, p.n_int, c.n));
                 }
             }
         }
     }
     rho = lane_group_sum<LPA>(rho);
+    if constexpr (ZONE) ep = lane_group_sum<LPA>(ep);
     if (sub != 0) return;
     b.rho[i] = rho;
-    b.e_a[i] = -sqrt(rho);
+    energy_a<T>(b)[i] = -sqrt(rho);
+    if constexpr (ZONE) energy_b<T>(b)[i] = ep;
     const float isr32 = rho < 1.0e-8 ? -1.0f : float(1.0 / sqrt(rho));
     const double isr64 = rho < 1.0e-8 ? -1.0 : 1.0 / sqrt(rho);
     b.inv_sqrt_rho[i] = isr32;
@@ -856,7 +963,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
     if (!ONE) stage_sc_table(F, tab);
     const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
     WalkerDev W;
-    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage);
+    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage, 2);
     if (!sl.ok) return;
     const int i = sl.i, sub = sl.sub;
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
@@ -1000,7 +1107,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
     b.fx[i] = fx;
     b.fy[i] = fy;
     b.fz[i] = fz;
-    b.e_b[i] = ep;
+    energy_b<T>(b)[i] = ep;
     if (STRESS) {
         b.st_atom[0 * (size_t)b.A + i] = s0;
         b.st_atom[1 * (size_t)b.A + i] = s1;
@@ -1043,7 +1150,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_pair_energy(BatchDev b)
         const T u2 = c.a * c.a * recip<T>(T(r2));
         ep += double(T(0.5) * c.eps * pow_half<T>(u2, p.n_int, c.n));
     }
-    b.e_b[i] = ep;
+    energy_b<T>(b)[i] = ep;
 }
 
 // ------------------------------------------------------- tabulated EAM
@@ -1116,7 +1223,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_eam_dens
         if (idx < 0 || idx >= T_.nrho) atomicOr(b.overflow, AKMC_OV_EAM_RHO);
         if (rho >= kEamTiny) Fp = eam_interp(T_.emb + 1, 2, T_.nrho, rho, T_.drho, idx);
     }
-    b.e_a[i] = Fi;
+    energy_a<T>(b)[i] = Fi;
     if (ONE) reinterpret_cast<double *>(&b.p4[i])[3] = Fp;
 }
 
@@ -1213,7 +1320,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_eam_forc
     b.fx[i] = fx;
     b.fy[i] = fy;
     b.fz[i] = fz;
-    b.e_b[i] = ep;
+    energy_b<T>(b)[i] = ep;
     if (STRESS) {
         b.st_atom[0 * (size_t)b.A + i] = s0;
         b.st_atom[1 * (size_t)b.A + i] = s1;
@@ -1486,7 +1593,6 @@ __global__ void __launch_bounds__(AKMC_STAGE_BLOCK) k_ewald_sk(BatchDev b)
     const int nx = b.kn[kk], ny = b.kn[b.nk_total + kk], nz = b.kn[2 * b.nk_total + kk];
     double C = 0.0, S = 0.0;
     if (has) {
-#pragma unroll 4
         for (int i = W.off + wave; i < W.off + W.n; i += kSkWaves) {
             const double2 p = phase_of(b.ptab + (size_t)i * 3 * b.G1, b.G1, nx, ny, nz);
             const double q = b.q[i];
@@ -1525,7 +1631,6 @@ __global__ void __launch_bounds__(kBlock) k_ewald_force(BatchDev b)
     const double q = b.q[i];
     const int nk = b.nk_total;
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-#pragma unroll 2
     for (int k = sub; k < W.nk; k += kIonLanes) {
         const int kk = W.k_off + k;
         const double2 p = phase_of(t, b.G1, b.kn[kk], b.kn[nk + kk], b.kn[2 * nk + kk]);
@@ -1584,10 +1689,13 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
     if (skipped(b, w)) return;
     const WalkerDev W = b.walkers[w];
     const FieldDev &F = *b.field;
+    // the many-body kernels keep FP64 walkers' energies apart
+    const bool e64 = F.kind == 0 && b.active[w] == 2 && b.e_a64;
+    const double *ea = e64 ? b.e_a64 : b.e_a, *eb = e64 ? b.e_b64 : b.e_b;
     double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = W.off + threadIdx.x; i < W.off + W.n; i += blockDim.x) {
-        v[0] += b.e_a[i];
-        v[1] += b.e_b[i];
+        v[0] += ea[i];
+        v[1] += eb[i];
         if (F.kind == 1 && F.do_ewald == 1) {
             v[2] += b.e_c[i];
             v[3] += b.q[i] * b.q[i];
@@ -1645,6 +1753,8 @@ __global__ void __launch_bounds__(kBlock) k_walker_reduce(BatchDev b, int stress
             o.stress[2] -= r7[0];
         }
         b.out[w] = o;
+        // the walker's full evaluation in this precision is done
+        if (b.zflag) b.zflag[w] &= ~(b.active[w] == 2 ? AKMC_ZONE_FULL64 : AKMC_ZONE_FULL32);
     }
 }
 
@@ -1676,6 +1786,7 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
     } else {
         k_nbr_build_images<<<(b.A + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(b);
     }
+    if (b.zflag) k_zone_build<<<b.W, kBlock, 0, s>>>(b);
     k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, b.rebuild_count);
 }
 
@@ -1699,12 +1810,15 @@ static ScLaunch sc_launch(const BatchDev &b)
     return {xcd_grid((b.A * LPA + kBlock - 1) / kBlock), kBlock, 0};
 }
 
+// search micro-steps with active zones (minimum-image fields only) take the
+// ZONE variants of pass 1
 template <typename T, int LPA, int U, bool ONE, bool STAGED>
 static void launch_sc_density_one(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     const ScLaunch L = sc_launch<LPA>(b);
-    if (hf.min_image) k_sc_density<T, true, LPA, U, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
-    else k_sc_density<T, false, LPA, U, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+    if (hf.min_image && b.zflag) k_sc_density<T, true, LPA, U, ONE, STAGED, true><<<L.grid, L.block, L.lds, s>>>(b);
+    else if (hf.min_image) k_sc_density<T, true, LPA, U, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
+    else k_sc_density<T, false, LPA, U, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
 }
 
 template <typename T, int LPA, int U>

@@ -187,6 +187,9 @@ def load() -> C.CDLL:
     L.akmc_batch_time_phases.argtypes = [C.c_void_p, C.c_int, _D]
     L.akmc_batch_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.akmc_batch_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_set_search_zone.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_time_search_phases.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.akmc_batch_get_zone.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 4
     L.akmc_batch_get_walkers_positions.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.akmc_batch_kernel_timer.argtypes = [C.c_void_p, C.c_int]
     L.akmc_batch_kernel_time.argtypes = [C.c_void_p, _D, C.POINTER(C.c_longlong)]
@@ -517,6 +520,23 @@ class Batch:
 
     def set_graphs(self, enable: bool):
         check(load().akmc_batch_set_graphs(self._h, int(bool(enable))))
+
+    def set_search_zone(self, enable: bool):
+        """Active zones of the search micro-steps (on by default)."""
+        check(load().akmc_batch_set_search_zone(self._h, int(bool(enable))))
+
+    def time_search_phases(self, nrep):
+        """per-phase ms of one evaluation of the search walkers as the micro-steps run it"""
+        ms = np.zeros(4)
+        check(load().akmc_batch_time_search_phases(self._h, int(nrep), _dp(ms)))
+        return ms / nrep
+
+    def zone(self):
+        """(member [A], size [W], flags [W], active [W]) of the active zones"""
+        member, size = np.zeros(self.A, np.int32), np.zeros(self.W, np.int32)
+        flags, active = np.zeros(self.W, np.int32), np.zeros(self.W, np.int32)
+        check(load().akmc_batch_get_zone(self._h, _ip(member), _ip(size), _ip(flags), _ip(active)))
+        return member, size, flags, active
 
     def kernel_timer(self, enable: bool):
         check(load().akmc_batch_kernel_timer(self._h, int(bool(enable))))

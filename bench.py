@@ -46,6 +46,11 @@ sys.path.insert(0, ROOT)
 METRIC = "FP64-verified saddles/sec at ≤0.01 eV barrier error; force evals/sec vs CPU"
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 FP64_VECTOR_PEAK_TFLOPS = 78.6
+# algorithmic FLOP per ordered in-cutoff pair of the zoned pass 1 of the
+# search micro-steps: the density's 20 (1/r^2, (a/r)^2, (a/r)^6, the
+# coefficient, the sum, with the FP64 displacement and cutoff test) plus 5
+# for the pair energy (a/r)^9 = (a/r)^8 sqrt((a/r)^2), 1/2 eps (a/r)^9, its sum
+ZONE_PASS1_FLOP = 25
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per ordered pair of one all-walker k_sc_force launch, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on `bench.py --forces-only 5`,
@@ -65,6 +70,49 @@ def ordered_pairs(cells, positions, rc):
         t = cKDTree(q, boxsize=L)
         P.append(int(t.count_neighbors(t, rc)) - c.n)
     return np.array(P)
+
+
+def atom_pairs(cells, positions, rc):
+    """Per walker, each atom's in-cutoff neighbours (r <= rc, minimum image)."""
+    from scipy.spatial import cKDTree
+    out = []
+    for c, p in zip(cells, positions):
+        L = np.array([c.lat[0], c.lat[4], c.lat[8]])
+        q = np.mod(p.T, L)
+        q = np.where(q >= L, 0.0, q)
+        t = cKDTree(q, boxsize=L)
+        out.append(np.asarray(t.query_ball_point(q, rc, return_length=True)) - 1)
+    return out
+
+
+def zone_work(camp, cells, rc):
+    """The algorithmic work of one evaluation of the search walkers as the
+    micro-steps run it: ordered in-cutoff pairs of pass 1 (each walker's
+    zone, or all its atoms when it evaluates in full) and of pass 2 (its
+    free atoms, or all), over the walkers in FP32 mode, at the current
+    probes; zone atoms and free atoms per walker."""
+    member, size, flags, active = camp.batch.zone()
+    probes, _ = camp.batch.download()
+    cnt = atom_pairs(cells, probes, rc)
+    p1 = p2 = n_zone = n_free = n_w = 0
+    for w, c in enumerate(cells):
+        if active[w] != 1:
+            continue
+        o, k = camp.off[w], cnt[w]
+        n_w += 1
+        if (flags[w] & 1) and not (flags[w] & 2):
+            zm = member[o:o + c.n] != 0
+            fr = camp._frozen[o:o + c.n] == 0
+            p1 += int(k[zm].sum())
+            p2 += int(k[fr].sum())
+            n_zone += int(zm.sum())
+            n_free += int(fr.sum())
+        else:
+            p1 += int(k.sum())
+            p2 += int(k.sum())
+            n_zone += c.n
+            n_free += c.n
+    return dict(pairs_pass1=p1, pairs_pass2=p2, zone_atoms=n_zone, free_atoms=n_free, walkers=n_w)
 
 
 def plain_dimer_rate(L, W, text, cells, basins, proposal_fn, params, n_search, basin32, basin64, eager=False):
@@ -626,11 +674,20 @@ def main():
     k_ms, k_launches = camp.batch.kernel_time()
     evals = camp.batch.eval_counts() - counts0
     camp.batch.kernel_timer(False)
-    timed_flops = 40.0 * float((evals * P).sum())  # SC pass 2: 40 FLOP per ordered pair of each evaluated walker
     timed_ms = k_ms / max(k_launches, 1)
-    timed_tf = timed_flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
-    print(f"[bench] k_sc_force in the roofline window: {k_launches} launches, {timed_ms:.4f} ms per launch, "
-          f"{timed_tf:.3f} TFLOP/s ({int(evals.sum())} walker evaluations)", file=sys.stderr, flush=True)
+    print(f"[bench] k_sc_force in the roofline window: {k_launches} launches, {timed_ms:.4f} ms per launch "
+          f"({int(evals.sum())} walker evaluations)", file=sys.stderr, flush=True)
+    # ---- the pass kernels as the search micro-steps run them (active zones),
+    # timed with HIP events on the batch's stream at the searches' current
+    # probes, after one warm evaluation that does any pending full one; the
+    # algorithmic work from the device's zones and the free-atom masks
+    camp.batch.time_search_phases(1)
+    zw = zone_work(camp, cells, rc)
+    zph = camp.batch.time_search_phases(20)
+    z_tf1 = ZONE_PASS1_FLOP * zw["pairs_pass1"] / (zph[1] * 1e-3) / 1e12
+    z_tf2 = 40.0 * zw["pairs_pass2"] / (zph[2] * 1e-3) / 1e12
+    print(f"[bench] zoned search evaluation: {zw}; ms lists {zph[0]:.4f} pass 1 {zph[1]:.4f} ({z_tf1:.2f} TFLOP/s) "
+          f"pass 2 {zph[2]:.4f} ({z_tf2:.2f} TFLOP/s) reduce {zph[3]:.4f}", file=sys.stderr, flush=True)
     value = tot.distinct / t_timed
     verified_rate = tot.verified / t_timed
     moves_timed = camp.basin_moves
@@ -716,21 +773,34 @@ def main():
                      "no matrix form and uses no MFMA); peak is MI355X's published FP32 vector figure (packed "
                      "FP32). This build compiles without packed FP32 instructions, so its reachable FP32 "
                      "vector ceiling is half that: frac_vs_unpacked_ceiling",
-                     "kernel": "k_sc_force (Sutton-Chen pass 2, autotuned variant)",
-                     "achieved": round(timed_tf, 3), "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(timed_tf / FP32_VECTOR_PEAK_TFLOPS, 4),
+                     "kernel": "k_sc_density, zoned (Sutton-Chen pass 1 of the search micro-steps: density and "
+                               "pair energy of each walker's active zone, autotuned variant)",
+                     "achieved": round(z_tf1, 3), "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(z_tf1 / FP32_VECTOR_PEAK_TFLOPS, 4),
                      "unpacked_ceiling": FP32_VECTOR_PEAK_TFLOPS / 2,
-                     "frac_vs_unpacked_ceiling": round(timed_tf / (FP32_VECTOR_PEAK_TFLOPS / 2), 4),
-                     "traffic": round(TRAFFIC_BYTES_PER_PAIR * Ptot) if TRAFFIC_BYTES_PER_PAIR else None,
-                     "traffic_note": "HBM bytes of one all-walker launch at the basins (FETCH_SIZE x 2.0 + "
-                                     "WRITE_SIZE x 1.0, scales from a 1 GiB copy), from the hardware-counter run",
-                     "measured_over": f"{k_launches} launches in a {args.roofline_window}-micro-step window of the "
-                                      "same searches after the timed windows (HIP events, eager launches)",
-                     "ms_per_launch": round(timed_ms, 4),
-                     "flops_per_launch": round(timed_flops / max(k_launches, 1)),
-                     "basin_all_walkers": {"ms_per_launch": round(force_ms, 4), "flops_per_launch": 40 * Ptot,
-                                           "tflops": round(achieved_tf, 3), "density_pass_tflops": round(dens_tf, 3),
-                                           "hbm_algorithmic_GBs": round(hbm_gbs, 1)},
+                     "frac_vs_unpacked_ceiling": round(z_tf1 / (FP32_VECTOR_PEAK_TFLOPS / 2), 4),
+                     "traffic": None,
+                     "traffic_note": "no counter run of the zoned kernels; full_evaluation.traffic holds the HBM "
+                                     "bytes of k_sc_force over every walker at the basins",
+                     "measured_over": "20 evaluations of the search walkers at their probes after the timed "
+                                      "windows, as the micro-steps launch them (HIP events on the batch's stream)",
+                     "ms_per_launch": round(float(zph[1]), 4),
+                     "flops_per_launch": round(ZONE_PASS1_FLOP * zw["pairs_pass1"]),
+                     "flop_per_pair": ZONE_PASS1_FLOP,
+                     "zones": zw,
+                     "pass2_zoned": {"kernel": "k_sc_force (free atoms)", "ms_per_launch": round(float(zph[2]), 4),
+                                     "flops_per_launch": 40 * zw["pairs_pass2"], "tflops": round(z_tf2, 3)},
+                     "window_k_sc_force_ms_per_launch": round(timed_ms, 4),
+                     "full_evaluation": {"kernel": "k_sc_force over every atom of every walker at the basins",
+                                         "ms_per_launch": round(force_ms, 4), "flops_per_launch": 40 * Ptot,
+                                         "tflops": round(achieved_tf, 3),
+                                         "frac": round(achieved_tf / FP32_VECTOR_PEAK_TFLOPS, 4),
+                                         "density_pass_tflops": round(dens_tf, 3),
+                                         "traffic": round(TRAFFIC_BYTES_PER_PAIR * Ptot) if TRAFFIC_BYTES_PER_PAIR
+                                         else None,
+                                         "traffic_note": "FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from a "
+                                                         "1 GiB copy), from the hardware-counter run",
+                                         "hbm_algorithmic_GBs": round(hbm_gbs, 1)},
                      "hbm_peak_GBs": HBM_PEAK_GBS},
     }
     if learned:

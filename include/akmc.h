@@ -293,6 +293,28 @@ int akmc_batch_run_wait(akmc_batch_t b, int *steps_done, int *running);
  * disabled the same kernels are launched one by one on the batch's stream
  * (for tools that cannot trace graph launches).  Results are identical. */
 int akmc_batch_set_graphs(akmc_batch_t b, int enable);
+/* enable (default) or disable the active zones of searches on minimum-image
+ * Sutton-Chen fields (no counterpart in the reference, whose force call
+ * evaluates every atom).  A search moves only its free atoms, so its micro-
+ * steps evaluate forces on the free atoms and densities and energies on the
+ * atoms within the list radius of a free atom; every other atom keeps the
+ * values of the walker's first full evaluation since its start, which no
+ * free atom can change.  The searches' forces on free atoms, energies and
+ * trajectories are bit for bit those of full evaluations; forces on frozen
+ * atoms are not re-evaluated during a search (the optimizers zero them), so
+ * akmc_batch_download after a search returns their values from its start:
+ * call akmc_batch_forces first for fresh ones. */
+int akmc_batch_set_search_zone(akmc_batch_t b, int enable);
+/* akmc_batch_time_phases for the search walkers at their current probes, as
+ * the search micro-steps launch the kernels (active zones included), without
+ * advancing any search (no counterpart in the reference: measurement). */
+int akmc_batch_time_search_phases(akmc_batch_t b, int nrep, double ms[4]);
+/* The active zones (no counterpart in the reference: inspection): member[A]
+ * 1 for the atoms in their walker's zone, size[W] zone sizes, flags[W] the
+ * zone bits (1 on, 2 / 4 a full FP32 / FP64 evaluation pending, 8 reset
+ * pending), active[W] the walkers' evaluation modes (0 none, 1 FP32 pair
+ * terms, 2 FP64); any pointer may be NULL. */
+int akmc_batch_get_zone(akmc_batch_t b, int *member, int *size, int *flags, int *active);
 /* Kernel timer of akmc_batch_run: HIP events on the batch's stream around
  * the FP32 Sutton-Chen pass-2 launch (k_sc_force) of every micro-step.
  * Events inside a replayed graph cannot be timed, so while the timer runs
