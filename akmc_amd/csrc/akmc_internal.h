@@ -180,6 +180,12 @@ struct BatchDev {
     // no free atom within rc since the walker's first full evaluation in
     // each precision, so its energy, kept from that evaluation, is the one a
     // full evaluation would give, bit for bit.
+    // search micro-steps on minimum-image Sutton-Chen fields: pass 1 also
+    // writes each atom's pair energy and pass 2 writes none, so the pass-2
+    // variant may differ from pass 1's without changing any energy
+    int pe1;
+    int lpa2, unroll2, staged2;  // pass-2 variant (0: the batch's)
+    const int *zfrozen;   // [A] the search's frozen mask (zone marking in the list build)
     int *zflag;           // [W] AKMC_ZONE_* bits
     int *zmark;           // [A] zone membership
     int *zlist;           // [A] walker w's zone (local ids, ascending) at zlist[off_w ..)
@@ -274,6 +280,9 @@ struct WalkerState {
     double fnpar, fnperp, fmpar, fmperp, dotfv, half, lam, lamold, lbeta;
     int lconv;
     int nuphill;  // FIRE2: uphill steps in a row
+    // relax of a walker with frozen atoms: their common velocity (they all
+    // start at rest and get the same centre-of-mass corrections and resets)
+    double vf[3];
     // basin 2 of a converged saddle: energy (search precision, FP64),
     // largest squared displacement from the basin, new-basin flag, FIRE
     // status, force calls and iterations of its relax

@@ -156,6 +156,9 @@ struct akmc_batch_s {
     bool zone_on = true;
     int *zflag = nullptr, *zmark = nullptr, *zlist = nullptr, *nz = nullptr;
     std::vector<int> hzflag;
+    // kernel variants of the search micro-steps' Sutton-Chen passes:
+    // (lanes, unroll, staged) of pass 1, then of pass 2; 0: the batch's
+    int sv[6] = {0, 0, 0, 0, 0, 0};
     // pinned status staging: one stream sync per host check of a run
     WalkerState *hws_pin = nullptr;
     int *ov_pin = nullptr;
@@ -1031,11 +1034,27 @@ bool zone_usable(akmc_batch_t b)
     return b->zone_on && b->zflag && h.kind == 0 && !h.eam && h.min_image;
 }
 
-// the batch view of the search micro-steps: the plain one plus the zones
+// the batch view of the search micro-steps: the plain one plus the pair
+// energies in pass 1, the search variants and the zones
 BatchDev step_dev(akmc_batch_t b)
 {
     BatchDev d = b->d;
+    const FieldDev &h = b->f->host;
+    if (h.kind == 0 && !h.eam && h.min_image) {
+        d.pe1 = 1;
+        if (b->sv[0]) {
+            d.lpa = b->sv[0];
+            d.unroll = b->sv[1];
+            d.staged = b->sv[2];
+        }
+        if (b->sv[3]) {
+            d.lpa2 = b->sv[3];
+            d.unroll2 = b->sv[4];
+            d.staged2 = b->sv[5];
+        }
+    }
     if (zone_usable(b)) {
+        d.zfrozen = b->s.frozen;
         d.zflag = b->zflag;
         d.zmark = b->zmark;
         d.zlist = b->zlist;
@@ -1121,6 +1140,16 @@ int start_walkers(akmc_batch_t b, const int *walkers, int kind, double dt0, doub
                            : 0;
     }
     HIPCHK(hipMemcpyAsync(b->zflag, b->hzflag.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    // a started walker rebuilds its lists on its first micro-step (a zoned
+    // one marks its zone there), with or without zones, so both list the
+    // same neighbours in the same order (flags already raised stay raised)
+    std::vector<int> reb(b->W, 0);
+    HIPCHK(hipMemcpyAsync(reb.data(), b->d.rebuild, b->W * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int w = 0; w < b->W; ++w)
+        if (!walkers || walkers[w]) reb[w] = 1;
+    HIPCHK(hipMemcpyAsync(b->d.rebuild, reb.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     int run;
     TRY(refresh_active(b, &run));
     akmc_launch_probe_init(b->d, b->s, b->stream);
@@ -1820,6 +1849,60 @@ int akmc_batch_set_search_zone(akmc_batch_t b, int enable)
     if (!b) return fail(AKMC_EINVAL, "null batch");
     b->zone_on = enable != 0;
     drop_graph(b);
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_set_search_variant(akmc_batch_t b, const int v[6])
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    int nv[6] = {0, 0, 0, 0, 0, 0};
+    if (v) {
+        for (int p = 0; p < 2; ++p) {
+            const int l = v[3 * p], u = v[3 * p + 1], st = v[3 * p + 2];
+            if (l == 0) continue;
+            if ((l != 1 && l != 2 && l != 4 && l != 8) || (u != 1 && u != 2 && u != 4 && u != 8))
+                return fail(AKMC_EINVAL, "pass %d: lanes and unroll must be 1, 2, 4 or 8", p + 1);
+            if (st && !b->stage_ok) return fail(AKMC_EUNSUPPORTED, "LDS staging is not possible for this batch");
+            nv[3 * p] = l;
+            nv[3 * p + 1] = u;
+            nv[3 * p + 2] = st ? 1 : 0;
+        }
+    }
+    std::memcpy(b->sv, nv, sizeof nv);
+    drop_graph(b);
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[6], double ms1[AKMC_NVARIANTS],
+                               double ms2[AKMC_NVARIANTS])
+{
+    if (!b || nrep <= 0) return fail(AKMC_EINVAL, "bad arguments");
+    if (!b->search_alloc) return fail(AKMC_EINVAL, "no search has been started");
+    static const int lanes[4] = {1, 2, 4, 8}, unroll[4] = {1, 2, 4, 8};
+    const int nv = b->stage_ok ? AKMC_NVARIANTS : AKMC_NVARIANTS / 2;
+    double t1[AKMC_NVARIANTS], t2[AKMC_NVARIANTS], ms[4];
+    for (int v = 0; v < AKMC_NVARIANTS; ++v) {
+        t1[v] = t2[v] = -1.0;
+        if (v >= nv) continue;
+        const int sv[6] = {lanes[v / 4 % 4], unroll[v % 4], v / 16, lanes[v / 4 % 4], unroll[v % 4], v / 16};
+        TRY(akmc_batch_set_search_variant(b, sv));
+        TRY(akmc_batch_time_search_phases(b, 1, ms));  // first launch of this instantiation
+        TRY(akmc_batch_time_search_phases(b, nrep, ms));
+        t1[v] = ms[1];
+        t2[v] = ms[2];
+    }
+    int k1 = 0, k2 = 0;
+    for (int v = 1; v < nv; ++v) {
+        if (t1[v] < t1[k1]) k1 = v;
+        if (t2[v] < t2[k2]) k2 = v;
+    }
+    const int sv[6] = {lanes[k1 / 4 % 4], unroll[k1 % 4], k1 / 16, lanes[k2 / 4 % 4], unroll[k2 % 4], k2 / 16};
+    TRY(akmc_batch_set_search_variant(b, sv));
+    if (best) std::memcpy(best, sv, sizeof sv);
+    for (int v = 0; v < AKMC_NVARIANTS; ++v) {
+        if (ms1) ms1[v] = t1[v];
+        if (ms2) ms2[v] = t2[v];
+    }
     return AKMC_SUCCESS;
 }
 

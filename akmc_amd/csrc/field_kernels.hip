@@ -229,6 +229,9 @@ __global__ void k_pack_check(BatchDev b)
     b.p4[i] = make_double4(x, y, z, (double)b.type[i]);
     double dx = x - b.xr[i], dy = y - b.yr[i], dz = z - b.zr[i];
     if (dx * dx + dy * dy + dz * dz > b.field->half_skin2) b.rebuild[w] = 1;
+    // a walker's fresh zone is marked by its list build (its start already
+    // asks for one)
+    if (b.zflag && (b.zflag[w] & AKMC_ZONE_RESET)) b.rebuild[w] = 1;
 }
 
 // record the positions the lists are built from (positions are never
@@ -280,6 +283,9 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
     const int nc = W.nc[0] * W.nc[1] * W.nc[2];
     const int tid = threadIdx.x;
     for (int c = tid; c < nc; c += kBlock) fill[c] = 0;
+    // a restarted walker's zone starts empty (the list build marks it)
+    if (b.zflag && (b.zflag[w] & AKMC_ZONE_RESET))
+        for (int l = tid; l < W.n; l += kBlock) b.zmark[W.off + l] = 0;
     __syncthreads();
     for (int l = tid; l < W.n; l += kBlock) atomicAdd(&fill[atom_cell_of(b, W, W.off + l)], 1);
     if (tid == 0) carry = W.off;
@@ -484,6 +490,10 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
     const bool have = q < n0;
     const int i = have ? b.cell_atoms[s0 + q] : 0;
     const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
+    // a free atom of a zoned walker puts itself and every atom of its list
+    // in the walker's zone
+    const bool mark = have && b.zflag && (b.zflag[w] & AKMC_ZONE_ON) && b.zfrozen[i] == 0;
+    if (mark && lane == 0) b.zmark[i] = 1;
     int cnt = 0, nnear = 0;
     // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
     for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
@@ -525,8 +535,10 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
                     // ballot: a lane's slot is the count of hits below it
                     const unsigned long long bal = __ballot(hit[u]);
                     const unsigned grp = unsigned(bal >> (wlane & ~(kNbLanes - 1))) & 0xFFFFu;
-                    if (hit[u])
+                    if (hit[u]) {
                         put_nbr(b, i, cnt + __popc(grp & ((1u << lane) - 1u)), uint32_t(jj[u] - W.off) | AKMC_NB_ZERO_SHIFT);
+                        if (mark) b.zmark[jj[u]] = 1;
+                    }
                     cnt += __popc(grp);
                 }
             }
@@ -642,49 +654,35 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_images(BatchDev b)
     put_sentinel(b, i, n, W.off);
 }
 
-// The active zone of a search walker (one workgroup per walker), after its
-// lists were rebuilt or at its start (membership reset first): its free
-// atoms and every atom on a free atom's list join the zone, then the
-// members are listed in ascending index order.
+// The active zone of a search walker whose lists were rebuilt (one
+// workgroup per walker): the list build has marked its free atoms and every
+// atom on a free atom's list (on top of the marks since the walker's start,
+// cleared by k_cell_bin at a restart); the members are listed in ascending
+// index order, each thread a contiguous run of the walker's atoms.
 __global__ void __launch_bounds__(kBlock) k_zone_build(BatchDev b)
 {
-    __shared__ int wcount[kBlock / 64];
-    __shared__ int total;
+    __shared__ int part[kBlock];
     const int w = blockIdx.x;
     const int f = b.zflag[w];
-    if (!(f & AKMC_ZONE_ON) || !((f & AKMC_ZONE_RESET) || b.rebuild[w])) return;
+    if (!(f & AKMC_ZONE_ON) || !b.rebuild[w]) return;
     const WalkerDev W = b.walkers[w];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (f & AKMC_ZONE_RESET) {
-        for (int l = threadIdx.x; l < W.n; l += kBlock) b.zmark[W.off + l] = 0;
-        __syncthreads();
-    }
-    const int nf = b.znact[w];
-    for (int k = threadIdx.x; k < nf; k += kBlock) b.zmark[b.zact[W.off + k]] = 1;
-    // one wavefront per free atom, its lanes over the atom's list
-    for (int k = wave; k < nf; k += kBlock / 64) {
-        const int i = b.zact[W.off + k];
-        const int cnt = b.nnb[i] < b.maxnb ? b.nnb[i] : b.maxnb;
-        for (int e = lane; e < cnt; e += 64) b.zmark[W.off + int(nbr_entry(b, e, i) & AKMC_NB_JMASK)] = 1;
-    }
-    if (threadIdx.x == 0) total = 0;
+    const int per = (W.n + kBlock - 1) / kBlock;
+    const int l0 = (int)threadIdx.x * per, l1 = min(W.n, l0 + per);
+    int c = 0;
+    for (int l = l0; l < l1; ++l) c += b.zmark[W.off + l] != 0;
+    part[threadIdx.x] = c;
     __syncthreads();
-    for (int l0 = 0; l0 < W.n; l0 += kBlock) {
-        const int l = l0 + (int)threadIdx.x;
-        const bool m = l < W.n && b.zmark[W.off + l] != 0;
-        const unsigned long long bal = __ballot(m);
-        if (lane == 0) wcount[wave] = __popcll(bal);
+    for (int o = 1; o < kBlock; o <<= 1) {
+        const int t = (int)threadIdx.x >= o ? part[threadIdx.x - o] : 0;
         __syncthreads();
-        int at = total;
-        for (int v = 0; v < wave; ++v) at += wcount[v];
-        if (m) b.zlist[W.off + at + __popcll(bal & ((1ull << lane) - 1ull))] = l;
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int v = 0; v < kBlock / 64; ++v) total += wcount[v];
+        part[threadIdx.x] += t;
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        b.nz[w] = total;
+    int at = part[threadIdx.x] - c;
+    for (int l = l0; l < l1; ++l)
+        if (b.zmark[W.off + l]) b.zlist[W.off + at++] = l;
+    if (threadIdx.x == kBlock - 1) {
+        b.nz[w] = part[kBlock - 1];
         b.zflag[w] = f & ~AKMC_ZONE_RESET;
     }
 }
@@ -846,10 +844,11 @@ __device__ inline AtomSlot sc_slot(const BatchDev &b, WalkerDev &W, double4 *sta
 // LPA lanes share one atom's neighbour list (strided) and combine by shuffles.
 // ONE: the field has a single species pair with n = 9, m = 6, whose
 // coefficients stay in registers and whose powers are straight products.
-// ZONE (search micro-steps with active zones): zoned walkers visit their
-// zone, and every visited atom also gets its half of its pairs' V, the
-// value pass 2 writes (the same terms in the same order), since pass 2
-// visits only the free atoms.
+// ZONE (search micro-steps): zoned walkers visit their zone, and every
+// visited atom also gets its half of its pairs' V (pass 2 then writes no
+// energy, so the energies do not depend on pass 2's variant, and the
+// frozen zone atoms' pair energies stay current while pass 2 visits only
+// the free atoms).
 template <typename T, bool MINIMG, int LPA, int U, bool ONE, bool STAGED, bool ZONE>
 __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_density(BatchDev b)
 {
@@ -1107,7 +1106,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
     b.fx[i] = fx;
     b.fy[i] = fy;
     b.fz[i] = fz;
-    energy_b<T>(b)[i] = ep;
+    if (!b.pe1) energy_b<T>(b)[i] = ep;  // search micro-steps: pass 1 wrote it
     if (STRESS) {
         b.st_atom[0 * (size_t)b.A + i] = s0;
         b.st_atom[1 * (size_t)b.A + i] = s1;
@@ -1320,7 +1319,7 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_eam_forc
     b.fx[i] = fx;
     b.fy[i] = fy;
     b.fz[i] = fz;
-    energy_b<T>(b)[i] = ep;
+    if (!b.pe1) energy_b<T>(b)[i] = ep;  // search micro-steps: pass 1 wrote it
     if (STRESS) {
         b.st_atom[0 * (size_t)b.A + i] = s0;
         b.st_atom[1 * (size_t)b.A + i] = s1;
@@ -1810,13 +1809,13 @@ static ScLaunch sc_launch(const BatchDev &b)
     return {xcd_grid((b.A * LPA + kBlock - 1) / kBlock), kBlock, 0};
 }
 
-// search micro-steps with active zones (minimum-image fields only) take the
-// ZONE variants of pass 1
+// search micro-steps (minimum-image fields only) take the ZONE variants of
+// pass 1, which also write the pair energies and visit active zones
 template <typename T, int LPA, int U, bool ONE, bool STAGED>
 static void launch_sc_density_one(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     const ScLaunch L = sc_launch<LPA>(b);
-    if (hf.min_image && b.zflag) k_sc_density<T, true, LPA, U, ONE, STAGED, true><<<L.grid, L.block, L.lds, s>>>(b);
+    if (hf.min_image && b.pe1) k_sc_density<T, true, LPA, U, ONE, STAGED, true><<<L.grid, L.block, L.lds, s>>>(b);
     else if (hf.min_image) k_sc_density<T, true, LPA, U, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
     else k_sc_density<T, false, LPA, U, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
 }
@@ -1947,6 +1946,24 @@ static void launch_pass1(const BatchDev &b, const FieldDev &hf, int want_stress,
     }
 }
 
+// pass 2 with its own variant (search micro-steps)
+static BatchDev pass2_view(const BatchDev &b)
+{
+    BatchDev v = b;
+    v.lpa = b.lpa2;
+    v.unroll = b.unroll2;
+    v.staged = b.staged2;
+    return v;
+}
+
+template <typename T>
+static void launch_sc_force_variant(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
+{
+#define AKMC_FORCE(L_, U_) launch_sc_force<T, L_, U_>(b, hf, want_stress, s)
+    AKMC_SC_VARIANT(AKMC_FORCE);
+#undef AKMC_FORCE
+}
+
 template <typename T>
 static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
@@ -1968,9 +1985,7 @@ static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress,
     if (hf.kind == 0 && hf.eam) {
         launch_eam_force<T>(b, hf, want_stress, s);
     } else if (hf.kind == 0) {
-#define AKMC_FORCE(L_, U_) launch_sc_force<T, L_, U_>(b, hf, want_stress, s)
-        AKMC_SC_VARIANT(AKMC_FORCE);
-#undef AKMC_FORCE
+        launch_sc_force_variant<T>(b.lpa2 ? pass2_view(b) : b, hf, want_stress, s);
     } else if (hf.do_ewald == 1) {
         k_ewald_tables<<<gA, kBlock, 0, s>>>(b);
         dim3 g((b.maxnk + 63) / 64, b.W);
@@ -1992,6 +2007,7 @@ static BatchDev fp64_view(const BatchDev &b, int prec)
         v.staged = 0;
         v.lpa = 4;  // measured best for this pass on the batched dimer workload
         v.unroll = 4;
+        v.lpa2 = 0;  // both passes
     }
     return v;
 }

@@ -58,6 +58,10 @@ struct Ctx {
     const SearchDev &s;
     int off, n, A;
     int nact;  // free atoms of the walker: s.act[off .. off + nact)
+    // the walker has frozen atoms: its relax loops visit the free atoms and
+    // carry the frozen atoms' common velocity as WalkerState.vf
+    __device__ bool part() const { return nact < n; }
+    __device__ int nfrozen() const { return n - nact; }
     __device__ double f(int c, int i) const
     {
         if (s.frozen[i]) return 0.0;
@@ -79,6 +83,16 @@ struct Ctx {
 #define FOR_ACTIVE(i)                                                                   \
     for (int t_##i = (int)threadIdx.x, i = 0;                                           \
          t_##i < C_.nact && ((i = C_.s.act[C_.off + t_##i]), true); t_##i += kBlock)
+// the per-iteration relax loops: every atom, or the free atoms when the
+// walker has frozen atoms.  A frozen atom has no force and never moves; its
+// velocity, which enters the centre-of-mass correction and |V|^2 of the
+// reference's all-atom sums (Relax.cpp:125-160, :964-996), is the same for
+// every frozen atom, so it is kept once per walker.
+#define FOR_RELAX(i)                                                                          \
+    for (int t_##i = (int)threadIdx.x, i = 0;                                                 \
+         t_##i < (C_.part() ? C_.nact : C_.n) &&                                              \
+         ((i = C_.part() ? C_.s.act[C_.off + t_##i] : C_.off + t_##i), true);                 \
+         t_##i += kBlock)
 #define V3(arr, c, i) (arr)[(size_t)(c) * C_.A + (i)]
 
 // probe = X + a * D (D a [3][A] vector) or X when D is null
@@ -143,6 +157,7 @@ __device__ __attribute__((always_inline)) void enter_basin2(const Ctx &C_, Walke
     w.it = 0;
     w.nstep = 1;  // Relax.cpp:49
     w.nuphill = 0;
+    w.vf[0] = w.vf[1] = w.vf[2] = 0.0;
     w.dt = C_.s.rp.initTimeStep;
     w.alpha = C_.s.rp.alphaStart;
     w.b2_calls = 0;
@@ -196,7 +211,7 @@ __device__ __attribute__((always_inline)) void relax_pre(const Ctx &C_, WalkerSt
     const RelaxParamsDev &p = C_.s.rp;
     const bool fire2 = p.method == 1;
     double acc[3] = {0, 0, 0};  // P, |F|^2, |V|^2
-    FOR_ATOMS(i)
+    FOR_RELAX(i)
     {
         for (int c = 0; c < 3; ++c) {
             double F = V3(C_.s.F0, c, i), v = V3(C_.s.V, c, i);
@@ -206,6 +221,7 @@ __device__ __attribute__((always_inline)) void relax_pre(const Ctx &C_, WalkerSt
         }
     }
     bsum<3>(acc, sh);
+    if (C_.part()) acc[2] += C_.nfrozen() * (w.vf[0] * w.vf[0] + w.vf[1] * w.vf[1] + w.vf[2] * w.vf[2]);
     double P = acc[0];
     double af = w.alpha * sqrt(acc[2]) / sqrt(acc[1]);
     double oma = 1.0 - w.alpha;
@@ -238,7 +254,9 @@ __device__ __attribute__((always_inline)) void relax_pre(const Ctx &C_, WalkerSt
         w.alpha = p.alphaStart;
     }
     const double dt = w.dt, fact = dt * p.timeStepUnit, half = 0.5 * dt * p.timeStepUnit;
-    FOR_ATOMS(i)
+    // frozen velocities: kept when mixing, zeroed on an uphill step and by Verlet
+    if (!down || p.verlet) w.vf[0] = w.vf[1] = w.vf[2] = 0.0;
+    FOR_RELAX(i)
     {
         bool fr = C_.s.frozen[i] != 0;
         for (int c = 0; c < 3; ++c) {
@@ -317,8 +335,9 @@ __device__ __attribute__((always_inline)) void relax_logic(const Ctx &C_, Walker
         finish(C_, w);
         return;
     }
-    // forces at X (frozen atoms zeroed, Relax.cpp:230)
-    FOR_ATOMS(i)
+    // forces at X (frozen atoms zeroed, Relax.cpp:230; with frozen atoms only
+    // the free atoms' forces are read)
+    FOR_RELAX(i)
     {
         for (int c = 0; c < 3; ++c) V3(C_.s.F0, c, i) = C_.f(c, i);
     }
@@ -341,7 +360,7 @@ __device__ __attribute__((always_inline)) void relax_logic(const Ctx &C_, Walker
         // second half kick with the new forces; frozen velocities stay zero
         // (Relax.cpp:233-257, :569-585)
         const double half = 0.5 * w.dt * p.timeStepUnit;
-        FOR_ATOMS(i)
+        FOR_RELAX(i)
         {
             const bool fr = C_.s.frozen[i] != 0;
             for (int c = 0; c < 3; ++c) {
@@ -349,17 +368,24 @@ __device__ __attribute__((always_inline)) void relax_logic(const Ctx &C_, Walker
                 else V3(C_.s.V, c, i) = 0.0;
             }
         }
+        w.vf[0] = w.vf[1] = w.vf[2] = 0.0;
         __syncthreads();
     }
     // centre-of-mass velocity removal over all atoms (Relax.cpp:964-996)
     double m[3] = {0, 0, 0};
-    FOR_ATOMS(i)
+    FOR_RELAX(i)
     {
         for (int c = 0; c < 3; ++c) m[c] += V3(C_.s.V, c, i);
     }
     bsum<3>(m, sh);
+    if (C_.part()) {
+        for (int c = 0; c < 3; ++c) {
+            m[c] += C_.nfrozen() * w.vf[c];
+            w.vf[c] -= m[c] / (double)C_.n;
+        }
+    }
     double fn2 = 0.0, fm2 = 0.0;
-    FOR_ATOMS(i)
+    FOR_RELAX(i)
     {
         double q = 0.0;
         for (int c = 0; c < 3; ++c) {

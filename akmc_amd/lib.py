@@ -190,6 +190,9 @@ def load() -> C.CDLL:
     L.akmc_batch_set_search_zone.argtypes = [C.c_void_p, C.c_int]
     L.akmc_batch_time_search_phases.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
     L.akmc_batch_get_zone.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 4
+    L.akmc_batch_set_search_variant.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.akmc_batch_autotune_search.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double)]
     L.akmc_batch_get_walkers_positions.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.akmc_batch_kernel_timer.argtypes = [C.c_void_p, C.c_int]
     L.akmc_batch_kernel_time.argtypes = [C.c_void_p, _D, C.POINTER(C.c_longlong)]
@@ -530,6 +533,22 @@ class Batch:
         ms = np.zeros(4)
         check(load().akmc_batch_time_search_phases(self._h, int(nrep), _dp(ms)))
         return ms / nrep
+
+    def set_search_variant(self, v=None):
+        """(lanes, unroll, staged) of pass 1 then pass 2 for the search micro-steps; None resets"""
+        a = None if v is None else np.ascontiguousarray(v, dtype=np.int32)
+        check(load().akmc_batch_set_search_variant(self._h, _ip(a)))
+
+    def autotune_search(self, nrep=5):
+        """fastest variant per pass on the running searches: (best[6], {variant: (ms1, ms2)})"""
+        best = np.zeros(6, np.int32)
+        n = 32
+        m1, m2 = np.zeros(n), np.zeros(n)
+        check(load().akmc_batch_autotune_search(self._h, int(nrep), _ip(best), _dp(m1), _dp(m2)))
+        table = {(l, u, st): (m1[16 * st + 4 * li + ui], m2[16 * st + 4 * li + ui]) for st in (0, 1)
+                 for li, l in enumerate((1, 2, 4, 8)) for ui, u in enumerate((1, 2, 4, 8))
+                 if m1[16 * st + 4 * li + ui] >= 0}
+        return tuple(int(x) for x in best), table
 
     def zone(self):
         """(member [A], size [W], flags [W], active [W]) of the active zones"""

@@ -651,7 +651,18 @@ def main():
     camp.begin()
     tot = S.CampaignStats()
     t_timed = 0.0
+    search_variant = None
     for r in range(args.warmup + args.steps):
+        if r == 1 or (r == 0 and args.warmup == 0):
+            # the search micro-steps' own variants, tuned per pass on the
+            # searches under way (zoned passes): untimed, between windows
+            t0 = time.perf_counter()
+            sv, sv_ms = camp.batch.autotune_search(5)
+            search_variant = {"pass1": sv[:3], "pass2": sv[3:], "seconds": round(time.perf_counter() - t0, 2),
+                              "ms_pass1_pass2": {f"{k[0]}x{k[1]}{' lds' if k[2] else ''}":
+                                                 [round(v[0], 4), round(v[1], 4)] for k, v in sv_ms.items()}}
+            print(f"[bench] search variants (lanes, unroll, staged): pass 1 {sv[:3]} pass 2 {sv[3:]}; "
+                  f"{search_variant['ms_pass1_pass2']}", file=sys.stderr, flush=True)
         rb0 = camp.batch.rebuilds
         st = camp.advance(args.window, sync_every=args.sync_every)
         print(f"[bench] window {r}: {camp.batch.rebuilds - rb0} walker list rebuilds; "
@@ -766,6 +777,7 @@ def main():
         "force_evals_per_s_batched_cells": round(batched_force_evals, 1),
         "force_eval_ms_all_walkers": round(eval_ms, 4),
         "force_evals_per_s_cells_device": round(len(cells) / (eval_ms * 1e-3), 1),
+        "search_variant": search_variant,
         "autotune": {"lanes_per_atom": variant[0], "gathers_in_flight": variant[1], "lds_staged": bool(variant[2]),
                      "ms_per_eval": {f"{k[0]}x{k[1]}{' lds' if k[2] else ''}": round(v, 4)
                                      for k, v in variant_ms.items()}},
