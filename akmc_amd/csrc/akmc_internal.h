@@ -185,6 +185,7 @@ struct BatchDev {
     // variant may differ from pass 1's without changing any energy
     int pe1;
     int lpa2, unroll2, staged2;  // pass-2 variant (0: the batch's)
+    int lpa64, unroll64;         // FP64 variant of mixed steps (unstaged; 0: 4 x 4)
     const int *zfrozen;   // [A] the search's frozen mask (zone marking in the list build)
     int *zflag;           // [W] AKMC_ZONE_* bits
     int *zmark;           // [A] zone membership
@@ -192,6 +193,12 @@ struct BatchDev {
     int *nz;              // [W] zone size
     const int *zact;      // [A] walker w's free atoms (global ids) at zact[off_w ..)
     const int *znact;     // [W] free atoms per walker
+    // work lists of the search micro-steps' Sutton-Chen pair kernels
+    // (k_work_list): FP32 pass 1, FP32 pass 2, FP64 pass 1, FP64 pass 2
+    int2 *wl_tab[4];      // (walker, first slot) per chunk
+    int *wl_count;        // [4] chunks per list
+    int wl_grid_staged;   // persistent grids of those kernels (multiples of 8)
+    int wl_grid_plain;
 };
 
 // entry k of atom i's neighbour list in the 32-bit form (local index and

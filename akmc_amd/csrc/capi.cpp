@@ -157,8 +157,13 @@ struct akmc_batch_s {
     int *zflag = nullptr, *zmark = nullptr, *zlist = nullptr, *nz = nullptr;
     std::vector<int> hzflag;
     // kernel variants of the search micro-steps' Sutton-Chen passes:
-    // (lanes, unroll, staged) of pass 1, then of pass 2; 0: the batch's
-    int sv[6] = {0, 0, 0, 0, 0, 0};
+    // (lanes, unroll, staged) of pass 1, then of pass 2, then (lanes,
+    // unroll) of the FP64 passes of mixed steps; 0: the default
+    int sv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // work lists of the search micro-steps' pair kernels and their grids
+    int2 *wl_tab[4] = {nullptr, nullptr, nullptr, nullptr};
+    int *wl_count = nullptr;
+    int wl_grid_staged = 0, wl_grid_plain = 0;
     // pinned status staging: one stream sync per host check of a run
     WalkerState *hws_pin = nullptr;
     int *ov_pin = nullptr;
@@ -1002,6 +1007,21 @@ int search_alloc(akmc_batch_t b)
     TRY(dalloc(b, &b->zmark, b->A));
     TRY(dalloc(b, &b->zlist, b->A));
     TRY(dalloc(b, &b->nz, b->W));
+    // a list chunk holds at least 32 slots (256 threads at 8 lanes per atom)
+    for (int k = 0; k < 4; ++k) TRY(dalloc(b, &b->wl_tab[k], (size_t)b->A / 32 + b->W + 1));
+    TRY(dalloc(b, &b->wl_count, 4));
+    HIPCHK(hipMemset(b->wl_count, 0, 4 * sizeof(int)));
+    {
+        // persistent grids: every CU, as many staged workgroups as its 160 KB
+        // of LDS holds (at most 2), 8 plain ones; multiples of the 8 XCDs
+        int dev = 0, ncu = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const size_t lds = (size_t)b->d.stage_bytes + sizeof(SCPairDev) * AKMC_MAXPAIR;
+        const int res = std::max(1, std::min(2, (int)(163840 / std::max<size_t>(lds, 1))));
+        b->wl_grid_staged = ((ncu * res + 7) / 8) * 8;
+        b->wl_grid_plain = ((ncu * 8 + 7) / 8) * 8;
+    }
     HIPCHK(hipMemset(b->zflag, 0, b->W * sizeof(int)));
     HIPCHK(hipMemset(b->nz, 0, b->W * sizeof(int)));
     b->hzflag.assign(b->W, 0);
@@ -1042,6 +1062,10 @@ BatchDev step_dev(akmc_batch_t b)
     const FieldDev &h = b->f->host;
     if (h.kind == 0 && !h.eam && h.min_image) {
         d.pe1 = 1;
+        for (int k = 0; k < 4; ++k) d.wl_tab[k] = b->wl_tab[k];
+        d.wl_count = b->wl_count;
+        d.wl_grid_staged = b->wl_grid_staged;
+        d.wl_grid_plain = b->wl_grid_plain;
         if (b->sv[0]) {
             d.lpa = b->sv[0];
             d.unroll = b->sv[1];
@@ -1051,6 +1075,10 @@ BatchDev step_dev(akmc_batch_t b)
             d.lpa2 = b->sv[3];
             d.unroll2 = b->sv[4];
             d.staged2 = b->sv[5];
+        }
+        if (b->sv[6]) {
+            d.lpa64 = b->sv[6];
+            d.unroll64 = b->sv[7];
         }
     }
     if (zone_usable(b)) {
@@ -1852,20 +1880,25 @@ int akmc_batch_set_search_zone(akmc_batch_t b, int enable)
     return AKMC_SUCCESS;
 }
 
-int akmc_batch_set_search_variant(akmc_batch_t b, const int v[6])
+int akmc_batch_set_search_variant(akmc_batch_t b, const int v[8])
 {
     if (!b) return fail(AKMC_EINVAL, "null batch");
-    int nv[6] = {0, 0, 0, 0, 0, 0};
+    int nv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto ok = [](int x) { return x == 1 || x == 2 || x == 4 || x == 8; };
     if (v) {
         for (int p = 0; p < 2; ++p) {
             const int l = v[3 * p], u = v[3 * p + 1], st = v[3 * p + 2];
             if (l == 0) continue;
-            if ((l != 1 && l != 2 && l != 4 && l != 8) || (u != 1 && u != 2 && u != 4 && u != 8))
-                return fail(AKMC_EINVAL, "pass %d: lanes and unroll must be 1, 2, 4 or 8", p + 1);
+            if (!ok(l) || !ok(u)) return fail(AKMC_EINVAL, "pass %d: lanes and unroll must be 1, 2, 4 or 8", p + 1);
             if (st && !b->stage_ok) return fail(AKMC_EUNSUPPORTED, "LDS staging is not possible for this batch");
             nv[3 * p] = l;
             nv[3 * p + 1] = u;
             nv[3 * p + 2] = st ? 1 : 0;
+        }
+        if (v[6]) {
+            if (!ok(v[6]) || !ok(v[7])) return fail(AKMC_EINVAL, "FP64: lanes and unroll must be 1, 2, 4 or 8");
+            nv[6] = v[6];
+            nv[7] = v[7];
         }
     }
     std::memcpy(b->sv, nv, sizeof nv);
@@ -1873,8 +1906,10 @@ int akmc_batch_set_search_variant(akmc_batch_t b, const int v[6])
     return AKMC_SUCCESS;
 }
 
-int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[6], double ms1[AKMC_NVARIANTS],
-                               double ms2[AKMC_NVARIANTS])
+static int time_search_phases_prec(akmc_batch_t b, int prec, int nrep, double ms[4]);
+
+int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[8], double ms1[AKMC_NVARIANTS],
+                               double ms2[AKMC_NVARIANTS], double ms64[AKMC_NVARIANTS / 2])
 {
     if (!b || nrep <= 0) return fail(AKMC_EINVAL, "bad arguments");
     if (!b->search_alloc) return fail(AKMC_EINVAL, "no search has been started");
@@ -1884,7 +1919,7 @@ int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[6], double ms1
     for (int v = 0; v < AKMC_NVARIANTS; ++v) {
         t1[v] = t2[v] = -1.0;
         if (v >= nv) continue;
-        const int sv[6] = {lanes[v / 4 % 4], unroll[v % 4], v / 16, lanes[v / 4 % 4], unroll[v % 4], v / 16};
+        const int sv[8] = {lanes[v / 4 % 4], unroll[v % 4], v / 16, lanes[v / 4 % 4], unroll[v % 4], v / 16, 0, 0};
         TRY(akmc_batch_set_search_variant(b, sv));
         TRY(akmc_batch_time_search_phases(b, 1, ms));  // first launch of this instantiation
         TRY(akmc_batch_time_search_phases(b, nrep, ms));
@@ -1896,12 +1931,26 @@ int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[6], double ms1
         if (t1[v] < t1[k1]) k1 = v;
         if (t2[v] < t2[k2]) k2 = v;
     }
-    const int sv[6] = {lanes[k1 / 4 % 4], unroll[k1 % 4], k1 / 16, lanes[k2 / 4 % 4], unroll[k2 % 4], k2 / 16};
+    // the FP64 passes of mixed steps (unstaged), on the walkers under FP64
+    // verification at the moment: both passes together
+    double t64[AKMC_NVARIANTS / 2];
+    int k64 = -1;
+    for (int v = 0; v < AKMC_NVARIANTS / 2; ++v) {
+        const int sv[8] = {0, 0, 0, 0, 0, 0, lanes[v / 4], unroll[v % 4]};
+        TRY(akmc_batch_set_search_variant(b, sv));
+        TRY(time_search_phases_prec(b, 2 | 4, 1, ms));
+        TRY(time_search_phases_prec(b, 2 | 4, nrep, ms));
+        t64[v] = ms[1] + ms[2];
+        if (k64 < 0 || t64[v] < t64[k64]) k64 = v;
+    }
+    const int sv[8] = {lanes[k1 / 4 % 4], unroll[k1 % 4], k1 / 16, lanes[k2 / 4 % 4], unroll[k2 % 4], k2 / 16,
+                       lanes[k64 / 4], unroll[k64 % 4]};
     TRY(akmc_batch_set_search_variant(b, sv));
     if (best) std::memcpy(best, sv, sizeof sv);
     for (int v = 0; v < AKMC_NVARIANTS; ++v) {
         if (ms1) ms1[v] = t1[v];
         if (ms2) ms2[v] = t2[v];
+        if (ms64 && v < AKMC_NVARIANTS / 2) ms64[v] = t64[v];
     }
     return AKMC_SUCCESS;
 }
@@ -2078,6 +2127,13 @@ int akmc_batch_time_search_phases(akmc_batch_t b, int nrep, double ms[4])
 {
     if (!b || nrep <= 0 || !ms) return fail(AKMC_EINVAL, "bad arguments");
     if (!b->search_alloc) return fail(AKMC_EINVAL, "no search has been started");
+    return time_search_phases_prec(b, batch_mode(b), nrep, ms);
+}
+
+// per-phase times of nrep evaluations of the search walkers of precision
+// mask prec (2 | 4: the FP64 passes of mixed steps) at their probes
+static int time_search_phases_prec(akmc_batch_t b, int prec, int nrep, double ms[4])
+{
     for (auto &e : b->evp)
         if (!e) HIPCHK(hipEventCreate(&e));
     const BatchDev d = step_dev(b);
@@ -2085,7 +2141,7 @@ int akmc_batch_time_search_phases(akmc_batch_t b, int nrep, double ms[4])
     for (int r = 0; r < nrep; ++r) {
         HIPCHK(hipEventRecord(b->evp[0], b->stream));
         for (int ph = 0; ph < 4; ++ph) {
-            akmc_launch_phase(d, b->f->host, batch_mode(b), 0, ph, b->stream);
+            akmc_launch_phase(d, b->f->host, prec, 0, ph, b->stream);
             HIPCHK(hipEventRecord(b->evp[ph + 1], b->stream));
         }
         HIPCHK(hipEventSynchronize(b->evp[4]));

@@ -225,6 +225,10 @@ __global__ void k_pack_check(BatchDev b)
     if (i >= b.A) return;
     int w = walker_of(b, i);
     if (skipped(b, w)) return;
+    // an atom outside a running search's zone has not moved since the
+    // walker's start and its record is only read by full evaluations, whose
+    // pass 1 rewrites the fourth word it reads (single pair type)
+    if (b.zflag && (b.zflag[w] & (AKMC_ZONE_ON | AKMC_ZONE_RESET)) == AKMC_ZONE_ON && !b.zmark[i]) return;
     const double x = b.x[i], y = b.y[i], z = b.z[i];
     b.p4[i] = make_double4(x, y, z, (double)b.type[i]);
     double dx = x - b.xr[i], dy = y - b.yr[i], dz = z - b.zr[i];
@@ -687,6 +691,56 @@ __global__ void __launch_bounds__(kBlock) k_zone_build(BatchDev b)
     }
 }
 
+// The work lists of a search micro-step (one 1024-thread workgroup, after
+// the zones): for each list li (FP32 pass 1, FP32 pass 2, FP64 pass 1,
+// FP64 pass 2) with chunk size apc[li] > 0, the chunks of every walker in
+// that precision, walker after walker, each chunk apc[li] consecutive slots
+// of the walker's pass list.
+__global__ void __launch_bounds__(1024) k_work_list(BatchDev b, int4 apc)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        const int li = blockIdx.x;  // one workgroup per list
+        const int a = li == 0 ? apc.x : li == 1 ? apc.y : li == 2 ? apc.z : apc.w;
+        if (tid == 0) carry = 0;
+        __syncthreads();
+        if (a > 0) {
+            const int mode = li < 2 ? 1 : 2, pass = (li & 1) + 1;
+            const int full = mode == 2 ? AKMC_ZONE_FULL64 : AKMC_ZONE_FULL32;
+            for (int w0 = 0; w0 < b.W; w0 += 1024) {
+                const int w = w0 + tid;
+                int k = 0;
+                if (w < b.W && b.active[w] == mode) {
+                    int len = b.walkers[w].n;
+                    if (b.zflag) {
+                        const int f = b.zflag[w];
+                        if ((f & AKMC_ZONE_ON) && !(f & full)) len = pass == 1 ? b.nz[w] : b.znact[w];
+                    }
+                    k = (len + a - 1) / a;
+                }
+                int incl = k;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(incl, o);
+                    if (lane >= o) incl += t;
+                }
+                if (lane == 63) wsum[wave] = incl;
+                __syncthreads();
+                int at = carry + incl - k;
+                for (int v = 0; v < wave; ++v) at += wsum[v];
+                for (int j = 0; j < k; ++j) b.wl_tab[li][at + j] = make_int2(w, j * a);
+                __syncthreads();
+                if (tid == 0)
+                    for (int v = 0; v < 16; ++v) carry += wsum[v];
+                __syncthreads();
+            }
+        }
+        if (tid == 0) b.wl_count[li] = carry;
+        __syncthreads();
+    }
+}
+
 __global__ void k_rebuild_done(BatchDev b, unsigned long long *count)
 {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -787,6 +841,28 @@ __device__ inline float isr32_of(double w) { return __uint_as_float(uint32_t(__d
 // (p4) into LDS, so every neighbour gather of the pair loop is an LDS read
 // instead of a texture-unit gather from L2.  A workgroup whose walker runs
 // in the other precision, or is inactive, leaves before staging.
+// Work lists of the search micro-steps (k_work_list): item c of list li is
+// (walker, first slot) of a chunk of the walker's pass list -- its zone
+// (pass 1) or free atoms (pass 2) when zoned, else all its atoms.  The pair
+// kernels of a search micro-step run a persistent grid over the list; the
+// XCD-ordered first item keeps consecutive items (one walker's chunks) on
+// one XCD.  The grid is a multiple of kXCD.
+__device__ inline int wl_first()
+{
+    const int per = gridDim.x / kXCD;
+    return (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+}
+template <typename T>
+__device__ inline int wl_len(const BatchDev &b, const WalkerDev &W, int w, int pass)
+{
+    return zoned<T>(b, w) ? (pass == 1 ? b.nz[w] : b.znact[w]) : W.n;
+}
+template <typename T>
+__device__ inline int wl_atom(const BatchDev &b, const WalkerDev &W, int w, int k, int pass)
+{
+    return zoned<T>(b, w) ? (pass == 1 ? W.off + b.zlist[W.off + k] : b.zact[W.off + k]) : W.off + k;
+}
+
 // With zp (1: pass 1, 2: pass 2) a zoned walker's slots run over its zone
 // (pass 1) or its free atoms (pass 2) instead of all its atoms: slot k of
 // the walker serves the k-th listed atom, and workgroups whose slots all
@@ -850,18 +926,11 @@ __device__ inline AtomSlot sc_slot(const BatchDev &b, WalkerDev &W, double4 *sta
 // frozen zone atoms' pair energies stay current while pass 2 visits only
 // the free atoms).
 template <typename T, bool MINIMG, int LPA, int U, bool ONE, bool STAGED, bool ZONE>
-__global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_density(BatchDev b)
+__device__ __forceinline__ void sc_density_atom(const BatchDev &b, const FieldDev &F, const SCPairDev *tab,
+                                                const SCCoef<T> &c1, const WalkerDev &W, const double4 *stage,
+                                                int i, int sub)
 {
-    __shared__ SCPairDev tab[ONE ? 1 : AKMC_MAXPAIR];
-    extern __shared__ double4 stage[];
     constexpr bool NB16 = STAGED && MINIMG;  // staged batches store 16-bit minimum-image entries
-    const FieldDev &F = *b.field;
-    if (!ONE) stage_sc_table(F, tab);
-    const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
-    WalkerDev W;
-    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage, ZONE ? 1 : 0);
-    if (!sl.ok) return;
-    const int i = sl.i, sub = sl.sub;
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
     const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
@@ -948,23 +1017,50 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
     if (ONE) reinterpret_cast<double *>(&b.p4[i])[3] = sizeof(T) == 8 ? isr64 : isr32_word(isr32);
 }
 
+template <typename T, bool MINIMG, int LPA, int U, bool ONE, bool STAGED, bool ZONE>
+__global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_density(BatchDev b)
+{
+    __shared__ SCPairDev tab[ONE ? 1 : AKMC_MAXPAIR];
+    extern __shared__ double4 stage[];
+    const FieldDev &F = *b.field;
+    if (!ONE) stage_sc_table(F, tab);
+    const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
+    WalkerDev W;
+    if constexpr (ZONE) {
+        // search micro-steps: a persistent grid over the step's work list
+        const int li = sizeof(T) == 8 ? 2 : 0;
+        for (int c = wl_first(); c < b.wl_count[li]; c += gridDim.x) {
+            const int2 it = b.wl_tab[li][c];
+            W = b.walkers[it.x];
+            const int n = wl_len<T>(b, W, it.x, 1);
+            if constexpr (STAGED) {
+                __syncthreads();  // the previous item is done with the stage
+                for (int k = threadIdx.x; k < W.n; k += blockDim.x) stage[k] = b.p4[W.off + k];
+                __syncthreads();
+            }
+            const int la = it.y + (int)threadIdx.x / LPA;
+            if (la < n)
+                sc_density_atom<T, MINIMG, LPA, U, ONE, STAGED, ZONE>(b, F, tab, c1, W, stage,
+                                                                       wl_atom<T>(b, W, it.x, la, 1),
+                                                                       (int)threadIdx.x % LPA);
+        }
+        return;
+    }
+    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage, 0);
+    if (!sl.ok) return;
+    sc_density_atom<T, MINIMG, LPA, U, ONE, STAGED, ZONE>(b, F, tab, c1, W, stage, sl.i, sl.sub);
+}
+
 // pass 2: F_i = sum_j r_ij (fs - fm)/r^2, fs = n eps (a/r)^n,
 // fm = 1/2 m (eps c)^2 (1/sqrt(rho_i) + 1/sqrt(rho_j)) (a/r)^m
 // (potentials/metal/Field.cpp:587-640, ManyBody.cpp:621-757); each atom
 // also carries half of each pair's V = eps (a/r)^n
 template <typename T, bool MINIMG, bool STRESS, int LPA, int U, bool ONE, bool STAGED>
-__global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force(BatchDev b)
+__device__ __forceinline__ void sc_force_atom(const BatchDev &b, const FieldDev &F, const SCPairDev *tab,
+                                              const SCCoef<T> &c1, const WalkerDev &W, const double4 *stage,
+                                              int i, int sub)
 {
-    __shared__ SCPairDev tab[ONE ? 1 : AKMC_MAXPAIR];
-    extern __shared__ double4 stage[];
     constexpr bool NB16 = STAGED && MINIMG;
-    const FieldDev &F = *b.field;
-    if (!ONE) stage_sc_table(F, tab);
-    const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
-    WalkerDev W;
-    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage, 2);
-    if (!sl.ok) return;
-    const int i = sl.i, sub = sl.sub;
     const double xi = b.x[i], yi = b.y[i], zi = b.z[i];
     const int row = b.type[i] * F.nspec;
     const double rc2 = F.rc2;
@@ -1115,6 +1211,40 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
         b.st_atom[4 * (size_t)b.A + i] = s4;
         b.st_atom[5 * (size_t)b.A + i] = s5;
     }
+}
+
+// WL: search micro-steps, a persistent grid over the step's work list
+template <typename T, bool MINIMG, bool STRESS, int LPA, int U, bool ONE, bool STAGED, bool WL>
+__global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force(BatchDev b)
+{
+    __shared__ SCPairDev tab[ONE ? 1 : AKMC_MAXPAIR];
+    extern __shared__ double4 stage[];
+    const FieldDev &F = *b.field;
+    if (!ONE) stage_sc_table(F, tab);
+    const SCCoef<T> c1 = sc_coef<T>(F.sc[0]);
+    WalkerDev W;
+    if constexpr (WL) {
+        const int li = sizeof(T) == 8 ? 3 : 1;
+        for (int c = wl_first(); c < b.wl_count[li]; c += gridDim.x) {
+            const int2 it = b.wl_tab[li][c];
+            W = b.walkers[it.x];
+            const int n = wl_len<T>(b, W, it.x, 2);
+            if constexpr (STAGED) {
+                __syncthreads();  // the previous item is done with the stage
+                for (int k = threadIdx.x; k < W.n; k += blockDim.x) stage[k] = b.p4[W.off + k];
+                __syncthreads();
+            }
+            const int la = it.y + (int)threadIdx.x / LPA;
+            if (la < n)
+                sc_force_atom<T, MINIMG, STRESS, LPA, U, ONE, STAGED>(b, F, tab, c1, W, stage,
+                                                                       wl_atom<T>(b, W, it.x, la, 2),
+                                                                       (int)threadIdx.x % LPA);
+        }
+        return;
+    }
+    const AtomSlot sl = sc_slot<T, LPA, STAGED>(b, W, stage, 2);
+    if (!sl.ok) return;
+    sc_force_atom<T, MINIMG, STRESS, LPA, U, ONE, STAGED>(b, F, tab, c1, W, stage, sl.i, sl.sub);
 }
 
 // energy-only pass 2 of Sutton-Chen (Field::calculateEnergy,
@@ -1815,7 +1945,8 @@ template <typename T, int LPA, int U, bool ONE, bool STAGED>
 static void launch_sc_density_one(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     const ScLaunch L = sc_launch<LPA>(b);
-    if (hf.min_image && b.pe1) k_sc_density<T, true, LPA, U, ONE, STAGED, true><<<L.grid, L.block, L.lds, s>>>(b);
+    const int gwl = STAGED ? b.wl_grid_staged : b.wl_grid_plain;
+    if (hf.min_image && b.pe1) k_sc_density<T, true, LPA, U, ONE, STAGED, true><<<gwl, L.block, L.lds, s>>>(b);
     else if (hf.min_image) k_sc_density<T, true, LPA, U, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
     else k_sc_density<T, false, LPA, U, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
 }
@@ -1836,13 +1967,16 @@ template <typename T, int LPA, int U, bool ONE, bool STAGED>
 static void launch_sc_force_one(const BatchDev &b, const FieldDev &hf, int want_stress, hipStream_t s)
 {
     const ScLaunch L = sc_launch<LPA>(b);
-    // the stress variant exists for U = 1 only: U does not change results
+    const int gwl = STAGED ? b.wl_grid_staged : b.wl_grid_plain;
+    // the stress variant exists for U = 1 only: U does not change results;
+    // search micro-steps (minimum image, no stress) run the work list
     if (hf.min_image) {
-        if (want_stress) k_sc_force<T, true, true, LPA, 1, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
-        else k_sc_force<T, true, false, LPA, U, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+        if (want_stress) k_sc_force<T, true, true, LPA, 1, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
+        else if (b.pe1) k_sc_force<T, true, false, LPA, U, ONE, STAGED, true><<<gwl, L.block, L.lds, s>>>(b);
+        else k_sc_force<T, true, false, LPA, U, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
     } else {
-        if (want_stress) k_sc_force<T, false, true, LPA, 1, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
-        else k_sc_force<T, false, false, LPA, U, ONE, STAGED><<<L.grid, L.block, L.lds, s>>>(b);
+        if (want_stress) k_sc_force<T, false, true, LPA, 1, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
+        else k_sc_force<T, false, false, LPA, U, ONE, STAGED, false><<<L.grid, L.block, L.lds, s>>>(b);
     }
 }
 
@@ -2005,8 +2139,8 @@ static BatchDev fp64_view(const BatchDev &b, int prec)
     BatchDev v = b;
     if (prec & 5) {
         v.staged = 0;
-        v.lpa = 4;  // measured best for this pass on the batched dimer workload
-        v.unroll = 4;
+        v.lpa = b.lpa64 ? b.lpa64 : 4;  // 4 x 4: measured best on the batched dimer workload
+        v.unroll = b.lpa64 ? b.unroll64 : 4;
         v.lpa2 = 0;  // both passes
     }
     return v;
@@ -2018,12 +2152,33 @@ static BatchDev fp64_view(const BatchDev &b, int prec)
 // the walkers whose active mode matches.  Bit 2 marks an FP64-only launch
 // that belongs to a mixed micro-step (the FP32 kernels launched apart), so
 // the FP64 kernels take their mixed-step variant.
+// chunk size (slots per workgroup) of a Sutton-Chen pair launch
+static int chunk_atoms(const BatchDev &v) { return (v.staged ? AKMC_STAGE_BLOCK : kBlock) / v.lpa; }
+
+// the search micro-step's work lists, for the variants the pair launches
+// of this precision mask will use
+static void launch_work_list(const BatchDev &b, int prec, hipStream_t s)
+{
+    int4 apc = make_int4(0, 0, 0, 0);
+    if (prec & 1) {
+        apc.x = chunk_atoms(b);
+        apc.y = chunk_atoms(b.lpa2 ? pass2_view(b) : b);
+    }
+    if (prec & 2) {
+        const BatchDev v = fp64_view(b, prec);
+        apc.z = chunk_atoms(v);
+        apc.w = chunk_atoms(v.lpa2 ? pass2_view(v) : v);
+    }
+    k_work_list<<<4, 1024, 0, s>>>(b, apc);
+}
+
 void akmc_launch_phase(const BatchDev &b, const FieldDev &hf, int prec, int want_stress, int phase,
                        hipStream_t s)
 {
     switch (phase) {
     case 0:
         launch_lists(b, hf, s);
+        if (b.pe1 && hf.kind == 0 && !hf.eam && hf.min_image) launch_work_list(b, prec, s);
         break;
     case 1:
         if (prec & 1) launch_pass1<float>(b, hf, want_stress, s);

@@ -752,17 +752,34 @@ __device__ __attribute__((always_inline)) void verify_logic(const Ctx &C_, Walke
             V3(C_.s.T, c, i) = r;
         }
     }
-    for (int j = 0; j <= k; ++j) {
-        const double *Qj = C_.s.Q + j * A3;
-        double dj[1] = {0.0};
+    // full re-orthogonalisation against q_0 .. q_k: classical Gram-Schmidt
+    // applied twice, every projection of a pass from one block reduction
+    // (instead of k + 1 reductions in a row)
+    for (int pass = 0; pass < 2; ++pass) {
+        double dj[AKMC_LZ_MAX];
+#pragma unroll
+        for (int j = 0; j < AKMC_LZ_MAX; ++j) dj[j] = 0.0;
         FOR_ACTIVE(i)
         {
-            for (int c = 0; c < 3; ++c) dj[0] += V3(C_.s.T, c, i) * V3(Qj, c, i);
+            const double t0 = V3(C_.s.T, 0, i), t1 = V3(C_.s.T, 1, i), t2 = V3(C_.s.T, 2, i);
+#pragma unroll
+            for (int j = 0; j < AKMC_LZ_MAX; ++j) {
+                if (j > k) break;
+                const double *Qj = C_.s.Q + j * A3;
+                dj[j] += t0 * V3(Qj, 0, i) + t1 * V3(Qj, 1, i) + t2 * V3(Qj, 2, i);
+            }
         }
-        bsum<1>(dj, sh);
+        bsum<AKMC_LZ_MAX>(dj, sh);
         FOR_ACTIVE(i)
         {
-            for (int c = 0; c < 3; ++c) V3(C_.s.T, c, i) -= dj[0] * V3(Qj, c, i);
+            double r[3] = {V3(C_.s.T, 0, i), V3(C_.s.T, 1, i), V3(C_.s.T, 2, i)};
+#pragma unroll
+            for (int j = 0; j < AKMC_LZ_MAX; ++j) {
+                if (j > k) break;
+                const double *Qj = C_.s.Q + j * A3;
+                for (int c = 0; c < 3; ++c) r[c] -= dj[j] * V3(Qj, c, i);
+            }
+            for (int c = 0; c < 3; ++c) V3(C_.s.T, c, i) = r[c];
         }
     }
     double bb[1] = {0.0};

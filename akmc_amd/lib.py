@@ -192,7 +192,7 @@ def load() -> C.CDLL:
     L.akmc_batch_get_zone.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 4
     L.akmc_batch_set_search_variant.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.akmc_batch_autotune_search.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
-                                             C.POINTER(C.c_double)]
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.akmc_batch_get_walkers_positions.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
     L.akmc_batch_kernel_timer.argtypes = [C.c_void_p, C.c_int]
     L.akmc_batch_kernel_time.argtypes = [C.c_void_p, _D, C.POINTER(C.c_longlong)]
@@ -535,17 +535,23 @@ class Batch:
         return ms / nrep
 
     def set_search_variant(self, v=None):
-        """(lanes, unroll, staged) of pass 1 then pass 2 for the search micro-steps; None resets"""
-        a = None if v is None else np.ascontiguousarray(v, dtype=np.int32)
+        """(lanes, unroll, staged) of pass 1, of pass 2, then (lanes, unroll) of
+        the mixed steps' FP64 passes for the search micro-steps; None resets"""
+        a = None
+        if v is not None:
+            a = np.zeros(8, np.int32)
+            a[:len(v)] = v
         check(load().akmc_batch_set_search_variant(self._h, _ip(a)))
 
     def autotune_search(self, nrep=5):
-        """fastest variant per pass on the running searches: (best[6], {variant: (ms1, ms2)})"""
-        best = np.zeros(6, np.int32)
+        """fastest variant per part on the running searches: (best[8],
+        {variant: (ms pass 1, ms pass 2, ms FP64 passes or None)}), ms per nrep evaluations"""
+        best = np.zeros(8, np.int32)
         n = 32
-        m1, m2 = np.zeros(n), np.zeros(n)
-        check(load().akmc_batch_autotune_search(self._h, int(nrep), _ip(best), _dp(m1), _dp(m2)))
-        table = {(l, u, st): (m1[16 * st + 4 * li + ui], m2[16 * st + 4 * li + ui]) for st in (0, 1)
+        m1, m2, m64 = np.zeros(n), np.zeros(n), np.zeros(n // 2)
+        check(load().akmc_batch_autotune_search(self._h, int(nrep), _ip(best), _dp(m1), _dp(m2), _dp(m64)))
+        table = {(l, u, st): (m1[16 * st + 4 * li + ui], m2[16 * st + 4 * li + ui],
+                              m64[4 * li + ui] if st == 0 else None) for st in (0, 1)
                  for li, l in enumerate((1, 2, 4, 8)) for ui, u in enumerate((1, 2, 4, 8))
                  if m1[16 * st + 4 * li + ui] >= 0}
         return tuple(int(x) for x in best), table

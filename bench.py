@@ -658,11 +658,14 @@ def main():
             # searches under way (zoned passes): untimed, between windows
             t0 = time.perf_counter()
             sv, sv_ms = camp.batch.autotune_search(5)
-            search_variant = {"pass1": sv[:3], "pass2": sv[3:], "seconds": round(time.perf_counter() - t0, 2),
-                              "ms_pass1_pass2": {f"{k[0]}x{k[1]}{' lds' if k[2] else ''}":
-                                                 [round(v[0], 4), round(v[1], 4)] for k, v in sv_ms.items()}}
-            print(f"[bench] search variants (lanes, unroll, staged): pass 1 {sv[:3]} pass 2 {sv[3:]}; "
-                  f"{search_variant['ms_pass1_pass2']}", file=sys.stderr, flush=True)
+            search_variant = {"pass1": sv[:3], "pass2": sv[3:6], "fp64": sv[6:],
+                              "seconds": round(time.perf_counter() - t0, 2),
+                              "ms_per_5_evals_pass1_pass2_fp64":
+                                  {f"{k[0]}x{k[1]}{' lds' if k[2] else ''}":
+                                   [round(x, 4) if x is not None else None for x in v] for k, v in sv_ms.items()}}
+            print(f"[bench] search variants (lanes, unroll, staged): pass 1 {sv[:3]} pass 2 {sv[3:6]} "
+                  f"FP64 {sv[6:]}; {search_variant['ms_per_5_evals_pass1_pass2_fp64']}", file=sys.stderr,
+                  flush=True)
         rb0 = camp.batch.rebuilds
         st = camp.advance(args.window, sync_every=args.sync_every)
         print(f"[bench] window {r}: {camp.batch.rebuilds - rb0} walker list rebuilds; "

@@ -318,17 +318,20 @@ int akmc_batch_get_zone(akmc_batch_t b, int *member, int *size, int *flags, int 
 /* Kernel variants of the search micro-steps' Sutton-Chen passes (no
  * counterpart in the reference): v = {lanes, unroll, staged} of pass 1
  * (density; it also writes the pair energies in search micro-steps), then
- * of pass 2 (forces); lanes 0 keeps the batch's variant for that pass, NULL
- * resets both.  Results do not depend on unroll or staging; the lanes of
- * pass 2 change only the summation order of the forces. */
-int akmc_batch_set_search_variant(akmc_batch_t b, const int v[6]);
+ * of pass 2 (forces), then {lanes, unroll} of both FP64 passes of steps
+ * that mix FP32 and FP64 walkers (unstaged); lanes 0 keeps the default for
+ * that part, NULL resets all.  Results do not depend on unroll or staging;
+ * lanes change only the summation order of the sums they split. */
+int akmc_batch_set_search_variant(akmc_batch_t b, const int v[8]);
 /* Times every variant of each pass on the search walkers at their current
- * probes (akmc_batch_time_search_phases, nrep evaluations each), keeps the
- * fastest per pass and returns it in best[6] (may be NULL) with the per-
- * variant ms of pass 1 and pass 2 (indexed like akmc_batch_autotune; NULL
- * allowed).  Call it with searches under way. */
-int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[6], double ms1[AKMC_NVARIANTS],
-                               double ms2[AKMC_NVARIANTS]);
+ * probes (akmc_batch_time_search_phases, nrep evaluations each; the FP64
+ * variants on the walkers under FP64 verification), keeps the fastest per
+ * part and returns it in best[8] (may be NULL) with the per-variant ms of
+ * pass 1, pass 2 (indexed like akmc_batch_autotune) and the FP64 passes
+ * (unstaged variants only; any of them NULL).  Call it with searches under
+ * way. */
+int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[8], double ms1[AKMC_NVARIANTS],
+                               double ms2[AKMC_NVARIANTS], double ms64[AKMC_NVARIANTS / 2]);
 /* Kernel timer of akmc_batch_run: HIP events on the batch's stream around
  * the FP32 Sutton-Chen pass-2 launch (k_sc_force) of every micro-step.
  * Events inside a replayed graph cannot be timed, so while the timer runs
