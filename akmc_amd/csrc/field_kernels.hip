@@ -291,7 +291,17 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
     if (b.zflag && (b.zflag[w] & AKMC_ZONE_RESET))
         for (int l = tid; l < W.n; l += kBlock) b.zmark[W.off + l] = 0;
     __syncthreads();
-    for (int l = tid; l < W.n; l += kBlock) atomicAdd(&fill[atom_cell_of(b, W, W.off + l)], 1);
+    // the counts, and the positions the walker's lists are built from
+    for (int l = tid; l < W.n; l += kBlock) {
+        const int i = W.off + l;
+        const double x = b.x[i], y = b.y[i], z = b.z[i];
+        b.xr[i] = x;
+        b.yr[i] = y;
+        b.zr[i] = z;
+        const int cell = (cell_coord(x, W.invL[0], W.nc[0]) * W.nc[1] + cell_coord(y, W.invL[1], W.nc[1])) * W.nc[2] +
+                         cell_coord(z, W.invL[2], W.nc[2]);
+        atomicAdd(&fill[cell], 1);
+    }
     if (tid == 0) carry = W.off;
     __syncthreads();
     // exclusive scan of the counts; fill[c] becomes the cell's next free slot
@@ -436,7 +446,7 @@ constexpr int kNbLanes = 16;
 constexpr int kNbSlots = kBlock / kNbLanes;  // home atoms per work item
 static_assert(kNbSlots == kNbSlotsQ, "queued items and the list build agree on the chunk size");
 constexpr int kNbIlp = 4;                    // candidates per lane per round
-constexpr int kNbCap = 2048;                 // candidates staged in LDS at a time
+constexpr int kNbCap = 1024;                 // candidates staged in LDS at a time (28 KB)
 
 __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
 {
@@ -562,7 +572,7 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
 // buffers is claimed only by workgroups with work.  An item is encoded as
 // (cell << 8) | chunk; chunks beyond 255 of very full cells are taken by the
 // same item in strides of 256.
-constexpr int kNbGrid = 512;
+constexpr int kNbGrid = 1024;
 
 __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
 {
@@ -663,12 +673,20 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_images(BatchDev b)
 // atom on a free atom's list (on top of the marks since the walker's start,
 // cleared by k_cell_bin at a restart); the members are listed in ascending
 // index order, each thread a contiguous run of the walker's atoms.
-__global__ void __launch_bounds__(kBlock) k_zone_build(BatchDev b)
+__global__ void __launch_bounds__(kBlock) k_zone_build(BatchDev b, unsigned long long *count)
 {
     __shared__ int part[kBlock];
     const int w = blockIdx.x;
+    if (!b.rebuild[w]) return;
     const int f = b.zflag[w];
-    if (!(f & AKMC_ZONE_ON) || !b.rebuild[w]) return;
+    // the walker's rebuild is done (k_rebuild_done's bookkeeping)
+    if (!(f & AKMC_ZONE_ON)) {
+        if (threadIdx.x == 0) {
+            b.rebuild[w] = 0;
+            atomicAdd(count, 1ull);
+        }
+        return;
+    }
     const WalkerDev W = b.walkers[w];
     const int per = (W.n + kBlock - 1) / kBlock;
     const int l0 = (int)threadIdx.x * per, l1 = min(W.n, l0 + per);
@@ -688,6 +706,8 @@ __global__ void __launch_bounds__(kBlock) k_zone_build(BatchDev b)
     if (threadIdx.x == kBlock - 1) {
         b.nz[w] = part[kBlock - 1];
         b.zflag[w] = f & ~AKMC_ZONE_RESET;
+        b.rebuild[w] = 0;
+        atomicAdd(count, 1ull);
     }
 }
 
@@ -1908,15 +1928,16 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     int gA = blocks(b.A);
     k_pack_check<<<gA, kBlock, 0, s>>>(b);
-    k_ref_positions<<<gA, kBlock, 0, s>>>(b);
     if (hf.min_image) {
-        k_cell_bin<<<b.W, kBlock, 0, s>>>(b);
+        k_cell_bin<<<b.W, kBlock, 0, s>>>(b);  // records the reference positions too
         k_nbr_build_cells<<<kNbGrid, kBlock, 0, s>>>(b);
     } else {
+        k_ref_positions<<<gA, kBlock, 0, s>>>(b);
         k_nbr_build_images<<<(b.A + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(b);
     }
-    if (b.zflag) k_zone_build<<<b.W, kBlock, 0, s>>>(b);
-    k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, b.rebuild_count);
+    // the zones of search micro-steps, with the rebuild bookkeeping
+    if (b.zflag) k_zone_build<<<b.W, kBlock, 0, s>>>(b, b.rebuild_count);
+    else k_rebuild_done<<<blocks(b.W), kBlock, 0, s>>>(b, b.rebuild_count);
 }
 
 // a single species pair with the exponents n = 9, m = 6 (the ONE kernels;

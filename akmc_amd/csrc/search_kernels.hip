@@ -673,6 +673,53 @@ __device__ __attribute__((always_inline)) double tridiag_kth(int n, const double
     return 0.5 * (lo + hi);
 }
 
+// Full re-orthogonalisation of the Lanczos residual T against q_0 .. q_k
+// with the vectors in LDS: the free atoms' components of q_0 .. q_k and T
+// are loaded once (all loads in flight together), then classical
+// Gram-Schmidt runs twice, each wavefront taking every fourth projection
+// (a wavefront sum, no block barrier per projection), and T goes back to
+// global memory.  Element e of a vector is component e / nact of free atom
+// e % nact.  Modified Gram-Schmidt in global memory, with k + 1 block
+// reductions in a row, stays for vectors too large for the LDS budget.
+constexpr int kLzLds = 8192;  // doubles of LDS for q_0 .. q_k and T
+__device__ __attribute__((always_inline)) void reorthogonalise_lds(const Ctx &C_, int k)
+{
+    __shared__ double lq[kLzLds];
+    __shared__ double dj[AKMC_LZ_MAX];
+    const int na = C_.nact, ne = 3 * na;
+    const size_t A3 = 3 * (size_t)C_.A;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int idx = threadIdx.x; idx < (k + 2) * ne; idx += kBlock) {
+        const int j = idx / ne, e = idx - j * ne;
+        const int c = e / na, t = e - c * na;
+        const int a = C_.s.act[C_.off + t];
+        lq[idx] = j <= k ? V3(C_.s.Q + j * A3, c, a) : V3(C_.s.T, c, a);
+    }
+    double *T = lq + (k + 1) * ne;
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int j = wave; j <= k; j += kWaves) {
+            const double *q = lq + j * ne;
+            double d = 0.0;
+            for (int e = lane; e < ne; e += 64) d += T[e] * q[e];
+            for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+            if (lane == 0) dj[j] = d;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < ne; e += kBlock) {
+            double r = T[e];
+            for (int j = 0; j <= k; ++j) r -= dj[j] * lq[j * ne + e];
+            T[e] = r;
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < ne; e += kBlock) {
+        const int c = e / na, t = e - c * na;
+        V3(C_.s.T, c, C_.s.act[C_.off + t]) = T[e];
+    }
+    __syncthreads();
+}
+
 // FP64 check of a converged saddle: max force on the free atoms, energy,
 // and the two lowest Ritz values of the finite-difference Hessian from a
 // Lanczos run started on the dimer mode (full re-orthogonalisation).  The
@@ -752,34 +799,23 @@ __device__ __attribute__((always_inline)) void verify_logic(const Ctx &C_, Walke
             V3(C_.s.T, c, i) = r;
         }
     }
-    // full re-orthogonalisation against q_0 .. q_k: classical Gram-Schmidt
-    // applied twice, every projection of a pass from one block reduction
-    // (instead of k + 1 reductions in a row)
-    for (int pass = 0; pass < 2; ++pass) {
-        double dj[AKMC_LZ_MAX];
-#pragma unroll
-        for (int j = 0; j < AKMC_LZ_MAX; ++j) dj[j] = 0.0;
-        FOR_ACTIVE(i)
-        {
-            const double t0 = V3(C_.s.T, 0, i), t1 = V3(C_.s.T, 1, i), t2 = V3(C_.s.T, 2, i);
-#pragma unroll
-            for (int j = 0; j < AKMC_LZ_MAX; ++j) {
-                if (j > k) break;
-                const double *Qj = C_.s.Q + j * A3;
-                dj[j] += t0 * V3(Qj, 0, i) + t1 * V3(Qj, 1, i) + t2 * V3(Qj, 2, i);
+    // full re-orthogonalisation against q_0 .. q_k
+    if (3 * C_.nact * (k + 2) <= kLzLds) {
+        __syncthreads();  // the update above is in T
+        reorthogonalise_lds(C_, k);
+    } else {
+        for (int j = 0; j <= k; ++j) {
+            const double *Qj = C_.s.Q + j * A3;
+            double dj[1] = {0.0};
+            FOR_ACTIVE(i)
+            {
+                for (int c = 0; c < 3; ++c) dj[0] += V3(C_.s.T, c, i) * V3(Qj, c, i);
             }
-        }
-        bsum<AKMC_LZ_MAX>(dj, sh);
-        FOR_ACTIVE(i)
-        {
-            double r[3] = {V3(C_.s.T, 0, i), V3(C_.s.T, 1, i), V3(C_.s.T, 2, i)};
-#pragma unroll
-            for (int j = 0; j < AKMC_LZ_MAX; ++j) {
-                if (j > k) break;
-                const double *Qj = C_.s.Q + j * A3;
-                for (int c = 0; c < 3; ++c) r[c] -= dj[j] * V3(Qj, c, i);
+            bsum<1>(dj, sh);
+            FOR_ACTIVE(i)
+            {
+                for (int c = 0; c < 3; ++c) V3(C_.s.T, c, i) -= dj[0] * V3(Qj, c, i);
             }
-            for (int c = 0; c < 3; ++c) V3(C_.s.T, c, i) = r[c];
         }
     }
     double bb[1] = {0.0};
@@ -795,22 +831,20 @@ __device__ __attribute__((always_inline)) void verify_logic(const Ctx &C_, Walke
     }
     w.lk = k + 1;
     if (w.lk >= nl || beta < 1.0e-12) {
-        __shared__ double ev[2];
-        if (threadIdx.x < 64) {  // one wavefront: the two lowest Ritz values
-            double d[AKMC_LZ_MAX], e[AKMC_LZ_MAX];
-            const int n = w.lk;
-            for (int t = 0; t < k; ++t) {
-                d[t] = lz[t];
-                e[t] = lz[AKMC_LZ_MAX + t];
-            }
-            d[k] = alpha;
-            e[k] = 0.0;
-            const double l1 = tridiag_kth(n, d, e, 0);
-            const double l2 = n > 1 ? tridiag_kth(n, d, e, 1) : 1.0e300;
-            if (threadIdx.x == 0) {
-                ev[0] = l1;
-                ev[1] = l2;
-            }
+        // the two lowest Ritz values, one wavefront each, the tridiagonal in
+        // LDS (every lane reads the same entries)
+        __shared__ double ev[2], td[AKMC_LZ_MAX], te[AKMC_LZ_MAX];
+        const int n = w.lk;
+        if (threadIdx.x < (unsigned)n) {
+            const int t = threadIdx.x;
+            td[t] = t < k ? lz[t] : alpha;
+            te[t] = t < k ? lz[AKMC_LZ_MAX + t] : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x < 128) {
+            const int which = threadIdx.x >> 6;
+            const double l = which == 0 || n > 1 ? tridiag_kth(n, td, te, which) : 1.0e300;
+            if ((threadIdx.x & 63) == 0) ev[which] = l;
         }
         __syncthreads();
         w.lam1 = ev[0];
@@ -978,17 +1012,17 @@ __device__ __attribute__((always_inline)) void art_lanczos_step(const Ctx &C_, W
         lz[i] = alpha;
         if (i > 0) lz[AKMC_LZ_MAX + i - 1] = beta;
     }
+    __shared__ double td[AKMC_LZ_MAX], te[AKMC_LZ_MAX];  // the tridiagonal, in LDS
+    if (i > 0 && threadIdx.x <= (unsigned)i) {
+        const int t = threadIdx.x;
+        td[t] = t < i ? lz[t] : alpha;
+        te[t] = t < i - 1 ? lz[AKMC_LZ_MAX + t] : (t == i - 1 ? beta : 0.0);
+    }
+    __syncthreads();
     if (i == 0) {
         if (threadIdx.x == 0) evs[0] = alpha;
     } else if (threadIdx.x < 64) {  // one wavefront: the lowest Ritz value
-        double d[AKMC_LZ_MAX], e[AKMC_LZ_MAX];
-        for (int t = 0; t < i; ++t) {
-            d[t] = lz[t];
-            e[t] = t < i - 1 ? lz[AKMC_LZ_MAX + t] : beta;
-        }
-        d[i] = alpha;
-        e[i] = 0.0;
-        const double l1 = tridiag_kth(i + 1, d, e, 0);
+        const double l1 = tridiag_kth(i + 1, td, te, 0);
         if (threadIdx.x == 0) evs[0] = l1;
     }
     __syncthreads();

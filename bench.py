@@ -248,6 +248,31 @@ def forces_only(L, W, args):
           flush=True)
 
 
+def zone_passes(L, S, W, args):
+    """The zoned pass kernels of the search micro-steps alone, for hardware
+    counter runs: the batched campaign (3 pooled basins per walker) after
+    a 250-micro-step warm window and the per-pass variant tuning, then
+    args.zone_passes evaluations of the running searches at their probes."""
+    cu = W.SC_CU
+    rc = 2 * cu["a"]
+    text = L.sc_potentials_text([("Cu", "Cu", cu["eps"], cu["a"], cu["n"], cu["m"], cu["c"])], rc)
+    cells = W.batched_sc_cells(n_walkers=args.walkers, seed=args.seed)
+    camp = S.SaddleCampaign(text, cells, params=L.dimer_defaults(maxParIter=400), seed=args.seed)
+    camp.batch.forces()
+    camp.batch.autotune(5)
+    camp.relax_basins()
+    camp.relax_pool(3, seed=args.seed + 17)
+    camp.searches_per_basin = args.searches_per_basin
+    camp.begin()
+    camp.advance(250)
+    sv, _ = camp.batch.autotune_search(3)
+    camp.batch.time_search_phases(1)
+    zw = zone_work(camp, cells, rc)
+    ph = camp.batch.time_search_phases(args.zone_passes)
+    print(json.dumps({"zone_passes": args.zone_passes, "search_variant": sv, "zones": zw,
+                      "phase_ms": [round(float(x), 4) for x in ph]}), flush=True)
+
+
 def drain(camp, stats):
     """Let every running search end (no refill)."""
     while camp.pending:
@@ -585,6 +610,8 @@ def main():
                          "(for kernel tracers that cannot follow graph launches)")
     ap.add_argument("--forces-only", type=int, default=0, metavar="N",
                     help="only N force evaluations of all walkers (for hardware-counter runs)")
+    ap.add_argument("--zone-passes", type=int, default=0, metavar="N",
+                    help="only N evaluations of running searches as the micro-steps run them (counter runs)")
     ap.add_argument("--copy-mib", type=int, default=0, metavar="MIB",
                     help="only a plain device copy of MIB MiB, 5 times (counter calibration)")
     args = ap.parse_args()
@@ -602,6 +629,9 @@ def main():
     L.load()
     if args.forces_only:
         forces_only(L, W, args)
+        return
+    if args.zone_passes:
+        zone_passes(L, S, W, args)
         return
     if args.c2_only:
         print(json.dumps({"c2": c2_section(L, W, args)}), flush=True)
