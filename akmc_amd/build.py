@@ -15,8 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libakmc_hip.so")
-SOURCES = ["field_kernels.hip", "search_kernels.hip", "mlp_kernels.hip", "capi.cpp", "kmc_host.cpp", "io_host.cpp",
-           "noalloc.hip"]
+SOURCES = ["field_kernels.hip", "search_kernels.hip", "search_kernels_wide.hip", "mlp_kernels.hip", "capi.cpp",
+           "kmc_host.cpp", "io_host.cpp", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions are disabled for every HIP source of this library
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-x", "hip",

@@ -324,6 +324,8 @@ struct SearchDev {
 };
 
 void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st);
+// the same state machines in 1024-thread workgroups (large walkers)
+void akmc_launch_search_logic_wide(const BatchDev &b, const SearchDev &s, hipStream_t st);
 // one walker start packed for akmc_launch_load_walkers: doubles at dbl
 // (mode [3][n], then positions [3][n] when has_x), ints at ints (frozen [n],
 // then the free-atom list [nact])

@@ -154,6 +154,8 @@ struct akmc_batch_s {
     // active zones of the search micro-steps (BatchDev.zflag ...): on by
     // default for minimum-image Sutton-Chen fields
     bool zone_on = true;
+    // the batch's largest walker has at least kWideLogicAtoms atoms
+    bool logic_wide = false;
     int *zflag = nullptr, *zmark = nullptr, *zlist = nullptr, *nz = nullptr;
     std::vector<int> hzflag;
     // kernel variants of the search micro-steps' Sutton-Chen passes:
@@ -686,6 +688,10 @@ int batch_build(akmc_field_t f, int W, const int *natoms, const double *lat, akm
         int nmax = 0;
         for (int w = 0; w < W; ++w) nmax = std::max(nmax, natoms[w]);
         d.nb16 = (f->kind == 0 && f->min_image && nmax <= 65535) ? 1 : 0;
+        // above this size a 256-thread workgroup loops 64 times over a
+        // walker's atoms per vector operation of its state machine
+        constexpr int kWideLogicAtoms = 16384;
+        b->logic_wide = nmax >= kWideLogicAtoms;
     }
     d.ncells_total = ncells;
     d.nk_total = nk;
@@ -1110,7 +1116,9 @@ void enqueue_micro_step(akmc_batch_t b, int j)
     } else {
         akmc_launch_forces_impl(d, b->f->host, prec, 0, b->stream);
     }
-    akmc_launch_search_logic(d, b->s, b->stream);
+    // walkers of tens of thousands of atoms: 1024-thread state machines
+    if (b->logic_wide) akmc_launch_search_logic_wide(d, b->s, b->stream);
+    else akmc_launch_search_logic(d, b->s, b->stream);
 }
 
 int fetch_states(akmc_batch_t b)

@@ -864,13 +864,22 @@ __device__ inline float isr32_of(double w) { return __uint_as_float(uint32_t(__d
 // Work lists of the search micro-steps (k_work_list): item c of list li is
 // (walker, first slot) of a chunk of the walker's pass list -- its zone
 // (pass 1) or free atoms (pass 2) when zoned, else all its atoms.  The pair
-// kernels of a search micro-step run a persistent grid over the list; the
-// XCD-ordered first item keeps consecutive items (one walker's chunks) on
-// one XCD.  The grid is a multiple of kXCD.
+// kernels of a search micro-step run a persistent grid over the list, each
+// workgroup a contiguous range of it.  The grid is a multiple of kXCD.
 __device__ inline int wl_first()
 {
     const int per = gridDim.x / kXCD;
     return (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+}
+// the workgroup's contiguous range [x, y) of a list of cnt items: equal
+// shares in XCD order, so one walker's consecutive chunks mostly fall to one
+// workgroup (one staging) and neighbouring ranges to one XCD
+__device__ inline int2 wl_range(int cnt)
+{
+    const int L = wl_first(), G = gridDim.x;
+    const int per = cnt / G, extra = cnt % G;
+    const int x = L * per + min(L, extra);
+    return make_int2(x, x + per + (L < extra ? 1 : 0));
 }
 template <typename T>
 __device__ inline int wl_len(const BatchDev &b, const WalkerDev &W, int w, int pass)
@@ -1049,14 +1058,19 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_densi
     if constexpr (ZONE) {
         // search micro-steps: a persistent grid over the step's work list
         const int li = sizeof(T) == 8 ? 2 : 0;
-        for (int c = wl_first(); c < b.wl_count[li]; c += gridDim.x) {
+        const int2 rg = wl_range(b.wl_count[li]);
+        int staged_w = -1;
+        for (int c = rg.x; c < rg.y; ++c) {
             const int2 it = b.wl_tab[li][c];
             W = b.walkers[it.x];
             const int n = wl_len<T>(b, W, it.x, 1);
             if constexpr (STAGED) {
-                __syncthreads();  // the previous item is done with the stage
-                for (int k = threadIdx.x; k < W.n; k += blockDim.x) stage[k] = b.p4[W.off + k];
-                __syncthreads();
+                if (it.x != staged_w) {  // a walker's chunks are consecutive: stage it once
+                    __syncthreads();     // the previous items are done with the stage
+                    for (int k = threadIdx.x; k < W.n; k += blockDim.x) stage[k] = b.p4[W.off + k];
+                    __syncthreads();
+                    staged_w = it.x;
+                }
             }
             const int la = it.y + (int)threadIdx.x / LPA;
             if (la < n)
@@ -1245,14 +1259,19 @@ __global__ void __launch_bounds__(STAGED ? AKMC_STAGE_BLOCK : kBlock) k_sc_force
     WalkerDev W;
     if constexpr (WL) {
         const int li = sizeof(T) == 8 ? 3 : 1;
-        for (int c = wl_first(); c < b.wl_count[li]; c += gridDim.x) {
+        const int2 rg = wl_range(b.wl_count[li]);
+        int staged_w = -1;
+        for (int c = rg.x; c < rg.y; ++c) {
             const int2 it = b.wl_tab[li][c];
             W = b.walkers[it.x];
             const int n = wl_len<T>(b, W, it.x, 2);
             if constexpr (STAGED) {
-                __syncthreads();  // the previous item is done with the stage
-                for (int k = threadIdx.x; k < W.n; k += blockDim.x) stage[k] = b.p4[W.off + k];
-                __syncthreads();
+                if (it.x != staged_w) {  // a walker's chunks are consecutive: stage it once
+                    __syncthreads();     // the previous items are done with the stage
+                    for (int k = threadIdx.x; k < W.n; k += blockDim.x) stage[k] = b.p4[W.off + k];
+                    __syncthreads();
+                    staged_w = it.x;
+                }
             }
             const int la = it.y + (int)threadIdx.x / LPA;
             if (la < n)

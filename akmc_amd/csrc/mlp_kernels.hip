@@ -31,13 +31,16 @@ __device__ inline float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
 enum { EPI_SILU = 0, EPI_LINEAR = 1, EPI_DSILU = 2 };
 
 // C[m][n] = sum_k A[m*lda + k] * Bt[n*ldb + k] (A FP32, Bt BF16 bits);
-// K is a multiple of 16; n < N are stored.  Epilogues:
+// K = 16 KS; n < N are stored; A rows 16-byte aligned.  Epilogues:
 //   EPI_SILU   Z = acc + bias, stored; Y = silu(Z)
 //   EPI_LINEAR Y = acc + bias
 //   EPI_DSILU  Y = acc * silu'(aux[m*ldc + n])
-template <int EPI>
+// KS = K / 16 k-steps: every operand of the tile is loaded first (one
+// 16-byte A load and one 8-byte B load per lane and k-step, all in flight
+// together), then the KS MFMAs run back to back in k order.
+template <int EPI, int KS>
 __global__ void __launch_bounds__(kB) k_gemm(const float *__restrict__ A, int lda, const uint16_t *__restrict__ Bt,
-                                             int ldb, int M, int N, int K, const float *__restrict__ bias,
+                                             int ldb, int M, int N, const float *__restrict__ bias,
                                              const float *__restrict__ aux, float *__restrict__ Z,
                                              float *__restrict__ Y, int ldc)
 {
@@ -45,15 +48,24 @@ __global__ void __launch_bounds__(kB) k_gemm(const float *__restrict__ A, int ld
     const int r0 = blockIdx.x * 64 + wave * 16, c0 = blockIdx.y * 16;
     const int li = lane & 15, lk = (lane >> 4) * 4;
     const int arow = r0 + li, bcol = c0 + li;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x4 a, b;
-        for (int e = 0; e < 4; ++e) {
-            float av = arow < M ? A[(size_t)arow * lda + k0 + lk + e] : 0.0f;
-            a[e] = (short)to_bf16(av);
-            b[e] = bcol < N ? (short)Bt[(size_t)bcol * ldb + k0 + lk + e] : (short)0;
+    float4 av[KS];
+    bf16x4 bv[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        av[s] = arow < M ? *reinterpret_cast<const float4 *>(A + (size_t)arow * lda + 16 * s + lk)
+                         : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (bcol < N) {
+            bv[s] = *reinterpret_cast<const bf16x4 *>(Bt + (size_t)bcol * ldb + 16 * s + lk);
+        } else {
+            bv[s] = bf16x4{0, 0, 0, 0};
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const bf16x4 a = {(short)to_bf16(av[s].x), (short)to_bf16(av[s].y), (short)to_bf16(av[s].z),
+                          (short)to_bf16(av[s].w)};
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[s], acc, 0, 0, 0);
     }
     const int col = c0 + li;
     if (col >= N) return;
@@ -88,16 +100,28 @@ __global__ void __launch_bounds__(kB) k_wgrad(const float *__restrict__ A, int l
     const int m_end = min(M, m_begin + rows_per_chunk);
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     const int ai = i0 + li, bj = j0 + li;
-    for (int m0 = m_begin; m0 < m_end; m0 += 16) {
-        bf16x4 a, b;
-        for (int e = 0; e < 4; ++e) {
-            int m = m0 + lk + e;
-            float av = (m < m_end && ai < Kd) ? A[(size_t)m * lda + ai] : 0.0f;
-            float bv = (m < m_end && bj < Nd) ? G[(size_t)m * ldg + bj] : 0.0f;
-            a[e] = (short)to_bf16(av);
-            b[e] = (short)to_bf16(bv);
+    // four k-steps (64 rows) of operands loaded before their MFMAs
+    constexpr int S = 4;
+    for (int m00 = m_begin; m00 < m_end; m00 += 16 * S) {
+        float av[S][4], gv[S][4];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int m = m00 + 16 * s + lk + e;
+                av[s][e] = (m < m_end && ai < Kd) ? A[(size_t)m * lda + ai] : 0.0f;
+                gv[s][e] = (m < m_end && bj < Nd) ? G[(size_t)m * ldg + bj] : 0.0f;
+            }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (m00 + 16 * s >= m_end) break;
+            bf16x4 a, b;
+            for (int e = 0; e < 4; ++e) {
+                a[e] = (short)to_bf16(av[s][e]);
+                b[e] = (short)to_bf16(gv[s][e]);
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
     const int col = j0 + li;
     if (col >= Nd) return;
@@ -316,10 +340,10 @@ void akmc_mlp_launch_forward(const float *X, int M, const float *p, const uint16
 {
     const float *b1 = p + 48 * 128, *b2 = b1 + 128 + 128 * 128, *b3 = b2 + 128 + 128 * 3;
     dim3 g((M + 63) / 64, 8);
-    k_gemm<EPI_SILU><<<g, kB, 0, s>>>(X, 48, w1t, 48, M, 128, 48, b1, nullptr, Z1, H1, 128);
-    k_gemm<EPI_SILU><<<g, kB, 0, s>>>(H1, 128, w2t, 128, M, 128, 128, b2, nullptr, Z2, H2, 128);
+    k_gemm<EPI_SILU, 3><<<g, kB, 0, s>>>(X, 48, w1t, 48, M, 128, b1, nullptr, Z1, H1, 128);
+    k_gemm<EPI_SILU, 8><<<g, kB, 0, s>>>(H1, 128, w2t, 128, M, 128, b2, nullptr, Z2, H2, 128);
     dim3 g3((M + 63) / 64, 1);
-    k_gemm<EPI_LINEAR><<<g3, kB, 0, s>>>(H2, 128, w3t, 128, M, 3, 128, b3, nullptr, nullptr, Y, 16);
+    k_gemm<EPI_LINEAR, 8><<<g3, kB, 0, s>>>(H2, 128, w3t, 128, M, 3, b3, nullptr, nullptr, Y, 16);
 }
 
 // loss partials and gradient of the mean-squared loss; grad laid out like p
@@ -331,8 +355,8 @@ void akmc_mlp_launch_backward(const float *X, const float *T, int M, const uint1
     k_mse<<<(M + kB - 1) / kB, kB, 0, s>>>(Y, T, M, dY, loss_part);
     dim3 g((M + 63) / 64, 8);
     // dZ2 = (dY W3^T) * silu'(Z2): K = 16 (padded output), B^T = W3 row-major padded
-    k_gemm<EPI_DSILU><<<g, kB, 0, s>>>(dY, 16, w3r, 16, M, 128, 16, nullptr, Z2, nullptr, dZ2, 128);
-    k_gemm<EPI_DSILU><<<g, kB, 0, s>>>(dZ2, 128, w2r, 128, M, 128, 128, nullptr, Z1, nullptr, dZ1, 128);
+    k_gemm<EPI_DSILU, 1><<<g, kB, 0, s>>>(dY, 16, w3r, 16, M, 128, nullptr, Z2, nullptr, dZ2, 128);
+    k_gemm<EPI_DSILU, 8><<<g, kB, 0, s>>>(dZ2, 128, w2r, 128, M, 128, nullptr, Z1, nullptr, dZ1, 128);
     const int nchunk = (M + rows_per_chunk - 1) / rows_per_chunk;
     float *gW1 = grad, *gb1 = gW1 + 48 * 128, *gW2 = gb1 + 128, *gb2 = gW2 + 128 * 128, *gW3 = gb2 + 128,
           *gb3 = gW3 + 128 * 3;

@@ -16,7 +16,13 @@
 
 namespace {
 
-constexpr int kBlock = 256;
+// workgroup size of the walker state machines: 256 threads, or 1024 in the
+// wide build of this file (search_kernels_wide.hip) for batches of walkers
+// so large that the strided loops of 256 threads dominate a micro-step
+#ifndef AKMC_LOGIC_BLOCK
+#define AKMC_LOGIC_BLOCK 256
+#endif
+constexpr int kBlock = AKMC_LOGIC_BLOCK;
 
 // Block-wide sums and maxima in a fixed order: a butterfly over the 64
 // lanes of each wavefront, then the four wavefront partials in order, so
@@ -1215,6 +1221,7 @@ __global__ void __launch_bounds__(kBlock) k_probe_init(BatchDev b, SearchDev s)
     FOR_ATOMS(i) C_.probe(i, V3(s.X, 0, i), V3(s.X, 1, i), V3(s.X, 2, i));
 }
 
+#if AKMC_LOGIC_BLOCK == 256
 // scatter of packed walker starts (one workgroup per walker): mode N,
 // zero velocity, positions X when given, the frozen mask, the free-atom
 // list and its length
@@ -1241,8 +1248,10 @@ __global__ void __launch_bounds__(kBlock) k_load_walkers(BatchDev b, SearchDev s
     if (threadIdx.x == 0) const_cast<int *>(s.nact)[it.walker] = it.nact;
 }
 
+#endif
 }  // namespace
 
+#if AKMC_LOGIC_BLOCK == 256
 void akmc_launch_load_walkers(const BatchDev &b, const SearchDev &s, const LoadItem *items, int nitems,
                               const double *D, const int *I, hipStream_t st)
 {
@@ -1270,12 +1279,18 @@ void akmc_launch_gather_walkers(const WalkerDev *walkers, const double *const sr
     if (nw > 0) k_gather_walkers<<<nw, kBlock, 0, st>>>(walkers, src[0], src[1], src[2], ws, dst, out);
 }
 
-void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st)
-{
-    k_search_logic<<<b.W, kBlock, 0, st>>>(b, s);
-}
-
 void akmc_launch_probe_init(const BatchDev &b, const SearchDev &s, hipStream_t st)
 {
     k_probe_init<<<b.W, kBlock, 0, st>>>(b, s);
 }
+
+void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st)
+{
+    k_search_logic<<<b.W, kBlock, 0, st>>>(b, s);
+}
+#else
+void akmc_launch_search_logic_wide(const BatchDev &b, const SearchDev &s, hipStream_t st)
+{
+    k_search_logic<<<b.W, kBlock, 0, st>>>(b, s);
+}
+#endif

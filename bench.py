@@ -250,14 +250,16 @@ def forces_only(L, W, args):
 
 def zone_passes(L, S, W, args):
     """The zoned pass kernels of the search micro-steps alone, for hardware
-    counter runs: the batched campaign (3 pooled basins per walker) after
-    a 250-micro-step warm window and the per-pass variant tuning, then
-    args.zone_passes evaluations of the running searches at their probes."""
+    counter runs: the batched campaign (3 pooled basins per walker, kernels
+    launched one by one) after a 250-micro-step warm window and the per-pass
+    variant tuning, then args.zone_passes evaluations of the running searches
+    at their probes."""
     cu = W.SC_CU
     rc = 2 * cu["a"]
     text = L.sc_potentials_text([("Cu", "Cu", cu["eps"], cu["a"], cu["n"], cu["m"], cu["c"])], rc)
     cells = W.batched_sc_cells(n_walkers=args.walkers, seed=args.seed)
     camp = S.SaddleCampaign(text, cells, params=L.dimer_defaults(maxParIter=400), seed=args.seed)
+    camp.batch.set_graphs(False)  # counter and trace tools do not follow graph launches
     camp.batch.forces()
     camp.batch.autotune(5)
     camp.relax_basins()
