@@ -81,6 +81,7 @@ struct FieldDev {
     double rc;
     double rc2;      // FP64 cutoff decision
     double rlist2;   // (rc + skin)^2
+    double rzone2;   // (rc + skin/2 + 1e-6)^2: a free atom's list entries within it join its walker's zone
     double half_skin2;
     double rnear2;   // (r_near + skin)^2: list entries inside come first and
                      // are evaluated in FP64 on the FP32 path (0: none)
@@ -185,7 +186,7 @@ struct BatchDev {
     // variant may differ from pass 1's without changing any energy
     int pe1;
     int lpa2, unroll2, staged2;  // pass-2 variant (0: the batch's)
-    int lpa64, unroll64;         // FP64 variant of mixed steps (unstaged; 0: 4 x 4)
+    int lpa64, unroll64, staged64;  // FP64 variant of mixed steps (lpa64 0: 4 x 4, unstaged)
     const int *zfrozen;   // [A] the search's frozen mask (zone marking in the list build)
     int *zflag;           // [W] AKMC_ZONE_* bits
     int *zmark;           // [A] zone membership

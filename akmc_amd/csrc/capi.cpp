@@ -159,9 +159,9 @@ struct akmc_batch_s {
     int *zflag = nullptr, *zmark = nullptr, *zlist = nullptr, *nz = nullptr;
     std::vector<int> hzflag;
     // kernel variants of the search micro-steps' Sutton-Chen passes:
-    // (lanes, unroll, staged) of pass 1, then of pass 2, then (lanes,
-    // unroll) of the FP64 passes of mixed steps; 0: the default
-    int sv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // (lanes, unroll, staged) of pass 1, of pass 2, then of the FP64 passes
+    // of mixed steps; lanes 0: the default
+    int sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     // work lists of the search micro-steps' pair kernels and their grids
     int2 *wl_tab[4] = {nullptr, nullptr, nullptr, nullptr};
     int *wl_count = nullptr;
@@ -368,6 +368,10 @@ void build_tables(akmc_field_t f)
     h.rc2 = f->rc * f->rc;
     double rl = f->rc + f->skin;
     h.rlist2 = rl * rl;
+    // between rebuilds an atom moves at most skin/2, so every atom that comes
+    // within rc of a free atom was within rc + skin/2 of it at the rebuild
+    const double rz = f->rc + 0.5 * f->skin + 1.0e-6;
+    h.rzone2 = rz * rz;
     h.half_skin2 = 0.25 * f->skin * f->skin;
     // ionic lists put the first shells (r <= 3.5 A) first; the FP32 path
     // evaluates them in FP64
@@ -1085,6 +1089,7 @@ BatchDev step_dev(akmc_batch_t b)
         if (b->sv[6]) {
             d.lpa64 = b->sv[6];
             d.unroll64 = b->sv[7];
+            d.staged64 = b->sv[8];
         }
     }
     if (zone_usable(b)) {
@@ -1880,6 +1885,15 @@ int akmc_batch_set_graphs(akmc_batch_t b, int enable)
     return AKMC_SUCCESS;
 }
 
+int akmc_batch_set_logic_block(akmc_batch_t b, int threads)
+{
+    if (!b) return fail(AKMC_EINVAL, "null batch");
+    if (threads != 256 && threads != 1024) return fail(AKMC_EINVAL, "threads must be 256 or 1024, got %d", threads);
+    b->logic_wide = threads == 1024;
+    drop_graph(b);
+    return AKMC_SUCCESS;
+}
+
 int akmc_batch_set_search_zone(akmc_batch_t b, int enable)
 {
     if (!b) return fail(AKMC_EINVAL, "null batch");
@@ -1888,25 +1902,20 @@ int akmc_batch_set_search_zone(akmc_batch_t b, int enable)
     return AKMC_SUCCESS;
 }
 
-int akmc_batch_set_search_variant(akmc_batch_t b, const int v[8])
+int akmc_batch_set_search_variant(akmc_batch_t b, const int v[9])
 {
     if (!b) return fail(AKMC_EINVAL, "null batch");
-    int nv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int nv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     auto ok = [](int x) { return x == 1 || x == 2 || x == 4 || x == 8; };
     if (v) {
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < 3; ++p) {
             const int l = v[3 * p], u = v[3 * p + 1], st = v[3 * p + 2];
             if (l == 0) continue;
-            if (!ok(l) || !ok(u)) return fail(AKMC_EINVAL, "pass %d: lanes and unroll must be 1, 2, 4 or 8", p + 1);
+            if (!ok(l) || !ok(u)) return fail(AKMC_EINVAL, "part %d: lanes and unroll must be 1, 2, 4 or 8", p + 1);
             if (st && !b->stage_ok) return fail(AKMC_EUNSUPPORTED, "LDS staging is not possible for this batch");
             nv[3 * p] = l;
             nv[3 * p + 1] = u;
             nv[3 * p + 2] = st ? 1 : 0;
-        }
-        if (v[6]) {
-            if (!ok(v[6]) || !ok(v[7])) return fail(AKMC_EINVAL, "FP64: lanes and unroll must be 1, 2, 4 or 8");
-            nv[6] = v[6];
-            nv[7] = v[7];
         }
     }
     std::memcpy(b->sv, nv, sizeof nv);
@@ -1916,8 +1925,8 @@ int akmc_batch_set_search_variant(akmc_batch_t b, const int v[8])
 
 static int time_search_phases_prec(akmc_batch_t b, int prec, int nrep, double ms[4]);
 
-int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[8], double ms1[AKMC_NVARIANTS],
-                               double ms2[AKMC_NVARIANTS], double ms64[AKMC_NVARIANTS / 2])
+int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[9], double ms1[AKMC_NVARIANTS],
+                               double ms2[AKMC_NVARIANTS], double ms64[AKMC_NVARIANTS])
 {
     if (!b || nrep <= 0) return fail(AKMC_EINVAL, "bad arguments");
     if (!b->search_alloc) return fail(AKMC_EINVAL, "no search has been started");
@@ -1927,7 +1936,7 @@ int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[8], double ms1
     for (int v = 0; v < AKMC_NVARIANTS; ++v) {
         t1[v] = t2[v] = -1.0;
         if (v >= nv) continue;
-        const int sv[8] = {lanes[v / 4 % 4], unroll[v % 4], v / 16, lanes[v / 4 % 4], unroll[v % 4], v / 16, 0, 0};
+        const int sv[9] = {lanes[v / 4 % 4], unroll[v % 4], v / 16, lanes[v / 4 % 4], unroll[v % 4], v / 16, 0, 0, 0};
         TRY(akmc_batch_set_search_variant(b, sv));
         TRY(akmc_batch_time_search_phases(b, 1, ms));  // first launch of this instantiation
         TRY(akmc_batch_time_search_phases(b, nrep, ms));
@@ -1939,26 +1948,28 @@ int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[8], double ms1
         if (t1[v] < t1[k1]) k1 = v;
         if (t2[v] < t2[k2]) k2 = v;
     }
-    // the FP64 passes of mixed steps (unstaged), on the walkers under FP64
-    // verification at the moment: both passes together
-    double t64[AKMC_NVARIANTS / 2];
-    int k64 = -1;
-    for (int v = 0; v < AKMC_NVARIANTS / 2; ++v) {
-        const int sv[8] = {0, 0, 0, 0, 0, 0, lanes[v / 4], unroll[v % 4]};
+    // the FP64 passes of mixed steps, on the walkers under FP64 verification
+    // at the moment: both passes together
+    double t64[AKMC_NVARIANTS];
+    int k64 = 0;
+    for (int v = 0; v < AKMC_NVARIANTS; ++v) {
+        t64[v] = -1.0;
+        if (v >= nv) continue;
+        const int sv[9] = {0, 0, 0, 0, 0, 0, lanes[v / 4 % 4], unroll[v % 4], v / 16};
         TRY(akmc_batch_set_search_variant(b, sv));
         TRY(time_search_phases_prec(b, 2 | 4, 1, ms));
         TRY(time_search_phases_prec(b, 2 | 4, nrep, ms));
         t64[v] = ms[1] + ms[2];
-        if (k64 < 0 || t64[v] < t64[k64]) k64 = v;
+        if (t64[v] < t64[k64]) k64 = v;
     }
-    const int sv[8] = {lanes[k1 / 4 % 4], unroll[k1 % 4], k1 / 16, lanes[k2 / 4 % 4], unroll[k2 % 4], k2 / 16,
-                       lanes[k64 / 4], unroll[k64 % 4]};
+    const int sv[9] = {lanes[k1 / 4 % 4], unroll[k1 % 4], k1 / 16, lanes[k2 / 4 % 4], unroll[k2 % 4], k2 / 16,
+                       lanes[k64 / 4 % 4], unroll[k64 % 4], k64 / 16};
     TRY(akmc_batch_set_search_variant(b, sv));
     if (best) std::memcpy(best, sv, sizeof sv);
     for (int v = 0; v < AKMC_NVARIANTS; ++v) {
         if (ms1) ms1[v] = t1[v];
         if (ms2) ms2[v] = t2[v];
-        if (ms64 && v < AKMC_NVARIANTS / 2) ms64[v] = t64[v];
+        if (ms64) ms64[v] = t64[v];
     }
     return AKMC_SUCCESS;
 }

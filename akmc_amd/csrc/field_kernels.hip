@@ -496,6 +496,7 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
     const int ncand = c_pref[n_cells];
     const double rl2 = b.field->rlist2;
     const double rn2 = b.field->rnear2;
+    const double rz2 = b.field->rzone2;
     const int s0 = b.cell_start[cg], n0 = b.cell_count[cg];
     const int lane = threadIdx.x % kNbLanes, slot = threadIdx.x / kNbLanes;
     const int wlane = threadIdx.x & 63;
@@ -505,7 +506,7 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
     const int i = have ? b.cell_atoms[s0 + q] : 0;
     const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
     // a free atom of a zoned walker puts itself and every atom of its list
-    // in the walker's zone
+    // within rc + skin/2 in the walker's zone
     const bool mark = have && b.zflag && (b.zflag[w] & AKMC_ZONE_ON) && b.zfrozen[i] == 0;
     if (mark && lane == 0) b.zmark[i] = 1;
     int cnt = 0, nnear = 0;
@@ -529,11 +530,13 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
             }
             for (int g = 0; g < m; g += kNbLanes * kNbIlp) {
                 int hit[kNbIlp], jj[kNbIlp];
+                bool zin[kNbIlp];
 #pragma unroll
                 for (int u = 0; u < kNbIlp; ++u) {
                     const int t = g + kNbLanes * u + lane;
                     hit[u] = 0;
                     jj[u] = 0;
+                    zin[u] = false;
                     if (have && t < m) {
                         jj[u] = sj[t];
                         double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
@@ -541,6 +544,7 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
                         const double r2 = dx * dx + dy * dy + dz * dz;
                         const bool near = r2 <= rn2;
                         hit[u] = (jj[u] != i && r2 <= rl2 && (pass == 0 ? near : !near)) ? 1 : 0;
+                        zin[u] = r2 <= rz2;
                     }
                 }
 #pragma unroll
@@ -551,7 +555,7 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
                     const unsigned grp = unsigned(bal >> (wlane & ~(kNbLanes - 1))) & 0xFFFFu;
                     if (hit[u]) {
                         put_nbr(b, i, cnt + __popc(grp & ((1u << lane) - 1u)), uint32_t(jj[u] - W.off) | AKMC_NB_ZERO_SHIFT);
-                        if (mark) b.zmark[jj[u]] = 1;
+                        if (mark && zin[u]) b.zmark[jj[u]] = 1;
                     }
                     cnt += __popc(grp);
                 }
@@ -2169,17 +2173,17 @@ static void launch_pass2(const BatchDev &b, const FieldDev &hf, int want_stress,
 }
 
 // The FP64 pair kernels of a mixed launch (prec 3) serve the few walkers
-// under FP64 verification: they run unstaged at 4 lanes per atom and 4
-// gathers in flight, whose 256-thread workgroups of 64 atoms spread those
-// walkers over more CUs than staged chunks would (the batch's own variant is
-// tuned for the FP32 pass over every walker).  An FP64-only launch serves
-// every walker and keeps the batch's variant and staging.
+// under FP64 verification with their own variant: the search micro-steps'
+// (lpa64, unroll64, staged64) when set, else unstaged at 4 lanes per atom
+// and 4 gathers in flight (the batch's own variant is tuned for the FP32
+// pass over every walker).  An FP64-only launch serves every walker and
+// keeps the batch's variant and staging.
 static BatchDev fp64_view(const BatchDev &b, int prec)
 {
     BatchDev v = b;
     if (prec & 5) {
-        v.staged = 0;
-        v.lpa = b.lpa64 ? b.lpa64 : 4;  // 4 x 4: measured best on the batched dimer workload
+        v.staged = b.lpa64 ? b.staged64 : 0;
+        v.lpa = b.lpa64 ? b.lpa64 : 4;  // 4 x 4 unstaged: measured best before the search variants
         v.unroll = b.lpa64 ? b.unroll64 : 4;
         v.lpa2 = 0;  // both passes
     }

@@ -188,6 +188,7 @@ def load() -> C.CDLL:
     L.akmc_batch_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.akmc_batch_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.akmc_batch_set_search_zone.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_set_logic_block.argtypes = [C.c_void_p, C.c_int]
     L.akmc_batch_time_search_phases.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
     L.akmc_batch_get_zone.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 4
     L.akmc_batch_set_search_variant.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
@@ -524,6 +525,10 @@ class Batch:
     def set_graphs(self, enable: bool):
         check(load().akmc_batch_set_graphs(self._h, int(bool(enable))))
 
+    def set_logic_block(self, threads: int):
+        """workgroup size of the walker state machines (256 or 1024)"""
+        check(load().akmc_batch_set_logic_block(self._h, int(threads)))
+
     def set_search_zone(self, enable: bool):
         """Active zones of the search micro-steps (on by default)."""
         check(load().akmc_batch_set_search_zone(self._h, int(bool(enable))))
@@ -535,24 +540,23 @@ class Batch:
         return ms / nrep
 
     def set_search_variant(self, v=None):
-        """(lanes, unroll, staged) of pass 1, of pass 2, then (lanes, unroll) of
-        the mixed steps' FP64 passes for the search micro-steps; None resets"""
+        """(lanes, unroll, staged) of pass 1, of pass 2, then of the mixed steps'
+        FP64 passes for the search micro-steps; None resets"""
         a = None
         if v is not None:
-            a = np.zeros(8, np.int32)
+            a = np.zeros(9, np.int32)
             a[:len(v)] = v
         check(load().akmc_batch_set_search_variant(self._h, _ip(a)))
 
     def autotune_search(self, nrep=5):
         """fastest variant per part on the running searches: (best[8],
         {variant: (ms pass 1, ms pass 2, ms FP64 passes or None)}), ms per nrep evaluations"""
-        best = np.zeros(8, np.int32)
+        best = np.zeros(9, np.int32)
         n = 32
-        m1, m2, m64 = np.zeros(n), np.zeros(n), np.zeros(n // 2)
+        m1, m2, m64 = np.zeros(n), np.zeros(n), np.zeros(n)
         check(load().akmc_batch_autotune_search(self._h, int(nrep), _ip(best), _dp(m1), _dp(m2), _dp(m64)))
-        table = {(l, u, st): (m1[16 * st + 4 * li + ui], m2[16 * st + 4 * li + ui],
-                              m64[4 * li + ui] if st == 0 else None) for st in (0, 1)
-                 for li, l in enumerate((1, 2, 4, 8)) for ui, u in enumerate((1, 2, 4, 8))
+        table = {(l, u, st): (m1[16 * st + 4 * li + ui], m2[16 * st + 4 * li + ui], m64[16 * st + 4 * li + ui])
+                 for st in (0, 1) for li, l in enumerate((1, 2, 4, 8)) for ui, u in enumerate((1, 2, 4, 8))
                  if m1[16 * st + 4 * li + ui] >= 0}
         return tuple(int(x) for x in best), table
 

@@ -19,14 +19,19 @@ jitter) and moves to the next after --searches-per-basin searches (5, the
 kmcevents of the reference's example control file), each basin with its
 own de-duplication catalogue, so the rate is a steady state rather than a
 catalogue that saturates as the run goes on.
+Searches evaluate their active zones (the free atoms' forces, the
+densities and energies within the list radius of a free atom), bit for bit
+the results of full evaluations.
 Also reported: the plain unbatched GPU dimer (one walker, host-driven
 micro-steps) on the same cells, force evaluations per second, the roofline
-of the dominant kernel (Sutton-Chen pass-2 force) timed with HIP events on
-the batch's stream, the learned-proposal A/B test, the KMC step (64
-systems x 8 walkers taking kmcSteps steps of 5 valid events each), the
-tabulated-EAM field's force evaluation on the same walkers, and the
-CPU baseline (the FP64 C restatement of the reference, OpenMP) on a
-bounded sample of the same searches.
+of the dominant kernel of the search micro-steps (the zoned Sutton-Chen
+pass 1) timed with HIP events on the batch's stream, with the full
+evaluation's pass 2 beside it, the learned-proposal A/B test, the KMC step
+(64 systems x 8 walkers taking kmcSteps steps of 5 valid events each), the
+tabulated-EAM field's force evaluation on the same walkers, the ionic
+(rock-salt Ewald) and full-loop (31,980-atom alloy) sections, and the CPU
+baseline (the FP64 C restatement of the reference, OpenMP) on a bounded
+sample of the same searches.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 2]
 """
@@ -57,6 +62,9 @@ HBM_PEAK_GBS = 8000.0
 # calibrated on a 1 GiB device copy (FETCH_SIZE reads half the copied bytes,
 # WRITE_SIZE all of them); the summary is under profiles/
 TRAFFIC_BYTES_PER_PAIR = 3.35
+# the same for the zoned pass 1 of the search micro-steps (k_sc_density with
+# pair energies), from the counter runs of `bench.py --zone-passes 10`
+ZONE_TRAFFIC_BYTES_PER_PAIR = 9.72
 
 
 def ordered_pairs(cells, positions, rc):
@@ -690,13 +698,13 @@ def main():
             # searches under way (zoned passes): untimed, between windows
             t0 = time.perf_counter()
             sv, sv_ms = camp.batch.autotune_search(5)
-            search_variant = {"pass1": sv[:3], "pass2": sv[3:6], "fp64": sv[6:],
+            search_variant = {"pass1": sv[:3], "pass2": sv[3:6], "fp64": sv[6:9],
                               "seconds": round(time.perf_counter() - t0, 2),
                               "ms_per_5_evals_pass1_pass2_fp64":
                                   {f"{k[0]}x{k[1]}{' lds' if k[2] else ''}":
                                    [round(x, 4) if x is not None else None for x in v] for k, v in sv_ms.items()}}
             print(f"[bench] search variants (lanes, unroll, staged): pass 1 {sv[:3]} pass 2 {sv[3:6]} "
-                  f"FP64 {sv[6:]}; {search_variant['ms_per_5_evals_pass1_pass2_fp64']}", file=sys.stderr,
+                  f"FP64 {sv[6:9]}; {search_variant['ms_per_5_evals_pass1_pass2_fp64']}", file=sys.stderr,
                   flush=True)
         rb0 = camp.batch.rebuilds
         st = camp.advance(args.window, sync_every=args.sync_every)
@@ -826,9 +834,10 @@ def main():
                      "frac": round(z_tf1 / FP32_VECTOR_PEAK_TFLOPS, 4),
                      "unpacked_ceiling": FP32_VECTOR_PEAK_TFLOPS / 2,
                      "frac_vs_unpacked_ceiling": round(z_tf1 / (FP32_VECTOR_PEAK_TFLOPS / 2), 4),
-                     "traffic": None,
-                     "traffic_note": "no counter run of the zoned kernels; full_evaluation.traffic holds the HBM "
-                                     "bytes of k_sc_force over every walker at the basins",
+                     "traffic": round(ZONE_TRAFFIC_BYTES_PER_PAIR * zw["pairs_pass1"]),
+                     "traffic_note": "HBM bytes of one zoned pass-1 launch: 9.72 B per ordered pair (FETCH_SIZE x "
+                                     "2.0 + WRITE_SIZE x 1.0, scales from a 1 GiB copy) from the counter runs of "
+                                     "bench.py --zone-passes 10, times this launch's pairs",
                      "measured_over": "20 evaluations of the search walkers at their probes after the timed "
                                       "windows, as the micro-steps launch them (HIP events on the batch's stream)",
                      "ms_per_launch": round(float(zph[1]), 4),

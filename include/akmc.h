@@ -305,6 +305,11 @@ int akmc_batch_set_graphs(akmc_batch_t b, int enable);
  * akmc_batch_download after a search returns their values from its start:
  * call akmc_batch_forces first for fresh ones. */
 int akmc_batch_set_search_zone(akmc_batch_t b, int enable);
+/* Workgroup size of the walker state machines: 256 threads, or 1024 (the
+ * default for batches whose largest walker has at least 16 384 atoms).  No
+ * counterpart in the reference; the two differ only in the order of their
+ * block sums. */
+int akmc_batch_set_logic_block(akmc_batch_t b, int threads);
 /* akmc_batch_time_phases for the search walkers at their current probes, as
  * the search micro-steps launch the kernels (active zones included), without
  * advancing any search (no counterpart in the reference: measurement). */
@@ -317,21 +322,20 @@ int akmc_batch_time_search_phases(akmc_batch_t b, int nrep, double ms[4]);
 int akmc_batch_get_zone(akmc_batch_t b, int *member, int *size, int *flags, int *active);
 /* Kernel variants of the search micro-steps' Sutton-Chen passes (no
  * counterpart in the reference): v = {lanes, unroll, staged} of pass 1
- * (density; it also writes the pair energies in search micro-steps), then
- * of pass 2 (forces), then {lanes, unroll} of both FP64 passes of steps
- * that mix FP32 and FP64 walkers (unstaged); lanes 0 keeps the default for
- * that part, NULL resets all.  Results do not depend on unroll or staging;
- * lanes change only the summation order of the sums they split. */
-int akmc_batch_set_search_variant(akmc_batch_t b, const int v[8]);
+ * (density; it also writes the pair energies in search micro-steps), of
+ * pass 2 (forces), then of both FP64 passes of steps that mix FP32 and FP64
+ * walkers; lanes 0 keeps the default for that part, NULL resets all.
+ * Results do not depend on unroll or staging; lanes change only the
+ * summation order of the sums they split. */
+int akmc_batch_set_search_variant(akmc_batch_t b, const int v[9]);
 /* Times every variant of each pass on the search walkers at their current
  * probes (akmc_batch_time_search_phases, nrep evaluations each; the FP64
  * variants on the walkers under FP64 verification), keeps the fastest per
- * part and returns it in best[8] (may be NULL) with the per-variant ms of
- * pass 1, pass 2 (indexed like akmc_batch_autotune) and the FP64 passes
- * (unstaged variants only; any of them NULL).  Call it with searches under
- * way. */
-int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[8], double ms1[AKMC_NVARIANTS],
-                               double ms2[AKMC_NVARIANTS], double ms64[AKMC_NVARIANTS / 2]);
+ * part and returns it in best[9] (may be NULL) with the per-variant ms of
+ * pass 1, pass 2 and the FP64 passes, indexed like akmc_batch_autotune (any
+ * of them NULL).  Call it with searches under way. */
+int akmc_batch_autotune_search(akmc_batch_t b, int nrep, int best[9], double ms1[AKMC_NVARIANTS],
+                               double ms2[AKMC_NVARIANTS], double ms64[AKMC_NVARIANTS]);
 /* Kernel timer of akmc_batch_run: HIP events on the batch's stream around
  * the FP32 Sutton-Chen pass-2 launch (k_sc_force) of every micro-step.
  * Events inside a replayed graph cannot be timed, so while the timer runs

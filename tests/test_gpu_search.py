@@ -55,6 +55,50 @@ def test_relax_matches_oracle(lib, oracle, c1):
     assert fmax < 2e-4
 
 
+def test_wide_state_machines_match_oracle(lib, oracle, c1):
+    """The 1024-thread build of the state machines (used for walkers of
+    16 384 atoms and more) on the small cell: its relax meets the oracle's
+    like the 256-thread one, and a dimer search with FP64 verification and
+    the basin-2 stage reaches the same saddle and basin-2 energies as the
+    256-thread build (the two differ only in the order of their block
+    sums)."""
+    c, of, h, r = c1
+    f = lib.Field(lib.sc_potentials_text(_pots(), 2 * CU["a"]))
+    b = lib.Batch(f, [c])
+    b.set_logic_block(1024)
+    b.start_relax()
+    b.run(4000, sync_every=50)
+    st = b.status()[0]
+    pos = b.positions()[0]
+    ref = oracle.sc_forces(of, pos, c.types, c.lat)
+    fmax = np.sqrt((ref["force"] ** 2).sum(0).max())
+    print(f"wide relax: status {st['status']} iters {st['iterations']} E {st['energy']:.8f} "
+          f"(oracle {r['energy']:.8f}); FP64 |F|max {fmax:.2e}", flush=True)
+    assert st["status"] == 0
+    assert abs(st["energy"] - r["energy"]) / c.n < 1e-6
+    assert fmax < 2e-4
+    basin = r["pos"]
+    frozen, disp, mode = _dimer_start(c, basin, 2)
+    out = []
+    for threads in (256, 1024):
+        bb = lib.Batch(f, [W.Cell(basin, c.types, c.lat)])
+        bb.set_logic_block(threads)
+        bb.set_frozen([frozen])
+        bb.set_basins(None, np.ascontiguousarray(basin).reshape(-1))
+        bb.set_basin2(lib.basin2_defaults())
+        bb.set_verify(16, 0.01)
+        bb.start_dimer([mode], [disp])
+        bb.run(20000, sync_every=25)
+        out.append(bb.status()[0])
+    a, w = out
+    print(f"256: E64 {a['energy64']:.9f} l1 {a['lambda1']:.5f} b2 {a['energy_basin2_64']:.9f}; "
+          f"1024: E64 {w['energy64']:.9f} l1 {w['lambda1']:.5f} b2 {w['energy_basin2_64']:.9f}", flush=True)
+    assert a["status"] == 0 and w["status"] == 0 and a["verified"] == 1 and w["verified"] == 1
+    assert abs(a["energy64"] - w["energy64"]) < 1e-6
+    assert abs(a["lambda1"] - w["lambda1"]) < 1e-3 * abs(a["lambda1"])
+    assert abs(a["energy_basin2_64"] - w["energy_basin2_64"]) < 1e-6
+
+
 @pytest.mark.parametrize("method,verlet", [("fire", 1), ("fire2", 0), ("fire2", 1)])
 def test_relax_variants_match_oracle(lib, oracle, c1, method, verlet):
     """Relax::RelaxStructure's other branches on the device (FIRE with
