@@ -340,6 +340,18 @@ void akmc_launch_gather_walkers(const WalkerDev *walkers, const double *const sr
 void akmc_launch_load_walkers(const BatchDev &b, const SearchDev &s, const LoadItem *items, int nitems,
                               const double *D, const int *I, hipStream_t st);
 void akmc_launch_probe_init(const BatchDev &b, const SearchDev &s, hipStream_t st);
+// a walker start: its fresh state, zone flags and active flag
+struct StartItem {
+    int walker, zflag, active, pad;
+    WalkerState ws;
+};
+// scatters the starts and asks each started walker for a list rebuild on
+// its first micro-step (zflag and harvested may be null)
+void akmc_launch_start_walkers(const BatchDev &b, const SearchDev &s, const StartItem *items, int nitems, int *zflag,
+                               int *harvested, hipStream_t st);
+// copies the positions X of every walker that has ended and is not yet
+// harvested into out ([3][A], host memory mapped for the device) and marks it
+void akmc_launch_harvest_pack(const BatchDev &b, const SearchDev &s, int *harvested, double *out, hipStream_t st);
 
 // learned proposal (mlp_kernels.hip)
 void akmc_mlp_launch_weights(const float *p, uint16_t *w1t, uint16_t *w2t, uint16_t *w3t, uint16_t *w2r,

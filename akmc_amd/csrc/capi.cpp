@@ -177,6 +177,21 @@ struct akmc_batch_s {
     void *stage_pin = nullptr;  // pinned host / device staging of walker starts
     void *stage_dev = nullptr;
     size_t stage_n = 0;
+    // the fresh states of started walkers (one pinned transfer per start);
+    // each pinned buffer is rewritten only after its last transfer is done
+    StartItem *st_pin = nullptr, *st_dev = nullptr;
+    hipEvent_t stage_ev = nullptr, st_ev = nullptr;
+    bool stage_pending = false, st_pending = false;
+    // starts queued behind a chunk in flight: their states replace that
+    // chunk's status copy when it is waited for
+    std::vector<std::pair<int, WalkerState>> started_inflight;
+    // reads of ended walkers while a chunk is in flight
+    hipStream_t side = nullptr;
+    // harvest: each chunk ends by copying the positions of the walkers that
+    // ended in it into one of two mapped host buffers ([3][A] each), in turn
+    int hv_on = 0, hv_cur = 0, hv_inflight = -1, hv_last = -1;
+    int *harvested = nullptr;
+    double *hv_pin[2] = {nullptr, nullptr}, *hv_dev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -856,6 +871,12 @@ void batch_free(akmc_batch_t b)
     if (b->gat_dev) (void)hipFree(b->gat_dev);
     if (b->stage_pin) (void)hipHostFree(b->stage_pin);
     if (b->stage_dev) (void)hipFree(b->stage_dev);
+    if (b->st_pin) (void)hipHostFree(b->st_pin);
+    if (b->stage_ev) (void)hipEventDestroy(b->stage_ev);
+    if (b->st_ev) (void)hipEventDestroy(b->st_ev);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    for (double *p : b->hv_pin)
+        if (p) (void)hipHostFree(p);
     delete b;
 }
 
@@ -1044,6 +1065,10 @@ int search_alloc(akmc_batch_t b)
     TRY(upload_active(b, nullptr));
     HIPCHK(hipHostMalloc((void **)&b->hws_pin, b->W * sizeof(WalkerState), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&b->ov_pin, sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&b->st_pin, b->W * sizeof(StartItem), hipHostMallocDefault));
+    TRY(dalloc(b, &b->st_dev, b->W));
+    HIPCHK(hipEventCreateWithFlags(&b->stage_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->st_ev, hipEventDisableTiming));
     b->search_alloc = true;
     return AKMC_SUCCESS;
 }
@@ -1150,9 +1175,26 @@ int refresh_active(akmc_batch_t b, int *n_running)
     return AKMC_SUCCESS;
 }
 
+// the host may rewrite a pinned staging buffer once its last transfer,
+// recorded on ev, is done
+int wait_staging(hipEvent_t ev, bool &pending)
+{
+    if (pending) HIPCHK(hipEventSynchronize(ev));
+    pending = false;
+    return AKMC_SUCCESS;
+}
+
+// The listed walkers' fresh states go to the device in one pinned transfer
+// and one scatter kernel, with, per walker: a fresh zone when it has frozen
+// atoms (membership reset, and a full evaluation first in each precision),
+// a list rebuild on its first micro-step (zones on or off alike, so both
+// list the same neighbours in the same order) and its active flag.  Only the
+// started walkers' device state is written, so a start may be queued behind
+// a run chunk in flight (it then returns at once); otherwise it waits.
 int start_walkers(akmc_batch_t b, const int *walkers, int kind, double dt0, double alpha0)
 {
-    TRY(fetch_states(b));
+    TRY(wait_staging(b->st_ev, b->st_pending));
+    int k = 0;
     for (int w = 0; w < b->W; ++w) {
         if (walkers && !walkers[w]) continue;
         WalkerState ws{};
@@ -1167,35 +1209,25 @@ int start_walkers(akmc_batch_t b, const int *walkers, int kind, double dt0, doub
         ws.inloop = 0;
         ws.verified_state = 0;
         b->hws[w] = ws;
-    }
-    HIPCHK(hipMemcpyAsync(b->s.ws, b->hws.data(), b->W * sizeof(WalkerState), hipMemcpyHostToDevice, b->stream));
-    // a started walker with frozen atoms gets a fresh zone: membership reset,
-    // and a full evaluation first in each precision
-    HIPCHK(hipMemcpyAsync(b->hzflag.data(), b->zflag, b->W * sizeof(int), hipMemcpyDeviceToHost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    for (int w = 0; w < b->W; ++w) {
-        if (walkers && !walkers[w]) continue;
         const int m = b->hnact[w];
-        b->hzflag[w] = (m > 0 && m < b->hw[w].n)
-                           ? (AKMC_ZONE_ON | AKMC_ZONE_FULL32 | AKMC_ZONE_FULL64 | AKMC_ZONE_RESET)
-                           : 0;
+        StartItem &it = b->st_pin[k++];
+        it.walker = w;
+        it.zflag = (m > 0 && m < b->hw[w].n) ? (AKMC_ZONE_ON | AKMC_ZONE_FULL32 | AKMC_ZONE_FULL64 | AKMC_ZONE_RESET)
+                                             : 0;
+        it.active = batch_mode(b);
+        it.pad = 0;
+        it.ws = ws;
+        b->hzflag[w] = it.zflag;
+        if (b->inflight) b->started_inflight.emplace_back(w, ws);
     }
-    HIPCHK(hipMemcpyAsync(b->zflag, b->hzflag.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    // a started walker rebuilds its lists on its first micro-step (a zoned
-    // one marks its zone there), with or without zones, so both list the
-    // same neighbours in the same order (flags already raised stay raised)
-    std::vector<int> reb(b->W, 0);
-    HIPCHK(hipMemcpyAsync(reb.data(), b->d.rebuild, b->W * sizeof(int), hipMemcpyDeviceToHost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    for (int w = 0; w < b->W; ++w)
-        if (!walkers || walkers[w]) reb[w] = 1;
-    HIPCHK(hipMemcpyAsync(b->d.rebuild, reb.data(), b->W * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    int run;
-    TRY(refresh_active(b, &run));
+    if (k == 0) return AKMC_SUCCESS;
+    HIPCHK(hipMemcpyAsync(b->st_dev, b->st_pin, k * sizeof(StartItem), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipEventRecord(b->st_ev, b->stream));
+    b->st_pending = true;
+    akmc_launch_start_walkers(b->d, b->s, b->st_dev, k, b->zflag, b->hv_on ? b->harvested : nullptr, b->stream);
     akmc_launch_probe_init(b->d, b->s, b->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(b->stream));
+    if (!b->inflight) HIPCHK(hipStreamSynchronize(b->stream));
     return AKMC_SUCCESS;
 }
 
@@ -1229,6 +1261,7 @@ int ensure_stage(akmc_batch_t b, size_t n)
 int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const double *mode, const int *frozen)
 {
     const size_t A = b->A;
+    TRY(wait_staging(b->stage_ev, b->stage_pending));
     std::vector<LoadItem> items;
     size_t dwords = 0, iwords = 0;
     for (int w = 0; w < b->W; ++w) {
@@ -1282,11 +1315,14 @@ int load_walkers(akmc_batch_t b, const int *walkers, const double *x0, const dou
     }
     std::memcpy(base, items.data(), ibytes);
     HIPCHK(hipMemcpyAsync(b->stage_dev, b->stage_pin, ibytes + dbytes + nbytes, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipEventRecord(b->stage_ev, b->stream));
+    b->stage_pending = true;
     char *dbase = (char *)b->stage_dev;
     akmc_launch_load_walkers(b->d, b->s, (const LoadItem *)dbase, (int)items.size(), (const double *)(dbase + ibytes),
                              (const int *)(dbase + ibytes + dbytes), b->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(b->stream));
+    // queued behind a chunk in flight, the load returns at once
+    if (!b->inflight) HIPCHK(hipStreamSynchronize(b->stream));
     return AKMC_SUCCESS;
 }
 
@@ -1768,24 +1804,24 @@ int akmc_batch_start_art(akmc_batch_t b, const int *walkers, const double *x0, c
     return start_walkers(b, walkers, WK_ART, p->initTimeStep, p->alphaStart);
 }
 
-int akmc_batch_run_async(akmc_batch_t b, int steps)
+}  // extern "C"
+
+namespace {
+
+// launches one chunk of `steps` micro-steps and the copies of the walker
+// states and the overflow flag that its wait reads
+int launch_chunk(akmc_batch_t b, int steps, int running)
 {
-    if (!b || !b->search_alloc || steps < 1) return fail(AKMC_EINVAL, "bad arguments");
-    if (b->inflight) return fail(AKMC_EINVAL, "a run chunk is already in flight");
-    b->s.default_mode = batch_mode(b);  // the mode the state kernels give walkers they move on
-    TRY(fetch_states(b));
-    int running = 0;
-    TRY(refresh_active(b, &running));
     b->inflight_steps = 0;
-    if (running == 0) {
-        b->inflight = 1;
-        return AKMC_SUCCESS;
-    }
+    b->inflight = 1;
+    if (running == 0) return AKMC_SUCCESS;
     // k micro-steps as one graph launch; the state kernels keep the
     // per-walker active flags current (0 when a walker ends, FP64 while
     // it verifies), so the host only reads states and the overflow flag
-    if (b->ktimer && steps > akmc_batch_s::kTimerSteps)
+    if (b->ktimer && steps > akmc_batch_s::kTimerSteps) {
+        b->inflight = 0;
         return fail(AKMC_EINVAL, "sync_every is limited to %d while the kernel timer runs", akmc_batch_s::kTimerSteps);
+    }
     // HIP events recorded inside a replayed graph cannot be timed, so the
     // kernel timer launches the micro-steps one by one
     if (b->use_graphs && !b->ktimer) {
@@ -1799,11 +1835,44 @@ int akmc_batch_run_async(akmc_batch_t b, int steps)
         for (int j = 0; j < steps; ++j) enqueue_micro_step(b, j);
         HIPCHK(hipGetLastError());
     }
+    if (b->hv_on) {
+        // the buffer the host read two waits ago
+        b->hv_cur ^= 1;
+        akmc_launch_harvest_pack(b->d, b->s, b->harvested, b->hv_dev[b->hv_cur], b->stream);
+        HIPCHK(hipGetLastError());
+        b->hv_inflight = b->hv_cur;
+    }
     HIPCHK(hipMemcpyAsync(b->hws_pin, b->s.ws, b->W * sizeof(WalkerState), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipMemcpyAsync(b->ov_pin, b->d.overflow, sizeof(int), hipMemcpyDeviceToHost, b->stream));
-    b->inflight = 1;
     b->inflight_steps = steps;
     return AKMC_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int akmc_batch_run_async(akmc_batch_t b, int steps)
+{
+    if (!b || !b->search_alloc || steps < 1) return fail(AKMC_EINVAL, "bad arguments");
+    if (b->inflight) return fail(AKMC_EINVAL, "a run chunk is already in flight");
+    b->s.default_mode = batch_mode(b);  // the mode the state kernels give walkers they move on
+    TRY(fetch_states(b));
+    int running = 0;
+    TRY(refresh_active(b, &running));
+    return launch_chunk(b, steps, running);
+}
+
+int akmc_batch_run_next(akmc_batch_t b, int steps)
+{
+    if (!b || !b->search_alloc || steps < 1) return fail(AKMC_EINVAL, "bad arguments");
+    if (b->inflight) return fail(AKMC_EINVAL, "a run chunk is already in flight");
+    b->s.default_mode = batch_mode(b);
+    // the host's copy of the states is the last wait's, with the starts
+    // made since; the device keeps the active flags
+    int running = 0;
+    for (int w = 0; w < b->W; ++w) running += b->hws[w].kind != WK_IDLE && b->hws[w].state != ST_DONE;
+    return launch_chunk(b, steps, running);
 }
 
 int akmc_batch_run_wait(akmc_batch_t b, int *steps_done, int *running)
@@ -1823,10 +1892,51 @@ int akmc_batch_run_wait(akmc_batch_t b, int *steps_done, int *running)
                 b->klaunches += 1;
             }
         std::memcpy(b->hws.data(), b->hws_pin, b->W * sizeof(WalkerState));
-        for (int w = 0; w < b->W; ++w) run += b->hws[w].kind != WK_IDLE && b->hws[w].state != ST_DONE;
+        if (b->hv_on) b->hv_last = b->hv_inflight;
+    } else {
+        HIPCHK(hipStreamSynchronize(b->stream));  // starts queued since the launch
     }
+    // walkers started behind the chunk have not run yet
+    for (const auto &p : b->started_inflight) b->hws[p.first] = p.second;
+    b->started_inflight.clear();
+    for (int w = 0; w < b->W; ++w) run += b->hws[w].kind != WK_IDLE && b->hws[w].state != ST_DONE;
     if (steps_done) *steps_done = k;
     if (running) *running = run;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_set_harvest(akmc_batch_t b, int on)
+{
+    if (!b || !b->search_alloc) return fail(AKMC_EINVAL, "no search has been started");
+    if (b->inflight) return fail(AKMC_EINVAL, "a run chunk is in flight");
+    if (on && !b->harvested) {
+        TRY(dalloc(b, &b->harvested, b->W));
+        for (int k = 0; k < 2; ++k) {
+            HIPCHK(hipHostMalloc((void **)&b->hv_pin[k], 3 * (size_t)b->A * sizeof(double), hipHostMallocDefault));
+            HIPCHK(hipHostGetDevicePointer((void **)&b->hv_dev[k], b->hv_pin[k], 0));
+        }
+    }
+    // walkers that have already ended are harvested by the next chunk
+    if (on) HIPCHK(hipMemsetAsync(b->harvested, 0, b->W * sizeof(int), b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->hv_on = on ? 1 : 0;
+    b->hv_last = -1;
+    return AKMC_SUCCESS;
+}
+
+int akmc_batch_get_harvested_positions(akmc_batch_t b, int nw, const int *walkers, double *out)
+{
+    if (!b || nw < 0 || (nw && (!walkers || !out))) return fail(AKMC_EINVAL, "bad arguments");
+    if (!b->hv_on || b->hv_last < 0) return fail(AKMC_EINVAL, "no harvested chunk has been waited for");
+    const double *src = b->hv_pin[b->hv_last];
+    const size_t A = b->A;
+    size_t o = 0;
+    for (int k = 0; k < nw; ++k) {
+        const int w = walkers[k];
+        if (w < 0 || w >= b->W) return fail(AKMC_EINVAL, "walker %d out of range", w);
+        const int off = b->hw[w].off, n = b->hw[w].n;
+        for (int c = 0; c < 3; ++c, o += n) std::memcpy(out + o, src + c * A + off, n * sizeof(double));
+    }
     return AKMC_SUCCESS;
 }
 
@@ -1978,7 +2088,9 @@ int akmc_batch_status(akmc_batch_t b, akmc_walker_status *out)
 {
     if (!b || !out) return fail(AKMC_EINVAL, "bad arguments");
     if (!b->search_alloc) return fail(AKMC_EINVAL, "no search has been started");
-    TRY(fetch_states(b));
+    // while a run chunk is in flight: the states at the end of the previous
+    // chunk, with the starts made since
+    if (!b->inflight) TRY(fetch_states(b));
     for (int w = 0; w < b->W; ++w) {
         const WalkerState &ws = b->hws[w];
         out[w].kind = ws.kind;
@@ -2074,6 +2186,13 @@ int copy_walkers_out3(akmc_batch_t b, const double *const src[3], int nw, const 
         HIPCHK(hipMalloc((void **)&b->gat_dev, cap));
         b->gat_n = cap;
     }
+    // while a run chunk is in flight the copy runs beside it on a side
+    // stream (the walkers that had ended before the chunk are not written)
+    hipStream_t st = b->stream;
+    if (b->inflight) {
+        if (!b->side) HIPCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+        st = b->side;
+    }
     int *ws = reinterpret_cast<int *>(b->gat_pin);
     long long *dst = reinterpret_cast<long long *>(b->gat_pin + ibytes);
     long long o = 0;
@@ -2082,14 +2201,14 @@ int copy_walkers_out3(akmc_batch_t b, const double *const src[3], int nw, const 
         dst[k] = o;
         o += 3 * (long long)b->hw[walkers[k]].n;
     }
-    HIPCHK(hipMemcpyAsync(b->gat_dev, b->gat_pin, ibytes + obytes, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->gat_dev, b->gat_pin, ibytes + obytes, hipMemcpyHostToDevice, st));
     double *dout = reinterpret_cast<double *>(b->gat_dev + ibytes + obytes);
     akmc_launch_gather_walkers(b->d.walkers, src, reinterpret_cast<const int *>(b->gat_dev),
-                               reinterpret_cast<const long long *>(b->gat_dev + ibytes), nw, dout, b->stream);
+                               reinterpret_cast<const long long *>(b->gat_dev + ibytes), nw, dout, st);
     HIPCHK(hipGetLastError());
     double *hout = reinterpret_cast<double *>(b->gat_pin + ibytes + obytes);
-    HIPCHK(hipMemcpyAsync(hout, dout, total * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpyAsync(hout, dout, total * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     std::memcpy(out, hout, total * sizeof(double));
     return AKMC_SUCCESS;
 }

@@ -269,6 +269,12 @@ __device__ inline int atom_cell_of(const BatchDev &b, const WalkerDev &W, int i)
     return (ca * W.nc[1] + cb) * W.nc[2] + cc;
 }
 
+// k_cell_bin: one workgroup of kBinBlock threads per rebuilt walker; the
+// cells of walkers up to kBinAtoms atoms stay in LDS between the counting
+// and the placement
+constexpr int kBinBlock = 1024;
+constexpr int kBinAtoms = 8192;
+
 // home atoms per list-build work item (k_nbr_build_cells)
 constexpr int kNbSlotsQ = 16;
 
@@ -276,23 +282,25 @@ constexpr int kNbSlotsQ = 16;
 // counts, exclusive scan and a stable placement of the atoms in index order,
 // so every cell lists its atoms by ascending index and the neighbour lists
 // (and the FP64 sums that follow them) do not depend on scheduling.
-__global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
+__global__ void __launch_bounds__(kBinBlock) k_cell_bin(BatchDev b)
 {
     __shared__ int fill[AKMC_MAX_WALKER_CELLS];
-    __shared__ int chunk[kBlock], rank_[kBlock], gsize[kBlock];
+    __shared__ int chunk[kBinBlock], rank_[kBinBlock], gsize[kBinBlock];
+    __shared__ int acell[kBinAtoms];  // each atom's cell, kept from the counts (walkers up to kBinAtoms)
     __shared__ int carry;
     const int w = blockIdx.x;
     if (!b.rebuild[w]) return;
     const WalkerDev W = b.walkers[w];
     const int nc = W.nc[0] * W.nc[1] * W.nc[2];
     const int tid = threadIdx.x;
-    for (int c = tid; c < nc; c += kBlock) fill[c] = 0;
+    for (int c = tid; c < nc; c += kBinBlock) fill[c] = 0;
     // a restarted walker's zone starts empty (the list build marks it)
     if (b.zflag && (b.zflag[w] & AKMC_ZONE_RESET))
-        for (int l = tid; l < W.n; l += kBlock) b.zmark[W.off + l] = 0;
+        for (int l = tid; l < W.n; l += kBinBlock) b.zmark[W.off + l] = 0;
     __syncthreads();
+    const bool keep = W.n <= kBinAtoms;
     // the counts, and the positions the walker's lists are built from
-    for (int l = tid; l < W.n; l += kBlock) {
+    for (int l = tid; l < W.n; l += kBinBlock) {
         const int i = W.off + l;
         const double x = b.x[i], y = b.y[i], z = b.z[i];
         b.xr[i] = x;
@@ -300,17 +308,18 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
         b.zr[i] = z;
         const int cell = (cell_coord(x, W.invL[0], W.nc[0]) * W.nc[1] + cell_coord(y, W.invL[1], W.nc[1])) * W.nc[2] +
                          cell_coord(z, W.invL[2], W.nc[2]);
+        if (keep) acell[l] = cell;
         atomicAdd(&fill[cell], 1);
     }
     if (tid == 0) carry = W.off;
     __syncthreads();
     // exclusive scan of the counts; fill[c] becomes the cell's next free slot
-    for (int base = 0; base < nc; base += kBlock) {
+    for (int base = 0; base < nc; base += kBinBlock) {
         const int c = base + tid;
         const int v = c < nc ? fill[c] : 0;
         chunk[tid] = v;
         __syncthreads();
-        for (int o = 1; o < kBlock; o <<= 1) {
+        for (int o = 1; o < kBinBlock; o <<= 1) {
             const int t = tid >= o ? chunk[tid - o] : 0;
             __syncthreads();
             chunk[tid] += t;
@@ -323,17 +332,17 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
             fill[c] = start;
         }
         __syncthreads();
-        if (tid == kBlock - 1) carry += chunk[tid];
+        if (tid == kBinBlock - 1) carry += chunk[tid];
         __syncthreads();
     }
-    // stable placement in index order, kBlock atoms at a time: each
+    // stable placement in index order, kBinBlock atoms at a time: each
     // wavefront finds, by ballots, the lanes of its 64 atoms that hold the
     // same cell (rank below, group size, highest lane); then one wavefront
-    // walks the four groups of 64 in order: an atom's slot is the cell's fill
+    // walks the groups of 64 in order: an atom's slot is the cell's fill
     // plus its rank, and the highest lane of each group advances the fill
-    for (int base = 0; base < W.n; base += kBlock) {
+    for (int base = 0; base < W.n; base += kBinBlock) {
         const int l = base + tid;
-        const int c = l < W.n ? atom_cell_of(b, W, W.off + l) : -1;
+        const int c = l < W.n ? (keep ? acell[l] : atom_cell_of(b, W, W.off + l)) : -1;
         unsigned long long todo = __ballot(c >= 0);
         unsigned long long same = 0;
         while (todo) {
@@ -349,7 +358,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
         gsize[tid] = (c >= 0 && (same >> wl) == 1ull) ? __popcll(same) : 0;  // set on the group's highest lane
         __syncthreads();
         if (tid < 64) {
-            for (int wv = 0; wv < kBlock / 64; ++wv) {
+            for (int wv = 0; wv < kBinBlock / 64; ++wv) {
                 const int k = wv * 64 + tid;
                 const int ck = chunk[k];
                 if (ck >= 0) {
@@ -373,10 +382,10 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
         return k < 256 ? k : 256;
     };
     int mine = 0;
-    for (int c = tid; c < nc; c += kBlock) mine += items_of(c);
+    for (int c = tid; c < nc; c += kBinBlock) mine += items_of(c);
     chunk[tid] = mine;
     __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
+    for (int o = kBinBlock / 2; o > 0; o >>= 1) {
         if (tid < o) chunk[tid] += chunk[tid + o];
         __syncthreads();
     }
@@ -385,12 +394,12 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
         carry = 0;
     }
     __syncthreads();
-    for (int base = 0; base < nc; base += kBlock) {
+    for (int base = 0; base < nc; base += kBinBlock) {
         const int c = base + tid;
         const int v = c < nc ? items_of(c) : 0;
         chunk[tid] = v;
         __syncthreads();
-        for (int o = 1; o < kBlock; o <<= 1) {
+        for (int o = 1; o < kBinBlock; o <<= 1) {
             const int t = tid >= o ? chunk[tid - o] : 0;
             __syncthreads();
             chunk[tid] += t;
@@ -401,7 +410,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_bin(BatchDev b)
             for (int ch = 0; ch < v; ++ch) b.cell_work[at + ch] = ((W.cell_off + c) << 8) | ch;
         }
         __syncthreads();
-        if (tid == kBlock - 1) carry += chunk[tid];
+        if (tid == kBinBlock - 1) carry += chunk[tid];
         __syncthreads();
     }
 }
@@ -447,6 +456,8 @@ constexpr int kNbSlots = kBlock / kNbLanes;  // home atoms per work item
 static_assert(kNbSlots == kNbSlotsQ, "queued items and the list build agree on the chunk size");
 constexpr int kNbIlp = 4;                    // candidates per lane per round
 constexpr int kNbCap = 1024;                 // candidates staged in LDS at a time (28 KB)
+constexpr int kNbStage = kNbCap / kBlock;    // candidates per thread in one staging
+static_assert(kNbStage * kBlock == kNbCap, "the staging covers kNbCap candidates");
 
 __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
 {
@@ -465,46 +476,56 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
 {
     const int w = b.cell_wk[cg];
     const WalkerDev W = b.walkers[w];
-    if (threadIdx.x == 0) {
+    // the home atom's loads go out before the candidate cells are set up
+    const int s0 = b.cell_start[cg], n0 = b.cell_count[cg];
+    const int q = chunk * kNbSlots + threadIdx.x / kNbLanes;
+    const bool have = q < n0;
+    const int i = have ? b.cell_atoms[s0 + q] : 0;
+    const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
+    // candidate cells, one per lane of the first wavefront in (a, b, c)
+    // order, with an inclusive scan of their counts
+    if (threadIdx.x < 64) {
+        const int t = threadIdx.x;
         const int lc = cg - W.cell_off;
         const int cen[3] = {lc / (W.nc[1] * W.nc[2]), (lc / W.nc[2]) % W.nc[1], lc % W.nc[2]};
-        int lo[3], hi[3];
+        int lo[3], span[3];
         for (int a = 0; a < 3; ++a) {
-            if (W.nc[a] >= 3) {
-                lo[a] = cen[a] - 1;
-                hi[a] = cen[a] + 1;
-            } else {
-                lo[a] = 0;
-                hi[a] = W.nc[a] - 1;
-            }
+            lo[a] = W.nc[a] >= 3 ? cen[a] - 1 : 0;
+            span[a] = W.nc[a] >= 3 ? 3 : W.nc[a];
         }
-        int nc = 0;
-        c_pref[0] = 0;
-        for (int pa = lo[0]; pa <= hi[0]; ++pa)
-            for (int pb = lo[1]; pb <= hi[1]; ++pb)
-                for (int pc = lo[2]; pc <= hi[2]; ++pc) {
-                    const int qa = (pa + W.nc[0]) % W.nc[0], qb = (pb + W.nc[1]) % W.nc[1];
-                    const int qc = (pc + W.nc[2]) % W.nc[2];
-                    const int cid = W.cell_off + (qa * W.nc[1] + qb) * W.nc[2] + qc;
-                    c_start[nc] = b.cell_start[cid];
-                    c_pref[nc + 1] = c_pref[nc] + b.cell_count[cid];
-                    ++nc;
-                }
-        n_cells = nc;
+        const int ncell = span[0] * span[1] * span[2];
+        int st = 0, v = 0;
+        if (t < ncell) {
+            const int pa = lo[0] + t / (span[1] * span[2]), pb = lo[1] + (t / span[2]) % span[1];
+            const int pc = lo[2] + t % span[2];
+            const int qa = (pa + W.nc[0]) % W.nc[0], qb = (pb + W.nc[1]) % W.nc[1];
+            const int qc = (pc + W.nc[2]) % W.nc[2];
+            const int cid = W.cell_off + (qa * W.nc[1] + qb) * W.nc[2] + qc;
+            st = b.cell_start[cid];
+            v = b.cell_count[cid];
+        }
+        int sum = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(sum, o);
+            if (t >= o) sum += u;
+        }
+        if (t < ncell) {
+            c_start[t] = st;
+            c_pref[t + 1] = sum;
+        }
+        if (t == 0) {
+            c_pref[0] = 0;
+            n_cells = ncell;
+        }
     }
     __syncthreads();
     const int ncand = c_pref[n_cells];
     const double rl2 = b.field->rlist2;
     const double rn2 = b.field->rnear2;
     const double rz2 = b.field->rzone2;
-    const int s0 = b.cell_start[cg], n0 = b.cell_count[cg];
-    const int lane = threadIdx.x % kNbLanes, slot = threadIdx.x / kNbLanes;
+    const int lane = threadIdx.x % kNbLanes;
     const int wlane = threadIdx.x & 63;
     static_assert(kNbLanes == 16, "the hit ballot is split in 16-bit fields");
-    const int q = chunk * kNbSlots + slot;
-    const bool have = q < n0;
-    const int i = have ? b.cell_atoms[s0 + q] : 0;
-    const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
     // a free atom of a zoned walker puts itself and every atom of its list
     // within rc + skin/2 in the walker's zone
     const bool mark = have && b.zflag && (b.zflag[w] & AKMC_ZONE_ON) && b.zfrozen[i] == 0;
@@ -515,16 +536,44 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
         for (int seg = 0; seg < ncand; seg += kNbCap) {
             const int m = min(kNbCap, ncand - seg);
             if (!(pass == 1 && m == ncand && rn2 > 0.0)) {  // a single segment stays staged
+                // every thread's candidates: cell by binary search over the
+                // scan, index, then coordinates, each level's loads in flight
+                // together before the LDS stores
+                int jv[kNbStage];
+                double vx[kNbStage], vy[kNbStage], vz[kNbStage];
+#pragma unroll
+                for (int u = 0; u < kNbStage; ++u) {
+                    const int t = threadIdx.x + u * kBlock;
+                    jv[u] = -1;
+                    if (t < m) {
+                        const int qq = seg + t;
+                        int c = 0, hi = n_cells - 1;  // the last cell c with c_pref[c] <= qq
+                        while (c < hi) {
+                            const int mid = (c + hi + 1) >> 1;
+                            if (c_pref[mid] <= qq) c = mid;
+                            else hi = mid - 1;
+                        }
+                        jv[u] = b.cell_atoms[c_start[c] + qq - c_pref[c]];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kNbStage; ++u) {
+                    if (jv[u] >= 0) {
+                        vx[u] = b.x[jv[u]];
+                        vy[u] = b.y[jv[u]];
+                        vz[u] = b.z[jv[u]];
+                    }
+                }
                 __syncthreads();
-                for (int t = threadIdx.x; t < m; t += kBlock) {
-                    const int qq = seg + t;
-                    int c = 0;
-                    while (c_pref[c + 1] <= qq) ++c;
-                    const int j = b.cell_atoms[c_start[c] + qq - c_pref[c]];
-                    sj[t] = j;
-                    sx[t] = b.x[j];
-                    sy[t] = b.y[j];
-                    sz[t] = b.z[j];
+#pragma unroll
+                for (int u = 0; u < kNbStage; ++u) {
+                    const int t = threadIdx.x + u * kBlock;
+                    if (jv[u] >= 0) {
+                        sj[t] = jv[u];
+                        sx[t] = vx[u];
+                        sy[t] = vy[u];
+                        sz[t] = vz[u];
+                    }
                 }
                 __syncthreads();
             }
@@ -1952,7 +2001,7 @@ static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
     int gA = blocks(b.A);
     k_pack_check<<<gA, kBlock, 0, s>>>(b);
     if (hf.min_image) {
-        k_cell_bin<<<b.W, kBlock, 0, s>>>(b);  // records the reference positions too
+        k_cell_bin<<<b.W, kBinBlock, 0, s>>>(b);  // records the reference positions too
         k_nbr_build_cells<<<kNbGrid, kBlock, 0, s>>>(b);
     } else {
         k_ref_positions<<<gA, kBlock, 0, s>>>(b);

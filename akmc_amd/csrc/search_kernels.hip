@@ -1284,6 +1284,43 @@ void akmc_launch_probe_init(const BatchDev &b, const SearchDev &s, hipStream_t s
     k_probe_init<<<b.W, kBlock, 0, st>>>(b, s);
 }
 
+__global__ void k_start_walkers(BatchDev b, SearchDev s, const StartItem *items, int n, int *zflag, int *harvested)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const StartItem &it = items[k];
+    s.ws[it.walker] = it.ws;
+    if (zflag) zflag[it.walker] = it.zflag;
+    if (harvested) harvested[it.walker] = 0;
+    b.rebuild[it.walker] = 1;
+    b.active[it.walker] = it.active;
+}
+
+void akmc_launch_start_walkers(const BatchDev &b, const SearchDev &s, const StartItem *items, int nitems, int *zflag,
+                               int *harvested, hipStream_t st)
+{
+    if (nitems > 0)
+        k_start_walkers<<<(nitems + kBlock - 1) / kBlock, kBlock, 0, st>>>(b, s, items, nitems, zflag, harvested);
+}
+
+// one workgroup per walker
+__global__ void __launch_bounds__(kBlock) k_harvest_pack(BatchDev b, SearchDev s, int *harvested, double *out)
+{
+    const int w = blockIdx.x;
+    const WalkerState &ws = s.ws[w];
+    if (ws.kind == WK_IDLE || ws.state != ST_DONE || harvested[w]) return;
+    const WalkerDev W = b.walkers[w];
+    const size_t A = b.A;
+    for (int i = W.off + (int)threadIdx.x; i < W.off + W.n; i += kBlock)
+        for (int c = 0; c < 3; ++c) out[(size_t)c * A + i] = s.X[(size_t)c * A + i];
+    if (threadIdx.x == 0) harvested[w] = 1;
+}
+
+void akmc_launch_harvest_pack(const BatchDev &b, const SearchDev &s, int *harvested, double *out, hipStream_t st)
+{
+    k_harvest_pack<<<b.W, kBlock, 0, st>>>(b, s, harvested, out);
+}
+
 void akmc_launch_search_logic(const BatchDev &b, const SearchDev &s, hipStream_t st)
 {
     k_search_logic<<<b.W, kBlock, 0, st>>>(b, s);

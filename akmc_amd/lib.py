@@ -181,6 +181,9 @@ def load() -> C.CDLL:
     L.akmc_batch_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _I]
     L.akmc_batch_run_async.argtypes = [C.c_void_p, C.c_int]
     L.akmc_batch_run_wait.argtypes = [C.c_void_p, _I, _I]
+    L.akmc_batch_run_next.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_set_harvest.argtypes = [C.c_void_p, C.c_int]
+    L.akmc_batch_get_harvested_positions.argtypes = [C.c_void_p, C.c_int, _I, _D]
     L.akmc_batch_status.argtypes = [C.c_void_p, C.POINTER(WalkerStatus)]
     L.akmc_batch_get_positions.argtypes = [C.c_void_p, _D, _D, _D]
     L.akmc_batch_get_mode.argtypes = [C.c_void_p, _D]
@@ -448,6 +451,16 @@ class Batch:
         """{w: (3, n_w) search positions} for several walkers, one transfer."""
         return self._walkers_out(load().akmc_batch_get_walkers_positions, walkers)
 
+    def set_harvest(self, on=True):
+        """every run chunk copies the positions of the walkers that ended in
+        it to host memory (read with harvested_positions while the next
+        chunk runs)"""
+        check(load().akmc_batch_set_harvest(self._h, int(bool(on))))
+
+    def harvested_positions(self, walkers):
+        """{w: (3, n_w)} for walkers harvested by the last waited chunk"""
+        return self._walkers_out(load().akmc_batch_get_harvested_positions, walkers)
+
     def _walkers_out(self, fn, walkers):
         ws = np.ascontiguousarray(walkers, dtype=np.int32)
         n = self.natoms[ws]
@@ -493,6 +506,12 @@ class Batch:
     def run_async(self, steps):
         """enqueue one chunk of micro-steps and return at once"""
         check(load().akmc_batch_run_async(self._h, int(steps)))
+
+    def run_next(self, steps):
+        """run_async without the host round trip before the launch (the
+        states read back, the active flags written): for a caller that
+        changes walkers only through starts between chunks"""
+        check(load().akmc_batch_run_next(self._h, int(steps)))
 
     def run_wait(self):
         """wait for the chunk of run_async: (steps run, walkers still running)"""

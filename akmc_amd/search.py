@@ -147,6 +147,8 @@ class SaddleCampaign:
         self.basin_moves = 0
         self._regions = {}
         self._next = {}  # walker -> its next proposal, drawn ahead
+        self._inflight = False  # a pipelined chunk is running (advance(pipelined=True))
+        self._harvest = False  # chunks copy the ended walkers' positions to host memory
         self.pending = set()
         self.cur_site = {}
         A = self.batch.A
@@ -302,6 +304,8 @@ class SaddleCampaign:
     def harvest(self, stats: CampaignStats):
         """Collect ended searches (their FP64 verification already ran on
         the device); returns the walkers that ended."""
+        if self._inflight:
+            raise RuntimeError("a pipelined chunk is in flight: drain() first")
         done, raw = self.collect(stats)
         self.process(raw, stats)
         return done
@@ -314,7 +318,12 @@ class SaddleCampaign:
         st = self.batch.status_array()
         done = [w for w in sorted(self.pending) if st["done"][w]]
         good = [w for w in done if st["status"][w] == 0 and st["verified"][w]]
-        positions = self.batch.walkers_positions(good) if good else {}
+        if not good:
+            positions = {}
+        elif self._inflight:  # copied to host memory at the end of the chunk that was waited for
+            positions = self.batch.harvested_positions(good)
+        else:
+            positions = self.batch.walkers_positions(good)
         raw = []
         for w in done:
             s = st[w]
@@ -374,7 +383,7 @@ class SaddleCampaign:
         """Start a search on every walker."""
         self.start(range(self.W))
 
-    def advance(self, micro_steps, sync_every=25, refill=True):
+    def advance(self, micro_steps, sync_every=25, refill=True, pipelined=False):
         """Run micro_steps micro-steps (a multiple of sync_every), refilling
         walkers whose search ended; returns the searches that ended in this
         window.  Searches still running carry over to the next window.
@@ -382,7 +391,14 @@ class SaddleCampaign:
         Each chunk of sync_every micro-steps runs on the device while the
         host processes the previous chunk's records and draws the next
         proposals; between chunks the host only reads the ended searches
-        and uploads the drawn proposals."""
+        and uploads the drawn proposals.  Pipelined, the next chunk is
+        launched before that: the device does not wait for the host, and a
+        search that ends waits one more chunk for its refill (a chunk stays
+        in flight between calls; drain() ends it)."""
+        if pipelined:
+            return self._advance_pipelined(micro_steps, sync_every, refill)
+        if self._inflight:
+            raise RuntimeError("a pipelined chunk is in flight: drain() first")
         stats = CampaignStats()
         t0 = time.perf_counter()
         steps = 0
@@ -418,6 +434,69 @@ class SaddleCampaign:
         self.process(raw, stats)
         stats.search_time = time.perf_counter() - t0
         stats.micro_steps = steps
+        return stats
+
+    def _advance_pipelined(self, micro_steps, sync_every, refill):
+        stats = CampaignStats()
+        t0 = time.perf_counter()
+        steps = 0
+        raw = []
+        if not self._harvest:
+            self.batch.set_harvest(True)
+            self._harvest = True
+        if not self._inflight and self.pending:
+            self.batch.run_next(sync_every)
+            self._inflight = True
+        while steps < micro_steps and self._inflight:
+            ta = time.perf_counter()
+            self.process(raw, stats)
+            tb = time.perf_counter()
+            if refill:
+                self.precompute(sorted(self.pending))
+            tc = time.perf_counter()
+            got, _ = self.batch.run_wait()
+            self._inflight = False
+            steps += got
+            t1 = time.perf_counter()
+            # the next chunk at once; the walkers that ended wait through it
+            if self.pending:
+                self.batch.run_next(sync_every)
+                self._inflight = True
+            t2 = time.perf_counter()
+            done, raw = self.collect(stats)
+            t3 = time.perf_counter()
+            if refill and done:
+                self.start(done)
+                if not self._inflight:
+                    self.batch.run_next(sync_every)
+                    self._inflight = True
+            t4 = time.perf_counter()
+            stats.device_time += t1 - tc
+            stats.harvest_time += (t3 - t2) + (tb - ta)
+            stats.refill_time += (t4 - t3) + (tc - tb)
+            stats.t_launch += t2 - t1
+            stats.t_overlap += tc - ta
+            stats.t_wait += t1 - tc
+            stats.t_collect += t3 - t2
+            stats.t_start += t4 - t3
+            if got == 0 and not done:
+                break
+        self.process(raw, stats)  # beside the chunk left in flight
+        stats.search_time = time.perf_counter() - t0
+        stats.micro_steps = steps
+        return stats
+
+    def drain(self, stats: CampaignStats = None, refill=True):
+        """End a pipelined run: wait for the chunk in flight, record the
+        searches it ended and (refill) start their walkers' next ones."""
+        stats = stats if stats is not None else CampaignStats()
+        if self._inflight:
+            self.batch.run_wait()
+            self._inflight = False
+            done, raw = self.collect(stats)
+            if refill and done:
+                self.start(done)
+            self.process(raw, stats)
         return stats
 
     def run(self, sync_every=25, max_steps=10**9):

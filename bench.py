@@ -597,6 +597,8 @@ def main():
     ap.add_argument("--plain", type=int, default=32, help="plain GPU dimer searches")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skin", type=float, default=None, help="Verlet skin of the device lists (A)")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="windows wait for the refills before the next chunk (no chunk launched ahead)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal section")
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--train-saddles", type=int, default=50000,
@@ -697,6 +699,7 @@ def main():
             # the search micro-steps' own variants, tuned per pass on the
             # searches under way (zoned passes): untimed, between windows
             t0 = time.perf_counter()
+            camp.drain()
             sv, sv_ms = camp.batch.autotune_search(5)
             search_variant = {"pass1": sv[:3], "pass2": sv[3:6], "fp64": sv[6:9],
                               "seconds": round(time.perf_counter() - t0, 2),
@@ -707,7 +710,7 @@ def main():
                   f"FP64 {sv[6:9]}; {search_variant['ms_per_5_evals_pass1_pass2_fp64']}", file=sys.stderr,
                   flush=True)
         rb0 = camp.batch.rebuilds
-        st = camp.advance(args.window, sync_every=args.sync_every)
+        st = camp.advance(args.window, sync_every=args.sync_every, pipelined=not args.no_pipeline)
         print(f"[bench] window {r}: {camp.batch.rebuilds - rb0} walker list rebuilds; "
               f"{st.searches} ended, {st.converged} converged, {st.verified} verified, "
               f"{st.distinct} distinct, {st.micro_steps} micro-steps in {st.search_time:.2f}s, "
@@ -722,6 +725,7 @@ def main():
     # window of the same searches that follows the timed ones (events inside
     # replayed graphs cannot be timed, so this window launches eagerly), with
     # the FP32 evaluations each walker consumed there
+    camp.drain()
     counts0 = camp.batch.eval_counts()
     camp.batch.kernel_timer(True)
     rst = camp.advance(args.roofline_window, sync_every=25)

@@ -285,10 +285,32 @@ int akmc_batch_run(akmc_batch_t b, int max_steps, int sync_every, int *steps_don
  * device runs it (no counterpart in the reference): run_async enqueues
  * steps micro-steps and the status copy and returns; run_wait waits for
  * them and reports the steps run (0 when no walker was running) and the
- * walkers still running.  Between the two, no other call may be made on
- * the batch. */
+ * walkers still running.  Between the two, the only calls allowed on the
+ * batch are akmc_batch_status (which then returns the states at the end of
+ * the previous chunk, with the starts made since), the walker position and
+ * saddle copies (made beside the chunk: defined for walkers that had ended
+ * before it) and akmc_batch_start_dimer / akmc_batch_start_art of walkers
+ * that had ended (queued behind the chunk: they run from the next one). */
 int akmc_batch_run_async(akmc_batch_t b, int steps);
 int akmc_batch_run_wait(akmc_batch_t b, int *steps_done, int *running);
+/* akmc_batch_run_async without its host round trip (the states read back
+ * and the active flags written before the launch): for a caller that only
+ * changes walkers through the akmc_batch_start_* calls between chunks, so
+ * the device's active flags and the host's copy of the states (the last
+ * wait's, with the starts made since) are current.  A pipelined caller
+ * launches the next chunk right after a wait and reads and refills the
+ * ended searches while it runs.  No counterpart in the reference. */
+int akmc_batch_run_next(akmc_batch_t b, int steps);
+/* harvest (no counterpart in the reference): while on, every run chunk ends
+ * by copying the search positions of the walkers that ended in it (and of
+ * those that had ended before harvest was turned on) into one of two
+ * mapped host buffers, used in turn, so a pipelined caller reads them while
+ * the next chunk runs.  akmc_batch_get_harvested_positions returns, laid
+ * out as akmc_batch_get_walkers_positions, those of walkers harvested by
+ * the chunk of the last akmc_batch_run_wait (other walkers' slices hold
+ * older data).  Needs a started search; not while a chunk is in flight. */
+int akmc_batch_set_harvest(akmc_batch_t b, int on);
+int akmc_batch_get_harvested_positions(akmc_batch_t b, int nw, const int *walkers, double *out);
 /* enable (default) or disable the hipGraph replay of akmc_batch_run; when
  * disabled the same kernels are launched one by one on the batch's stream
  * (for tools that cannot trace graph launches).  Results are identical. */
