@@ -457,6 +457,7 @@ static_assert(kNbSlots == kNbSlotsQ, "queued items and the list build agree on t
 constexpr int kNbIlp = 4;                    // candidates per lane per round
 constexpr int kNbCap = 1024;                 // candidates staged in LDS at a time (28 KB)
 constexpr int kNbStage = kNbCap / kBlock;    // candidates per thread in one staging
+constexpr int kNbWaves = kBlock / 64;
 static_assert(kNbStage * kBlock == kNbCap, "the staging covers kNbCap candidates");
 
 __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
@@ -472,14 +473,14 @@ __device__ inline void put_nbr(const BatchDev &b, int i, int k, uint32_t e)
 
 // one work item: home atoms [chunk * kNbSlots, +kNbSlots) of cell cg
 __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx, double *sy, double *sz, int *sj,
-                               int *c_start, int *c_pref, int &n_cells)
+                               int *c_start, int *c_pref, int &n_cells, double *box, int *scnt)
 {
     const int w = b.cell_wk[cg];
     const WalkerDev W = b.walkers[w];
     // the home atom's loads go out before the candidate cells are set up
     const int s0 = b.cell_start[cg], n0 = b.cell_count[cg];
     const int q = chunk * kNbSlots + threadIdx.x / kNbLanes;
-    const bool have = q < n0;
+    const bool have = q < n0;  // slot 0 always has an atom
     const int i = have ? b.cell_atoms[s0 + q] : 0;
     const double xi = have ? b.x[i] : 0.0, yi = have ? b.y[i] : 0.0, zi = have ? b.z[i] : 0.0;
     // candidate cells, one per lane of the first wavefront in (a, b, c)
@@ -518,8 +519,33 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
             n_cells = ncell;
         }
     }
+    // the home atoms' box: their offsets from the item's first atom (all in
+    // one cell, so these are their true separations), as a centre and half
+    // widths per axis
+    const int slot = threadIdx.x / kNbLanes;
+    if (have && threadIdx.x % kNbLanes == 0) {
+        double dx = xi - b.x[b.cell_atoms[s0 + chunk * kNbSlots]], dy = yi - b.y[b.cell_atoms[s0 + chunk * kNbSlots]],
+               dz = zi - b.z[b.cell_atoms[s0 + chunk * kNbSlots]];
+        min_image(dx, dy, dz, W);
+        box[6 + 3 * slot] = dx;
+        box[6 + 3 * slot + 1] = dy;
+        box[6 + 3 * slot + 2] = dz;
+    }
     __syncthreads();
     const int ncand = c_pref[n_cells];
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x, ns = min(kNbSlots, n0 - chunk * kNbSlots);
+        double lo = box[6 + a], hi = lo;
+        for (int k = 1; k < ns; ++k) {
+            lo = fmin(lo, box[6 + 3 * k + a]);
+            hi = fmax(hi, box[6 + 3 * k + a]);
+        }
+        const int i0 = b.cell_atoms[s0 + chunk * kNbSlots];
+        box[a] = (a == 0 ? b.x[i0] : a == 1 ? b.y[i0] : b.z[i0]) + 0.5 * (lo + hi);
+        box[3 + a] = 0.5 * (hi - lo);
+    }
+    __syncthreads();
+    const double bcx = box[0], bcy = box[1], bcz = box[2], bhx = box[3], bhy = box[4], bhz = box[5];
     const double rl2 = b.field->rlist2;
     const double rn2 = b.field->rnear2;
     const double rz2 = b.field->rzone2;
@@ -531,6 +557,8 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
     const bool mark = have && b.zflag && (b.zflag[w] & AKMC_ZONE_ON) && b.zfrozen[i] == 0;
     if (mark && lane == 0) b.zmark[i] = 1;
     int cnt = 0, nnear = 0;
+    int ms = 0;  // candidates staged in LDS
+    const int wave = threadIdx.x / 64;
     // pass 0 lists the near shell (r^2 <= rnear2), pass 1 the rest
     for (int pass = (rn2 > 0.0 ? 0 : 1); pass < 2; ++pass) {
         for (int seg = 0; seg < ncand; seg += kNbCap) {
@@ -564,20 +592,52 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
                         vz[u] = b.z[jv[u]];
                     }
                 }
-                __syncthreads();
+                // only candidates within the list radius of the home atoms'
+                // box are staged, compacted in candidate order (the others
+                // are no home atom's neighbours, so the lists are unchanged)
+                bool keep[kNbStage];
+                int rank[kNbStage];
 #pragma unroll
                 for (int u = 0; u < kNbStage; ++u) {
-                    const int t = threadIdx.x + u * kBlock;
+                    keep[u] = false;
                     if (jv[u] >= 0) {
-                        sj[t] = jv[u];
-                        sx[t] = vx[u];
-                        sy[t] = vy[u];
-                        sz[t] = vz[u];
+                        double dx = vx[u] - bcx, dy = vy[u] - bcy, dz = vz[u] - bcz;
+                        min_image(dx, dy, dz, W);
+                        const double ex = fmax(fabs(dx) - bhx, 0.0), ey = fmax(fabs(dy) - bhy, 0.0),
+                                     ez = fmax(fabs(dz) - bhz, 0.0);
+                        keep[u] = ex * ex + ey * ey + ez * ez <= rl2 * (1.0 + 1.0e-9);
+                    }
+                    const unsigned long long bal = __ballot(keep[u]);
+                    rank[u] = __popcll(bal & ((1ull << wlane) - 1ull));
+                    if (wlane == 0) scnt[u * kNbWaves + wave] = __popcll(bal);
+                }
+                __syncthreads();
+                // a kept candidate's slot: the kept ones of earlier rounds,
+                // of earlier wavefronts in its round, and of lower lanes
+                int total = 0;
+#pragma unroll
+                for (int u = 0; u < kNbStage; ++u) {
+                    for (int v = 0; v < kNbWaves; ++v) {
+                        const int c = scnt[u * kNbWaves + v];
+                        if (v < wave) rank[u] += c;
+                        total += c;
+                    }
+                    if (u + 1 < kNbStage) rank[u + 1] += total;
+                }
+                ms = total;
+#pragma unroll
+                for (int u = 0; u < kNbStage; ++u) {
+                    if (keep[u]) {
+                        const int pos = rank[u];
+                        sj[pos] = jv[u];
+                        sx[pos] = vx[u];
+                        sy[pos] = vy[u];
+                        sz[pos] = vz[u];
                     }
                 }
                 __syncthreads();
             }
-            for (int g = 0; g < m; g += kNbLanes * kNbIlp) {
+            for (int g = 0; g < ms; g += kNbLanes * kNbIlp) {
                 int hit[kNbIlp], jj[kNbIlp];
                 bool zin[kNbIlp];
 #pragma unroll
@@ -586,7 +646,7 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
                     hit[u] = 0;
                     jj[u] = 0;
                     zin[u] = false;
-                    if (have && t < m) {
+                    if (have && t < ms) {
                         jj[u] = sj[t];
                         double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
                         min_image(dx, dy, dz, W);
@@ -633,13 +693,15 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_cells(BatchDev b)
     __shared__ int sj[kNbCap];
     __shared__ int c_start[27], c_pref[28];
     __shared__ int n_cells;
+    __shared__ double box[6 + 3 * kNbSlots];
+    __shared__ int scnt[kNbStage * kNbWaves];
     const int items = *b.cell_work_n;
     for (int it = blockIdx.x; it < items; it += gridDim.x) {
         const int code = b.cell_work[it];
         const int cg = code >> 8, chunk0 = code & 255;
         const int n0 = b.cell_count[cg];
         for (int chunk = chunk0; chunk * kNbSlots < n0; chunk += 256) {
-            nbr_build_item(b, cg, chunk, sx, sy, sz, sj, c_start, c_pref, n_cells);
+            nbr_build_item(b, cg, chunk, sx, sy, sz, sj, c_start, c_pref, n_cells, box, scnt);
             __syncthreads();
         }
     }
