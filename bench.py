@@ -816,6 +816,7 @@ def main():
                             "force_calls_per_search": round(plain_calls / max(1, args.plain), 1),
                             "ms_per_force_call": round(1e3 * t_plain / max(1, plain_calls), 4)},
         "batched_force_calls_per_verified": round(tot.force_calls / max(1, tot.verified), 1),
+        "windows": {"pipelined": not args.no_pipeline, "micro_steps_per_chunk": args.sync_every},
         "basin_turnover": {"searches_per_basin": args.searches_per_basin, "pool_basins_per_walker": args.basin_pool,
                            "basin_moves": int(moves_timed)},
         "batched_over_plain": round(value / plain_rate, 2) if plain_rate > 0 else None,
