@@ -242,9 +242,9 @@ class SaddleCampaign:
         if hit is None:
             c = self.cells[w]
             site = c.vacancies[v]
-            basin = self.basins[w]
-            frozen = active_region(c, basin, site, self.region_radius)
-            nb = W.neighbours_of_site(W.Cell(basin, c.types, c.lat), site, 12)
+            r = W.site_distances(W.Cell(self.basins[w], c.types, c.lat), site)  # both from one pass
+            frozen = (r >= self.region_radius).astype(np.int32)
+            nb = W.neighbours_of_site(c, site, 12, r=r)
             hit = self._regions[key] = (frozen, nb)
         return hit
 

@@ -91,10 +91,24 @@ def sc_cell(nx, ny, nz, n_vac=1, jitter=0.01, seed=0, a=SC_CU["a"], two_species=
                 vacancies=[sites[:, v].copy() for v in vac], meta=dict(seed=seed, nx=nx, ny=ny, nz=nz, jitter=jitter))
 
 
-def neighbours_of_site(cell: Cell, site: np.ndarray, count: int = 12) -> np.ndarray:
-    d = min_image(cell.pos - site[:, None], cell.lat)
-    r = np.sqrt((d * d).sum(0))
-    return np.argsort(r, kind="stable")[:count]
+def site_distances(cell: Cell, site: np.ndarray) -> np.ndarray:
+    """minimum-image distance of every atom from site"""
+    d = min_image(cell.pos - np.asarray(site)[:, None], cell.lat)
+    return np.sqrt((d * d).sum(0))
+
+
+def neighbours_of_site(cell: Cell, site: np.ndarray, count: int = 12, r: np.ndarray | None = None) -> np.ndarray:
+    """The count atoms nearest to site (minimum image), nearest first, ties
+    in index order: a stable sort of the atoms within the count-th distance
+    (the first count of a stable sort of all of them, without sorting all).
+    r: the distances from site_distances, when already at hand."""
+    if r is None:
+        r = site_distances(cell, site)
+    if count >= r.size:
+        return np.argsort(r, kind="stable")[:count]
+    cut = np.partition(r, count - 1)[count - 1]
+    near = np.nonzero(r <= cut)[0]
+    return near[np.argsort(r[near], kind="stable")[:count]]
 
 
 def local_displacement(cell: Cell, atoms: np.ndarray, d0: float, seed: int):
