@@ -519,13 +519,13 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
             n_cells = ncell;
         }
     }
-    // the home atoms' box: their offsets from the item's first atom (all in
-    // one cell, so these are their true separations), as a centre and half
-    // widths per axis
+    // the home atoms' box: their minimum-image offsets from the item's first
+    // atom, as a centre and half widths per axis (an arc of the periodic cell
+    // along each axis that holds every home atom, whatever the cell size)
     const int slot = threadIdx.x / kNbLanes;
+    const int i0 = b.cell_atoms[s0 + chunk * kNbSlots];
     if (have && threadIdx.x % kNbLanes == 0) {
-        double dx = xi - b.x[b.cell_atoms[s0 + chunk * kNbSlots]], dy = yi - b.y[b.cell_atoms[s0 + chunk * kNbSlots]],
-               dz = zi - b.z[b.cell_atoms[s0 + chunk * kNbSlots]];
+        double dx = xi - b.x[i0], dy = yi - b.y[i0], dz = zi - b.z[i0];
         min_image(dx, dy, dz, W);
         box[6 + 3 * slot] = dx;
         box[6 + 3 * slot + 1] = dy;
@@ -540,7 +540,6 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
             lo = fmin(lo, box[6 + 3 * k + a]);
             hi = fmax(hi, box[6 + 3 * k + a]);
         }
-        const int i0 = b.cell_atoms[s0 + chunk * kNbSlots];
         box[a] = (a == 0 ? b.x[i0] : a == 1 ? b.y[i0] : b.z[i0]) + 0.5 * (lo + hi);
         box[3 + a] = 0.5 * (hi - lo);
     }
@@ -593,8 +592,11 @@ __device__ void nbr_build_item(const BatchDev &b, int cg, int chunk, double *sx,
                     }
                 }
                 // only candidates within the list radius of the home atoms'
-                // box are staged, compacted in candidate order (the others
-                // are no home atom's neighbours, so the lists are unchanged)
+                // box are staged, compacted in candidate order; the distance
+                // to the box per axis is the candidate's minimum-image offset
+                // from the centre less the half width, never more than its
+                // offset from any home atom, so every neighbour stays and
+                // the lists are unchanged
                 bool keep[kNbStage];
                 int rank[kNbStage];
 #pragma unroll
