@@ -282,12 +282,35 @@ constexpr int kNbSlotsQ = 16;
 // counts, exclusive scan and a stable placement of the atoms in index order,
 // so every cell lists its atoms by ascending index and the neighbour lists
 // (and the FP64 sums that follow them) do not depend on scheduling.
+// inclusive scan of v over the kBinBlock threads: wavefront shuffles, then
+// the wavefront totals in order (two barriers); total gets the sum
+__device__ inline int bin_scan(int v, int *wsum, int &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(x, o);
+        if (lane >= o) x += t;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int k = 0; k < kBinBlock / 64; ++k) {
+        const int t = wsum[k];
+        if (k < wave) before += t;
+        all += t;
+    }
+    __syncthreads();  // the next scan rewrites wsum
+    total = all;
+    return x + before;
+}
+
 __global__ void __launch_bounds__(kBinBlock) k_cell_bin(BatchDev b)
 {
     __shared__ int fill[AKMC_MAX_WALKER_CELLS];
     __shared__ int chunk[kBinBlock], rank_[kBinBlock], gsize[kBinBlock];
     __shared__ int acell[kBinAtoms];  // each atom's cell, kept from the counts (walkers up to kBinAtoms)
-    __shared__ int carry;
+    __shared__ int wsum[kBinBlock / 64];
     const int w = blockIdx.x;
     if (!b.rebuild[w]) return;
     const WalkerDev W = b.walkers[w];
@@ -311,30 +334,23 @@ __global__ void __launch_bounds__(kBinBlock) k_cell_bin(BatchDev b)
         if (keep) acell[l] = cell;
         atomicAdd(&fill[cell], 1);
     }
-    if (tid == 0) carry = W.off;
     __syncthreads();
     // exclusive scan of the counts; fill[c] becomes the cell's next free slot
+    int carry = W.off;
     for (int base = 0; base < nc; base += kBinBlock) {
         const int c = base + tid;
         const int v = c < nc ? fill[c] : 0;
-        chunk[tid] = v;
-        __syncthreads();
-        for (int o = 1; o < kBinBlock; o <<= 1) {
-            const int t = tid >= o ? chunk[tid - o] : 0;
-            __syncthreads();
-            chunk[tid] += t;
-            __syncthreads();
-        }
+        int tot;
+        const int incl = bin_scan(v, wsum, tot);
         if (c < nc) {
-            const int start = carry + chunk[tid] - v;
+            const int start = carry + incl - v;
             b.cell_count[W.cell_off + c] = v;
             b.cell_start[W.cell_off + c] = start;
             fill[c] = start;
         }
-        __syncthreads();
-        if (tid == kBinBlock - 1) carry += chunk[tid];
-        __syncthreads();
+        carry += tot;
     }
+    __syncthreads();
     // stable placement in index order, kBinBlock atoms at a time: each
     // wavefront finds, by ballots, the lanes of its 64 atoms that hold the
     // same cell (rank below, group size, highest lane); then one wavefront
@@ -383,35 +399,21 @@ __global__ void __launch_bounds__(kBinBlock) k_cell_bin(BatchDev b)
     };
     int mine = 0;
     for (int c = tid; c < nc; c += kBinBlock) mine += items_of(c);
-    chunk[tid] = mine;
+    int items;
+    (void)bin_scan(mine, wsum, items);
+    if (tid == 0) qbase = atomicAdd(b.cell_work_n, items);
     __syncthreads();
-    for (int o = kBinBlock / 2; o > 0; o >>= 1) {
-        if (tid < o) chunk[tid] += chunk[tid + o];
-        __syncthreads();
-    }
-    if (tid == 0) {
-        qbase = atomicAdd(b.cell_work_n, chunk[0]);
-        carry = 0;
-    }
-    __syncthreads();
+    carry = 0;
     for (int base = 0; base < nc; base += kBinBlock) {
         const int c = base + tid;
         const int v = c < nc ? items_of(c) : 0;
-        chunk[tid] = v;
-        __syncthreads();
-        for (int o = 1; o < kBinBlock; o <<= 1) {
-            const int t = tid >= o ? chunk[tid - o] : 0;
-            __syncthreads();
-            chunk[tid] += t;
-            __syncthreads();
-        }
+        int tot;
+        const int incl = bin_scan(v, wsum, tot);
         if (c < nc) {
-            const int at = qbase + carry + chunk[tid] - v;
+            const int at = qbase + carry + incl - v;
             for (int ch = 0; ch < v; ++ch) b.cell_work[at + ch] = ((W.cell_off + c) << 8) | ch;
         }
-        __syncthreads();
-        if (tid == kBinBlock - 1) carry += chunk[tid];
-        __syncthreads();
+        carry += tot;
     }
 }
 
