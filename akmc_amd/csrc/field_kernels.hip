@@ -794,7 +794,7 @@ __global__ void __launch_bounds__(kBlock) k_nbr_build_images(BatchDev b)
 // index order, each thread a contiguous run of the walker's atoms.
 __global__ void __launch_bounds__(kBlock) k_zone_build(BatchDev b, unsigned long long *count)
 {
-    __shared__ int part[kBlock];
+    __shared__ int part[kBlock / 64];
     const int w = blockIdx.x;
     if (!b.rebuild[w]) return;
     const int f = b.zflag[w];
@@ -809,21 +809,31 @@ __global__ void __launch_bounds__(kBlock) k_zone_build(BatchDev b, unsigned long
     const WalkerDev W = b.walkers[w];
     const int per = (W.n + kBlock - 1) / kBlock;
     const int l0 = (int)threadIdx.x * per, l1 = min(W.n, l0 + per);
+    // the run's marks, kept as bits (the first 32) so the listing below
+    // stores without reloading them
     int c = 0;
-    for (int l = l0; l < l1; ++l) c += b.zmark[W.off + l] != 0;
-    part[threadIdx.x] = c;
-    __syncthreads();
-    for (int o = 1; o < kBlock; o <<= 1) {
-        const int t = (int)threadIdx.x >= o ? part[threadIdx.x - o] : 0;
-        __syncthreads();
-        part[threadIdx.x] += t;
-        __syncthreads();
+    unsigned bits = 0;
+    for (int l = l0; l < l1; ++l) {
+        const bool m = b.zmark[W.off + l] != 0;
+        c += m;
+        if (m && l - l0 < 32) bits |= 1u << (l - l0);
     }
-    int at = part[threadIdx.x] - c;
+    // inclusive scan of the runs' counts: wavefront shuffles, then the
+    // wavefront totals in order
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) part[wave] = incl;
+    __syncthreads();
+    for (int k = 0; k < wave; ++k) incl += part[k];
+    int at = incl - c;
     for (int l = l0; l < l1; ++l)
-        if (b.zmark[W.off + l]) b.zlist[W.off + at++] = l;
+        if (l - l0 < 32 ? ((bits >> (l - l0)) & 1u) != 0 : b.zmark[W.off + l] != 0) b.zlist[W.off + at++] = l;
     if (threadIdx.x == kBlock - 1) {
-        b.nz[w] = part[kBlock - 1];
+        b.nz[w] = incl;
         b.zflag[w] = f & ~AKMC_ZONE_RESET;
         b.rebuild[w] = 0;
         atomicAdd(count, 1ull);
