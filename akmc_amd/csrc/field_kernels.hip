@@ -238,6 +238,33 @@ __global__ void k_pack_check(BatchDev b)
     if (b.zflag && (b.zflag[w] & AKMC_ZONE_RESET)) b.rebuild[w] = 1;
 }
 
+// k_pack_check for batches with zones (search micro-steps): one workgroup
+// per walker, over its zone when the zone is established (the other atoms
+// have not moved since the walker's start), else over all its atoms
+constexpr int kPackBlock = 1024;
+__global__ void __launch_bounds__(kPackBlock) k_pack_check_zoned(BatchDev b)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *b.cell_work_n = 0;  // k_cell_bin queues this step's cells
+    const int w = blockIdx.x;
+    if (skipped(b, w)) return;
+    const WalkerDev W = b.walkers[w];
+    const int f = b.zflag[w];
+    const bool zoned = (f & (AKMC_ZONE_ON | AKMC_ZONE_RESET)) == AKMC_ZONE_ON;
+    const int n = zoned ? b.nz[w] : W.n;
+    const double h2 = b.field->half_skin2;
+    bool moved = false;
+    for (int k = threadIdx.x; k < n; k += kPackBlock) {
+        const int i = W.off + (zoned ? b.zlist[W.off + k] : k);
+        const double x = b.x[i], y = b.y[i], z = b.z[i];
+        b.p4[i] = make_double4(x, y, z, (double)b.type[i]);
+        const double dx = x - b.xr[i], dy = y - b.yr[i], dz = z - b.zr[i];
+        moved |= dx * dx + dy * dy + dz * dz > h2;
+    }
+    // a walker's fresh zone is marked by its list build (its start already
+    // asks for one)
+    if (moved || (f & AKMC_ZONE_RESET)) b.rebuild[w] = 1;
+}
+
 // record the positions the lists are built from (positions are never
 // wrapped on the device: the minimum-image and explicit-image kernels are
 // periodic in them)
@@ -2075,7 +2102,8 @@ extern "C" unsigned long long *akmc_rebuild_counter_ptr()
 static void launch_lists(const BatchDev &b, const FieldDev &hf, hipStream_t s)
 {
     int gA = blocks(b.A);
-    k_pack_check<<<gA, kBlock, 0, s>>>(b);
+    if (b.zflag) k_pack_check_zoned<<<b.W, kPackBlock, 0, s>>>(b);
+    else k_pack_check<<<gA, kBlock, 0, s>>>(b);
     if (hf.min_image) {
         k_cell_bin<<<b.W, kBinBlock, 0, s>>>(b);  // records the reference positions too
         k_nbr_build_cells<<<kNbGrid, kBlock, 0, s>>>(b);
