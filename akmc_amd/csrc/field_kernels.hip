@@ -1981,13 +1981,13 @@ __global__ void __launch_bounds__(kBlock) k_ewald_force(BatchDev b)
 // ----------------------------------------------------- walker reductions
 
 // Workgroup sums of NV values in a fixed order: a butterfly over the 64
-// lanes of each wavefront, then the kBlock / 64 wavefront partials in
-// order, so every thread gets the same bits whatever the batch holds.
-template <int NV>
+// lanes of each wavefront, then the NT / 64 wavefront partials in order, so
+// every thread gets the same bits whatever the batch holds.
+template <int NV, int NT = kBlock>
 __device__ inline void block_sum(double (&v)[NV], double *sh)
 {
-    constexpr int kW = kBlock / 64;
-    static_assert(NV * kW <= kBlock, "partials fit the scratch");
+    constexpr int kW = NT / 64;
+    static_assert(NV * kW <= NT, "partials fit the scratch");
 #pragma unroll
     for (int c = 0; c < NV; ++c)
         for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
@@ -2003,6 +2003,37 @@ __device__ inline void block_sum(double (&v)[NV], double *sh)
         v[c] = t;
     }
     __syncthreads();
+}
+
+// The many-body fields' walker sums without stress (the search micro-steps):
+// the two energy terms only, so a workgroup of kReduceBlock threads keeps its
+// registers; fixed summation order (deterministic)
+constexpr int kReduceBlock = 1024;
+__global__ void __launch_bounds__(kReduceBlock) k_walker_reduce_many(BatchDev b)
+{
+    __shared__ double sh[kReduceBlock];
+    const int w = blockIdx.x;
+    if (skipped(b, w)) return;
+    const WalkerDev W = b.walkers[w];
+    const bool e64 = b.active[w] == 2 && b.e_a64;
+    const double *ea = e64 ? b.e_a64 : b.e_a, *eb = e64 ? b.e_b64 : b.e_b;
+    double v[2] = {0.0, 0.0};
+    for (int i = W.off + threadIdx.x; i < W.off + W.n; i += kReduceBlock) {
+        v[0] += ea[i];
+        v[1] += eb[i];
+    }
+    block_sum<2, kReduceBlock>(v, sh);
+    if (threadIdx.x == 0) {
+        WalkerOut o;
+        o.many = v[0];
+        o.vdw = v[1];
+        o.real = o.rcp = o.self = 0.0;
+        o.total = o.many + o.vdw;
+        for (int c = 0; c < 6; ++c) o.stress[c] = 0.0;
+        b.out[w] = o;
+        // the walker's full evaluation in this precision is done
+        if (b.zflag) b.zflag[w] &= ~(b.active[w] == 2 ? AKMC_ZONE_FULL64 : AKMC_ZONE_FULL32);
+    }
 }
 
 // one workgroup per walker; fixed summation order (deterministic)
@@ -2391,7 +2422,8 @@ void akmc_launch_phase(const BatchDev &b, const FieldDev &hf, int prec, int want
         }
         break;
     default:
-        k_walker_reduce<<<b.W, kBlock, 0, s>>>(b, want_stress);
+        if (hf.kind == 0 && !want_stress) k_walker_reduce_many<<<b.W, kReduceBlock, 0, s>>>(b);
+        else k_walker_reduce<<<b.W, kBlock, 0, s>>>(b, want_stress);
     }
 }
 
