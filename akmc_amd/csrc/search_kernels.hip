@@ -695,11 +695,23 @@ __device__ __attribute__((always_inline)) void reorthogonalise_lds(const Ctx &C_
     const int na = C_.nact, ne = 3 * na;
     const size_t A3 = 3 * (size_t)C_.A;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int idx = threadIdx.x; idx < (k + 2) * ne; idx += kBlock) {
-        const int j = idx / ne, e = idx - j * ne;
+    // a thread's elements e: their free atom and component once, then the
+    // k + 2 vectors' values, kLzIlp loads in flight before their LDS stores
+    constexpr int kLzIlp = 4;
+    for (int e = threadIdx.x; e < ne; e += kBlock) {
         const int c = e / na, t = e - c * na;
-        const int a = C_.s.act[C_.off + t];
-        lq[idx] = j <= k ? V3(C_.s.Q + j * A3, c, a) : V3(C_.s.T, c, a);
+        const size_t o = (size_t)c * C_.A + C_.s.act[C_.off + t];
+        for (int j0 = 0; j0 < k + 2; j0 += kLzIlp) {
+            double v[kLzIlp];
+#pragma unroll
+            for (int u = 0; u < kLzIlp; ++u) {
+                const int j = j0 + u;
+                if (j < k + 2) v[u] = j <= k ? C_.s.Q[j * A3 + o] : C_.s.T[o];
+            }
+#pragma unroll
+            for (int u = 0; u < kLzIlp; ++u)
+                if (j0 + u < k + 2) lq[(j0 + u) * ne + e] = v[u];
+        }
     }
     double *T = lq + (k + 1) * ne;
     __syncthreads();
